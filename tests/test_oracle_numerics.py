@@ -1,0 +1,163 @@
+"""Pins the oracle's quadrature, shape functions, geometry and kernel against
+the reference's own test assertions (test/runtests.jl) and against
+independent computations (numpy's Gauss-Legendre, closed-form geometry)."""
+import math
+
+import numpy as np
+import pytest
+
+
+# --- test/runtests.jl:16-31 ----------------------------------------------------
+@pytest.mark.parametrize("n", [1, 2, 3, 4, 5])
+def test_gl_tables(orc, n):
+    p, w = orc.gl1d(n)
+    assert len(p) == n
+    assert abs(w.sum() - 2.0) <= 1e-12
+    qp, qw = orc.quad_rule(n)
+    assert qp.shape == (n * n, 2)
+    assert abs(qw.sum() - 4.0) <= 1e-12
+    # independent: numpy's Golub-Welsch/Newton nodes agree to ~1 ulp
+    rp, rw = np.polynomial.legendre.leggauss(n)
+    assert np.allclose(p, rp, rtol=0, atol=4e-16)
+    assert np.allclose(w, rw, rtol=0, atol=4e-16)
+    # xi fastest ordering (Quadrature.jl:102-107)
+    assert np.array_equal(qp[:n, 0], p) and np.all(qp[:n, 1] == p[0])
+    assert np.allclose(qw.reshape(n, n), np.outer(w, w))
+
+
+def test_gl_x4(orc):
+    p, w = orc.gl1d(3)
+    assert abs(np.dot(w, p ** 4) - 2 / 5) <= 1e-12
+
+
+def test_gl_closed_forms_bitwise(orc):
+    # the table values are closed forms evaluated with correctly rounded sqrt/div
+    p, w = orc.gl1d(4)
+    a = math.sqrt((3 + 2 * math.sqrt(6 / 5)) / 7)
+    assert p[3] == a and p[0] == -a and w[0] == (18 - math.sqrt(30)) / 36
+    p, w = orc.gl1d(5)
+    assert p[4] == math.sqrt(5 + 2 * math.sqrt(10 / 7)) / 3 and w[2] == 128 / 225
+
+
+# --- test/runtests.jl:34-48 (n > 5; reference uses LAPACK eigen) -----------------
+@pytest.mark.parametrize("n", [6, 7, 8])
+def test_gl_large_n(orc, n):
+    p, w = orc.gl1d(n)
+    assert abs(w.sum() - 2.0) <= 1e-12
+    rp, rw = np.polynomial.legendre.leggauss(n)
+    assert np.allclose(p, rp, atol=1e-14) and np.allclose(w, rw, atol=1e-14)
+    if n == 6:
+        assert abs(np.dot(w, p ** 10) - 2 / 11) <= 1e-12
+
+
+# --- test/runtests.jl:258-275 (Dunavant; Q3 constant kept) ------------------------
+@pytest.mark.parametrize("n", range(1, 9))
+def test_dunavant_sum(orc, n):
+    p, w = orc.tri_rule(n)
+    assert abs(w.sum() - 0.5) <= 1e-6
+    if n == 3:
+        # Q3: w2 = 0.132394440720100 (published 0.132394152788506) -> sum 0.500000431897
+        assert abs(w.sum() - 0.500000431897) < 1e-12
+    elif n != 3:
+        assert abs(w.sum() - 0.5) < 1e-14
+
+
+def test_dunavant_degree7(orc):
+    p, w = orc.tri_rule(4)
+    assert abs(np.sum(w * p[:, 0] ** 3 * p[:, 1] ** 4) - 1 / 2520) <= 1e-10
+    # and lower-degree monomials exactly (independent check of point/weight pairing)
+    for a in range(5):
+        for b in range(5 - a):
+            exact = math.factorial(a) * math.factorial(b) / math.factorial(a + b + 2)
+            assert abs(np.sum(w * p[:, 0] ** a * p[:, 1] ** b) - exact) < 1e-13
+
+
+def test_dunavant_degree5(orc):
+    p, w = orc.tri_rule(3)
+    # degree-5 exactness up to the Q3 weight deviation (~4.3e-7 * max monomial)
+    for a in range(4):
+        for b in range(4 - a):
+            exact = math.factorial(a) * math.factorial(b) / math.factorial(a + b + 2)
+            assert abs(np.sum(w * p[:, 0] ** a * p[:, 1] ** b) - exact) < 5e-7
+
+
+# --- shape functions: test/runtests.jl:51-67, 227-255 -----------------------------
+NODES = {
+    0: [(-1, -1), (1, -1), (1, 1), (-1, 1), (0, -1), (1, 0), (0, 1), (-1, 0)],
+    2: [(-1, -1), (1, -1), (1, 1), (-1, 1)],
+    1: [(0, 0), (1, 0), (0, 1), (0.5, 0), (0.5, 0.5), (0, 0.5)],
+    3: [(0, 0), (1, 0), (0, 1)],
+}
+
+
+@pytest.mark.parametrize("fam", [0, 1, 2, 3])
+def test_shape_partition_and_delta(orc, fam):
+    rng = np.random.default_rng(1)
+    for _ in range(20):
+        x, e = rng.uniform(-1, 1, 2) if fam in (0, 2) else rng.uniform(0, 0.5, 2)
+        N, dx, de = orc.shape(fam, x, e)
+        assert abs(N.sum() - 1) <= 1e-14
+        assert abs(dx.sum()) <= 1e-14 and abs(de.sum()) <= 1e-14
+        # derivatives vs central differences of N (independent of the formulas)
+        h = 1e-6
+        Np, _, _ = orc.shape(fam, x + h, e); Nm, _, _ = orc.shape(fam, x - h, e)
+        assert np.allclose((Np - Nm) / (2 * h), dx, atol=1e-8)
+        Np, _, _ = orc.shape(fam, x, e + h); Nm, _, _ = orc.shape(fam, x, e - h)
+        assert np.allclose((Np - Nm) / (2 * h), de, atol=1e-8)
+    for a, (x, e) in enumerate(NODES[fam]):
+        N, _, _ = orc.shape(fam, float(x), float(e))
+        assert np.allclose(N, np.eye(len(NODES[fam]))[a], atol=1e-13)
+
+
+# --- test/runtests.jl:81-99 ---------------------------------------------------------
+def test_quad8_geometry(orc):
+    coords = np.array([[0, 0, 0], [1, 0, 0], [1, 1, 0], [0, 1, 0], [0.5, 0, 0], [1, 0.5, 0], [0.5, 1, 0],
+                       [0, 0.5, 0]], np.float64)
+    nodes = np.arange(1, 9, dtype=np.int32)
+    x, n, dA = orc.eval_point(0, coords, nodes, 0.0, 0.0)
+    assert np.allclose(x, [0.5, 0.5, 0], atol=1e-14)
+    assert np.allclose(n, [0, 0, 1], atol=1e-14)
+    assert abs(dA - 0.25) <= 1e-14
+    qp, qw = orc.quad_rule(4)
+    A = sum(w * orc.eval_point(0, coords, nodes, p[0], p[1])[2] for p, w in zip(qp, qw))
+    assert abs(A - 1.0) <= 1e-12
+
+
+def test_curved_quad8_area(orc):
+    # quarter-cylinder patch as one Quad8 with exact mid-side nodes: GL area vs
+    # the closed-form area of the isoparametric surface computed independently
+    r = 1.0
+    th = [0.0, np.pi / 2]
+    c = lambda t, z: (r * np.cos(t), r * np.sin(t), z)  # noqa: E731
+    coords = np.array([c(th[0], 0), c(th[1], 0), c(th[1], 1), c(th[0], 1), c(np.pi / 4, 0), c(th[1], 0.5),
+                       c(np.pi / 4, 1), c(th[0], 0.5)])
+    nodes = np.arange(1, 9, dtype=np.int32)
+    qp, qw = orc.quad_rule(5)
+    A = sum(w * orc.eval_point(0, coords, nodes, p[0], p[1])[2] for p, w in zip(qp, qw))
+    # independent: the isoparametric edge is the parabola through the 3 nodes;
+    # surface area = height * arc length of that parabola; with the same 5-point
+    # rule the tensor sum factorises exactly, with 40 points it converges
+    P = coords[[0, 4, 1]]
+    for npt, tol in ((5, 1e-14), (40, 1e-5)):
+        g, gw = np.polynomial.legendre.leggauss(npt)
+        dpdx = np.outer(g - 0.5, P[0]) + np.outer(-2 * g, P[1]) + np.outer(g + 0.5, P[2])
+        L = np.sum(gw * np.linalg.norm(dpdx, axis=1))
+        assert abs(A - L * 1.0) < tol
+
+
+# --- test/runtests.jl:137-153 -------------------------------------------------------
+def test_vf_kernel(orc):
+    xi = np.array([0, 0, 0.]); ni = np.array([0, 0, 1.]); xj = np.array([0, 0, 1.]); nj = np.array([0, 0, -1.])
+    assert abs(orc.vf_kernel(xi, ni, xj, nj) - 1 / math.pi) <= 1e-14
+    assert orc.vf_kernel(xi, ni, xj, ni) == 0.0
+    assert orc.vf_kernel(xi, ni, xi, nj) == 0.0
+    # independent formula on random configurations
+    rng = np.random.default_rng(2)
+    for _ in range(200):
+        xi, xj = rng.normal(size=3), rng.normal(size=3)
+        ni, nj = rng.normal(size=3), rng.normal(size=3)
+        ni /= np.linalg.norm(ni); nj /= np.linalg.norm(nj)
+        r = xj - xi; d = np.linalg.norm(r)
+        ci, cj = ni @ r / d, -(nj @ r) / d
+        ref = ci * cj / (math.pi * d * d) if (ci > 0 and cj > 0) else 0.0
+        assert abs(orc.vf_kernel(xi, ni, xj, nj) - ref) <= 1e-14 * max(1.0, abs(ref))
