@@ -1,0 +1,151 @@
+/*
+ * rvf.h — C ABI of the MI355X element-pair view-factor library
+ * (librvf_hip.so).  Plain pointers and sizes only; no C++ exceptions cross
+ * it.  Every entry point returns 0 on success and a negative RVF_E* code on
+ * failure; rvf_last_error() then holds a message (thread-local).  The
+ * library never calls abort() or exit().
+ *
+ * Each entry point names the reference interface it replaces
+ * (rot4te/RadiativeViewFactor.jl v0.5.0, file:line).
+ */
+#ifndef RVF_H
+#define RVF_H
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define RVF_OK 0
+#define RVF_E_ARG (-1)       /* invalid argument / mesh */
+#define RVF_E_HIP (-2)       /* HIP runtime error (incl. device faults) */
+#define RVF_E_NOMEM (-3)     /* device allocation failed / admission refused */
+#define RVF_E_STATE (-4)     /* object used in a wrong state */
+#define RVF_E_CANCELLED (-5) /* serving: request cancelled */
+#define RVF_E_TIMEOUT (-6)   /* serving: wait timed out */
+
+/* family codes: src/GPUKernels.jl:344-375 */
+#define RVF_QUAD8 0
+#define RVF_TRI6 1
+#define RVF_QUAD4 2
+#define RVF_TRI3 3
+
+/* methods: compute_view_factors keywords monte_carlo / use_duffy
+ * (src/Assembly.jl:64-73) */
+#define RVF_METHOD_QUAD 0
+#define RVF_METHOD_MC 1
+#define RVF_METHOD_DUFFY 2
+
+#define RVF_FP64 0   /* reference arithmetic, bit-faithful */
+#define RVF_MIXED 1  /* FP32 bulk, FP64 accumulation and refinement */
+
+/* soup reading (src/BVH.jl:81-109 vs 232-240, quirk Q1) */
+#define RVF_SOUP_REFERENCE 0    /* AABB reads [xyz,vertex], Moller-Trumbore reads [vertex,xyz] */
+#define RVF_SOUP_XYZ_VERTEX 1   /* geometric reading of a [xyz,vertex,tri] soup (load_mesh) */
+#define RVF_SOUP_VERTEX_XYZ 2   /* geometric reading of a [vertex,xyz,tri] soup */
+
+#define RVF_WANT_F 1     /* materialise the N x N element matrix F on the device */
+#define RVF_AUDIT 2      /* per-pair blocked-ray count + hash (occluded solves) */
+#define RVF_TIMING 4     /* record per-kernel HIP event timings */
+
+typedef struct rvf_ctx rvf_ctx;
+typedef struct rvf_mesh rvf_mesh;
+typedef struct rvf_obs rvf_obs;
+typedef struct rvf_plan rvf_plan;
+
+typedef struct rvf_opts {
+    int32_t nquad;      /* Gauss points per direction (1..8) */
+    int32_t method;     /* RVF_METHOD_* */
+    int32_t precision;  /* RVF_FP64 / RVF_MIXED */
+    int32_t n_samples;  /* Monte Carlo samples per pair */
+    uint64_t seed;      /* Monte Carlo stream seed */
+    int32_t flags;      /* RVF_WANT_F | RVF_AUDIT | RVF_TIMING */
+    int32_t reserved;
+} rvf_opts;
+
+typedef struct rvf_stats {
+    double precompute_ms, pair_ms, duffy_ms, fill_ms, reduce_ms, total_ms;
+    int64_t n_elems, unique_pairs, tiles;
+    int64_t rays;       /* rays cast (occluded solves) */
+    int64_t retests;    /* FP64 ray-triangle re-tests (mixed mode) */
+    int64_t singular_pairs;
+    int64_t aabb_tests; /* FP64 mode: box tests in the stackless order */
+    int64_t tri_tests;  /* FP64 mode: triangle tests in the stackless order */
+} rvf_stats;
+
+/* --- lifecycle (no reference counterpart: the reference is one Julia process) */
+int rvf_ctx_create(int device, rvf_ctx** out);
+void rvf_ctx_destroy(rvf_ctx* ctx);
+const char* rvf_last_error(void);
+int rvf_version(void);
+
+/* --- mesh upload: replaces build_gpu_arrays (src/GPUKernels.jl:336-412).
+ * coords: 3 x n_nodes column-major; family/elem_node_idx/elem_group: per
+ * element; nodes[f]: (8|6|4|3) x n_per_family[f] column-major 1-based node ids,
+ * elem_node_idx is 1-based into its family's matrix (as the reference builds
+ * them). */
+int rvf_mesh_upload(rvf_ctx* ctx, const double* coords, int64_t n_nodes, int64_t n_elems,
+                    const int8_t* family, const int32_t* elem_node_idx, const int32_t* elem_group,
+                    const int32_t* const nodes[4], const int64_t n_per_family[4], rvf_mesh** out);
+void rvf_mesh_free(rvf_mesh* mesh);
+int rvf_mesh_info(const rvf_mesh* mesh, int64_t* n_elems, int32_t* n_groups, int32_t* group_tags);
+
+/* --- obstruction: replaces build_flat_bvh_from_mesh (src/GPUBVH.jl:146-174)
+ * with the CPU path's semantics (src/Assembly.jl:119-143): one BVH per
+ * distinct active set obstruction_groups \ {g_i, g_j}, built bit-identically
+ * to build_bvh (src/BVH.jl:71-148).  soup: (3,3,T) column-major, triangles of
+ * each group contiguous in ascending group order, tri_group: tag per triangle. */
+int rvf_obs_build(rvf_ctx* ctx, const rvf_mesh* mesh, const double* soup, const int32_t* tri_group, int64_t T,
+                  const int32_t* obstruction_groups, int32_t n_obs, int32_t soup_read, rvf_obs** out);
+void rvf_obs_free(rvf_obs* obs);
+/* inspection: number of distinct trees / nodes / triangles */
+int rvf_obs_info(const rvf_obs* obs, int32_t* n_trees, int64_t* n_nodes, int64_t* n_tris);
+/* export tree k in the reference's layout: box (6 x nodes: lo, hi), meta
+ * (5 x nodes: left, right, tri_start, tri_count, miss; 1-based, 0 = none),
+ * tri_idx (1-based permutation into the active set's merged soup). */
+int rvf_obs_export(const rvf_obs* obs, int32_t tree, int64_t cap, double* box, int32_t* meta, int32_t* tri_idx,
+                   int64_t* n_nodes, int64_t* n_tris);
+
+/* host-only build of one tree from a soup (no device needed), same layout as
+ * rvf_obs_export; for checking the builder against build_bvh (src/BVH.jl:71-148). */
+int rvf_bvh_build_host(const double* soup, int64_t T, int32_t soup_read, int64_t cap, double* box, int32_t* meta,
+                       int32_t* tri_idx, int64_t* n_nodes);
+
+/* --- solve: replaces the GPU hook target compute_view_factors_gpu
+ * (src/GPUAssembly.jl:32-105) through launch_vf_kernel! / launch_mc_kernel!
+ * (src/GPUKernels.jl:454-500, src/GPUMCKernels.jl:295-328).  A plan owns the
+ * device buffers; rvf_plan_run enqueues all kernels on the context stream. */
+int rvf_plan_create(rvf_ctx* ctx, const rvf_mesh* mesh, const rvf_obs* obs /*nullable*/, const rvf_opts* opts,
+                    rvf_plan** out);
+int rvf_plan_run(rvf_plan* plan);
+int rvf_plan_sync(rvf_plan* plan);
+/* Results, host copies (each pointer nullable):
+ *   F N x N column-major (requires RVF_WANT_F), A N, rowsum N,
+ *   F_group G x G column-major over sorted tags, A_group G, group_tags G,
+ *   audit_count / audit_hash N x N (upper triangle i<j at i + N*j). */
+int rvf_plan_fetch(rvf_plan* plan, double* F, double* A, double* rowsum, double* F_group, double* A_group,
+                   int32_t* group_tags, uint32_t* audit_count, uint32_t* audit_hash);
+int rvf_plan_stats(const rvf_plan* plan, rvf_stats* st);
+/* device pointer of an output (0 = F, 1 = A, 2 = rowsum) for device-side use */
+int rvf_plan_device_ptr(const rvf_plan* plan, int which, void** ptr);
+void rvf_plan_free(rvf_plan* plan);
+
+/* One-shot: the body of the Julia hook (compute_view_factors_gpu,
+ * src/GPUAssembly.jl:32-105) — upload, plan, run, fetch. */
+int rvf_compute_view_factors(rvf_ctx* ctx, const rvf_mesh* mesh, const rvf_obs* obs, const rvf_opts* opts,
+                             double* F, double* A, double* F_group, double* A_group, int32_t* group_tags,
+                             rvf_stats* st);
+
+/* --- quadrature tables as the device sees them (gauss_legendre_2d,
+ * src/Quadrature.jl:96-109; tri_quad_rule, src/ViewFactorKernel.jl:42-71) */
+int rvf_quad_rule(int32_t nquad, int32_t tri, double* pts, double* wts);
+
+/* --- timing on the context stream */
+int rvf_ctx_event_record(rvf_ctx* ctx, int32_t slot);
+int rvf_ctx_event_elapsed(rvf_ctx* ctx, int32_t slot_a, int32_t slot_b, float* ms);
+int rvf_ctx_sync(rvf_ctx* ctx);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

@@ -1,0 +1,96 @@
+"""ctypes binding of librvf_hip.so (the C ABI declared in include/rvf.h).
+
+Loading fails loudly when the library is missing: there is no CPU fallback
+in the product path.
+"""
+from __future__ import annotations
+
+import ctypes as C
+import os
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(HERE, "librvf_hip.so")
+
+RVF_WANT_F, RVF_AUDIT, RVF_TIMING = 1, 2, 4
+METHODS = {"quad": 0, "mc": 1, "duffy": 2}
+PRECISIONS = {"fp64": 0, "mixed": 1}
+
+
+class RVFError(RuntimeError):
+    """Error returned through the C ABI (message from rvf_last_error)."""
+
+    def __init__(self, code, msg):
+        super().__init__(f"rvf error {code}: {msg}")
+        self.code = code
+
+
+class Opts(C.Structure):
+    _fields_ = [("nquad", C.c_int32), ("method", C.c_int32), ("precision", C.c_int32), ("n_samples", C.c_int32),
+                ("seed", C.c_uint64), ("flags", C.c_int32), ("reserved", C.c_int32)]
+
+
+class Stats(C.Structure):
+    _fields_ = [("precompute_ms", C.c_double), ("pair_ms", C.c_double), ("duffy_ms", C.c_double),
+                ("fill_ms", C.c_double), ("reduce_ms", C.c_double), ("total_ms", C.c_double),
+                ("n_elems", C.c_int64), ("unique_pairs", C.c_int64), ("tiles", C.c_int64), ("rays", C.c_int64),
+                ("retests", C.c_int64), ("singular_pairs", C.c_int64), ("aabb_tests", C.c_int64),
+                ("tri_tests", C.c_int64)]
+
+    def as_dict(self):
+        return {k: getattr(self, k) for k, _ in self._fields_}
+
+
+_lib = None
+
+
+def lib():
+    global _lib
+    if _lib is not None:
+        return _lib
+    if not os.path.exists(LIB_PATH):
+        raise RuntimeError(f"librvf_hip.so not built ({LIB_PATH}); run __graft_entry__.build()")
+    L = C.CDLL(LIB_PATH)
+    P, I32, I64, D = C.c_void_p, C.c_int32, C.c_int64, C.c_double
+    PP = C.POINTER(C.c_void_p)
+    sig = {
+        "rvf_ctx_create": ([C.c_int, PP], C.c_int),
+        "rvf_ctx_destroy": ([P], None),
+        "rvf_last_error": ([], C.c_char_p),
+        "rvf_version": ([], C.c_int),
+        "rvf_mesh_upload": ([P, P, I64, I64, P, P, P, P, P, PP], C.c_int),
+        "rvf_mesh_free": ([P], None),
+        "rvf_mesh_info": ([P, P, P, P], C.c_int),
+        "rvf_obs_build": ([P, P, P, P, I64, P, I32, I32, PP], C.c_int),
+        "rvf_obs_free": ([P], None),
+        "rvf_obs_info": ([P, P, P, P], C.c_int),
+        "rvf_obs_export": ([P, I32, I64, P, P, P, P, P], C.c_int),
+        "rvf_bvh_build_host": ([P, I64, I32, I64, P, P, P, P], C.c_int),
+        "rvf_plan_create": ([P, P, P, P, PP], C.c_int),
+        "rvf_plan_run": ([P], C.c_int),
+        "rvf_plan_sync": ([P], C.c_int),
+        "rvf_plan_fetch": ([P, P, P, P, P, P, P, P, P], C.c_int),
+        "rvf_plan_stats": ([P, P], C.c_int),
+        "rvf_plan_device_ptr": ([P, C.c_int, PP], C.c_int),
+        "rvf_plan_free": ([P], None),
+        "rvf_compute_view_factors": ([P, P, P, P, P, P, P, P, P, P], C.c_int),
+        "rvf_quad_rule": ([I32, I32, P, P], C.c_int),
+        "rvf_ctx_event_record": ([P, I32], C.c_int),
+        "rvf_ctx_event_elapsed": ([P, I32, I32, C.POINTER(C.c_float)], C.c_int),
+        "rvf_ctx_sync": ([P], C.c_int),
+    }
+    for name, (args, res) in sig.items():
+        f = getattr(L, name)
+        f.argtypes = args
+        f.restype = res
+    _lib = L
+    return L
+
+
+def check(rc):
+    if rc is not None and rc < 0:
+        raise RVFError(rc, lib().rvf_last_error().decode())
+    return rc
+
+
+def ptr(a):
+    return None if a is None else a.ctypes.data_as(C.c_void_p)

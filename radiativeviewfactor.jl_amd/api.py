@@ -1,0 +1,342 @@
+"""Reference-shaped entry points over the C ABI.
+
+``compute_view_factors`` mirrors src/Assembly.jl:64-96 for the GPU branch
+(the hook ``compute_view_factors_gpu``, src/GPUAssembly.jl:32-105) with an
+``MI355XBackend`` carrying the knobs the reference hook drops (precision,
+Duffy, Monte Carlo seed, device; src/Assembly.jl:77-78,243-252).
+``ViewFactorResult``, ``aggregate_by_group``, ``check_reciprocity`` and
+``check_closure`` mirror src/Results.jl:35-127.
+"""
+from __future__ import annotations
+
+import ctypes as C
+from dataclasses import dataclass, field
+from typing import Dict, List, Optional, Sequence
+
+import numpy as np
+
+from . import _lib
+from ._lib import RVF_AUDIT, RVF_TIMING, RVF_WANT_F, RVFError, check, ptr
+from .mesh import MeshData
+
+SOUP_READ = {"reference": 0, ("geometric", "xyz_vertex"): 1, ("geometric", "vertex_xyz"): 2}
+
+
+@dataclass
+class MI355XBackend:
+    """Backend object passed through the reference's GPU hook unchanged.
+
+    precision: "fp64" (reference arithmetic) or "mixed" (FP32 bulk with FP64
+    accumulation and FP64 re-tests of undecided ray-triangle decisions).
+    parity: "reference" reproduces the reference's soup reading (quirk Q1:
+    boxes from ``[xyz, vertex]``, Moller-Trumbore from ``[vertex, xyz]``);
+    "geometric" reads the soup consistently by ``mesh.soup_layout``."""
+    precision: str = "fp64"
+    use_duffy: bool = False
+    seed: int = 0
+    device: int = 0
+    parity: str = "reference"
+    audit: bool = False
+
+
+@dataclass
+class ViewFactorResult:
+    F_elem: np.ndarray
+    A_elem: np.ndarray
+    F_group: np.ndarray
+    A_group: np.ndarray
+    group_tags: List[int]
+    group_names: List[str]
+    row_sums: Optional[np.ndarray] = None
+    stats: Dict = field(default_factory=dict)
+    audit_count: Optional[np.ndarray] = None
+    audit_hash: Optional[np.ndarray] = None
+
+
+# ----------------------------------------------------------------------------
+# device objects
+# ----------------------------------------------------------------------------
+class Context:
+    def __init__(self, device: int = 0):
+        L = _lib.lib()
+        h = C.c_void_p()
+        check(L.rvf_ctx_create(device, C.byref(h)))
+        self.h = h
+        self.device = device
+
+    def close(self):
+        if self.h:
+            _lib.lib().rvf_ctx_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def record(self, slot):
+        check(_lib.lib().rvf_ctx_event_record(self.h, slot))
+
+    def elapsed(self, a, b):
+        ms = C.c_float()
+        check(_lib.lib().rvf_ctx_event_elapsed(self.h, a, b, C.byref(ms)))
+        return ms.value
+
+    def sync(self):
+        check(_lib.lib().rvf_ctx_sync(self.h))
+
+
+def family_matrices(mesh: MeshData):
+    """Per-family node matrices and 1-based elem_node_idx, as build_gpu_arrays
+    (src/GPUKernels.jl:344-385) lays them out."""
+    nn = (8, 6, 4, 3)
+    mats, counts = [], np.zeros(4, np.int64)
+    idx = np.zeros(mesh.n_elems, np.int32)
+    for f in range(4):
+        sel = np.nonzero(mesh.family == f)[0]
+        counts[f] = sel.size
+        idx[sel] = np.arange(1, sel.size + 1, dtype=np.int32)
+        mats.append(np.ascontiguousarray(mesh.nodes[sel, :nn[f]], dtype=np.int32))
+    return mats, counts, idx
+
+
+class DeviceMesh:
+    def __init__(self, ctx: Context, mesh: MeshData):
+        if mesh.mesh_dim == 1:
+            raise ValueError("GPU backend does not support curve meshes (mesh_dim=1). "
+                             "Use the CPU backend for 2D per-unit-depth view factors.")
+        self.ctx = ctx
+        self.mesh = mesh
+        mats, counts, idx = family_matrices(mesh)
+        self._keep = (mats, counts, idx)
+        arr = (C.c_void_p * 4)(*[m.ctypes.data if m.size else None for m in mats])
+        h = C.c_void_p()
+        check(_lib.lib().rvf_mesh_upload(ctx.h, ptr(mesh.coords), mesh.n_nodes, mesh.n_elems, ptr(mesh.family),
+                                         ptr(idx), ptr(mesh.group), arr, ptr(counts), C.byref(h)))
+        self.h = h
+        G = C.c_int32()
+        _lib.lib().rvf_mesh_info(self.h, None, C.byref(G), None)
+        tags = np.zeros(max(G.value, 1), np.int32)
+        _lib.lib().rvf_mesh_info(self.h, None, None, ptr(tags))
+        self.tags = [int(t) for t in tags[:G.value]]
+
+    def close(self):
+        if self.h:
+            _lib.lib().rvf_mesh_free(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
+class DeviceObstruction:
+    def __init__(self, ctx: Context, dmesh: DeviceMesh, obstruction_groups: Sequence[int], parity: str):
+        mesh = dmesh.mesh
+        tags = sorted(set(int(g) for g in obstruction_groups if int(g) in mesh.group_tri_soup))
+        if tags:
+            soup = np.ascontiguousarray(np.concatenate([mesh.group_tri_soup[t] for t in tags]), np.float64)
+            tg = np.concatenate([np.full(mesh.group_tri_soup[t].shape[0], t, np.int32) for t in tags])
+        else:
+            soup = np.zeros((0, 3, 3)); tg = np.zeros(0, np.int32)
+        obs = np.ascontiguousarray(list(obstruction_groups), np.int32)
+        mode = SOUP_READ["reference"] if parity == "reference" else SOUP_READ[("geometric", mesh.soup_layout)]
+        self._keep = (soup, tg, obs)
+        h = C.c_void_p()
+        check(_lib.lib().rvf_obs_build(ctx.h, dmesh.h, ptr(soup), ptr(tg), soup.shape[0], ptr(obs), len(obs), mode,
+                                       C.byref(h)))
+        self.h = h
+        nt = C.c_int32(); nn = C.c_int64(); ntri = C.c_int64()
+        _lib.lib().rvf_obs_info(self.h, C.byref(nt), C.byref(nn), C.byref(ntri))
+        self.n_trees, self.n_nodes, self.n_tris = nt.value, nn.value, ntri.value
+
+    def export(self, tree=0):
+        L = _lib.lib()
+        cap = max(self.n_nodes, self.n_tris, 1)
+        box = np.zeros(6 * cap); meta = np.zeros(5 * cap, np.int32); tri = np.zeros(cap, np.int32)
+        nn = C.c_int64(); nt = C.c_int64()
+        check(L.rvf_obs_export(self.h, tree, cap, ptr(box), ptr(meta), ptr(tri), C.byref(nn), C.byref(nt)))
+        return box[:6 * nn.value].reshape(-1, 6), meta[:5 * nn.value].reshape(-1, 5), tri[:nt.value]
+
+    def close(self):
+        if self.h:
+            _lib.lib().rvf_obs_free(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
+class Plan:
+    """Device buffers + launch sequence for one (mesh, obstruction, options)."""
+
+    def __init__(self, ctx: Context, dmesh: DeviceMesh, dobs: Optional[DeviceObstruction], nquad: int,
+                 method: str = "quad", precision: str = "fp64", n_samples: int = 10000, seed: int = 0,
+                 want_F: bool = True, audit: bool = False, timing: bool = True):
+        o = _lib.Opts()
+        o.nquad = nquad
+        o.method = _lib.METHODS[method]
+        o.precision = _lib.PRECISIONS[precision]
+        o.n_samples = n_samples
+        o.seed = seed & ((1 << 64) - 1)
+        o.flags = (RVF_WANT_F if want_F else 0) | (RVF_AUDIT if audit else 0) | (RVF_TIMING if timing else 0)
+        self.opts = o
+        self.ctx, self.dmesh, self.dobs = ctx, dmesh, dobs
+        h = C.c_void_p()
+        check(_lib.lib().rvf_plan_create(ctx.h, dmesh.h, dobs.h if dobs is not None else None, C.byref(o),
+                                         C.byref(h)))
+        self.h = h
+        self.want_F, self.audit = want_F, audit and dobs is not None
+
+    def run(self):
+        check(_lib.lib().rvf_plan_run(self.h))
+
+    def sync(self):
+        check(_lib.lib().rvf_plan_sync(self.h))
+
+    def stats(self):
+        s = _lib.Stats()
+        check(_lib.lib().rvf_plan_stats(self.h, C.byref(s)))
+        return s.as_dict()
+
+    def fetch(self, F=True):
+        mesh = self.dmesh.mesh
+        N = mesh.n_elems
+        G = len(self.dmesh.tags)
+        Fm = np.empty((N, N), order="F") if (F and self.want_F) else None
+        A = np.empty(N); rs = np.empty(N)
+        Fg = np.empty((G, G), order="F"); Ag = np.empty(G); tags = np.empty(G, np.int32)
+        ac = np.empty((N, N), np.uint32, order="F") if self.audit else None
+        ah = np.empty((N, N), np.uint32, order="F") if self.audit else None
+        check(_lib.lib().rvf_plan_fetch(self.h, ptr(Fm), ptr(A), ptr(rs), ptr(Fg), ptr(Ag), ptr(tags), ptr(ac),
+                                        ptr(ah)))
+        return Fm, A, rs, Fg, Ag, [int(t) for t in tags], ac, ah
+
+    def close(self):
+        if self.h:
+            _lib.lib().rvf_plan_free(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
+_CTX: Dict[int, Context] = {}
+
+
+def context(device: int = 0) -> Context:
+    if device not in _CTX:
+        _CTX[device] = Context(device)
+    return _CTX[device]
+
+
+# ----------------------------------------------------------------------------
+# reference-shaped API
+# ----------------------------------------------------------------------------
+def compute_view_factors(mesh: MeshData, nquad: int = 4, obstruction_groups: Sequence[int] = (),
+                         backend: Optional[MI355XBackend] = None, self_vf: bool = False,
+                         monte_carlo: bool = False, n_samples: int = 10000, rng=None, use_duffy: bool = False,
+                         verbose: bool = True) -> ViewFactorResult:
+    """GPU branch of compute_view_factors (src/Assembly.jl:64-96) on MI355X."""
+    if backend is None:
+        backend = MI355XBackend()
+    if isinstance(backend, type):
+        backend = backend()
+    if not isinstance(backend, MI355XBackend):
+        raise TypeError("only the MI355X backend is provided here; the CPU path is the reference's own")
+    duffy = use_duffy or backend.use_duffy
+    if duffy and monte_carlo:
+        raise ValueError("use_duffy and monte_carlo cannot both be true.")
+    if self_vf and verbose:
+        print("self_vf is CPU-only; ignored for GPU backends.")
+    method = "mc" if monte_carlo else ("duffy" if duffy else "quad")
+    ctx = context(backend.device)
+    N = mesh.n_elems
+    if verbose:
+        what = f"n_samples={n_samples} (Monte Carlo)" if monte_carlo else f"nquad={nquad}"
+        print(f"GPU compute_view_factors: {N} elements, {what}, precision={backend.precision}, backend=MI355X")
+    dmesh = DeviceMesh(ctx, mesh)
+    dobs = None
+    if len(obstruction_groups) > 0:
+        dobs = DeviceObstruction(ctx, dmesh, obstruction_groups, backend.parity)
+        if dobs.n_trees == 0:
+            print("Warning: obstruction_groups specified but no triangle geometry found for those groups; "
+                  "proceeding without obstruction checking.")
+            dobs = None
+    plan = Plan(ctx, dmesh, dobs, nquad, method, backend.precision, n_samples, backend.seed, want_F=True,
+                audit=backend.audit)
+    plan.run()
+    plan.sync()
+    F, A, rs, Fg, Ag, tags, ac, ah = plan.fetch()
+    names = [mesh.group_tags.get(t, str(t)) for t in tags]
+    res = ViewFactorResult(F, A, Fg, Ag, tags, names, rs, plan.stats(), ac, ah)
+    if verbose:
+        print("  Row-sum check (element level) — max |Σⱼ Fᵢⱼ - 1| : ", float(np.max(np.abs(rs - 1.0))))
+        print("  Row-sum check (group level)   — max |Σⱼ Fᵢⱼ - 1| : ", float(np.max(np.abs(Fg.sum(1) - 1.0))))
+    plan.close(); dmesh.close()
+    if dobs is not None:
+        dobs.close()
+    return res
+
+
+def aggregate_by_group(result: ViewFactorResult, mesh: MeshData):
+    """Re-aggregate element results to group level (src/Results.jl:56-93)."""
+    tags = sorted(mesh.group_tags)
+    ge = mesh.group_elems
+    G = len(tags)
+    A = result.A_elem
+    Ag = np.array([A[ge[t]].sum() for t in tags])
+    Fg = np.zeros((G, G))
+    for a, ta in enumerate(tags):
+        ia = ge[ta]
+        for b, tb in enumerate(tags):
+            Fg[a, b] = (A[ia] * result.F_elem[np.ix_(ia, ge[tb])].sum(1)).sum() / Ag[a]
+    return Fg, Ag, tags, [mesh.group_tags[t] for t in tags]
+
+
+def check_reciprocity(result: ViewFactorResult, tol: float = 1e-4) -> bool:
+    """A_i F_ij ~ A_j F_ji (src/Results.jl:105-114)."""
+    F, A = result.F_elem, result.A_elem
+    AF = A[:, None] * F
+    iu = np.triu_indices(F.shape[0], 1)
+    err = np.abs(AF[iu] - AF.T[iu]) / np.maximum(AF[iu], 1e-30)
+    m = float(err.max()) if err.size else 0.0
+    print(f"Reciprocity max relative error: {m}")
+    return m < tol
+
+
+def check_closure(result: ViewFactorResult, tol: float = 1e-3) -> bool:
+    """No row of F sums to more than 1 + tol (src/Results.jl:122-127)."""
+    rs = result.F_elem.sum(1)
+    print(f"Row sums: min={round(float(rs.min()), 6)}, max={round(float(rs.max()), 6)}")
+    return float(rs.max()) <= 1.0 + tol
+
+
+def quad_rule(nquad: int, tri: bool = False):
+    """The quadrature table the device uses (host-computed, no GPU needed)."""
+    pts = np.zeros(2 * 64); w = np.zeros(64)
+    n = check(_lib.lib().rvf_quad_rule(nquad, 1 if tri else 0, ptr(pts), ptr(w)))
+    return pts[:2 * n].reshape(n, 2).copy(), w[:n].copy()
+
+
+def bvh_build_host(soup: np.ndarray, soup_read: int):
+    soup = np.ascontiguousarray(soup, np.float64).reshape(-1, 3, 3)
+    T = soup.shape[0]
+    cap = 2 * T + 4
+    box = np.zeros(6 * cap); meta = np.zeros(5 * cap, np.int32); tri = np.zeros(T, np.int32)
+    nn = C.c_int64()
+    check(_lib.lib().rvf_bvh_build_host(ptr(soup), T, soup_read, cap, ptr(box), ptr(meta), ptr(tri), C.byref(nn)))
+    return box[:6 * nn.value].reshape(-1, 6), meta[:5 * nn.value].reshape(-1, 5), tri
+
+
+__all__ = ["MI355XBackend", "ViewFactorResult", "compute_view_factors", "aggregate_by_group", "check_reciprocity",
+           "check_closure", "Context", "DeviceMesh", "DeviceObstruction", "Plan", "RVFError", "quad_rule"]

@@ -1,0 +1,104 @@
+// Shared device/host definitions for the MI355X view-factor library.
+#pragma once
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+#define RVF_STRIP 64          // i-elements per pair tile (one per lane of a wavefront)
+#define RVF_CHUNK 16          // max j-elements per pair tile (group-pure column chunk)
+#define RVF_MAXPTS 64         // quadrature points per element (nquad <= 8 quads)
+
+// Flattened BVH node, FP64 (64 B).  Preorder layout: an interior node's left
+// child is the next node; `miss` is the DFS successor (-1 = end of tree).
+struct __align__(16) BvhNode64 {
+    double lo[3];
+    double hi[3];
+    int32_t miss;
+    int32_t start;   // first triangle (absolute index into the triangle arrays)
+    int32_t count;   // 0 for interior nodes
+    int32_t pad;
+};
+
+// FP32 node (32 B) with conservatively padded bounds.
+struct __align__(16) BvhNode32 {
+    float lo[3];
+    int32_t miss;
+    float hi[3];
+    int32_t start_count;  // (start << 4) | count, count <= 8
+};
+
+// Triangle for Moller-Trumbore, FP64: v0, e1 = v1 - v0, e2 = v2 - v0 as the
+// reference's reading of the soup yields them (72 B, padded to 80).
+struct __align__(16) Tri64 {
+    double v0[3];
+    double e1[3];
+    double e2[3];
+    double pad;
+};
+
+// FP32 triangle (48 B): v0, e1, e2 and the determinant band scale.
+struct __align__(16) Tri32 {
+    float v0[3];
+    float sa;        // |e1|_1 * |e2|_1, scales the |a| < 1e-10 band
+    float e1[3];
+    float pad0;
+    float e2[3];
+    float pad1;
+};
+
+// Column chunk: a run of <= RVF_CHUNK consecutive elements of one group.
+struct Chunk {
+    int32_t begin;
+    int32_t end;     // exclusive
+    int32_t grp;     // group rank
+    int32_t pad;
+};
+
+struct PairArgs {
+    const int2* tiles;        // (strip, chunk)
+    int ntiles;
+    int NS;                   // slots = N rounded up to RVF_STRIP
+    int maxpts;               // per-element point stride in Q arrays
+    int64_t N;                // elements
+    const Chunk* chunks;
+    const int32_t* elem_np;   // points per element
+    const int32_t* elem_grp;  // group rank per element
+    const double* area;       // per element
+    // FP64 point data
+    const double* P64;        // point-major [(p*8+c)*NS + e]: x y z nx ny nz w dA
+    const double* Q64;        // element-major [(e*maxpts + p)*8 + c]
+    // FP32 point data (mixed)
+    const float* P32;         // point-major [(p*8+c)*NS + e]: dx dy dz nx ny nz wdA 0
+    const float* Q32;         // element-major [(e*maxpts + p)*8 + c]
+    const double* anchor;     // [e*4 + c]
+    // outputs
+    double* F;                // N*N col-major: raw_ij at [i + N*j], i < j (nullable)
+    double* part;             // [nchunks * NS]: sum_{j in chunk, j > i} F_ij
+    double* partT;            // [nstrips * NS]: sum_{i in strip, i < j} F_ji
+    // obstruction
+    int G;                    // radiating groups
+    const int32_t* tree_root; // [G*G] root node or -1
+    const BvhNode64* nodes64;
+    const Tri64* tris64;
+    const BvhNode32* nodes32;
+    const Tri32* tris32;
+    // audit (nullable): per pair blocked-ray count / order-free hash at [i + N*j]
+    uint32_t* audit_count;
+    uint32_t* audit_hash;
+    // [0] FP64 re-tests (mixed), [1] rays cast, [2] AABB tests, [3] triangle
+    // tests (FP64: canonical stackless order, GPUBVH.jl:202-285)
+    unsigned long long* counters;
+};
+
+// Inputs/outputs of the per-element precompute kernel.
+struct PrecompArgs {
+    const double* coords;
+    const int8_t* fam;
+    const int32_t* nodes8;     // N*8, 1-based
+    int64_t N;
+    int NS, maxpts;
+    int nq_quad, nq_tri;
+    const double* rule;        // [quad pts 2*64][quad w 64][tri pts 2*13][tri w 13]
+    double* P64; double* Q64; float* P32; float* Q32; double* anchor; double* area;
+    int32_t* elem_np;
+    int* bad;                  // set to 1 if any dA is 0 or not finite
+};

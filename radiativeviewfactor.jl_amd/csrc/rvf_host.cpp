@@ -1,0 +1,858 @@
+// Host side of librvf_hip.so: C ABI, mesh planning, BVH construction and
+// solve orchestration.  Compiled with -ffp-contract=off: the quadrature tables
+// and the BVH boxes must be bit-identical to the reference's (Float64, no FMA).
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cmath>
+#include <cstdio>
+#include <cstring>
+#include <map>
+#include <string>
+#include <vector>
+
+#include "../../include/rvf.h"
+#include "rvf_common.h"
+
+// device launch wrappers (rvf_kernels.hip)
+
+extern "C" int rvf_launch_precompute(const PrecompArgs* a, hipStream_t st);
+extern "C" int rvf_launch_pair(const PairArgs* a, int mixed, int occ, int pc, float Sm, hipStream_t st);
+extern "C" int rvf_launch_fill(double* F, const double* area, int64_t N, const int2* tiles, int ntiles,
+                               hipStream_t st);
+extern "C" int rvf_launch_reduce(const double* part, const double* partT, const Chunk* chunks, int nchunks,
+                                 const int32_t* chunk_of, int64_t N, int NS, int G, double* rowsum, double* W,
+                                 hipStream_t st);
+
+// ---------------------------------------------------------------------------
+// errors
+// ---------------------------------------------------------------------------
+static thread_local std::string g_err;
+
+static int fail(int code, const std::string& msg) {
+    g_err = msg;
+    return code;
+}
+
+#define HIPCHK(expr)                                                                          \
+    do {                                                                                      \
+        hipError_t _e = (expr);                                                               \
+        if (_e != hipSuccess) return fail(RVF_E_HIP, std::string(#expr ": ") + hipGetErrorString(_e)); \
+    } while (0)
+
+// ---------------------------------------------------------------------------
+// device buffer
+// ---------------------------------------------------------------------------
+struct DBuf {
+    void* p = nullptr;
+    size_t n = 0;
+    ~DBuf() { if (p) (void)hipFree(p); }
+    int alloc(size_t bytes) {
+        if (p) { (void)hipFree(p); p = nullptr; n = 0; }
+        if (bytes == 0) bytes = 16;
+        hipError_t e = hipMalloc(&p, bytes);
+        if (e != hipSuccess) { p = nullptr; return fail(RVF_E_NOMEM, std::string("hipMalloc: ") + hipGetErrorString(e)); }
+        n = bytes;
+        return 0;
+    }
+    template <class T> T* as() const { return (T*)p; }
+};
+
+static int upload(DBuf& b, const void* src, size_t bytes) {
+    int rc = b.alloc(bytes);
+    if (rc) return rc;
+    if (bytes) HIPCHK(hipMemcpy(b.p, src, bytes, hipMemcpyHostToDevice));
+    return 0;
+}
+
+// ---------------------------------------------------------------------------
+// context
+// ---------------------------------------------------------------------------
+struct rvf_ctx {
+    int device = 0;
+    hipStream_t stream = nullptr;
+    hipEvent_t ev[16] = {};
+};
+
+// ---------------------------------------------------------------------------
+// Quadrature tables (src/Quadrature.jl:35-109; src/ViewFactorKernel.jl:42-71),
+// evaluated from the same closed forms with correctly rounded sqrt/division.
+// ---------------------------------------------------------------------------
+static void legendre_nodes(int n, double* p, double* w) {
+    // n > 5: the reference uses Golub-Welsch (LAPACK eigen); here Newton on
+    // P_n in long double, rounded once.
+    for (int k = 0; k < n; k++) {
+        long double x = cosl(3.14159265358979323846264338327950288L * (k + 0.75L) / (n + 0.5L));
+        long double dp = 0;
+        for (int it = 0; it < 100; it++) {
+            long double p0 = 1, p1 = x;
+            for (int m = 2; m <= n; m++) {
+                long double p2 = ((2 * m - 1) * x * p1 - (m - 1) * p0) / m;
+                p0 = p1; p1 = p2;
+            }
+            dp = n * (x * p1 - p0) / (x * x - 1);
+            long double dx = p1 / dp;
+            x -= dx;
+            if (fabsl(dx) < 1e-30L) break;
+        }
+        long double p0 = 1, p1 = x;
+        for (int m = 2; m <= n; m++) {
+            long double p2 = ((2 * m - 1) * x * p1 - (m - 1) * p0) / m;
+            p0 = p1; p1 = p2;
+        }
+        dp = n * (x * p1 - p0) / (x * x - 1);
+        p[n - 1 - k] = (double)x;
+        w[n - 1 - k] = (double)(2.0L / ((1 - x * x) * dp * dp));
+    }
+}
+
+static int gl1d(int n, double* p, double* w) {
+    if (n == 1) { p[0] = 0.0; w[0] = 2.0; return 1; }
+    if (n == 2) { p[0] = -1.0 / std::sqrt(3.0); p[1] = 1.0 / std::sqrt(3.0); w[0] = w[1] = 1.0; return 2; }
+    if (n == 3) {
+        double a = std::sqrt(3.0 / 5.0);
+        p[0] = -a; p[1] = 0.0; p[2] = a; w[0] = 5.0 / 9.0; w[1] = 8.0 / 9.0; w[2] = 5.0 / 9.0;
+        return 3;
+    }
+    if (n == 4) {
+        double a = std::sqrt((3.0 + 2.0 * std::sqrt(6.0 / 5.0)) / 7.0);
+        double b = std::sqrt((3.0 - 2.0 * std::sqrt(6.0 / 5.0)) / 7.0);
+        double wa = (18.0 - std::sqrt(30.0)) / 36.0, wb = (18.0 + std::sqrt(30.0)) / 36.0;
+        p[0] = -a; p[1] = -b; p[2] = b; p[3] = a; w[0] = wa; w[1] = wb; w[2] = wb; w[3] = wa;
+        return 4;
+    }
+    if (n == 5) {
+        double a = std::sqrt(5.0 + 2.0 * std::sqrt(10.0 / 7.0)) / 3.0;
+        double b = std::sqrt(5.0 - 2.0 * std::sqrt(10.0 / 7.0)) / 3.0;
+        double wa = (322.0 - 13.0 * std::sqrt(70.0)) / 900.0, wb = (322.0 + 13.0 * std::sqrt(70.0)) / 900.0;
+        p[0] = -a; p[1] = -b; p[2] = 0.0; p[3] = b; p[4] = a;
+        w[0] = wa; w[1] = wb; w[2] = 128.0 / 225.0; w[3] = wb; w[4] = wa;
+        return 5;
+    }
+    if (n < 1 || n > 8) return -1;
+    legendre_nodes(n, p, w);
+    return n;
+}
+
+static int quad_rule(int nquad, double* pts, double* wts) {
+    double p[8], w[8];
+    int n = gl1d(nquad, p, w);
+    if (n < 0) return -1;
+    int k = 0;
+    for (int j = 0; j < n; j++)
+        for (int i = 0; i < n; i++) { pts[2 * k] = p[i]; pts[2 * k + 1] = p[j]; wts[k] = w[i] * w[j]; k++; }
+    return k;
+}
+
+static int tri_rule(int nquad, double* pts, double* wts) {
+    if (nquad <= 1) { pts[0] = pts[1] = 1.0 / 3.0; wts[0] = 0.5; return 1; }
+    if (nquad == 2) {
+        const double x[3] = {1.0 / 6.0, 2.0 / 3.0, 1.0 / 6.0}, y[3] = {1.0 / 6.0, 1.0 / 6.0, 2.0 / 3.0};
+        for (int k = 0; k < 3; k++) { pts[2 * k] = x[k]; pts[2 * k + 1] = y[k]; wts[k] = 0.5 / 3.0; }
+        return 3;
+    }
+    if (nquad == 3) {
+        double a1 = 0.101286507323456, b1 = 1.0 - 2.0 * a1, a2 = 0.470142064105115, b2 = 1.0 - 2.0 * a2;
+        const double x[7] = {a1, b1, a1, a2, b2, a2, 1.0 / 3.0}, y[7] = {a1, a1, b1, a2, a2, b2, 1.0 / 3.0};
+        double w1 = 0.125939180544827 / 2.0, w2 = 0.132394440720100 / 2.0, w3 = 0.225 / 2.0;
+        const double w[7] = {w1, w1, w1, w2, w2, w2, w3};
+        for (int k = 0; k < 7; k++) { pts[2 * k] = x[k]; pts[2 * k + 1] = y[k]; wts[k] = w[k]; }
+        return 7;
+    }
+    double cen = 1.0 / 3.0;
+    double a1 = 0.260345966079038, b1 = 1.0 - 2.0 * a1, a2 = 0.065130102902216, b2 = 1.0 - 2.0 * a2;
+    double a3 = 0.048690315425316, b3 = 0.312865496004875, c3 = (1.0 - a3) - b3;
+    const double x[13] = {a1, b1, a1, a2, b2, a2, a3, b3, a3, c3, b3, c3, cen};
+    const double y[13] = {a1, a1, b1, a2, a2, b2, b3, a3, c3, a3, c3, b3, cen};
+    double wc = -0.149570044467670 / 2.0, wA = 0.175615257433204 / 2.0, wB = 0.053347235608839 / 2.0,
+           wC = 0.077113760890257 / 2.0;
+    const double w[13] = {wA, wA, wA, wB, wB, wB, wC, wC, wC, wC, wC, wC, wc};
+    for (int k = 0; k < 13; k++) { pts[2 * k] = x[k]; pts[2 * k + 1] = y[k]; wts[k] = w[k]; }
+    return 13;
+}
+
+// ---------------------------------------------------------------------------
+// mesh
+// ---------------------------------------------------------------------------
+struct rvf_mesh {
+    rvf_ctx* ctx = nullptr;
+    int64_t N = 0, Nn = 0;
+    int NS = 0;
+    std::vector<int8_t> fam;
+    std::vector<int32_t> nodes8;   // N*8 1-based
+    std::vector<int32_t> grp_rank; // per element
+    std::vector<int32_t> tags;     // sorted group tags
+    std::vector<Chunk> chunks;
+    std::vector<int32_t> chunk_of;
+    std::vector<int2> tiles, fill_tiles;
+    double Sm = 1.0;               // max |coordinate|
+    DBuf coords, d_fam, d_nodes8, d_grp, d_chunks, d_chunk_of, d_tiles, d_fill_tiles;
+};
+
+int rvf_mesh_upload(rvf_ctx* ctx, const double* coords, int64_t n_nodes, int64_t n_elems, const int8_t* family,
+                    const int32_t* elem_node_idx, const int32_t* elem_group, const int32_t* const nodes[4],
+                    const int64_t n_per_family[4], rvf_mesh** out) {
+    if (!ctx || !out || n_elems < 0 || n_nodes < 0) return fail(RVF_E_ARG, "rvf_mesh_upload: bad arguments");
+    if (n_elems > ((int64_t)1 << 30)) return fail(RVF_E_ARG, "rvf_mesh_upload: too many elements");
+    HIPCHK(hipSetDevice(ctx->device));
+    const int NNF[4] = {8, 6, 4, 3};
+    rvf_mesh* m = new rvf_mesh();
+    m->ctx = ctx; m->N = n_elems; m->Nn = n_nodes;
+    m->fam.assign(family, family + n_elems);
+    m->nodes8.assign(8 * n_elems, 0);
+    double Sm = 0.0;
+    for (int64_t k = 0; k < 3 * n_nodes; k++) {
+        if (!std::isfinite(coords[k])) { delete m; return fail(RVF_E_ARG, "rvf_mesh_upload: non-finite coordinate"); }
+        Sm = std::max(Sm, std::fabs(coords[k]));
+    }
+    m->Sm = Sm > 0 ? Sm : 1.0;
+    for (int64_t e = 0; e < n_elems; e++) {
+        int f = family[e];
+        if (f < 0 || f > 3) { delete m; return fail(RVF_E_ARG, "GPU backend does not support element family code " + std::to_string(f)); }
+        int64_t col = (int64_t)elem_node_idx[e] - 1;
+        if (col < 0 || col >= n_per_family[f] || !nodes[f]) { delete m; return fail(RVF_E_ARG, "rvf_mesh_upload: elem_node_idx out of range"); }
+        for (int a = 0; a < NNF[f]; a++) {
+            int32_t nd = nodes[f][col * NNF[f] + a];
+            if (nd < 1 || nd > n_nodes) { delete m; return fail(RVF_E_ARG, "rvf_mesh_upload: node index out of range"); }
+            m->nodes8[8 * e + a] = nd;
+        }
+    }
+    // groups: sorted distinct tags (Results.jl:59) -> ranks
+    std::vector<int32_t> t(elem_group, elem_group + n_elems);
+    std::sort(t.begin(), t.end());
+    t.erase(std::unique(t.begin(), t.end()), t.end());
+    m->tags = t;
+    m->grp_rank.resize(n_elems);
+    for (int64_t e = 0; e < n_elems; e++)
+        m->grp_rank[e] = (int32_t)(std::lower_bound(t.begin(), t.end(), elem_group[e]) - t.begin());
+    // plan: strips of 64, group-pure column chunks of <= 16
+    m->NS = (int)(((n_elems + RVF_STRIP - 1) / RVF_STRIP) * RVF_STRIP);
+    m->chunk_of.resize(n_elems);
+    for (int64_t e = 0; e < n_elems;) {
+        int64_t b = e;
+        int g = m->grp_rank[e];
+        while (e < n_elems && e - b < RVF_CHUNK && m->grp_rank[e] == g) e++;
+        Chunk c; c.begin = (int32_t)b; c.end = (int32_t)e; c.grp = g; c.pad = 0;
+        for (int64_t k = b; k < e; k++) m->chunk_of[k] = (int32_t)m->chunks.size();
+        m->chunks.push_back(c);
+    }
+    // tiles (strip s, chunk c) exist iff 64 s < last(c); heavy-first order is
+    // irrelevant for uniform work, so order by chunk then strip.
+    for (size_t c = 0; c < m->chunks.size(); c++) {
+        int64_t last = m->chunks[c].end - 1;
+        for (int64_t s = 0; s * RVF_STRIP < last; s++) m->tiles.push_back(make_int2((int)s, (int)c));
+    }
+    int64_t nb = m->NS / RVF_STRIP;
+    for (int64_t bj = 0; bj < nb; bj++)
+        for (int64_t bi = 0; bi <= bj; bi++) m->fill_tiles.push_back(make_int2((int)bi, (int)bj));
+    int rc;
+    if ((rc = upload(m->coords, coords, sizeof(double) * 3 * n_nodes)) ||
+        (rc = upload(m->d_fam, m->fam.data(), n_elems)) ||
+        (rc = upload(m->d_nodes8, m->nodes8.data(), sizeof(int32_t) * 8 * n_elems)) ||
+        (rc = upload(m->d_grp, m->grp_rank.data(), sizeof(int32_t) * n_elems)) ||
+        (rc = upload(m->d_chunks, m->chunks.data(), sizeof(Chunk) * m->chunks.size())) ||
+        (rc = upload(m->d_chunk_of, m->chunk_of.data(), sizeof(int32_t) * n_elems)) ||
+        (rc = upload(m->d_tiles, m->tiles.data(), sizeof(int2) * m->tiles.size())) ||
+        (rc = upload(m->d_fill_tiles, m->fill_tiles.data(), sizeof(int2) * m->fill_tiles.size()))) {
+        delete m;
+        return rc;
+    }
+    *out = m;
+    return 0;
+}
+
+void rvf_mesh_free(rvf_mesh* m) { delete m; }
+
+int rvf_mesh_info(const rvf_mesh* m, int64_t* n, int32_t* G, int32_t* tags) {
+    if (!m) return fail(RVF_E_ARG, "null mesh");
+    if (n) *n = m->N;
+    if (G) *G = (int32_t)m->tags.size();
+    if (tags) for (size_t k = 0; k < m->tags.size(); k++) tags[k] = m->tags[k];
+    return 0;
+}
+
+// ---------------------------------------------------------------------------
+// BVH: build_bvh / _build_recursive! (src/BVH.jl:71-148), bit-identical.
+// ---------------------------------------------------------------------------
+static inline double jmin(double x, double y) {  // Julia Base.min for floats
+    double d = x - y;
+    double r = std::signbit(d) ? x : y;
+    return (std::isnan(x) || std::isnan(y)) ? d : r;
+}
+static inline double jmax(double x, double y) {
+    double d = x - y;
+    double r = std::signbit(d) ? y : x;
+    return (std::isnan(x) || std::isnan(y)) ? d : r;
+}
+static inline bool jisless(double a, double b) {
+    if (std::isnan(a)) return false;
+    if (std::isnan(b)) return true;
+    if (a < b) return true;
+    if (a > b) return false;
+    return std::signbit(a) && !std::signbit(b);
+}
+
+struct HostNode { double lo[3], hi[3]; int left, right, start, count; };  // 1-based like the reference
+
+struct Builder {
+    const double* soup;   // merged (3,3,T)
+    int av, ac;           // AABB read strides (vertex, coord)
+    std::vector<int> idx; // 1-based permutation
+    std::vector<HostNode> nodes;
+    std::vector<double> key;
+    double rd(int64_t t, int v, int c) const { return soup[9 * t + v * av + c * ac]; }
+    void tri_box(int64_t t, double* lo, double* hi) const {
+        for (int c = 0; c < 3; c++) lo[c] = hi[c] = rd(t, 0, c);
+        for (int v = 1; v < 3; v++)
+            for (int c = 0; c < 3; c++) { double x = rd(t, v, c); lo[c] = jmin(lo[c], x); hi[c] = jmax(hi[c], x); }
+    }
+    int build(int start, int stop) {  // 1-based inclusive
+        double lo[3], hi[3];
+        tri_box(idx[start - 1] - 1, lo, hi);
+        for (int i = start + 1; i <= stop; i++) {
+            double l2[3], h2[3];
+            tri_box(idx[i - 1] - 1, l2, h2);
+            for (int c = 0; c < 3; c++) { lo[c] = jmin(lo[c], l2[c]); hi[c] = jmax(hi[c], h2[c]); }
+        }
+        int count = stop - start + 1;
+        HostNode nd;
+        std::memcpy(nd.lo, lo, sizeof lo); std::memcpy(nd.hi, hi, sizeof hi);
+        if (count <= 8) {
+            nd.left = nd.right = 0; nd.start = start; nd.count = count;
+            nodes.push_back(nd);
+            return (int)nodes.size();
+        }
+        double ext[3] = {hi[0] - lo[0], hi[1] - lo[1], hi[2] - lo[2]};
+        int axis = 0;
+        for (int c = 1; c < 3; c++) if (jisless(ext[axis], ext[c])) axis = c;
+        int mid = (start + stop) / 2;
+        // stable sort by centroid: ((0 + x1) + x2) + x3, / 3 (BVH.jl:100-109)
+        std::vector<std::pair<double, int>> kv(count);
+        for (int i = start; i <= stop; i++) {
+            int64_t t = idx[i - 1] - 1;
+            double s = 0.0;
+            for (int v = 0; v < 3; v++) s += rd(t, v, axis);
+            kv[i - start] = std::make_pair(s / 3.0, idx[i - 1]);
+        }
+        std::stable_sort(kv.begin(), kv.end(),
+                         [](const std::pair<double, int>& a, const std::pair<double, int>& b) {
+                             return jisless(a.first, b.first);
+                         });
+        for (int i = start; i <= stop; i++) idx[i - 1] = kv[i - start].second;
+        nd.left = nd.right = nd.start = nd.count = 0;
+        nodes.push_back(nd);
+        int me = (int)nodes.size();
+        int l = build(start, mid);
+        int r = build(mid + 1, stop);
+        nodes[me - 1].left = l; nodes[me - 1].right = r;
+        return me;
+    }
+};
+
+struct Tree {
+    std::vector<int> groups;          // active set (sorted)
+    std::vector<double> soup;         // merged soup
+    std::vector<int> idx;             // 1-based permutation
+    std::vector<HostNode> nodes;
+    std::vector<int> miss;            // 1-based, 0 = end
+    int64_t node_base = 0, tri_base = 0;
+};
+
+static void assign_miss(Tree& t, int node, int next) {  // GPUBVH.jl:69-82
+    t.miss[node - 1] = next;
+    const HostNode& nd = t.nodes[node - 1];
+    if (nd.left != 0) {
+        int l = nd.left, r = nd.right;
+        assign_miss(t, l, r);
+        assign_miss(t, r, next);
+    }
+}
+
+struct rvf_obs {
+    rvf_ctx* ctx = nullptr;
+    int G = 0;
+    int soup_read = 0;
+    std::vector<Tree> trees;
+    std::vector<int32_t> root;   // G*G absolute node index or -1
+    int64_t n_nodes = 0, n_tris = 0;
+    float Sm = 1.0f;
+    DBuf d_root, d_n64, d_t64, d_n32, d_t32;
+};
+
+static float f32_down(double x) {
+    float f = (float)x;
+    if ((double)f > x) f = std::nextafter(f, -INFINITY);
+    return f;
+}
+static float f32_up(double x) {
+    float f = (float)x;
+    if ((double)f < x) f = std::nextafter(f, INFINITY);
+    return f;
+}
+
+int rvf_obs_build(rvf_ctx* ctx, const rvf_mesh* mesh, const double* soup, const int32_t* tri_group, int64_t T,
+                  const int32_t* obs_groups, int32_t n_obs, int32_t soup_read, rvf_obs** out) {
+    if (!ctx || !mesh || !out || T < 0 || n_obs < 0 || soup_read < 0 || soup_read > 2)
+        return fail(RVF_E_ARG, "rvf_obs_build: bad arguments");
+    HIPCHK(hipSetDevice(ctx->device));
+    for (int64_t k = 0; k < 9 * T; k++)
+        if (!std::isfinite(soup[k])) return fail(RVF_E_ARG, "rvf_obs_build: non-finite soup coordinate");
+    rvf_obs* o = new rvf_obs();
+    o->ctx = ctx; o->soup_read = soup_read;
+    int G = (int)mesh->tags.size();
+    o->G = G;
+    o->root.assign((size_t)G * G, -1);
+    // strides: coord c of vertex v at v*sv + c*sc
+    int aav, aac, mtv, mtc;
+    if (soup_read == RVF_SOUP_REFERENCE) { aac = 1; aav = 3; mtv = 1; mtc = 3; }
+    else if (soup_read == RVF_SOUP_XYZ_VERTEX) { aac = 1; aav = 3; mtc = 1; mtv = 3; }
+    else { aav = 1; aac = 3; mtv = 1; mtc = 3; }
+    double Sm = mesh->Sm;
+    for (int64_t k = 0; k < 9 * T; k++) Sm = std::max(Sm, std::fabs(soup[k]));
+    o->Sm = (float)(Sm * (1 + 1e-6));
+    std::map<std::vector<int>, int> seen;
+    std::vector<int> act;
+    for (int a = 0; a < G; a++)
+        for (int b = 0; b < G; b++) {
+            act.clear();
+            for (int k = 0; k < n_obs; k++)
+                if (obs_groups[k] != mesh->tags[a] && obs_groups[k] != mesh->tags[b]) act.push_back(obs_groups[k]);
+            std::sort(act.begin(), act.end());
+            if (act.empty()) continue;
+            auto it = seen.find(act);
+            if (it != seen.end()) { o->root[a * G + b] = it->second; continue; }
+            Tree tr;
+            tr.groups = act;
+            for (int g : act)
+                for (int64_t t = 0; t < T; t++)
+                    if (tri_group[t] == g) tr.soup.insert(tr.soup.end(), soup + 9 * t, soup + 9 * t + 9);
+            int64_t nt = (int64_t)tr.soup.size() / 9;
+            if (nt == 0) { seen[act] = -1; continue; }
+            if (nt >= ((int64_t)1 << 27)) { delete o; return fail(RVF_E_ARG, "rvf_obs_build: too many triangles"); }
+            Builder bd;
+            bd.soup = tr.soup.data(); bd.av = aav; bd.ac = aac;
+            bd.idx.resize(nt);
+            for (int64_t t = 0; t < nt; t++) bd.idx[t] = (int)(t + 1);
+            bd.build(1, (int)nt);
+            tr.idx = bd.idx; tr.nodes = bd.nodes;
+            tr.miss.assign(tr.nodes.size(), 0);
+            assign_miss(tr, 1, 0);
+            tr.node_base = o->n_nodes; tr.tri_base = o->n_tris;
+            o->n_nodes += (int64_t)tr.nodes.size();
+            o->n_tris += nt;
+            int id = (int)o->trees.size();
+            o->trees.push_back(std::move(tr));
+            seen[act] = (int)o->trees[id].node_base;
+            o->root[a * G + b] = (int)o->trees[id].node_base;
+        }
+    // flatten
+    std::vector<BvhNode64> n64(std::max<int64_t>(o->n_nodes, 1));
+    std::vector<BvhNode32> n32(std::max<int64_t>(o->n_nodes, 1));
+    std::vector<Tri64> t64(std::max<int64_t>(o->n_tris, 1));
+    std::vector<Tri32> t32(std::max<int64_t>(o->n_tris, 1));
+    const double u = 5.9604644775390625e-08;
+    const float pad = (float)(128.0 * u * Sm);
+    for (const Tree& tr : o->trees) {
+        for (size_t k = 0; k < tr.nodes.size(); k++) {
+            const HostNode& h = tr.nodes[k];
+            BvhNode64& d = n64[tr.node_base + k];
+            for (int c = 0; c < 3; c++) { d.lo[c] = h.lo[c]; d.hi[c] = h.hi[c]; }
+            d.miss = tr.miss[k] ? (int32_t)(tr.node_base + tr.miss[k] - 1) : -1;
+            d.start = h.count ? (int32_t)(tr.tri_base + h.start - 1) : 0;
+            d.count = h.count;
+            d.pad = 0;
+            BvhNode32& f = n32[tr.node_base + k];
+            for (int c = 0; c < 3; c++) { f.lo[c] = f32_down(h.lo[c]) - pad; f.hi[c] = f32_up(h.hi[c]) + pad; }
+            f.miss = d.miss;
+            f.start_count = (d.start << 4) | d.count;
+            if (h.left && h.left != (int)k + 2) { delete o; return fail(RVF_E_STATE, "bvh: non-preorder layout"); }
+        }
+        int64_t nt = (int64_t)tr.idx.size();
+        for (int64_t k = 0; k < nt; k++) {
+            int64_t t = tr.idx[k] - 1;
+            const double* s = tr.soup.data() + 9 * t;
+            double v[3][3];
+            for (int vv = 0; vv < 3; vv++)
+                for (int c = 0; c < 3; c++) v[vv][c] = s[vv * mtv + c * mtc];
+            Tri64& d = t64[tr.tri_base + k];
+            for (int c = 0; c < 3; c++) {
+                d.v0[c] = v[0][c]; d.e1[c] = v[1][c] - v[0][c]; d.e2[c] = v[2][c] - v[0][c];
+            }
+            d.pad = 0;
+            Tri32& f = t32[tr.tri_base + k];
+            double E1 = 0, E2 = 0;
+            for (int c = 0; c < 3; c++) {
+                f.v0[c] = (float)d.v0[c]; f.e1[c] = (float)d.e1[c]; f.e2[c] = (float)d.e2[c];
+                E1 += std::fabs(d.e1[c]); E2 += std::fabs(d.e2[c]);
+            }
+            f.sa = (float)(E1 * E2 * (1 + 1e-6));
+            f.pad0 = f.pad1 = 0;
+        }
+    }
+    int rc;
+    if ((rc = upload(o->d_root, o->root.data(), sizeof(int32_t) * std::max(1, G * G))) ||
+        (rc = upload(o->d_n64, n64.data(), sizeof(BvhNode64) * n64.size())) ||
+        (rc = upload(o->d_n32, n32.data(), sizeof(BvhNode32) * n32.size())) ||
+        (rc = upload(o->d_t64, t64.data(), sizeof(Tri64) * t64.size())) ||
+        (rc = upload(o->d_t32, t32.data(), sizeof(Tri32) * t32.size()))) {
+        delete o;
+        return rc;
+    }
+    *out = o;
+    return 0;
+}
+
+void rvf_obs_free(rvf_obs* o) { delete o; }
+
+int rvf_obs_info(const rvf_obs* o, int32_t* n_trees, int64_t* n_nodes, int64_t* n_tris) {
+    if (!o) return fail(RVF_E_ARG, "null obs");
+    if (n_trees) *n_trees = (int32_t)o->trees.size();
+    if (n_nodes) *n_nodes = o->n_nodes;
+    if (n_tris) *n_tris = o->n_tris;
+    return 0;
+}
+
+int rvf_obs_export(const rvf_obs* o, int32_t tree, int64_t cap, double* box, int32_t* meta, int32_t* tri_idx,
+                   int64_t* n_nodes, int64_t* n_tris) {
+    if (!o || tree < 0 || tree >= (int)o->trees.size()) return fail(RVF_E_ARG, "rvf_obs_export: bad tree");
+    const Tree& t = o->trees[tree];
+    if (n_nodes) *n_nodes = (int64_t)t.nodes.size();
+    if (n_tris) *n_tris = (int64_t)t.idx.size();
+    if ((int64_t)t.nodes.size() > cap || (int64_t)t.idx.size() > cap) return fail(RVF_E_ARG, "rvf_obs_export: cap too small");
+    for (size_t k = 0; k < t.nodes.size(); k++) {
+        for (int c = 0; c < 3; c++) { box[6 * k + c] = t.nodes[k].lo[c]; box[6 * k + 3 + c] = t.nodes[k].hi[c]; }
+        meta[5 * k] = t.nodes[k].left; meta[5 * k + 1] = t.nodes[k].right;
+        meta[5 * k + 2] = t.nodes[k].start; meta[5 * k + 3] = t.nodes[k].count; meta[5 * k + 4] = t.miss[k];
+    }
+    for (size_t k = 0; k < t.idx.size(); k++) tri_idx[k] = t.idx[k];
+    return 0;
+}
+
+int rvf_bvh_build_host(const double* soup, int64_t T, int32_t soup_read, int64_t cap, double* box, int32_t* meta,
+                       int32_t* tri_idx, int64_t* n_nodes) {
+    if (!soup || T <= 0 || soup_read < 0 || soup_read > 2) return fail(RVF_E_ARG, "rvf_bvh_build_host: bad arguments");
+    Builder bd;
+    bd.soup = soup;
+    if (soup_read == RVF_SOUP_VERTEX_XYZ) { bd.av = 1; bd.ac = 3; } else { bd.ac = 1; bd.av = 3; }
+    bd.idx.resize(T);
+    for (int64_t t = 0; t < T; t++) bd.idx[t] = (int)(t + 1);
+    bd.build(1, (int)T);
+    Tree tr;
+    tr.nodes = bd.nodes; tr.idx = bd.idx;
+    tr.miss.assign(tr.nodes.size(), 0);
+    assign_miss(tr, 1, 0);
+    if (n_nodes) *n_nodes = (int64_t)tr.nodes.size();
+    if ((int64_t)tr.nodes.size() > cap) return fail(RVF_E_ARG, "rvf_bvh_build_host: cap too small");
+    for (size_t k = 0; k < tr.nodes.size(); k++) {
+        for (int c = 0; c < 3; c++) { box[6 * k + c] = tr.nodes[k].lo[c]; box[6 * k + 3 + c] = tr.nodes[k].hi[c]; }
+        meta[5 * k] = tr.nodes[k].left; meta[5 * k + 1] = tr.nodes[k].right;
+        meta[5 * k + 2] = tr.nodes[k].start; meta[5 * k + 3] = tr.nodes[k].count; meta[5 * k + 4] = tr.miss[k];
+    }
+    for (int64_t t = 0; t < T; t++) tri_idx[t] = tr.idx[t];
+    return 0;
+}
+
+// ---------------------------------------------------------------------------
+// plan
+// ---------------------------------------------------------------------------
+struct rvf_plan {
+    rvf_ctx* ctx = nullptr;
+    const rvf_mesh* mesh = nullptr;
+    const rvf_obs* obs = nullptr;
+    rvf_opts opts{};
+    int nq_quad = 0, nq_tri = 0, maxpts = 0;
+    DBuf rule, P64, Q64, P32, Q32, anchor, area, elem_np, bad, F, part, partT, rowsum, W, audit_c, audit_h, counters;
+    hipEvent_t ev[8] = {};
+    rvf_stats st{};
+    bool ran = false;
+};
+
+int rvf_plan_create(rvf_ctx* ctx, const rvf_mesh* mesh, const rvf_obs* obs, const rvf_opts* opts, rvf_plan** out) {
+    if (!ctx || !mesh || !opts || !out) return fail(RVF_E_ARG, "rvf_plan_create: bad arguments");
+    if (opts->nquad < 1 || opts->nquad > 8) return fail(RVF_E_ARG, "nquad must be in 1..8");
+    if (opts->method < 0 || opts->method > 2) return fail(RVF_E_ARG, "unknown method");
+    if (opts->precision < 0 || opts->precision > 1) return fail(RVF_E_ARG, "unknown precision");
+    if (opts->method == RVF_METHOD_MC && opts->n_samples < 1) return fail(RVF_E_ARG, "n_samples must be >= 1");
+    if (obs && obs->G != (int)mesh->tags.size()) return fail(RVF_E_ARG, "obstruction set built for another mesh");
+    HIPCHK(hipSetDevice(ctx->device));
+    rvf_plan* p = new rvf_plan();
+    p->ctx = ctx; p->mesh = mesh; p->obs = obs; p->opts = *opts;
+    std::vector<double> rule(3 * 64 + 39, 0.0);
+    p->nq_quad = quad_rule(opts->nquad, rule.data(), rule.data() + 2 * 64);
+    p->nq_tri = tri_rule(opts->nquad, rule.data() + 3 * 64, rule.data() + 3 * 64 + 26);
+    bool anyq = false, anyt = false;
+    for (int8_t f : mesh->fam) { if (f == 0 || f == 2) anyq = true; else anyt = true; }
+    p->maxpts = std::max(anyq ? p->nq_quad : 0, anyt ? p->nq_tri : 0);
+    if (p->maxpts == 0) p->maxpts = 1;
+    int64_t N = mesh->N, NS = mesh->NS;
+    int G = (int)mesh->tags.size();
+    size_t nch = mesh->chunks.size(), nstrip = NS / RVF_STRIP;
+    int rc;
+    if ((rc = upload(p->rule, rule.data(), sizeof(double) * rule.size())) ||
+        (rc = p->P64.alloc(sizeof(double) * 8 * p->maxpts * NS)) ||
+        (rc = p->Q64.alloc(sizeof(double) * 8 * p->maxpts * NS)) ||
+        (rc = p->P32.alloc(sizeof(float) * 8 * p->maxpts * NS)) ||
+        (rc = p->Q32.alloc(sizeof(float) * 8 * p->maxpts * NS)) ||
+        (rc = p->anchor.alloc(sizeof(double) * 4 * NS)) || (rc = p->area.alloc(sizeof(double) * NS)) ||
+        (rc = p->elem_np.alloc(sizeof(int32_t) * NS)) || (rc = p->bad.alloc(sizeof(int))) ||
+        (rc = p->part.alloc(sizeof(double) * std::max<size_t>(nch, 1) * NS)) ||
+        (rc = p->partT.alloc(sizeof(double) * std::max<size_t>(nstrip, 1) * NS)) ||
+        (rc = p->rowsum.alloc(sizeof(double) * NS)) || (rc = p->W.alloc(sizeof(double) * std::max(G, 1) * NS)) ||
+        (rc = p->counters.alloc(sizeof(unsigned long long) * 4))) {
+        delete p;
+        return rc;
+    }
+    if (opts->flags & RVF_WANT_F) {
+        if ((rc = p->F.alloc(sizeof(double) * N * N))) { delete p; return rc; }
+    }
+    if ((opts->flags & RVF_AUDIT) && obs) {
+        if ((rc = p->audit_c.alloc(sizeof(uint32_t) * N * N)) || (rc = p->audit_h.alloc(sizeof(uint32_t) * N * N))) {
+            delete p;
+            return rc;
+        }
+        HIPCHK(hipMemset(p->audit_c.p, 0, p->audit_c.n));
+        HIPCHK(hipMemset(p->audit_h.p, 0, p->audit_h.n));
+    }
+    // zero point arrays once (padding lanes / points read but never accumulated)
+    HIPCHK(hipMemset(p->P64.p, 0, p->P64.n));
+    HIPCHK(hipMemset(p->P32.p, 0, p->P32.n));
+    HIPCHK(hipMemset(p->anchor.p, 0, p->anchor.n));
+    HIPCHK(hipMemset(p->area.p, 0, p->area.n));
+    HIPCHK(hipMemset(p->elem_np.p, 0, p->elem_np.n));
+    for (int k = 0; k < 8; k++) HIPCHK(hipEventCreate(&p->ev[k]));
+    p->st.n_elems = N;
+    p->st.unique_pairs = N * (N - 1) / 2;
+    p->st.tiles = (int64_t)mesh->tiles.size();
+    *out = p;
+    return 0;
+}
+
+static PairArgs pair_args(rvf_plan* p) {
+    const rvf_mesh* m = p->mesh;
+    PairArgs a{};
+    a.tiles = m->d_tiles.as<int2>(); a.ntiles = (int)m->tiles.size();
+    a.NS = m->NS; a.maxpts = p->maxpts; a.N = m->N;
+    a.chunks = m->d_chunks.as<Chunk>();
+    a.elem_np = p->elem_np.as<int32_t>(); a.elem_grp = m->d_grp.as<int32_t>(); a.area = p->area.as<double>();
+    a.P64 = p->P64.as<double>(); a.Q64 = p->Q64.as<double>(); a.P32 = p->P32.as<float>(); a.Q32 = p->Q32.as<float>();
+    a.anchor = p->anchor.as<double>();
+    a.F = p->F.as<double>(); a.part = p->part.as<double>(); a.partT = p->partT.as<double>();
+    a.G = (int)m->tags.size();
+    if (p->obs && !p->obs->trees.empty()) {
+        a.tree_root = p->obs->d_root.as<int32_t>();
+        a.nodes64 = p->obs->d_n64.as<BvhNode64>(); a.tris64 = p->obs->d_t64.as<Tri64>();
+        a.nodes32 = p->obs->d_n32.as<BvhNode32>(); a.tris32 = p->obs->d_t32.as<Tri32>();
+    }
+    a.audit_count = p->audit_c.as<uint32_t>(); a.audit_hash = p->audit_h.as<uint32_t>();
+    a.counters = p->counters.as<unsigned long long>();
+    return a;
+}
+
+int rvf_plan_run(rvf_plan* p) {
+    if (!p) return fail(RVF_E_ARG, "null plan");
+    rvf_ctx* c = p->ctx;
+    const rvf_mesh* m = p->mesh;
+    hipStream_t st = c->stream;
+    HIPCHK(hipSetDevice(c->device));
+    bool tim = (p->opts.flags & RVF_TIMING) != 0;
+    if (tim) HIPCHK(hipEventRecord(p->ev[0], st));
+    HIPCHK(hipMemsetAsync(p->bad.p, 0, sizeof(int), st));
+    HIPCHK(hipMemsetAsync(p->counters.p, 0, p->counters.n, st));
+    PrecompArgs pa{};
+    pa.coords = m->coords.as<double>(); pa.fam = m->d_fam.as<int8_t>(); pa.nodes8 = m->d_nodes8.as<int32_t>();
+    pa.N = m->N; pa.NS = m->NS; pa.maxpts = p->maxpts; pa.nq_quad = p->nq_quad; pa.nq_tri = p->nq_tri;
+    pa.rule = p->rule.as<double>();
+    pa.P64 = p->P64.as<double>(); pa.Q64 = p->Q64.as<double>(); pa.P32 = p->P32.as<float>(); pa.Q32 = p->Q32.as<float>();
+    pa.anchor = p->anchor.as<double>(); pa.area = p->area.as<double>(); pa.elem_np = p->elem_np.as<int32_t>();
+    pa.bad = p->bad.as<int>();
+    int rc = rvf_launch_precompute(&pa, st);
+    if (rc) return fail(RVF_E_HIP, std::string("precompute launch: ") + hipGetErrorString((hipError_t)rc));
+    if (tim) HIPCHK(hipEventRecord(p->ev[1], st));
+    PairArgs a = pair_args(p);
+    bool occ = p->obs && !p->obs->trees.empty();
+    if (p->opts.method == RVF_METHOD_QUAD || p->opts.method == RVF_METHOD_DUFFY) {
+        int pc = p->maxpts;
+        if (p->opts.precision == RVF_FP64 && pc > 9) pc = 9;  // register chunks of 9 points
+        rc = rvf_launch_pair(&a, p->opts.precision == RVF_MIXED, occ, pc, p->obs ? p->obs->Sm : 1.0f, st);
+        if (rc) return fail(RVF_E_HIP, std::string("pair launch: ") + hipGetErrorString((hipError_t)rc));
+    } else {
+        return fail(RVF_E_ARG, "Monte Carlo method not available in this build");
+    }
+    if (tim) HIPCHK(hipEventRecord(p->ev[2], st));
+    if (tim) HIPCHK(hipEventRecord(p->ev[3], st));  // duffy slot
+    if (p->F.p) {
+        rc = rvf_launch_fill(p->F.as<double>(), p->area.as<double>(), m->N, m->d_fill_tiles.as<int2>(),
+                             (int)m->fill_tiles.size(), st);
+        if (rc) return fail(RVF_E_HIP, std::string("fill launch: ") + hipGetErrorString((hipError_t)rc));
+    }
+    if (tim) HIPCHK(hipEventRecord(p->ev[4], st));
+    HIPCHK(hipMemsetAsync(p->W.p, 0, p->W.n, st));
+    rc = rvf_launch_reduce(p->part.as<double>(), p->partT.as<double>(), m->d_chunks.as<Chunk>(),
+                           (int)m->chunks.size(), m->d_chunk_of.as<int32_t>(), m->N, m->NS, (int)m->tags.size(),
+                           p->rowsum.as<double>(), p->W.as<double>(), st);
+    if (rc) return fail(RVF_E_HIP, std::string("reduce launch: ") + hipGetErrorString((hipError_t)rc));
+    if (tim) HIPCHK(hipEventRecord(p->ev[5], st));
+    p->ran = true;
+    return 0;
+}
+
+int rvf_plan_sync(rvf_plan* p) {
+    if (!p) return fail(RVF_E_ARG, "null plan");
+    HIPCHK(hipSetDevice(p->ctx->device));
+    HIPCHK(hipStreamSynchronize(p->ctx->stream));
+    int bad = 0;
+    HIPCHK(hipMemcpy(&bad, p->bad.p, sizeof(int), hipMemcpyDeviceToHost));
+    if (bad) return fail(RVF_E_ARG, "degenerate element: zero or non-finite area element at a quadrature point");
+    unsigned long long cnt[4];
+    HIPCHK(hipMemcpy(cnt, p->counters.p, sizeof cnt, hipMemcpyDeviceToHost));
+    p->st.retests = (int64_t)cnt[0];
+    p->st.rays = (int64_t)cnt[1];
+    p->st.aabb_tests = (int64_t)cnt[2];
+    p->st.tri_tests = (int64_t)cnt[3];
+    if (p->opts.flags & RVF_TIMING) {
+        float ms;
+        HIPCHK(hipEventElapsedTime(&ms, p->ev[0], p->ev[1])); p->st.precompute_ms = ms;
+        HIPCHK(hipEventElapsedTime(&ms, p->ev[1], p->ev[2])); p->st.pair_ms = ms;
+        HIPCHK(hipEventElapsedTime(&ms, p->ev[2], p->ev[3])); p->st.duffy_ms = ms;
+        HIPCHK(hipEventElapsedTime(&ms, p->ev[3], p->ev[4])); p->st.fill_ms = ms;
+        HIPCHK(hipEventElapsedTime(&ms, p->ev[4], p->ev[5])); p->st.reduce_ms = ms;
+        HIPCHK(hipEventElapsedTime(&ms, p->ev[0], p->ev[5])); p->st.total_ms = ms;
+    }
+    return 0;
+}
+
+int rvf_plan_fetch(rvf_plan* p, double* F, double* A, double* rowsum, double* F_group, double* A_group,
+                   int32_t* group_tags, uint32_t* audit_count, uint32_t* audit_hash) {
+    if (!p || !p->ran) return fail(RVF_E_STATE, "rvf_plan_fetch: plan has not run");
+    const rvf_mesh* m = p->mesh;
+    int64_t N = m->N;
+    int G = (int)m->tags.size();
+    HIPCHK(hipSetDevice(p->ctx->device));
+    if (F) {
+        if (!p->F.p) return fail(RVF_E_STATE, "F was not requested (RVF_WANT_F)");
+        HIPCHK(hipMemcpy(F, p->F.p, sizeof(double) * N * N, hipMemcpyDeviceToHost));
+    }
+    std::vector<double> hA(N), hW((size_t)G * m->NS);
+    HIPCHK(hipMemcpy(hA.data(), p->area.p, sizeof(double) * N, hipMemcpyDeviceToHost));
+    if (A) std::memcpy(A, hA.data(), sizeof(double) * N);
+    if (rowsum) HIPCHK(hipMemcpy(rowsum, p->rowsum.p, sizeof(double) * N, hipMemcpyDeviceToHost));
+    if (F_group || A_group) {
+        HIPCHK(hipMemcpy(hW.data(), p->W.p, sizeof(double) * G * m->NS, hipMemcpyDeviceToHost));
+        // A_g F_gh = sum_{i in g} A_i sum_{j in h} F_ij (Results.jl:56-82), with the
+        // j < i part from the symmetric raw integral: sum_{j in h} A_j W[g][j]
+        std::vector<double> Ag(G, 0.0), R((size_t)G * G, 0.0);
+        for (int64_t i = 0; i < N; i++) Ag[m->grp_rank[i]] += hA[i];
+        for (int64_t i = 0; i < N; i++) {
+            int g = m->grp_rank[i];
+            for (int h = 0; h < G; h++) {
+                double v = hA[i] * hW[(size_t)h * m->NS + i];
+                R[(size_t)g * G + h] += v;   // pairs (i in g, j in h, j > i)
+                R[(size_t)h * G + g] += v;   // the same pairs seen from row j
+            }
+        }
+        if (A_group) for (int g = 0; g < G; g++) A_group[g] = Ag[g];
+        if (F_group)
+            for (int g = 0; g < G; g++)
+                for (int h = 0; h < G; h++) F_group[g + (size_t)G * h] = R[(size_t)g * G + h] / Ag[g];
+    }
+    if (group_tags) for (int g = 0; g < G; g++) group_tags[g] = m->tags[g];
+    if (audit_count || audit_hash) {
+        if (!p->audit_c.p) return fail(RVF_E_STATE, "audit was not requested (RVF_AUDIT with an obstruction set)");
+        if (audit_count) HIPCHK(hipMemcpy(audit_count, p->audit_c.p, sizeof(uint32_t) * N * N, hipMemcpyDeviceToHost));
+        if (audit_hash) HIPCHK(hipMemcpy(audit_hash, p->audit_h.p, sizeof(uint32_t) * N * N, hipMemcpyDeviceToHost));
+    }
+    return 0;
+}
+
+int rvf_plan_stats(const rvf_plan* p, rvf_stats* st) {
+    if (!p || !st) return fail(RVF_E_ARG, "null plan");
+    *st = p->st;
+    return 0;
+}
+
+int rvf_plan_device_ptr(const rvf_plan* p, int which, void** ptr) {
+    if (!p || !ptr) return fail(RVF_E_ARG, "null plan");
+    *ptr = which == 0 ? p->F.p : which == 1 ? p->area.p : which == 2 ? p->rowsum.p : nullptr;
+    return 0;
+}
+
+void rvf_plan_free(rvf_plan* p) {
+    if (!p) return;
+    for (int k = 0; k < 8; k++) if (p->ev[k]) (void)hipEventDestroy(p->ev[k]);
+    delete p;
+}
+
+int rvf_compute_view_factors(rvf_ctx* ctx, const rvf_mesh* mesh, const rvf_obs* obs, const rvf_opts* opts,
+                             double* F, double* A, double* F_group, double* A_group, int32_t* group_tags,
+                             rvf_stats* st) {
+    rvf_opts o = *opts;
+    if (F) o.flags |= RVF_WANT_F;
+    rvf_plan* p = nullptr;
+    int rc = rvf_plan_create(ctx, mesh, obs, &o, &p);
+    if (rc) return rc;
+    if ((rc = rvf_plan_run(p)) || (rc = rvf_plan_sync(p)) ||
+        (rc = rvf_plan_fetch(p, F, A, nullptr, F_group, A_group, group_tags, nullptr, nullptr))) {
+        rvf_plan_free(p);
+        return rc;
+    }
+    if (st) *st = p->st;
+    rvf_plan_free(p);
+    return 0;
+}
+
+// ---------------------------------------------------------------------------
+// context + misc
+// ---------------------------------------------------------------------------
+int rvf_ctx_create(int device, rvf_ctx** out) {
+    if (!out) return fail(RVF_E_ARG, "null out");
+    int n = 0;
+    HIPCHK(hipGetDeviceCount(&n));
+    if (device < 0 || device >= n) return fail(RVF_E_ARG, "no such device");
+    HIPCHK(hipSetDevice(device));
+    rvf_ctx* c = new rvf_ctx();
+    c->device = device;
+    hipError_t e = hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking);
+    if (e != hipSuccess) { delete c; return fail(RVF_E_HIP, "hipStreamCreate failed"); }
+    for (int k = 0; k < 16; k++) (void)hipEventCreate(&c->ev[k]);
+    *out = c;
+    return 0;
+}
+
+void rvf_ctx_destroy(rvf_ctx* c) {
+    if (!c) return;
+    (void)hipSetDevice(c->device);
+    (void)hipStreamSynchronize(c->stream);
+    for (int k = 0; k < 16; k++) if (c->ev[k]) (void)hipEventDestroy(c->ev[k]);
+    (void)hipStreamDestroy(c->stream);
+    delete c;
+}
+
+const char* rvf_last_error(void) { return g_err.c_str(); }
+int rvf_version(void) { return 1; }
+
+int rvf_quad_rule(int32_t nquad, int32_t tri, double* pts, double* wts) {
+    if (!pts || !wts) return fail(RVF_E_ARG, "null output");
+    int n = tri ? tri_rule(nquad, pts, wts) : quad_rule(nquad, pts, wts);
+    if (n < 0) return fail(RVF_E_ARG, "nquad out of range");
+    return n;
+}
+
+int rvf_ctx_event_record(rvf_ctx* c, int32_t slot) {
+    if (!c || slot < 0 || slot >= 16) return fail(RVF_E_ARG, "bad event slot");
+    HIPCHK(hipEventRecord(c->ev[slot], c->stream));
+    return 0;
+}
+
+int rvf_ctx_event_elapsed(rvf_ctx* c, int32_t a, int32_t b, float* ms) {
+    if (!c || a < 0 || a >= 16 || b < 0 || b >= 16 || !ms) return fail(RVF_E_ARG, "bad event slot");
+    HIPCHK(hipEventSynchronize(c->ev[b]));
+    HIPCHK(hipEventElapsedTime(ms, c->ev[a], c->ev[b]));
+    return 0;
+}
+
+int rvf_ctx_sync(rvf_ctx* c) {
+    if (!c) return fail(RVF_E_ARG, "null ctx");
+    HIPCHK(hipSetDevice(c->device));
+    HIPCHK(hipStreamSynchronize(c->stream));
+    return 0;
+}

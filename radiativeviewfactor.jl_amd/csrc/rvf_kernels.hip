@@ -1,0 +1,808 @@
+// HIP kernels for the element-pair view-factor path on MI355X (gfx950).
+//
+// Layout and mapping
+//   * Element e's quadrature points are precomputed once (k_precompute) into
+//     two layouts: point-major SoA P[(p*8+c)*NS + e] read by lanes (coalesced
+//     over e) and element-major Q[(e*maxpts+p)*8 + c] read with wave-uniform
+//     (scalar) loads.
+//   * A pair tile is one wavefront: lane = row element i of a 64-element strip,
+//     the column element j (a run of <= 16 elements of one group) is uniform
+//     across the wave.  Each lane owns its pair sum, so the FP64 mode sums in
+//     exactly the reference's order (ViewFactorKernel.jl:215-238).
+//   * Row sums and group sums come from per-tile partials written without
+//     atomics (deterministic), reduced by k_reduce_rows.
+//
+// FP64 mode: reference arithmetic verbatim (vf_kernel, ViewFactorKernel.jl:
+// 155-166; is_visible / _ray_aabb / _ray_triangle, RayCast.jl:20-32,
+// BVH.jl:155-201), compiled with -ffp-contract=off.
+// Mixed mode: FP32 anchored coordinates for the kernel, FP64 outer sums, FP32
+// box traversal on padded bounds with an FP32 ray-triangle filter whose
+// undecided cases are re-tested in FP64 with the reference arithmetic.
+#include "rvf_common.h"
+
+#define PI_D 3.141592653589793
+
+// ---------------------------------------------------------------------------
+// wave helpers
+// ---------------------------------------------------------------------------
+__device__ __forceinline__ double wave_sum_d(double v) {
+#pragma unroll
+    for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o, 64);
+    return v;
+}
+
+__device__ __forceinline__ void neumaier(double& s, double& c, double x) {
+    double t = s + x;
+    c += (fabs(s) >= fabs(x)) ? ((s - t) + x) : ((x - t) + s);
+    s = t;
+}
+
+__device__ __forceinline__ uint32_t mix32(uint32_t x) {
+    x ^= x >> 16; x *= 0x7feb352dU; x ^= x >> 15; x *= 0x846ca68bU; x ^= x >> 16;
+    return x;
+}
+
+// ---------------------------------------------------------------------------
+// Shape functions (Geometry.jl:36-71,124-134; ViewFactorKernel.jl:77-83,
+// 116-121), same expression trees as the reference, no contraction.
+// ---------------------------------------------------------------------------
+__device__ __forceinline__ int shape_fn(int fam, double x, double e, double* N, double* dx, double* de) {
+    if (fam == 0) {
+        N[0] = ((0.25 * (1 - x)) * (1 - e)) * ((-x - e) - 1);
+        N[1] = ((0.25 * (1 + x)) * (1 - e)) * ((x - e) - 1);
+        N[2] = ((0.25 * (1 + x)) * (1 + e)) * ((x + e) - 1);
+        N[3] = ((0.25 * (1 - x)) * (1 + e)) * ((-x + e) - 1);
+        N[4] = (0.5 * (1 - x * x)) * (1 - e);
+        N[5] = (0.5 * (1 + x)) * (1 - e * e);
+        N[6] = (0.5 * (1 - x * x)) * (1 + e);
+        N[7] = (0.5 * (1 - x)) * (1 - e * e);
+        dx[0] = (0.25 * (1 - e)) * (2 * x + e);
+        dx[1] = (0.25 * (1 - e)) * (2 * x - e);
+        dx[2] = (0.25 * (1 + e)) * (2 * x + e);
+        dx[3] = (0.25 * (1 + e)) * (2 * x - e);
+        dx[4] = (-x) * (1 - e);
+        dx[5] = 0.5 * (1 - e * e);
+        dx[6] = (-x) * (1 + e);
+        dx[7] = (-0.5) * (1 - e * e);
+        de[0] = (0.25 * (1 - x)) * (x + 2 * e);
+        de[1] = (0.25 * (1 + x)) * (-x + 2 * e);
+        de[2] = (0.25 * (1 + x)) * (x + 2 * e);
+        de[3] = (0.25 * (1 - x)) * (-x + 2 * e);
+        de[4] = (-0.5) * (1 - x * x);
+        de[5] = (-(1 + x)) * e;
+        de[6] = 0.5 * (1 - x * x);
+        de[7] = (-(1 - x)) * e;
+        return 8;
+    } else if (fam == 1) {
+        double L1 = (1.0 - x) - e, L2 = x, L3 = e;
+        N[0] = L1 * (2 * L1 - 1); N[1] = L2 * (2 * L2 - 1); N[2] = L3 * (2 * L3 - 1);
+        N[3] = (4 * L1) * L2; N[4] = (4 * L2) * L3; N[5] = (4 * L1) * L3;
+        dx[0] = (4 * L1 - 1) * (-1.0); dx[1] = 4 * L2 - 1; dx[2] = 0.0;
+        dx[3] = 4 * (L2 * (-1.0) + L1); dx[4] = 4 * L3; dx[5] = (4 * L3) * (-1.0);
+        de[0] = (4 * L1 - 1) * (-1.0); de[1] = 0.0; de[2] = 4 * L3 - 1;
+        de[3] = (4 * L2) * (-1.0); de[4] = 4 * L2; de[5] = 4 * (L3 * (-1.0) + L1);
+        return 6;
+    } else if (fam == 2) {
+        N[0] = (0.25 * (1 - x)) * (1 - e);
+        N[1] = (0.25 * (1 + x)) * (1 - e);
+        N[2] = (0.25 * (1 + x)) * (1 + e);
+        N[3] = (0.25 * (1 - x)) * (1 + e);
+        dx[0] = (-0.25) * (1 - e); dx[1] = 0.25 * (1 - e); dx[2] = 0.25 * (1 + e); dx[3] = (-0.25) * (1 + e);
+        de[0] = (-0.25) * (1 - x); de[1] = (-0.25) * (1 + x); de[2] = 0.25 * (1 + x); de[3] = 0.25 * (1 - x);
+        return 4;
+    } else {
+        N[0] = (1.0 - x) - e; N[1] = x; N[2] = e;
+        dx[0] = -1.0; dx[1] = 1.0; dx[2] = 0.0;
+        de[0] = -1.0; de[1] = 0.0; de[2] = 1.0;
+        return 3;
+    }
+}
+
+// x, n = c/|c|, dA = |c| at (xi, eta) (Geometry.jl:78-110 and twins)
+__device__ __forceinline__ void eval_point(int fam, const double* __restrict__ coords, const int32_t* nodes,
+                                           double xi, double eta, double* x, double* n, double* dA) {
+    double N[8], dx[8], de[8];
+    int nn = shape_fn(fam, xi, eta, N, dx, de);
+    double p0 = 0.0, p1 = 0.0, p2 = 0.0, a0 = 0.0, a1 = 0.0, a2 = 0.0, b0 = 0.0, b1 = 0.0, b2 = 0.0;
+    for (int k = 0; k < nn; k++) {
+        const double* xa = coords + 3 * (int64_t)(nodes[k] - 1);
+        double X = xa[0], Y = xa[1], Z = xa[2];
+        p0 = p0 + N[k] * X; p1 = p1 + N[k] * Y; p2 = p2 + N[k] * Z;
+        a0 = a0 + dx[k] * X; a1 = a1 + dx[k] * Y; a2 = a2 + dx[k] * Z;
+        b0 = b0 + de[k] * X; b1 = b1 + de[k] * Y; b2 = b2 + de[k] * Z;
+    }
+    double c0 = a1 * b2 - a2 * b1, c1 = a2 * b0 - a0 * b2, c2 = a0 * b1 - a1 * b0;
+    double d = sqrt((c0 * c0 + c1 * c1) + c2 * c2);
+    x[0] = p0; x[1] = p1; x[2] = p2;
+    n[0] = c0 / d; n[1] = c1 / d; n[2] = c2 / d;
+    *dA = d;
+}
+
+// ---------------------------------------------------------------------------
+// Precompute: one thread per element, points in rule order.  Writes FP64 SoA
+// and AoS, the area A_e = sum_p w_p dA_p in p order (Assembly.jl:186-190 via
+// ViewFactorKernel.jl:221), and the FP32 anchored data for mixed mode.
+// ---------------------------------------------------------------------------
+
+
+__global__ __launch_bounds__(64) void k_precompute(PrecompArgs a) {
+    int64_t e = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
+    if (e >= a.N) return;
+    int f = a.fam[e];
+    int isq = (f == 0 || f == 2);
+    int np = isq ? a.nq_quad : a.nq_tri;
+    const double* pts = isq ? a.rule : a.rule + 3 * 64;
+    const double* wts = isq ? a.rule + 2 * 64 : a.rule + 3 * 64 + 26;
+    const int32_t* nd = a.nodes8 + 8 * e;
+    double A = 0.0, cx = 0.0, cy = 0.0, cz = 0.0;
+    for (int p = 0; p < np; p++) {
+        double x[3], n[3], dA;
+        eval_point(f, a.coords, nd, pts[2 * p], pts[2 * p + 1], x, n, &dA);
+        double w = wts[p];
+        A += w * dA;
+        if (!(dA > 0.0) || !isfinite(dA)) atomicExch(a.bad, 1);
+        double v[8] = {x[0], x[1], x[2], n[0], n[1], n[2], w, dA};
+        for (int c = 0; c < 8; c++) {
+            a.P64[(int64_t)(p * 8 + c) * a.NS + e] = v[c];
+            a.Q64[((int64_t)e * a.maxpts + p) * 8 + c] = v[c];
+        }
+        cx += x[0]; cy += x[1]; cz += x[2];
+    }
+    a.area[e] = A;
+    a.elem_np[e] = np;
+    double ax = cx / np, ay = cy / np, az = cz / np;
+    a.anchor[e * 4 + 0] = ax; a.anchor[e * 4 + 1] = ay; a.anchor[e * 4 + 2] = az; a.anchor[e * 4 + 3] = 0.0;
+    for (int p = 0; p < np; p++) {
+        const double* P = a.P64;
+        int64_t NS = a.NS;
+        float v[8];
+        v[0] = (float)(P[(int64_t)(p * 8 + 0) * NS + e] - ax);
+        v[1] = (float)(P[(int64_t)(p * 8 + 1) * NS + e] - ay);
+        v[2] = (float)(P[(int64_t)(p * 8 + 2) * NS + e] - az);
+        v[3] = (float)P[(int64_t)(p * 8 + 3) * NS + e];
+        v[4] = (float)P[(int64_t)(p * 8 + 4) * NS + e];
+        v[5] = (float)P[(int64_t)(p * 8 + 5) * NS + e];
+        v[6] = (float)(P[(int64_t)(p * 8 + 6) * NS + e] * P[(int64_t)(p * 8 + 7) * NS + e]);
+        v[7] = 0.0f;
+        for (int c = 0; c < 8; c++) {
+            a.P32[(int64_t)(p * 8 + c) * NS + e] = v[c];
+            a.Q32[((int64_t)e * a.maxpts + p) * 8 + c] = v[c];
+        }
+    }
+}
+
+// ---------------------------------------------------------------------------
+// FP64 reference kernel and visibility (exact reference arithmetic)
+// ---------------------------------------------------------------------------
+struct Ray64 { double ox, oy, oz, dx, dy, dz, L; };
+
+// vf_kernel (ViewFactorKernel.jl:155-166).  Also returns the normalised
+// direction and |r|, which equal is_visible's d/L and L (RayCast.jl:27-30).
+__device__ __forceinline__ double vf_exact(double xi, double yi, double zi, double nxi, double nyi, double nzi,
+                                           double xj, double yj, double zj, double nxj, double nyj, double nzj,
+                                           Ray64& ray) {
+    double r0 = xj - xi, r1 = yj - yi, r2 = zj - zi;
+    double rr = (r0 * r0 + r1 * r1) + r2 * r2;
+    double r = sqrt(rr);
+    double h0 = r0 / r, h1 = r1 / r, h2 = r2 / r;
+    double ci = (nxi * h0 + nyi * h1) + nzi * h2;
+    double cj = (nxj * (-h0) + nyj * (-h1)) + nzj * (-h2);
+    ray.ox = xi; ray.oy = yi; ray.oz = zi; ray.dx = h0; ray.dy = h1; ray.dz = h2; ray.L = r;
+    double K = (ci * cj) / (PI_D * rr);
+    return (rr < 1e-30 || ci <= 0.0 || cj <= 0.0) ? 0.0 : K;
+}
+
+// _ray_triangle with t_min = 0 and the caller's t < t_max (BVH.jl:181-201,241-242)
+__device__ __forceinline__ bool mt_hit64(const Tri64& T, const Ray64& R) {
+    double e1x = T.e1[0], e1y = T.e1[1], e1z = T.e1[2];
+    double e2x = T.e2[0], e2y = T.e2[1], e2z = T.e2[2];
+    double hx = R.dy * e2z - R.dz * e2y;
+    double hy = R.dz * e2x - R.dx * e2z;
+    double hz = R.dx * e2y - R.dy * e2x;
+    double av = (e1x * hx + e1y * hy) + e1z * hz;
+    if (fabs(av) < 1e-10) return false;
+    double f = 1.0 / av;
+    double sx = R.ox - T.v0[0], sy = R.oy - T.v0[1], sz = R.oz - T.v0[2];
+    double u = f * ((sx * hx + sy * hy) + sz * hz);
+    if (u < 0.0 || u > 1.0) return false;
+    double qx = sy * e1z - sz * e1y, qy = sz * e1x - sx * e1z, qz = sx * e1y - sy * e1x;
+    double v = f * ((R.dx * qx + R.dy * qy) + R.dz * qz);
+    if (v < 0.0 || u + v > 1.0) return false;
+    double t = f * ((e2x * qx + e2y * qy) + e2z * qz);
+    return t > 0.0 && t < R.L;
+}
+
+// intersect_ray_bvh (BVH.jl:213-250) in stackless order; the any-hit result
+// does not depend on visiting order.  _ray_aabb (BVH.jl:155-169) keeps Julia's
+// NaN-propagating min/max semantics: any NaN slab value makes the box a miss.
+__device__ bool blocked64(const BvhNode64* __restrict__ nodes, const Tri64* __restrict__ tris, int root,
+                          const Ray64& R, unsigned long long& naabb, unsigned long long& ntri) {
+    double ix = 1.0 / R.dx, iy = 1.0 / R.dy, iz = 1.0 / R.dz;
+    int n = root;
+    while (n >= 0) {
+        const BvhNode64 nd = nodes[n];
+        naabb++;
+        double t1x = (nd.lo[0] - R.ox) * ix, t2x = (nd.hi[0] - R.ox) * ix;
+        double t1y = (nd.lo[1] - R.oy) * iy, t2y = (nd.hi[1] - R.oy) * iy;
+        double t1z = (nd.lo[2] - R.oz) * iz, t2z = (nd.hi[2] - R.oz) * iz;
+        bool nan = (t1x != t1x) | (t2x != t2x) | (t1y != t1y) | (t2y != t2y) | (t1z != t1z) | (t2z != t2z);
+        double tmin = fmax(fmax(fmax(fmin(t1x, t2x), fmin(t1y, t2y)), fmin(t1z, t2z)), 0.0);
+        double tmax = fmin(fmin(fmin(fmax(t1x, t2x), fmax(t1y, t2y)), fmax(t1z, t2z)), R.L);
+        if (!nan && tmin <= tmax) {
+            if (nd.count > 0) {
+                for (int k = nd.start; k < nd.start + nd.count; k++) {
+                    ntri++;
+                    if (mt_hit64(tris[k], R)) return true;
+                }
+                n = nd.miss;
+            } else {
+                n = n + 1;
+            }
+        } else {
+            n = nd.miss;
+        }
+    }
+    return false;
+}
+
+// ---------------------------------------------------------------------------
+// Pair tile prologue shared by all pair kernels
+// ---------------------------------------------------------------------------
+struct TileCtx {
+    int s, chunk, lane, jb, je;
+    int64_t i;
+    bool vi;
+};
+
+__device__ __forceinline__ TileCtx tile_ctx(const PairArgs& a) {
+    TileCtx t;
+    int2 tl = a.tiles[blockIdx.x];
+    t.s = tl.x; t.chunk = tl.y;
+    t.lane = threadIdx.x;
+    t.i = (int64_t)t.s * RVF_STRIP + t.lane;
+    t.vi = t.i < a.N;
+    Chunk c = a.chunks[t.chunk];
+    t.jb = c.begin; t.je = c.end;
+    return t;
+}
+
+// Per-j epilogue: raw output, row partial (F_ij = raw/A_i) and the
+// transposed partial for row j (sum over lanes of raw/A_j).
+__device__ __forceinline__ void pair_out(const PairArgs& a, const TileCtx& t, int64_t j, bool act, double raw,
+                                         double Ai, double& ls, double& lc) {
+    if (act) {
+        if (a.F) a.F[t.i + a.N * j] = raw;
+        neumaier(ls, lc, raw / Ai);
+    }
+    double Aj = a.area[j];
+    double v = act ? raw / Aj : 0.0;
+    v = wave_sum_d(v);
+    if (t.lane == 0) a.partT[(int64_t)t.s * a.NS + j] = v;
+}
+
+// ---------------------------------------------------------------------------
+// FP64, unoccluded: i-points in registers, PC per chunk of the p loop.
+// ---------------------------------------------------------------------------
+template <int PC>
+__global__ __launch_bounds__(64) void k_pair_fp64(PairArgs a) {
+    const TileCtx t = tile_ctx(a);
+    const int64_t NS = a.NS;
+    const int npi = t.vi ? a.elem_np[t.i] : 0;
+    int npmax = npi;
+#pragma unroll
+    for (int o = 32; o > 0; o >>= 1) npmax = max(npmax, __shfl_xor(npmax, o, 64));
+    npmax = __builtin_amdgcn_readfirstlane(npmax);
+    const double Ai = t.vi ? a.area[t.i] : 1.0;
+    const double* __restrict__ P = a.P64;
+    double ls = 0.0, lc = 0.0;
+    for (int64_t j = t.jb; j < t.je; j++) {
+        if (j <= (int64_t)t.s * RVF_STRIP) {  // no lane has i < j
+            if (t.lane == 0) a.partT[(int64_t)t.s * a.NS + j] = 0.0;
+            continue;
+        }
+        const bool act = t.vi && t.i < j;
+        const int npj = a.elem_np[j];
+        const double* __restrict__ qj = a.Q64 + j * a.maxpts * 8;
+        double Fij = 0.0;
+        for (int c0 = 0; c0 < npmax; c0 += PC) {
+            double X[PC], Y[PC], Z[PC], NX[PC], NY[PC], NZ[PC], inner[PC];
+#pragma unroll
+            for (int p = 0; p < PC; p++) {
+                int pp = c0 + p;
+                inner[p] = 0.0;
+                if (pp < npmax) {
+                    X[p] = P[(int64_t)(pp * 8 + 0) * NS + t.i];
+                    Y[p] = P[(int64_t)(pp * 8 + 1) * NS + t.i];
+                    Z[p] = P[(int64_t)(pp * 8 + 2) * NS + t.i];
+                    NX[p] = P[(int64_t)(pp * 8 + 3) * NS + t.i];
+                    NY[p] = P[(int64_t)(pp * 8 + 4) * NS + t.i];
+                    NZ[p] = P[(int64_t)(pp * 8 + 5) * NS + t.i];
+                } else {
+                    X[p] = Y[p] = Z[p] = NX[p] = NY[p] = NZ[p] = 0.0;
+                }
+            }
+            for (int q = 0; q < npj; q++) {
+                const double xj = qj[q * 8 + 0], yj = qj[q * 8 + 1], zj = qj[q * 8 + 2];
+                const double nxj = qj[q * 8 + 3], nyj = qj[q * 8 + 4], nzj = qj[q * 8 + 5];
+                const double wj = qj[q * 8 + 6], dAj = qj[q * 8 + 7];
+#pragma unroll
+                for (int p = 0; p < PC; p++) {
+                    if (c0 + p < npmax) {
+                        Ray64 R;
+                        double K = vf_exact(X[p], Y[p], Z[p], NX[p], NY[p], NZ[p], xj, yj, zj, nxj, nyj, nzj, R);
+                        inner[p] += (wj * K) * dAj;
+                    }
+                }
+            }
+#pragma unroll
+            for (int p = 0; p < PC; p++) {
+                int pp = c0 + p;
+                if (pp < npmax && pp < npi) {
+                    double wi = P[(int64_t)(pp * 8 + 6) * NS + t.i], dAi = P[(int64_t)(pp * 8 + 7) * NS + t.i];
+                    Fij += (wi * inner[p]) * dAi;
+                }
+            }
+        }
+        pair_out(a, t, j, act, Fij, Ai, ls, lc);
+    }
+    a.part[(int64_t)t.chunk * NS + t.i] = ls + lc;
+}
+
+// ---------------------------------------------------------------------------
+// FP64, occluded: the reference's p-outer / q-inner loop, one ray per K != 0.
+// ---------------------------------------------------------------------------
+__global__ __launch_bounds__(64) void k_pair_fp64_occ(PairArgs a) {
+    const TileCtx t = tile_ctx(a);
+    const int64_t NS = a.NS;
+    const int npi = t.vi ? a.elem_np[t.i] : 0;
+    int npmax = npi;
+#pragma unroll
+    for (int o = 32; o > 0; o >>= 1) npmax = max(npmax, __shfl_xor(npmax, o, 64));
+    npmax = __builtin_amdgcn_readfirstlane(npmax);
+    const double Ai = t.vi ? a.area[t.i] : 1.0;
+    const int gi = t.vi ? a.elem_grp[t.i] : 0;
+    const double* __restrict__ P = a.P64;
+    double ls = 0.0, lc = 0.0;
+    unsigned long long nrays = 0, naabb = 0, ntri = 0;
+    for (int64_t j = t.jb; j < t.je; j++) {
+        if (j <= (int64_t)t.s * RVF_STRIP) {
+            if (t.lane == 0) a.partT[(int64_t)t.s * a.NS + j] = 0.0;
+            continue;
+        }
+        const bool act = t.vi && t.i < j;
+        const int npj = a.elem_np[j];
+        const int gj = a.elem_grp[j];
+        const int root = a.tree_root[gi * a.G + gj];
+        const double* __restrict__ qj = a.Q64 + j * a.maxpts * 8;
+        double Fij = 0.0;
+        uint32_t bc = 0, bh = 0;
+        for (int p = 0; p < npmax; p++) {
+            const bool pv = act && p < npi;
+            const double xi = P[(int64_t)(p * 8 + 0) * NS + t.i], yi = P[(int64_t)(p * 8 + 1) * NS + t.i];
+            const double zi = P[(int64_t)(p * 8 + 2) * NS + t.i];
+            const double nxi = P[(int64_t)(p * 8 + 3) * NS + t.i], nyi = P[(int64_t)(p * 8 + 4) * NS + t.i];
+            const double nzi = P[(int64_t)(p * 8 + 5) * NS + t.i];
+            double inner = 0.0;
+            for (int q = 0; q < npj; q++) {
+                const double xj = qj[q * 8 + 0], yj = qj[q * 8 + 1], zj = qj[q * 8 + 2];
+                const double nxj = qj[q * 8 + 3], nyj = qj[q * 8 + 4], nzj = qj[q * 8 + 5];
+                const double wj = qj[q * 8 + 6], dAj = qj[q * 8 + 7];
+                Ray64 R;
+                double K = vf_exact(xi, yi, zi, nxi, nyi, nzi, xj, yj, zj, nxj, nyj, nzj, R);
+                if (pv && K != 0.0 && root >= 0) {
+                    nrays++;
+                    if (blocked64(a.nodes64, a.tris64, root, R, naabb, ntri)) {
+                        K = 0.0;
+                        bc++;
+                        bh += mix32((uint32_t)(p * npj + q) + 0x9e3779b9U);
+                    }
+                }
+                inner += (wj * K) * dAj;
+            }
+            if (p < npi) {
+                double wi = P[(int64_t)(p * 8 + 6) * NS + t.i], dAi = P[(int64_t)(p * 8 + 7) * NS + t.i];
+                Fij += (wi * inner) * dAi;
+            }
+        }
+        if (act && a.audit_count) {
+            a.audit_count[t.i + a.N * j] = bc;
+            a.audit_hash[t.i + a.N * j] = bh;
+        }
+        pair_out(a, t, j, act, Fij, Ai, ls, lc);
+    }
+    a.part[(int64_t)t.chunk * NS + t.i] = ls + lc;
+    if (a.counters) {
+        unsigned long long c1 = nrays, c2 = naabb, c3 = ntri;
+        for (int o = 32; o > 0; o >>= 1) {
+            c1 += __shfl_xor(c1, o, 64); c2 += __shfl_xor(c2, o, 64); c3 += __shfl_xor(c3, o, 64);
+        }
+        if (t.lane == 0) { atomicAdd(&a.counters[1], c1); atomicAdd(&a.counters[2], c2); atomicAdd(&a.counters[3], c3); }
+    }
+}
+
+// ---------------------------------------------------------------------------
+// Mixed precision kernel evaluation.
+//   r = (a_j - a_i) + (d_j - d_i) with FP64 anchors a and FP32 offsets d;
+//   K' = max(n_i.r, 0) max(-n_j.r, 0) / r^4  (1/pi applied once per pair)
+// ---------------------------------------------------------------------------
+__device__ __forceinline__ float vf_mixed(float bx, float by, float bz, float nxi, float nyi, float nzi,
+                                          float qx, float qy, float qz, float nxj, float nyj, float nzj,
+                                          float& rx, float& ry, float& rz, float& r2) {
+#pragma clang fp contract(fast)
+    rx = bx + qx; ry = by + qy; rz = bz + qz;
+    r2 = rx * rx + ry * ry + rz * rz;
+    float di = nxi * rx + nyi * ry + nzi * rz;
+    float dj = nxj * rx + nyj * ry + nzj * rz;
+    float rr = __builtin_amdgcn_rcpf(r2);
+    float k = (fmaxf(di, 0.0f) * rr) * (fmaxf(-dj, 0.0f) * rr);
+    return (r2 >= 1e-30f) ? k : 0.0f;
+}
+
+template <int PC>
+__global__ __launch_bounds__(64) void k_pair_mixed(PairArgs a) {
+    const TileCtx t = tile_ctx(a);
+    const int64_t NS = a.NS;
+    const int npi = t.vi ? a.elem_np[t.i] : 0;
+    int npmax = npi;
+#pragma unroll
+    for (int o = 32; o > 0; o >>= 1) npmax = max(npmax, __shfl_xor(npmax, o, 64));
+    npmax = __builtin_amdgcn_readfirstlane(npmax);
+    const double Ai = t.vi ? a.area[t.i] : 1.0;
+    const float* __restrict__ P = a.P32;
+    const double* __restrict__ P64 = a.P64;
+    const double aix = a.anchor[t.i * 4 + 0], aiy = a.anchor[t.i * 4 + 1], aiz = a.anchor[t.i * 4 + 2];
+    double ls = 0.0, lc = 0.0;
+    for (int64_t j = t.jb; j < t.je; j++) {
+        if (j <= (int64_t)t.s * RVF_STRIP) {
+            if (t.lane == 0) a.partT[(int64_t)t.s * a.NS + j] = 0.0;
+            continue;
+        }
+        const bool act = t.vi && t.i < j;
+        const int npj = a.elem_np[j];
+        const float* __restrict__ qj = a.Q32 + j * a.maxpts * 8;
+        const float dax = (float)(a.anchor[j * 4 + 0] - aix);
+        const float day = (float)(a.anchor[j * 4 + 1] - aiy);
+        const float daz = (float)(a.anchor[j * 4 + 2] - aiz);
+        double Fs = 0.0, Fc = 0.0;
+        for (int c0 = 0; c0 < npmax; c0 += PC) {
+            float BX[PC], BY[PC], BZ[PC], NX[PC], NY[PC], NZ[PC], inner[PC];
+#pragma unroll
+            for (int p = 0; p < PC; p++) {
+                int pp = c0 + p;
+                inner[p] = 0.0f;
+                if (pp < npmax) {
+                    BX[p] = dax - P[(int64_t)(pp * 8 + 0) * NS + t.i];
+                    BY[p] = day - P[(int64_t)(pp * 8 + 1) * NS + t.i];
+                    BZ[p] = daz - P[(int64_t)(pp * 8 + 2) * NS + t.i];
+                    NX[p] = P[(int64_t)(pp * 8 + 3) * NS + t.i];
+                    NY[p] = P[(int64_t)(pp * 8 + 4) * NS + t.i];
+                    NZ[p] = P[(int64_t)(pp * 8 + 5) * NS + t.i];
+                } else {
+                    BX[p] = BY[p] = BZ[p] = NX[p] = NY[p] = NZ[p] = 0.0f;
+                }
+            }
+            for (int q = 0; q < npj; q++) {
+                const float qx = qj[q * 8 + 0], qy = qj[q * 8 + 1], qz = qj[q * 8 + 2];
+                const float nxj = qj[q * 8 + 3], nyj = qj[q * 8 + 4], nzj = qj[q * 8 + 5];
+                const float wdj = qj[q * 8 + 6];
+#pragma unroll
+                for (int p = 0; p < PC; p++) {
+                    if (c0 + p < npmax) {
+                        float rx, ry, rz, r2;
+                        float k = vf_mixed(BX[p], BY[p], BZ[p], NX[p], NY[p], NZ[p], qx, qy, qz, nxj, nyj, nzj,
+                                           rx, ry, rz, r2);
+                        inner[p] = fmaf(wdj, k, inner[p]);
+                    }
+                }
+            }
+#pragma unroll
+            for (int p = 0; p < PC; p++) {
+                int pp = c0 + p;
+                if (pp < npmax && pp < npi) {
+                    double wd = P64[(int64_t)(pp * 8 + 6) * NS + t.i] * P64[(int64_t)(pp * 8 + 7) * NS + t.i];
+                    neumaier(Fs, Fc, wd * (double)inner[p]);
+                }
+            }
+        }
+        pair_out(a, t, j, act, (Fs + Fc) * (1.0 / PI_D), Ai, ls, lc);
+    }
+    a.part[(int64_t)t.chunk * NS + t.i] = ls + lc;
+}
+
+// ---------------------------------------------------------------------------
+// Mixed precision occlusion: FP32 traversal of conservatively padded boxes,
+// FP32 Moller-Trumbore filter with rigorous error bounds; triangles the
+// filter cannot reject are decided by mt_hit64 on the exact FP64 ray.
+// ---------------------------------------------------------------------------
+struct Ray32 { float ox, oy, oz, dx, dy, dz, L; };
+
+// Returns 0 = definite miss, 1 = undecided (FP64 decides).  Error bounds
+// (u = 2^-24, safety factor 2): |d origin| <= 3u Sm, |d v0| <= u Sm,
+// |d e| <= u |e|, |d dir| <= dd (per ray, from the anchored difference), Sm =
+// scene coordinate magnitude, E1/E2 = |e1|_1/|e2|_1, sa = E1 E2.
+__device__ __forceinline__ int mt_filter32(const Tri32& T, const Ray32& R, float Sm, float dd) {
+#pragma clang fp contract(fast)
+    const float u = 5.9604645e-08f;
+    float e1x = T.e1[0], e1y = T.e1[1], e1z = T.e1[2];
+    float e2x = T.e2[0], e2y = T.e2[1], e2z = T.e2[2];
+    float hx = R.dy * e2z - R.dz * e2y;
+    float hy = R.dz * e2x - R.dx * e2z;
+    float hz = R.dx * e2y - R.dy * e2x;
+    float av = e1x * hx + e1y * hy + e1z * hz;
+    float E1 = fabsf(e1x) + fabsf(e1y) + fabsf(e1z);
+    float E2 = fabsf(e2x) + fabsf(e2y) + fabsf(e2z);
+    float KA = 2.0f * T.sa * (dd + 9.0f * u);
+    if (fabsf(av) <= KA + 1e-10f) return 1;  // grazing or near the 1e-10 cut
+    float f = __builtin_amdgcn_rcpf(av);
+    float af = fabsf(f) * 1.0000005f;
+    float sx = R.ox - T.v0[0], sy = R.oy - T.v0[1], sz = R.oz - T.v0[2];
+    float ub = f * (sx * hx + sy * hy + sz * hz);
+    float eu = (2.0f * E2 * Sm * (64.0f * u + 6.0f * dd) + 2.0f * KA) * af + 12.0f * u;
+    if (ub < -eu || ub > 1.0f + eu) return 0;
+    float qx = sy * e1z - sz * e1y, qy = sz * e1x - sx * e1z, qz = sx * e1y - sy * e1x;
+    float vb = f * (R.dx * qx + R.dy * qy + R.dz * qz);
+    float ev = (2.0f * E1 * Sm * (12.0f * dd + 68.0f * u) + 2.0f * KA) * af + 12.0f * u;
+    if (vb < -ev || ub + vb > 1.0f + eu + ev) return 0;
+    float tb = f * (e2x * qx + e2y * qy + e2z * qz);
+    float et = (132.0f * u * T.sa * Sm + fabsf(tb) * KA) * af + 6.0f * u * fabsf(tb);
+    if (tb < -et || tb > R.L * (1.0f + 2.0f * (dd + 3.0f * u)) + et) return 0;
+    return 1;
+}
+
+struct Exact64Ctx {
+    // lazily built exact FP64 ray for (i, p) -> (j, q)
+    const double* Pi;  // P64 base
+    int64_t NS, i;
+    int p;
+    const double* qjq; // Q64 row of (j, q)
+};
+
+__device__ __forceinline__ void exact_ray(const Exact64Ctx& c, Ray64& R) {
+    double xi = c.Pi[(int64_t)(c.p * 8 + 0) * c.NS + c.i];
+    double yi = c.Pi[(int64_t)(c.p * 8 + 1) * c.NS + c.i];
+    double zi = c.Pi[(int64_t)(c.p * 8 + 2) * c.NS + c.i];
+    double d0 = c.qjq[0] - xi, d1 = c.qjq[1] - yi, d2 = c.qjq[2] - zi;
+    double L = sqrt((d0 * d0 + d1 * d1) + d2 * d2);
+    R.ox = xi; R.oy = yi; R.oz = zi;
+    R.dx = d0 / L; R.dy = d1 / L; R.dz = d2 / L; R.L = L;
+}
+
+__device__ bool blocked_mixed(const BvhNode32* __restrict__ nodes, const Tri32* __restrict__ tris32,
+                              const Tri64* __restrict__ tris64, int root, const Ray32& R, float Sm, float dd,
+                              const Exact64Ctx& ec, unsigned long long& nretest) {
+    float ix = __builtin_amdgcn_rcpf(R.dx), iy = __builtin_amdgcn_rcpf(R.dy), iz = __builtin_amdgcn_rcpf(R.dz);
+    bool have64 = false;
+    Ray64 R64;
+    int n = root;
+    while (n >= 0) {
+        const BvhNode32 nd = nodes[n];
+        float t1x = (nd.lo[0] - R.ox) * ix, t2x = (nd.hi[0] - R.ox) * ix;
+        float t1y = (nd.lo[1] - R.oy) * iy, t2y = (nd.hi[1] - R.oy) * iy;
+        float t1z = (nd.lo[2] - R.oz) * iz, t2z = (nd.hi[2] - R.oz) * iz;
+        float tmin = fmaxf(fmaxf(fmaxf(fminf(t1x, t2x), fminf(t1y, t2y)), fminf(t1z, t2z)), 0.0f);
+        float tmax = fminf(fminf(fminf(fmaxf(t1x, t2x), fmaxf(t1y, t2y)), fmaxf(t1z, t2z)), R.L * 1.0001f);
+        if (tmin <= tmax) {
+            int cnt = nd.start_count & 15;
+            if (cnt > 0) {
+                int st = nd.start_count >> 4;
+                for (int k = st; k < st + cnt; k++) {
+                    if (mt_filter32(tris32[k], R, Sm, dd)) {
+                        if (!have64) { exact_ray(ec, R64); have64 = true; }
+                        nretest++;
+                        if (mt_hit64(tris64[k], R64)) return true;
+                    }
+                }
+                n = nd.miss;
+            } else {
+                n = n + 1;
+            }
+        } else {
+            n = nd.miss;
+        }
+    }
+    return false;
+}
+
+__global__ __launch_bounds__(64) void k_pair_mixed_occ(PairArgs a, float Sm) {
+    const TileCtx t = tile_ctx(a);
+    const int64_t NS = a.NS;
+    const int npi = t.vi ? a.elem_np[t.i] : 0;
+    int npmax = npi;
+#pragma unroll
+    for (int o = 32; o > 0; o >>= 1) npmax = max(npmax, __shfl_xor(npmax, o, 64));
+    npmax = __builtin_amdgcn_readfirstlane(npmax);
+    const double Ai = t.vi ? a.area[t.i] : 1.0;
+    const int gi = t.vi ? a.elem_grp[t.i] : 0;
+    const float* __restrict__ P = a.P32;
+    const double* __restrict__ P64 = a.P64;
+    const double aix = a.anchor[t.i * 4 + 0], aiy = a.anchor[t.i * 4 + 1], aiz = a.anchor[t.i * 4 + 2];
+    const float fax = (float)aix, fay = (float)aiy, faz = (float)aiz;
+    double ls = 0.0, lc = 0.0;
+    unsigned long long nret = 0, nrays = 0;
+    for (int64_t j = t.jb; j < t.je; j++) {
+        if (j <= (int64_t)t.s * RVF_STRIP) {
+            if (t.lane == 0) a.partT[(int64_t)t.s * a.NS + j] = 0.0;
+            continue;
+        }
+        const bool act = t.vi && t.i < j;
+        const int npj = a.elem_np[j];
+        const int gj = a.elem_grp[j];
+        const int root = a.tree_root[gi * a.G + gj];
+        const float* __restrict__ qj = a.Q32 + j * a.maxpts * 8;
+        const double* __restrict__ qj64 = a.Q64 + j * a.maxpts * 8;
+        const float dax = (float)(a.anchor[j * 4 + 0] - aix);
+        const float day = (float)(a.anchor[j * 4 + 1] - aiy);
+        const float daz = (float)(a.anchor[j * 4 + 2] - aiz);
+        double Fs = 0.0, Fc = 0.0;
+        uint32_t bc = 0, bh = 0;
+        for (int p = 0; p < npmax; p++) {
+            const bool pv = act && p < npi;
+            const float dxi = P[(int64_t)(p * 8 + 0) * NS + t.i], dyi = P[(int64_t)(p * 8 + 1) * NS + t.i];
+            const float dzi = P[(int64_t)(p * 8 + 2) * NS + t.i];
+            const float nxi = P[(int64_t)(p * 8 + 3) * NS + t.i], nyi = P[(int64_t)(p * 8 + 4) * NS + t.i];
+            const float nzi = P[(int64_t)(p * 8 + 5) * NS + t.i];
+            const float bx = dax - dxi, by = day - dyi, bz = daz - dzi;
+            float inner = 0.0f;
+            for (int q = 0; q < npj; q++) {
+                const float qx = qj[q * 8 + 0], qy = qj[q * 8 + 1], qz = qj[q * 8 + 2];
+                const float nxj = qj[q * 8 + 3], nyj = qj[q * 8 + 4], nzj = qj[q * 8 + 5];
+                const float wdj = qj[q * 8 + 6];
+                float rx, ry, rz, r2;
+                float k = vf_mixed(bx, by, bz, nxi, nyi, nzi, qx, qy, qz, nxj, nyj, nzj, rx, ry, rz, r2);
+                if (pv && k != 0.0f && root >= 0) {
+                    nrays++;
+                    float rs = __builtin_amdgcn_rsqf(r2);
+                    Ray32 R;
+                    R.ox = fax + dxi; R.oy = fay + dyi; R.oz = faz + dzi;
+                    R.dx = rx * rs; R.dy = ry * rs; R.dz = rz * rs; R.L = r2 * rs;
+                    const float uu = 5.9604645e-08f;
+                    float dd = 3.0f * uu * (fabsf(dax) + fabsf(day) + fabsf(daz) + fabsf(bx) + fabsf(by) + fabsf(bz) +
+                                            fabsf(rx) + fabsf(ry) + fabsf(rz)) * rs + 4.0f * uu;
+                    Exact64Ctx ec{P64, NS, t.i, p, qj64 + q * 8};
+                    if (blocked_mixed(a.nodes32, a.tris32, a.tris64, root, R, Sm, dd, ec, nret)) {
+                        k = 0.0f;
+                        bc++;
+                        bh += mix32((uint32_t)(p * npj + q) + 0x9e3779b9U);
+                    }
+                }
+                inner = fmaf(wdj, k, inner);
+            }
+            if (p < npi) {
+                double wd = P64[(int64_t)(p * 8 + 6) * NS + t.i] * P64[(int64_t)(p * 8 + 7) * NS + t.i];
+                neumaier(Fs, Fc, wd * (double)inner);
+            }
+        }
+        if (act && a.audit_count) {
+            a.audit_count[t.i + a.N * j] = bc;
+            a.audit_hash[t.i + a.N * j] = bh;
+        }
+        pair_out(a, t, j, act, (Fs + Fc) * (1.0 / PI_D), Ai, ls, lc);
+    }
+    a.part[(int64_t)t.chunk * NS + t.i] = ls + lc;
+    if (a.counters) {
+        unsigned long long r0 = nret, r1 = nrays;
+        for (int o = 32; o > 0; o >>= 1) { r0 += __shfl_xor(r0, o, 64); r1 += __shfl_xor(r1, o, 64); }
+        if (t.lane == 0) { atomicAdd(&a.counters[0], r0); atomicAdd(&a.counters[1], r1); }
+    }
+}
+// ---------------------------------------------------------------------------
+// Fill: F[i,j] = raw/A_i and F[j,i] = raw/A_j for i < j (Assembly.jl:211),
+// F[i,i] = 0.  One 64x64 tile per workgroup, transposed half through LDS.
+// ---------------------------------------------------------------------------
+__global__ __launch_bounds__(256) void k_fill(double* __restrict__ F, const double* __restrict__ area, int64_t N,
+                                              const int2* __restrict__ tiles) {
+    __shared__ double T[64][65];
+    int2 tl = tiles[blockIdx.x];
+    int64_t i0 = (int64_t)tl.x * 64, j0 = (int64_t)tl.y * 64;
+    int tx = threadIdx.x & 63, ty = threadIdx.x >> 6;  // 64 x 4
+    int64_t i = i0 + tx;
+    double Ai = i < N ? area[i] : 1.0;
+    for (int r = ty; r < 64; r += 4) {
+        int64_t j = j0 + r;
+        double out = 0.0;
+        if (i < N && j < N) {
+            if (i < j) {
+                double raw = F[i + N * j];
+                F[i + N * j] = raw / Ai;
+                out = raw / area[j];
+            } else if (i == j) {
+                F[i + N * j] = 0.0;
+            }
+        }
+        T[tx][r] = out;  // value for F[j, i]
+    }
+    __syncthreads();
+    // write F[j, i] for i < j: column i = i0 + c, rows j0 + lane
+    int64_t j = j0 + tx;
+    for (int c = ty; c < 64; c += 4) {
+        int64_t ii = i0 + c;
+        if (ii < N && j < N && ii < j) F[j + N * ii] = T[c][tx];
+    }
+}
+
+// ---------------------------------------------------------------------------
+// Row reduction: rowsum[r] and W[h][r] = sum_{j in group h, j > r} F_rj.
+//   upper part: part[c][r] over chunks c with chunk_last(c) > 64*strip(r)
+//   lower part: partT[s][r] over strips s <= (chunk_last(chunk(r)) - 1) / 64
+// Compensated (Neumaier) sums in fixed order.
+// ---------------------------------------------------------------------------
+__global__ __launch_bounds__(256) void k_reduce_rows(const double* __restrict__ part,
+                                                     const double* __restrict__ partT, const Chunk* __restrict__ chunks,
+                                                     int nchunks, const int32_t* __restrict__ chunk_of,
+                                                     int64_t N, int NS, int G, double* __restrict__ rowsum,
+                                                     double* __restrict__ W) {
+    int64_t r = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
+    if (r >= N) return;
+    int64_t sr = r / RVF_STRIP;
+    // first chunk with last index > 64*sr: chunks are sorted; binary search
+    int lo = 0, hi = nchunks;
+    while (lo < hi) {
+        int mid = (lo + hi) >> 1;
+        if (chunks[mid].end - 1 > sr * RVF_STRIP) hi = mid; else lo = mid + 1;
+    }
+    double ts = 0.0, tc = 0.0;
+    int curg = -1;
+    double gs = 0.0, gc = 0.0;
+    for (int c = lo; c < nchunks; c++) {
+        int g = chunks[c].grp;
+        if (g != curg) {
+            if (curg >= 0) W[(int64_t)curg * NS + r] += gs + gc;
+            curg = g; gs = 0.0; gc = 0.0;
+        }
+        double v = part[(int64_t)c * NS + r];
+        neumaier(gs, gc, v);
+        neumaier(ts, tc, v);
+    }
+    if (curg >= 0) W[(int64_t)curg * NS + r] += gs + gc;
+    int cr = chunk_of[r];
+    // tile (s, cr) exists iff 64 s < last(cr), i.e. s <= (end - 2) / 64
+    int64_t smax = chunks[cr].end >= 2 ? (int64_t)(chunks[cr].end - 2) / RVF_STRIP : -1;
+    double ls = 0.0, lcmp = 0.0;
+    for (int64_t s = 0; s <= smax; s++) neumaier(ls, lcmp, partT[s * NS + r]);
+    neumaier(ts, tc, ls + lcmp);
+    rowsum[r] = ts + tc;
+}
+
+// ---------------------------------------------------------------------------
+// Launch wrappers (host side of this translation unit)
+// ---------------------------------------------------------------------------
+extern "C" {
+int rvf_launch_precompute(const PrecompArgs* a, hipStream_t st) {
+    int64_t nb = (a->N + 63) / 64;
+    if (nb == 0) return 0;
+    hipLaunchKernelGGL(k_precompute, dim3((unsigned)nb), dim3(64), 0, st, *a);
+    return (int)hipGetLastError();
+}
+
+int rvf_launch_pair(const PairArgs* a, int mixed, int occ, int pc, float Sm, hipStream_t st) {
+    if (a->ntiles == 0) return 0;
+    dim3 g((unsigned)a->ntiles), b(64);
+    if (!mixed) {
+        if (occ) hipLaunchKernelGGL(k_pair_fp64_occ, g, b, 0, st, *a);
+        else if (pc <= 4) hipLaunchKernelGGL(k_pair_fp64<4>, g, b, 0, st, *a);
+        else if (pc <= 9) hipLaunchKernelGGL(k_pair_fp64<9>, g, b, 0, st, *a);
+        else hipLaunchKernelGGL(k_pair_fp64<16>, g, b, 0, st, *a);
+    } else {
+        if (occ) hipLaunchKernelGGL(k_pair_mixed_occ, g, b, 0, st, *a, Sm);
+        else if (pc <= 4) hipLaunchKernelGGL(k_pair_mixed<4>, g, b, 0, st, *a);
+        else if (pc <= 9) hipLaunchKernelGGL(k_pair_mixed<9>, g, b, 0, st, *a);
+        else hipLaunchKernelGGL(k_pair_mixed<16>, g, b, 0, st, *a);
+    }
+    return (int)hipGetLastError();
+}
+
+int rvf_launch_fill(double* F, const double* area, int64_t N, const int2* tiles, int ntiles, hipStream_t st) {
+    if (ntiles == 0) return 0;
+    hipLaunchKernelGGL(k_fill, dim3((unsigned)ntiles), dim3(256), 0, st, F, area, N, tiles);
+    return (int)hipGetLastError();
+}
+
+int rvf_launch_reduce(const double* part, const double* partT, const Chunk* chunks, int nchunks,
+                      const int32_t* chunk_of, int64_t N, int NS, int G, double* rowsum, double* W,
+                      hipStream_t st) {
+    if (N == 0) return 0;
+    hipLaunchKernelGGL(k_reduce_rows, dim3((unsigned)((N + 255) / 256)), dim3(256), 0, st, part, partT, chunks,
+                       nchunks, chunk_of, N, NS, G, rowsum, W);
+    return (int)hipGetLastError();
+}
+}

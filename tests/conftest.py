@@ -26,3 +26,13 @@ def orc():
     import oracle
     oracle.lib()
     return oracle
+
+
+def pytest_collection_modifyitems(config, items):
+    # GPU tests need an MI355X; without /dev/kfd they are skipped, not failed
+    if os.path.exists("/dev/kfd"):
+        return
+    skip = pytest.mark.skip(reason="no GPU in this container")
+    for it in items:
+        if "gpu" in it.keywords:
+            it.add_marker(skip)
