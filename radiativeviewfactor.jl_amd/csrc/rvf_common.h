@@ -84,6 +84,13 @@ struct PairArgs {
     // audit (nullable): per pair blocked-ray count / order-free hash at [i + N*j]
     uint32_t* audit_count;
     uint32_t* audit_hash;
+    // element topology (Duffy singular-pair skip, Monte Carlo sampling)
+    const double* coords;     // 3 x Nn
+    const int8_t* fam;        // N
+    const int32_t* nodes8;    // 8 x N, 1-based
+    int duffy;                // skip Quad8 pairs sharing a corner (done by k_duffy)
+    int mc_nsamp, mc_s;       // Monte Carlo samples per pair, strata per side
+    uint64_t mc_seed;
     // [0] FP64 re-tests (mixed), [1] rays cast, [2] AABB tests, [3] triangle
     // tests (FP64: canonical stackless order, GPUBVH.jl:202-285)
     unsigned long long* counters;
@@ -101,4 +108,12 @@ struct PrecompArgs {
     double* P64; double* Q64; float* P32; float* Q32; double* anchor; double* area;
     int32_t* elem_np;
     int* bad;                  // set to 1 if any dA is 0 or not finite
+};
+
+// One singular Quad8 pair for the Duffy / Sauter-Schwab kernel.
+struct DuffyPair {
+    int32_t i, j;        // i < j
+    int32_t kind;        // 1 = common vertex, 2 = common edge
+    int32_t ci0, ci1, cj0, cj1;  // shared local corners (1-based), first two in scan order
+    int32_t root;        // obstruction tree or -1
 };

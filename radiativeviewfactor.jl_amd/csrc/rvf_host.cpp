@@ -18,6 +18,10 @@
 
 extern "C" int rvf_launch_precompute(const PrecompArgs* a, hipStream_t st);
 extern "C" int rvf_launch_pair(const PairArgs* a, int mixed, int occ, int pc, float Sm, hipStream_t st);
+extern "C" int rvf_launch_mc(const PairArgs* a, double* area, hipStream_t st);
+extern "C" int rvf_launch_duffy(const PairArgs* a, const DuffyPair* pairs, int npairs, const double* gl01, int nq,
+                                double* val, const int32_t* rptr, const int32_t* rlist, double* rowsum, double* W,
+                                int phase, hipStream_t st);
 extern "C" int rvf_launch_fill(double* F, const double* area, int64_t N, const int2* tiles, int ntiles,
                                hipStream_t st);
 extern "C" int rvf_launch_reduce(const double* part, const double* partT, const Chunk* chunks, int nchunks,
@@ -562,6 +566,9 @@ struct rvf_plan {
     rvf_opts opts{};
     int nq_quad = 0, nq_tri = 0, maxpts = 0;
     DBuf rule, P64, Q64, P32, Q32, anchor, area, elem_np, bad, F, part, partT, rowsum, W, audit_c, audit_h, counters;
+    // Duffy singular pairs
+    int npairs = 0;
+    DBuf d_pairs, d_gl01, d_dval, d_rptr, d_rlist;
     hipEvent_t ev[8] = {};
     rvf_stats st{};
     bool ran = false;
@@ -605,6 +612,57 @@ int rvf_plan_create(rvf_ctx* ctx, const rvf_mesh* mesh, const rvf_obs* obs, cons
     if (opts->flags & RVF_WANT_F) {
         if ((rc = p->F.alloc(sizeof(double) * N * N))) { delete p; return rc; }
     }
+    if (opts->method == RVF_METHOD_DUFFY) {
+        // singular Quad8 pairs (singularity_type, DuffyKernel.jl:95-115)
+        std::map<int32_t, std::vector<int32_t>> by_corner;
+        for (int64_t e = 0; e < N; e++)
+            if (mesh->fam[e] == 0)
+                for (int q = 0; q < 4; q++) by_corner[mesh->nodes8[8 * e + q]].push_back((int32_t)e);
+        std::vector<DuffyPair> pairs;
+        std::vector<int32_t> cand;
+        for (int64_t i = 0; i < N; i++) {
+            if (mesh->fam[i] != 0) continue;
+            cand.clear();
+            for (int q = 0; q < 4; q++)
+                for (int32_t j : by_corner[mesh->nodes8[8 * i + q]])
+                    if (j > i) cand.push_back(j);
+            std::sort(cand.begin(), cand.end());
+            cand.erase(std::unique(cand.begin(), cand.end()), cand.end());
+            for (int32_t j : cand) {
+                DuffyPair d{};
+                d.i = (int32_t)i; d.j = j;
+                int n = 0, ci[2] = {0, 0}, cj[2] = {0, 0};
+                for (int a = 0; a < 4; a++)
+                    for (int b = 0; b < 4; b++)
+                        if (mesh->nodes8[8 * i + a] == mesh->nodes8[8 * (int64_t)j + b]) {
+                            if (n < 2) { ci[n] = a + 1; cj[n] = b + 1; }
+                            n++;
+                        }
+                d.kind = n == 1 ? 1 : 2;
+                d.ci0 = ci[0]; d.ci1 = ci[1]; d.cj0 = cj[0]; d.cj1 = cj[1];
+                d.root = obs && !obs->trees.empty() ? obs->root[(size_t)mesh->grp_rank[i] * G + mesh->grp_rank[j]] : -1;
+                pairs.push_back(d);
+            }
+        }
+        p->npairs = (int)pairs.size();
+        std::vector<int32_t> rptr(N + 1, 0), rlist;
+        for (const DuffyPair& d : pairs) { rptr[d.i + 1]++; rptr[d.j + 1]++; }
+        for (int64_t r = 0; r < N; r++) rptr[r + 1] += rptr[r];
+        rlist.assign(rptr[N], 0);
+        std::vector<int32_t> fillp(rptr.begin(), rptr.end() - 1);
+        for (int k = 0; k < (int)pairs.size(); k++) { rlist[fillp[pairs[k].i]++] = k; rlist[fillp[pairs[k].j]++] = k; }
+        double p1[8], w1[8], g01[16] = {0};
+        int nq = gl1d(opts->nquad, p1, w1);
+        for (int k = 0; k < nq; k++) { g01[k] = (p1[k] + 1.0) / 2.0; g01[8 + k] = w1[k] / 2.0; }
+        if ((rc = upload(p->d_pairs, pairs.data(), sizeof(DuffyPair) * pairs.size())) ||
+            (rc = upload(p->d_gl01, g01, sizeof g01)) || (rc = p->d_dval.alloc(sizeof(double) * std::max<size_t>(pairs.size(), 1))) ||
+            (rc = upload(p->d_rptr, rptr.data(), sizeof(int32_t) * rptr.size())) ||
+            (rc = upload(p->d_rlist, rlist.data(), sizeof(int32_t) * std::max<size_t>(rlist.size(), 1)))) {
+            delete p;
+            return rc;
+        }
+        p->st.singular_pairs = p->npairs;
+    }
     if ((opts->flags & RVF_AUDIT) && obs) {
         if ((rc = p->audit_c.alloc(sizeof(uint32_t) * N * N)) || (rc = p->audit_h.alloc(sizeof(uint32_t) * N * N))) {
             delete p;
@@ -645,6 +703,11 @@ static PairArgs pair_args(rvf_plan* p) {
     }
     a.audit_count = p->audit_c.as<uint32_t>(); a.audit_hash = p->audit_h.as<uint32_t>();
     a.counters = p->counters.as<unsigned long long>();
+    a.coords = m->coords.as<double>(); a.fam = m->d_fam.as<int8_t>(); a.nodes8 = m->d_nodes8.as<int32_t>();
+    a.duffy = p->opts.method == RVF_METHOD_DUFFY ? 1 : 0;
+    a.mc_nsamp = p->opts.n_samples;
+    a.mc_s = (int)std::floor(std::sqrt((double)p->opts.n_samples));
+    a.mc_seed = p->opts.seed;
     return a;
 }
 
@@ -676,21 +739,33 @@ int rvf_plan_run(rvf_plan* p) {
         rc = rvf_launch_pair(&a, p->opts.precision == RVF_MIXED, occ, pc, p->obs ? p->obs->Sm : 1.0f, st);
         if (rc) return fail(RVF_E_HIP, std::string("pair launch: ") + hipGetErrorString((hipError_t)rc));
     } else {
-        return fail(RVF_E_ARG, "Monte Carlo method not available in this build");
+        rc = rvf_launch_mc(&a, p->area.as<double>(), st);
+        if (rc) return fail(RVF_E_HIP, std::string("mc launch: ") + hipGetErrorString((hipError_t)rc));
     }
     if (tim) HIPCHK(hipEventRecord(p->ev[2], st));
-    if (tim) HIPCHK(hipEventRecord(p->ev[3], st));  // duffy slot
-    if (p->F.p) {
-        rc = rvf_launch_fill(p->F.as<double>(), p->area.as<double>(), m->N, m->d_fill_tiles.as<int2>(),
-                             (int)m->fill_tiles.size(), st);
-        if (rc) return fail(RVF_E_HIP, std::string("fill launch: ") + hipGetErrorString((hipError_t)rc));
+    if (p->npairs) {
+        rc = rvf_launch_duffy(&a, p->d_pairs.as<DuffyPair>(), p->npairs, p->d_gl01.as<double>(), p->opts.nquad,
+                              p->d_dval.as<double>(), nullptr, nullptr, nullptr, nullptr, 0, st);
+        if (rc) return fail(RVF_E_HIP, std::string("duffy launch: ") + hipGetErrorString((hipError_t)rc));
     }
-    if (tim) HIPCHK(hipEventRecord(p->ev[4], st));
+    if (tim) HIPCHK(hipEventRecord(p->ev[3], st));
     HIPCHK(hipMemsetAsync(p->W.p, 0, p->W.n, st));
     rc = rvf_launch_reduce(p->part.as<double>(), p->partT.as<double>(), m->d_chunks.as<Chunk>(),
                            (int)m->chunks.size(), m->d_chunk_of.as<int32_t>(), m->N, m->NS, (int)m->tags.size(),
                            p->rowsum.as<double>(), p->W.as<double>(), st);
     if (rc) return fail(RVF_E_HIP, std::string("reduce launch: ") + hipGetErrorString((hipError_t)rc));
+    if (p->npairs) {
+        rc = rvf_launch_duffy(&a, p->d_pairs.as<DuffyPair>(), p->npairs, nullptr, 0, p->d_dval.as<double>(),
+                              p->d_rptr.as<int32_t>(), p->d_rlist.as<int32_t>(), p->rowsum.as<double>(),
+                              p->W.as<double>(), 1, st);
+        if (rc) return fail(RVF_E_HIP, std::string("duffy apply launch: ") + hipGetErrorString((hipError_t)rc));
+    }
+    if (tim) HIPCHK(hipEventRecord(p->ev[4], st));
+    if (p->F.p) {
+        rc = rvf_launch_fill(p->F.as<double>(), p->area.as<double>(), m->N, m->d_fill_tiles.as<int2>(),
+                             (int)m->fill_tiles.size(), st);
+        if (rc) return fail(RVF_E_HIP, std::string("fill launch: ") + hipGetErrorString((hipError_t)rc));
+    }
     if (tim) HIPCHK(hipEventRecord(p->ev[5], st));
     p->ran = true;
     return 0;
@@ -714,8 +789,8 @@ int rvf_plan_sync(rvf_plan* p) {
         HIPCHK(hipEventElapsedTime(&ms, p->ev[0], p->ev[1])); p->st.precompute_ms = ms;
         HIPCHK(hipEventElapsedTime(&ms, p->ev[1], p->ev[2])); p->st.pair_ms = ms;
         HIPCHK(hipEventElapsedTime(&ms, p->ev[2], p->ev[3])); p->st.duffy_ms = ms;
-        HIPCHK(hipEventElapsedTime(&ms, p->ev[3], p->ev[4])); p->st.fill_ms = ms;
-        HIPCHK(hipEventElapsedTime(&ms, p->ev[4], p->ev[5])); p->st.reduce_ms = ms;
+        HIPCHK(hipEventElapsedTime(&ms, p->ev[3], p->ev[4])); p->st.reduce_ms = ms;
+        HIPCHK(hipEventElapsedTime(&ms, p->ev[4], p->ev[5])); p->st.fill_ms = ms;
         HIPCHK(hipEventElapsedTime(&ms, p->ev[0], p->ev[5])); p->st.total_ms = ms;
     }
     return 0;

@@ -266,6 +266,27 @@ __device__ __forceinline__ TileCtx tile_ctx(const PairArgs& a) {
     return t;
 }
 
+// Quad8 pairs sharing a corner node are integrated by k_duffy instead
+// (singularity_type, DuffyKernel.jl:95-115).
+struct Corners { int32_t c[4]; int fam; };
+
+__device__ __forceinline__ Corners corners_of(const PairArgs& a, int64_t e, bool valid) {
+    Corners k;
+    k.fam = valid ? a.fam[e] : -1;
+    for (int q = 0; q < 4; q++) k.c[q] = (valid && k.fam == 0) ? a.nodes8[8 * e + q] : -1 - q;
+    return k;
+}
+
+__device__ __forceinline__ bool shares_corner(const Corners& x, const Corners& y) {
+    if (x.fam != 0 || y.fam != 0) return false;
+    bool s = false;
+#pragma unroll
+    for (int p = 0; p < 4; p++)
+#pragma unroll
+        for (int q = 0; q < 4; q++) s |= (x.c[p] == y.c[q]);
+    return s;
+}
+
 // Per-j epilogue: raw output, row partial (F_ij = raw/A_i) and the
 // transposed partial for row j (sum over lanes of raw/A_j).
 __device__ __forceinline__ void pair_out(const PairArgs& a, const TileCtx& t, int64_t j, bool act, double raw,
@@ -286,6 +307,7 @@ __device__ __forceinline__ void pair_out(const PairArgs& a, const TileCtx& t, in
 template <int PC>
 __global__ __launch_bounds__(64) void k_pair_fp64(PairArgs a) {
     const TileCtx t = tile_ctx(a);
+    const Corners cwi = corners_of(a, t.i, t.vi && a.duffy);
     const int64_t NS = a.NS;
     const int npi = t.vi ? a.elem_np[t.i] : 0;
     int npmax = npi;
@@ -300,7 +322,8 @@ __global__ __launch_bounds__(64) void k_pair_fp64(PairArgs a) {
             if (t.lane == 0) a.partT[(int64_t)t.s * a.NS + j] = 0.0;
             continue;
         }
-        const bool act = t.vi && t.i < j;
+        bool act = t.vi && t.i < j;
+        if (a.duffy && shares_corner(cwi, corners_of(a, j, true))) act = false;
         const int npj = a.elem_np[j];
         const double* __restrict__ qj = a.Q64 + j * a.maxpts * 8;
         double Fij = 0.0;
@@ -353,6 +376,7 @@ __global__ __launch_bounds__(64) void k_pair_fp64(PairArgs a) {
 // ---------------------------------------------------------------------------
 __global__ __launch_bounds__(64) void k_pair_fp64_occ(PairArgs a) {
     const TileCtx t = tile_ctx(a);
+    const Corners cwi = corners_of(a, t.i, t.vi && a.duffy);
     const int64_t NS = a.NS;
     const int npi = t.vi ? a.elem_np[t.i] : 0;
     int npmax = npi;
@@ -369,7 +393,8 @@ __global__ __launch_bounds__(64) void k_pair_fp64_occ(PairArgs a) {
             if (t.lane == 0) a.partT[(int64_t)t.s * a.NS + j] = 0.0;
             continue;
         }
-        const bool act = t.vi && t.i < j;
+        bool act = t.vi && t.i < j;
+        if (a.duffy && shares_corner(cwi, corners_of(a, j, true))) act = false;
         const int npj = a.elem_np[j];
         const int gj = a.elem_grp[j];
         const int root = a.tree_root[gi * a.G + gj];
@@ -438,9 +463,13 @@ __device__ __forceinline__ float vf_mixed(float bx, float by, float bz, float nx
     return (r2 >= 1e-30f) ? k : 0.0f;
 }
 
-template <int PC>
+template <int PC, bool SINGLE>
 __global__ __launch_bounds__(64) void k_pair_mixed(PairArgs a) {
+    // SINGLE: every element has <= PC points, so the strip's points stay in
+    // registers for the whole tile; otherwise chunks of PC points are reloaded
+    // per column element (L1/L2 hits).
     const TileCtx t = tile_ctx(a);
+    const Corners cwi = corners_of(a, t.i, t.vi && a.duffy);
     const int64_t NS = a.NS;
     const int npi = t.vi ? a.elem_np[t.i] : 0;
     int npmax = npi;
@@ -451,35 +480,48 @@ __global__ __launch_bounds__(64) void k_pair_mixed(PairArgs a) {
     const float* __restrict__ P = a.P32;
     const double* __restrict__ P64 = a.P64;
     const double aix = a.anchor[t.i * 4 + 0], aiy = a.anchor[t.i * 4 + 1], aiz = a.anchor[t.i * 4 + 2];
+    float DX[PC], DY[PC], DZ[PC], NX[PC], NY[PC], NZ[PC];
+    double WD[PC];
+    auto load_chunk = [&](int c0) {
+#pragma unroll
+        for (int p = 0; p < PC; p++) {
+            int pp = c0 + p;
+            if (pp < npmax) {
+                DX[p] = P[(int64_t)(pp * 8 + 0) * NS + t.i];
+                DY[p] = P[(int64_t)(pp * 8 + 1) * NS + t.i];
+                DZ[p] = P[(int64_t)(pp * 8 + 2) * NS + t.i];
+                NX[p] = P[(int64_t)(pp * 8 + 3) * NS + t.i];
+                NY[p] = P[(int64_t)(pp * 8 + 4) * NS + t.i];
+                NZ[p] = P[(int64_t)(pp * 8 + 5) * NS + t.i];
+                WD[p] = pp < npi ? P64[(int64_t)(pp * 8 + 6) * NS + t.i] * P64[(int64_t)(pp * 8 + 7) * NS + t.i] : 0.0;
+            } else {
+                DX[p] = DY[p] = DZ[p] = NX[p] = NY[p] = NZ[p] = 0.0f;
+                WD[p] = 0.0;
+            }
+        }
+    };
+    if (SINGLE) load_chunk(0);
     double ls = 0.0, lc = 0.0;
     for (int64_t j = t.jb; j < t.je; j++) {
         if (j <= (int64_t)t.s * RVF_STRIP) {
             if (t.lane == 0) a.partT[(int64_t)t.s * a.NS + j] = 0.0;
             continue;
         }
-        const bool act = t.vi && t.i < j;
+        bool act = t.vi && t.i < j;
+        if (a.duffy && shares_corner(cwi, corners_of(a, j, true))) act = false;
         const int npj = a.elem_np[j];
         const float* __restrict__ qj = a.Q32 + j * a.maxpts * 8;
         const float dax = (float)(a.anchor[j * 4 + 0] - aix);
         const float day = (float)(a.anchor[j * 4 + 1] - aiy);
         const float daz = (float)(a.anchor[j * 4 + 2] - aiz);
         double Fs = 0.0, Fc = 0.0;
-        for (int c0 = 0; c0 < npmax; c0 += PC) {
-            float BX[PC], BY[PC], BZ[PC], NX[PC], NY[PC], NZ[PC], inner[PC];
+        for (int c0 = 0; c0 < (SINGLE ? 1 : npmax); c0 += PC) {
+            if (!SINGLE) load_chunk(c0);
+            float BX[PC], BY[PC], BZ[PC], inner[PC];
 #pragma unroll
             for (int p = 0; p < PC; p++) {
-                int pp = c0 + p;
+                BX[p] = dax - DX[p]; BY[p] = day - DY[p]; BZ[p] = daz - DZ[p];
                 inner[p] = 0.0f;
-                if (pp < npmax) {
-                    BX[p] = dax - P[(int64_t)(pp * 8 + 0) * NS + t.i];
-                    BY[p] = day - P[(int64_t)(pp * 8 + 1) * NS + t.i];
-                    BZ[p] = daz - P[(int64_t)(pp * 8 + 2) * NS + t.i];
-                    NX[p] = P[(int64_t)(pp * 8 + 3) * NS + t.i];
-                    NY[p] = P[(int64_t)(pp * 8 + 4) * NS + t.i];
-                    NZ[p] = P[(int64_t)(pp * 8 + 5) * NS + t.i];
-                } else {
-                    BX[p] = BY[p] = BZ[p] = NX[p] = NY[p] = NZ[p] = 0.0f;
-                }
             }
             for (int q = 0; q < npj; q++) {
                 const float qx = qj[q * 8 + 0], qy = qj[q * 8 + 1], qz = qj[q * 8 + 2];
@@ -496,13 +538,8 @@ __global__ __launch_bounds__(64) void k_pair_mixed(PairArgs a) {
                 }
             }
 #pragma unroll
-            for (int p = 0; p < PC; p++) {
-                int pp = c0 + p;
-                if (pp < npmax && pp < npi) {
-                    double wd = P64[(int64_t)(pp * 8 + 6) * NS + t.i] * P64[(int64_t)(pp * 8 + 7) * NS + t.i];
-                    neumaier(Fs, Fc, wd * (double)inner[p]);
-                }
-            }
+            for (int p = 0; p < PC; p++)
+                if (c0 + p < npmax) neumaier(Fs, Fc, WD[p] * (double)inner[p]);
         }
         pair_out(a, t, j, act, (Fs + Fc) * (1.0 / PI_D), Ai, ls, lc);
     }
@@ -605,6 +642,7 @@ __device__ bool blocked_mixed(const BvhNode32* __restrict__ nodes, const Tri32* 
 
 __global__ __launch_bounds__(64) void k_pair_mixed_occ(PairArgs a, float Sm) {
     const TileCtx t = tile_ctx(a);
+    const Corners cwi = corners_of(a, t.i, t.vi && a.duffy);
     const int64_t NS = a.NS;
     const int npi = t.vi ? a.elem_np[t.i] : 0;
     int npmax = npi;
@@ -624,7 +662,8 @@ __global__ __launch_bounds__(64) void k_pair_mixed_occ(PairArgs a, float Sm) {
             if (t.lane == 0) a.partT[(int64_t)t.s * a.NS + j] = 0.0;
             continue;
         }
-        const bool act = t.vi && t.i < j;
+        bool act = t.vi && t.i < j;
+        if (a.duffy && shares_corner(cwi, corners_of(a, j, true))) act = false;
         const int npj = a.elem_np[j];
         const int gj = a.elem_grp[j];
         const int root = a.tree_root[gi * a.G + gj];
@@ -685,6 +724,263 @@ __global__ __launch_bounds__(64) void k_pair_mixed_occ(PairArgs a, float Sm) {
         if (t.lane == 0) { atomicAdd(&a.counters[0], r0); atomicAdd(&a.counters[1], r1); }
     }
 }
+// ---------------------------------------------------------------------------
+// Monte Carlo (GPUMCKernels.jl:37-50,146-281): the reference's per-pair
+// stream (splitmix64-seeded xorshift64, thread id (i-1)N + j), stratum k of
+// element i paired with stratum k of element j, FP64 throughout; visibility
+// with the CPU path's semantics (per-active-set trees, t > 0).
+// ---------------------------------------------------------------------------
+__device__ __forceinline__ uint64_t init_rng(uint64_t seed, int64_t tid) {
+    uint64_t z = seed + (uint64_t)tid * 0x9E3779B97F4A7C15ULL;
+    z = (z ^ (z >> 30)) * 0xBF58476D1CE4E5B9ULL;
+    z = (z ^ (z >> 27)) * 0x94D049BB133111EBULL;
+    return z ^ (z >> 31);
+}
+__device__ __forceinline__ double xorshift64(uint64_t& s) {
+    uint64_t x = s;
+    x ^= x << 13; x ^= x >> 7; x ^= x << 17;
+    s = x;
+    return (double)(x >> 11) / (double)0x001FFFFFFFFFFFFFULL;
+}
+// _sample_on_element (GPUMCKernels.jl:266-281)
+__device__ __forceinline__ void mc_sample(const PairArgs& a, int f, int64_t e, double u1, double u2, double* x,
+                                          double* n, double* dA) {
+    double xi, eta;
+    if (f == 0 || f == 2) { xi = 2.0 * u1 - 1.0; eta = 2.0 * u2 - 1.0; }
+    else { xi = u1; eta = u2; if (xi + eta > 1.0) { xi = 1.0 - xi; eta = 1.0 - eta; } }
+    eval_point(f, a.coords, a.nodes8 + 8 * e, xi, eta, x, n, dA);
+}
+
+// MC area of element e from the stream of the pair (e, e), mirroring the CPU
+// path's area pass element_pair_view_factor_mc(i, i) (Assembly.jl:164-169).
+__global__ __launch_bounds__(64) void k_mc_area(PairArgs a, double* area) {
+    int64_t e = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
+    if (e >= a.N) return;
+    int f = a.fam[e];
+    uint64_t st = init_rng(a.mc_seed, e * a.N + e + 1);
+    int s = a.mc_s;
+    double Ai = 0.0;
+    int k = 0;
+    double x[3], n[3], dA;
+    for (int si = 0; si < s; si++)
+        for (int sj = 0; sj < s; sj++) {
+            k++;
+            double u1 = xorshift64(st), u2 = xorshift64(st);
+            mc_sample(a, f, e, (si + u1) / s, (sj + u2) / s, x, n, &dA);
+            Ai += dA;
+            xorshift64(st); xorshift64(st);
+        }
+    for (; k < a.mc_nsamp; k++) {
+        double u1 = xorshift64(st), u2 = xorshift64(st);
+        mc_sample(a, f, e, u1, u2, x, n, &dA);
+        Ai += dA;
+        xorshift64(st); xorshift64(st);
+    }
+    double ri = (f == 0 || f == 2) ? 4.0 : 0.5;
+    area[e] = (Ai * ri) / a.mc_nsamp;
+}
+
+__global__ __launch_bounds__(64) void k_pair_mc(PairArgs a) {
+    const TileCtx t = tile_ctx(a);
+    const Corners cwi = corners_of(a, t.i, false);
+    (void)cwi;
+    const int64_t NS = a.NS;
+    const int fi = t.vi ? a.fam[t.i] : 2;
+    const double Ai = t.vi ? a.area[t.i] : 1.0;
+    const int gi = t.vi ? a.elem_grp[t.i] : 0;
+    const int64_t ii = t.vi ? t.i : 0;
+    const int s = a.mc_s;
+    double ls = 0.0, lc = 0.0;
+    unsigned long long nrays = 0, naabb = 0, ntri = 0;
+    for (int64_t j = t.jb; j < t.je; j++) {
+        if (j <= (int64_t)t.s * RVF_STRIP) {
+            if (t.lane == 0) a.partT[(int64_t)t.s * NS + j] = 0.0;
+            continue;
+        }
+        const bool act = t.vi && t.i < j;
+        const int fj = a.fam[j];
+        const int root = a.tree_root ? a.tree_root[gi * a.G + a.elem_grp[j]] : -1;
+        uint64_t st = init_rng(a.mc_seed, ii * a.N + j + 1);
+        double Ks = 0.0;
+        int k = 0;
+        double xi[3], ni[3], ai, xj[3], nj[3], aj;
+        for (int kk = 0; kk < a.mc_nsamp; kk++) {
+            double u1 = xorshift64(st), u2 = xorshift64(st);
+            double si = (double)(kk / s), sj = (double)(kk % s);
+            bool strat = kk < s * s;
+            mc_sample(a, fi, ii, strat ? (si + u1) / s : u1, strat ? (sj + u2) / s : u2, xi, ni, &ai);
+            double u3 = xorshift64(st), u4 = xorshift64(st);
+            mc_sample(a, fj, j, strat ? (si + u3) / s : u3, strat ? (sj + u4) / s : u4, xj, nj, &aj);
+            Ray64 R;
+            double K = vf_exact(xi[0], xi[1], xi[2], ni[0], ni[1], ni[2], xj[0], xj[1], xj[2], nj[0], nj[1], nj[2], R);
+            if (act && K > 0.0 && root >= 0) {
+                nrays++;
+                if (blocked64(a.nodes64, a.tris64, root, R, naabb, ntri)) K = 0.0;
+            }
+            Ks += (K * ai) * aj;
+            k++;
+        }
+        (void)k;
+        double ri = (fi == 0 || fi == 2) ? 4.0 : 0.5;
+        double rj = (fj == 0 || fj == 2) ? 4.0 : 0.5;
+        double raw = ((Ks * ri) * rj) / a.mc_nsamp;
+        pair_out(a, t, j, act, raw, Ai, ls, lc);
+    }
+    a.part[(int64_t)t.chunk * NS + t.i] = ls + lc;
+    if (a.counters) {
+        unsigned long long c1 = nrays;
+        for (int o = 32; o > 0; o >>= 1) c1 += __shfl_xor(c1, o, 64);
+        if (t.lane == 0) atomicAdd(&a.counters[1], c1);
+    }
+}
+
+// ---------------------------------------------------------------------------
+// Duffy / Sauter-Schwab for Quad8 pairs sharing corners (DuffyKernel.jl:
+// 222-487, maps restated verbatim including the reference's Jacobians).  One
+// wavefront per singular pair; lanes split the regions x nq^4 points and the
+// non-negative terms are summed (per-lane sequential, then a wave tree).
+// ---------------------------------------------------------------------------
+__device__ __forceinline__ void vertex_map(int region, double r, double e1, double e2, double e3, double u0i,
+                                           double v0i, double u0j, double v0j, double& u, double& v, double& s,
+                                           double& t) {
+    double du, dv, ds, dt;
+    switch (region) {
+    case 1: du = r; dv = r * e1; ds = r * e2; dt = (r * e2) * e3; break;
+    case 2: du = r; dv = r * e1; ds = (r * e2) * e3; dt = r * e2; break;
+    case 3: du = r * e1; dv = r; ds = r * e2; dt = (r * e2) * e3; break;
+    case 4: du = r * e1; dv = r; ds = (r * e2) * e3; dt = r * e2; break;
+    case 5: du = r * e2; dv = (r * e2) * e3; ds = r; dt = r * e1; break;
+    case 6: du = (r * e2) * e3; dv = r * e2; ds = r; dt = r * e1; break;
+    case 7: du = r * e2; dv = (r * e2) * e3; ds = r * e1; dt = r; break;
+    default: du = (r * e2) * e3; dv = r * e2; ds = r * e1; dt = r; break;
+    }
+    u = u0i + (u0i == 0.0 ? du : -du);
+    v = v0i + (v0i == 0.0 ? dv : -dv);
+    s = u0j + (u0j == 0.0 ? ds : -ds);
+    t = v0j + (v0j == 0.0 ? dt : -dt);
+}
+
+__device__ __forceinline__ void edge_map(int region, double r, double e1, double e2, double e3, double& u, double& v,
+                                         double& s, double& t) {
+    switch (region) {
+    case 1:
+        u = r * (1 - e1 * (1 - e3)); v = ((r * e1) * e2) * e3;
+        s = (r * (1 - e1 * (1 - e3)) - ((r * e1) * (1 - e2)) * e3) + ((r * e1) * e2) * e3;
+        t = ((r * e1) * (1 - e2)) * e3;
+        break;
+    case 2:
+        u = r * (1 - (e1 * e2) * e3); v = ((r * e1) * e3) * (1 - e2);
+        s = (r * (1 - (e1 * e2) * e3) + ((r * e1) * e2) * e3) - (r * e1) * e3;
+        t = ((r * e1) * e2) * e3;
+        break;
+    case 3:
+        u = r * (1 - e1 * e2); v = ((r * e1) * e2) * e3;
+        s = r * (1 - e1 * e2) - ((r * e1) * e2) * (1 - e3);
+        t = ((r * e1) * e2) * (1 - e3);
+        break;
+    case 4:
+        u = r * (e2 * e3 + e1 * (1 - e2 * e3)); v = ((r * (1 - e1)) * e2) * e3;
+        s = (r * e2) * e3; t = ((r * (1 - e1)) * e2) * e3;
+        break;
+    default:
+        u = (r * e1) * e3; v = ((r * (1 - e1)) * e2) * e3;
+        s = ((r * e1) * e3) * (1 - e2) + ((r * e1) * e2) * e3;
+        t = ((r * (1 - e1)) * e2) * e3;
+        break;
+    }
+}
+
+__device__ __forceinline__ int corners_to_edge(int c1, int c2) {
+    int a = c1 < c2 ? c1 : c2, b = c1 < c2 ? c2 : c1;
+    if (a == 1 && b == 2) return 1;
+    if (a == 2 && b == 3) return 2;
+    if (a == 3 && b == 4) return 3;
+    if (a == 1 && b == 4) return 4;
+    return 1;
+}
+__device__ __forceinline__ void rotate_to_edge(double& u, double& v, int e) {
+    double a = u, b = v;
+    if (e == 2) { u = 1.0 - b; v = a; }
+    else if (e == 3) { u = 1.0 - a; v = 1.0 - b; }
+    else if (e == 4) { u = b; v = 1.0 - a; }
+}
+
+__global__ __launch_bounds__(64) void k_duffy(PairArgs a, const DuffyPair* __restrict__ pairs, int npairs,
+                                              const double* __restrict__ gl01, int nq, double* __restrict__ out) {
+    int pidx = blockIdx.x;
+    if (pidx >= npairs) return;
+    const DuffyPair P = pairs[pidx];
+    const double* p01 = gl01;
+    const double* w01 = gl01 + 8;
+    const double CUV[4][2] = {{0.0, 0.0}, {1.0, 0.0}, {1.0, 1.0}, {0.0, 1.0}};
+    double u0i = 0, v0i = 0, u0j = 0, v0j = 0;
+    int edge_i = 1, edge_j = 1;
+    if (P.kind == 1) {
+        u0i = CUV[P.ci0 - 1][0]; v0i = CUV[P.ci0 - 1][1];
+        u0j = CUV[P.cj0 - 1][0]; v0j = CUV[P.cj0 - 1][1];
+    } else {
+        edge_i = corners_to_edge(P.ci0, P.ci1);
+        edge_j = corners_to_edge(P.cj0, P.cj1);
+    }
+    const int nreg = P.kind == 1 ? 8 : 5;
+    const int nq4 = nq * nq * nq * nq;
+    const int total = nreg * nq4;
+    double s = 0.0, c = 0.0;
+    unsigned long long naabb = 0, ntri = 0;
+    for (int idx = threadIdx.x; idx < total; idx += 64) {
+        int region = idx / nq4 + 1;
+        int rem = idx % nq4;
+        int ia = rem / (nq * nq * nq), ib = (rem / (nq * nq)) % nq, ic = (rem / nq) % nq, id = rem % nq;
+        double r = p01[ia], wr = w01[ia], e1 = p01[ib], we1 = w01[ib];
+        double e2 = p01[ic], we2 = w01[ic], e3 = p01[id], we3 = w01[id];
+        double u, v, ss, tt;
+        if (P.kind == 1) vertex_map(region, r, e1, e2, e3, u0i, v0i, u0j, v0j, u, v, ss, tt);
+        else {
+            edge_map(region, r, e1, e2, e3, u, v, ss, tt);
+            rotate_to_edge(u, v, edge_i);
+            rotate_to_edge(ss, tt, edge_j);
+        }
+        if (u < 0 || u > 1 || v < 0 || v > 1 || ss < 0 || ss > 1 || tt < 0 || tt > 1) continue;
+        double xi[3], ni[3], ai, xj[3], nj[3], aj;
+        eval_point(0, a.coords, a.nodes8 + 8 * (int64_t)P.i, 2 * u - 1.0, 2 * v - 1.0, xi, ni, &ai);
+        ai = ai * 4.0;
+        eval_point(0, a.coords, a.nodes8 + 8 * (int64_t)P.j, 2 * ss - 1.0, 2 * tt - 1.0, xj, nj, &aj);
+        aj = aj * 4.0;
+        Ray64 R;
+        double K = vf_exact(xi[0], xi[1], xi[2], ni[0], ni[1], ni[2], xj[0], xj[1], xj[2], nj[0], nj[1], nj[2], R);
+        if (K == 0.0) continue;
+        if (P.root >= 0 && blocked64(a.nodes64, a.tris64, P.root, R, naabb, ntri)) continue;
+        double jac = (r * r) * r;
+        neumaier(s, c, ((((((wr * we1) * we2) * we3) * jac) * K) * ai) * aj);
+    }
+    double v = wave_sum_d(s + c);
+    if (threadIdx.x == 0) out[pidx] = v;
+}
+
+// Write Duffy raw values into F (before the fill converts raw to F) and add
+// their row / group contributions in a fixed order per row.
+__global__ __launch_bounds__(256) void k_duffy_apply(PairArgs a, const DuffyPair* __restrict__ pairs,
+                                                     const double* __restrict__ val, const int32_t* __restrict__ rptr,
+                                                     const int32_t* __restrict__ rlist, double* __restrict__ rowsum,
+                                                     double* __restrict__ W) {
+    int64_t r = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
+    if (r >= a.N) return;
+    double Ar = a.area[r];
+    double ts = 0.0, tc = 0.0;
+    for (int k = rptr[r]; k < rptr[r + 1]; k++) {
+        int pidx = rlist[k];
+        const DuffyPair P = pairs[pidx];
+        double raw = val[pidx];
+        double f = raw / Ar;
+        neumaier(ts, tc, f);
+        if (P.i == r) {  // partner j > r: upper part, group of j
+            W[(int64_t)a.elem_grp[P.j] * a.NS + r] += f;
+            if (a.F) a.F[(int64_t)P.i + a.N * (int64_t)P.j] = raw;
+        }
+    }
+    rowsum[r] += ts + tc;
+}
+
 // ---------------------------------------------------------------------------
 // Fill: F[i,j] = raw/A_i and F[j,i] = raw/A_j for i < j (Assembly.jl:211),
 // F[i,i] = 0.  One 64x64 tile per workgroup, transposed half through LDS.
@@ -784,10 +1080,29 @@ int rvf_launch_pair(const PairArgs* a, int mixed, int occ, int pc, float Sm, hip
         else hipLaunchKernelGGL(k_pair_fp64<16>, g, b, 0, st, *a);
     } else {
         if (occ) hipLaunchKernelGGL(k_pair_mixed_occ, g, b, 0, st, *a, Sm);
-        else if (pc <= 4) hipLaunchKernelGGL(k_pair_mixed<4>, g, b, 0, st, *a);
-        else if (pc <= 9) hipLaunchKernelGGL(k_pair_mixed<9>, g, b, 0, st, *a);
-        else hipLaunchKernelGGL(k_pair_mixed<16>, g, b, 0, st, *a);
+        else if (pc <= 4) hipLaunchKernelGGL((k_pair_mixed<4, true>), g, b, 0, st, *a);
+        else if (pc <= 9) hipLaunchKernelGGL((k_pair_mixed<9, true>), g, b, 0, st, *a);
+        else if (pc <= 16) hipLaunchKernelGGL((k_pair_mixed<16, true>), g, b, 0, st, *a);
+        else hipLaunchKernelGGL((k_pair_mixed<16, false>), g, b, 0, st, *a);
     }
+    return (int)hipGetLastError();
+}
+
+int rvf_launch_mc(const PairArgs* a, double* area, hipStream_t st) {
+    if (a->N > 0) hipLaunchKernelGGL(k_mc_area, dim3((unsigned)((a->N + 63) / 64)), dim3(64), 0, st, *a, area);
+    if (a->ntiles > 0) hipLaunchKernelGGL(k_pair_mc, dim3((unsigned)a->ntiles), dim3(64), 0, st, *a);
+    return (int)hipGetLastError();
+}
+
+int rvf_launch_duffy(const PairArgs* a, const DuffyPair* pairs, int npairs, const double* gl01, int nq, double* val,
+                     const int32_t* rptr, const int32_t* rlist, double* rowsum, double* W, int phase,
+                     hipStream_t st) {
+    if (npairs == 0) return 0;
+    if (phase == 0)
+        hipLaunchKernelGGL(k_duffy, dim3((unsigned)npairs), dim3(64), 0, st, *a, pairs, npairs, gl01, nq, val);
+    else
+        hipLaunchKernelGGL(k_duffy_apply, dim3((unsigned)((a->N + 255) / 256)), dim3(256), 0, st, *a, pairs, val,
+                           rptr, rlist, rowsum, W);
     return (int)hipGetLastError();
 }
 
