@@ -109,6 +109,7 @@ def main():
     ap.add_argument("--cpu-seconds", type=float, default=15.0)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-verify", action="store_true")
+    ap.add_argument("--no-secondary", dest="secondary", action="store_false")
     args = ap.parse_args()
     if args.gpus != 1:
         print("bench.py: this build runs on one GPU; multi-GPU sharding of pair tiles is designed "
@@ -188,7 +189,36 @@ def main():
     }
     if cpu:
         out["gpu_vs_cpu_unique_pairs"] = (N * (N - 1) / 2) / (ms_m * 1e-3) / cpu["value"]
+    plan_m.close(); plan_d.close()
+    if args.secondary and args.workload != "cube":
+        out["unoccluded_cube"] = secondary_cube(api, ctx, args.steps)
     print(json.dumps(out))
+
+
+def secondary_cube(api, ctx, steps):
+    """The unoccluded 13,824-element cube (nquad 3): mixed and FP64 solve
+    times, the pair kernels' durations and mixed-vs-FP64 differences."""
+    mesh, obs, nquad, parity, desc = build_workload("cube")
+    N = mesh.n_elems
+    dm = api.DeviceMesh(ctx, mesh)
+    res = {"workload": desc}
+    Fs = {}
+    for prec in ("mixed", "fp64"):
+        plan, ms, lat, pair = run_mode(api, ctx, dm, None, nquad, prec, steps, 1)
+        st = plan.stats()
+        F, A, rs, Fg, *_ = plan.fetch()
+        Fs[prec] = (F, rs)
+        res[prec] = {"ms_per_step": ms, "pairs_per_s": float(N) * N / (ms * 1e-3),
+                     "pair_kernel_ms": float(pair.mean()), "fill_ms": st["fill_ms"], "reduce_ms": st["reduce_ms"],
+                     "precompute_ms": st["precompute_ms"],
+                     "pair_kernel_tflops": flop_model(N, nquad * nquad) / (float(pair.mean()) * 1e-3) / 1e12}
+        plan.close()
+    res["speedup_mixed_vs_fp64"] = res["fp64"]["ms_per_step"] / res["mixed"]["ms_per_step"]
+    res["max_abs_dF"] = float(np.abs(Fs["mixed"][0] - Fs["fp64"][0]).max())
+    res["closure_err"] = float(np.abs(Fs["mixed"][1] - Fs["fp64"][1]).max())
+    res["rowsum_minus_1_max_fp64"] = float(np.abs(Fs["fp64"][1] - 1.0).max())
+    dm.close()
+    return res
 
 
 if __name__ == "__main__":

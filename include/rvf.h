@@ -47,6 +47,7 @@ extern "C" {
 #define RVF_WANT_F 1     /* materialise the N x N element matrix F on the device */
 #define RVF_AUDIT 2      /* per-pair blocked-ray count + hash (occluded solves) */
 #define RVF_TIMING 4     /* record per-kernel HIP event timings */
+#define RVF_GRAPH 8      /* capture the solve's launch sequence once and replay it as a hipGraph */
 
 typedef struct rvf_ctx rvf_ctx;
 typedef struct rvf_mesh rvf_mesh;
@@ -144,6 +145,11 @@ int rvf_quad_rule(int32_t nquad, int32_t tri, double* pts, double* wts);
 int rvf_ctx_event_record(rvf_ctx* ctx, int32_t slot);
 int rvf_ctx_event_elapsed(rvf_ctx* ctx, int32_t slot_a, int32_t slot_b, float* ms);
 int rvf_ctx_sync(rvf_ctx* ctx);
+
+/* measurement aid (no reference counterpart): copy `bytes` device-to-device
+ * `reps` times with the library's 16-byte-vector copy kernel; average ms.
+ * Used to calibrate HBM traffic counters against a kernel of known size. */
+int rvf_copy_probe(rvf_ctx* ctx, int64_t bytes, int32_t reps, float* ms);
 
 #ifdef __cplusplus
 }

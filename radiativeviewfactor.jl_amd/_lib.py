@@ -11,7 +11,7 @@ import os
 HERE = os.path.dirname(os.path.abspath(__file__))
 LIB_PATH = os.path.join(HERE, "librvf_hip.so")
 
-RVF_WANT_F, RVF_AUDIT, RVF_TIMING = 1, 2, 4
+RVF_WANT_F, RVF_AUDIT, RVF_TIMING, RVF_GRAPH = 1, 2, 4, 8
 METHODS = {"quad": 0, "mc": 1, "duffy": 2}
 PRECISIONS = {"fp64": 0, "mixed": 1}
 
@@ -77,6 +77,7 @@ def lib():
         "rvf_ctx_event_record": ([P, I32], C.c_int),
         "rvf_ctx_event_elapsed": ([P, I32, I32, C.POINTER(C.c_float)], C.c_int),
         "rvf_ctx_sync": ([P], C.c_int),
+        "rvf_copy_probe": ([P, I64, I32, C.POINTER(C.c_float)], C.c_int),
     }
     for name, (args, res) in sig.items():
         f = getattr(L, name)

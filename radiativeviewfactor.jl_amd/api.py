@@ -178,14 +178,15 @@ class Plan:
 
     def __init__(self, ctx: Context, dmesh: DeviceMesh, dobs: Optional[DeviceObstruction], nquad: int,
                  method: str = "quad", precision: str = "fp64", n_samples: int = 10000, seed: int = 0,
-                 want_F: bool = True, audit: bool = False, timing: bool = True):
+                 want_F: bool = True, audit: bool = False, timing: bool = True, graph: bool = False):
         o = _lib.Opts()
         o.nquad = nquad
         o.method = _lib.METHODS[method]
         o.precision = _lib.PRECISIONS[precision]
         o.n_samples = n_samples
         o.seed = seed & ((1 << 64) - 1)
-        o.flags = (RVF_WANT_F if want_F else 0) | (RVF_AUDIT if audit else 0) | (RVF_TIMING if timing else 0)
+        o.flags = (RVF_WANT_F if want_F else 0) | (RVF_AUDIT if audit else 0) | (RVF_TIMING if timing else 0) | \
+            (_lib.RVF_GRAPH if graph else 0)
         self.opts = o
         self.ctx, self.dmesh, self.dobs = ctx, dmesh, dobs
         h = C.c_void_p()

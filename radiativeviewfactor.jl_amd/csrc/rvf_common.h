@@ -15,7 +15,7 @@ struct __align__(16) BvhNode64 {
     int32_t miss;
     int32_t start;   // first triangle (absolute index into the triangle arrays)
     int32_t count;   // 0 for interior nodes
-    int32_t pad;
+    int32_t pad;     // interior nodes: right child (absolute)
 };
 
 // FP32 node (32 B) with conservatively padded bounds.
@@ -23,7 +23,7 @@ struct __align__(16) BvhNode32 {
     float lo[3];
     int32_t miss;
     float hi[3];
-    int32_t start_count;  // (start << 4) | count, count <= 8
+    int32_t start_count;  // leaves: (start << 4) | count (count <= 8); interior: right child << 4
 };
 
 // Triangle for Moller-Trumbore, FP64: v0, e1 = v1 - v0, e2 = v2 - v0 as the
@@ -60,9 +60,10 @@ struct PairArgs {
     int maxpts;               // per-element point stride in Q arrays
     int64_t N;                // elements
     const Chunk* chunks;
-    const int32_t* elem_np;   // points per element
-    const int32_t* elem_grp;  // group rank per element
-    const double* area;       // per element
+    const int32_t* slot_orig; // slot -> original element index (-1 = padding)
+    const int32_t* elem_np;   // points per slot
+    const int32_t* elem_grp;  // group rank per slot
+    const double* area;       // per slot
     // FP64 point data
     const double* P64;        // point-major [(p*8+c)*NS + e]: x y z nx ny nz w dA
     const double* Q64;        // element-major [(e*maxpts + p)*8 + c]
@@ -81,6 +82,8 @@ struct PairArgs {
     const Tri64* tris64;
     const BvhNode32* nodes32;
     const Tri32* tris32;
+    const int32_t* parent;    // per node, absolute; -1 at a root
+    int verify;               // mixed: confirm FP64 hits by the leaf's exact slab chain
     // audit (nullable): per pair blocked-ray count / order-free hash at [i + N*j]
     uint32_t* audit_count;
     uint32_t* audit_hash;
@@ -108,6 +111,8 @@ struct PrecompArgs {
     double* P64; double* Q64; float* P32; float* Q32; double* anchor; double* area;
     int32_t* elem_np;
     int* bad;                  // set to 1 if any dA is 0 or not finite
+    const int32_t* slot_orig;  // slot -> original element
+    double* area_orig;         // areas in original element order
 };
 
 // One singular Quad8 pair for the Duffy / Sauter-Schwab kernel.

@@ -18,15 +18,18 @@
 
 extern "C" int rvf_launch_precompute(const PrecompArgs* a, hipStream_t st);
 extern "C" int rvf_launch_pair(const PairArgs* a, int mixed, int occ, int pc, float Sm, hipStream_t st);
-extern "C" int rvf_launch_mc(const PairArgs* a, double* area, hipStream_t st);
+extern "C" int rvf_launch_mc(const PairArgs* a, double* area, double* area_orig, hipStream_t st);
 extern "C" int rvf_launch_duffy(const PairArgs* a, const DuffyPair* pairs, int npairs, const double* gl01, int nq,
                                 double* val, const int32_t* rptr, const int32_t* rlist, double* rowsum, double* W,
                                 int phase, hipStream_t st);
+extern "C" int rvf_launch_zero(void* p, int64_t bytes, hipStream_t st);
+extern "C" int rvf_launch_copy(const void* src, void* dst, int64_t bytes, hipStream_t st);
 extern "C" int rvf_launch_fill(double* F, const double* area, int64_t N, const int2* tiles, int ntiles,
                                hipStream_t st);
 extern "C" int rvf_launch_reduce(const double* part, const double* partT, const Chunk* chunks, int nchunks,
                                  const int32_t* chunk_of, int64_t N, int NS, int G, double* rowsum, double* W,
-                                 hipStream_t st);
+                                 double* Tseg, double* Wseg, hipStream_t st);
+#define RVF_RSEG 16
 
 // ---------------------------------------------------------------------------
 // errors
@@ -65,7 +68,7 @@ struct DBuf {
 static int upload(DBuf& b, const void* src, size_t bytes) {
     int rc = b.alloc(bytes);
     if (rc) return rc;
-    if (bytes) HIPCHK(hipMemcpy(b.p, src, bytes, hipMemcpyHostToDevice));
+    if (bytes && src) HIPCHK(hipMemcpy(b.p, src, bytes, hipMemcpyHostToDevice));
     return 0;
 }
 
@@ -178,6 +181,18 @@ static int tri_rule(int nquad, double* pts, double* wts) {
 // ---------------------------------------------------------------------------
 // mesh
 // ---------------------------------------------------------------------------
+// Slot layout of a solve: slot k of the pair tiling holds original element
+// slot_orig[k].  Identity for unoccluded solves; occluded quadrature solves
+// order elements by (group, Morton code of the centroid) so that a strip's 64
+// lanes cover a compact patch and their rays form coherent bundles.
+struct Layout {
+    bool built = false;
+    std::vector<int32_t> slot_orig, orig_slot, slot_grp, chunk_of;
+    std::vector<Chunk> chunks;
+    std::vector<int2> tiles;
+    DBuf d_slot_orig, d_slot_grp, d_chunks, d_chunk_of, d_tiles;
+};
+
 struct rvf_mesh {
     rvf_ctx* ctx = nullptr;
     int64_t N = 0, Nn = 0;
@@ -186,12 +201,85 @@ struct rvf_mesh {
     std::vector<int32_t> nodes8;   // N*8 1-based
     std::vector<int32_t> grp_rank; // per element
     std::vector<int32_t> tags;     // sorted group tags
-    std::vector<Chunk> chunks;
-    std::vector<int32_t> chunk_of;
-    std::vector<int2> tiles, fill_tiles;
+    std::vector<double> hcoords;
+    std::vector<int2> fill_tiles;
     double Sm = 1.0;               // max |coordinate|
-    DBuf coords, d_fam, d_nodes8, d_grp, d_chunks, d_chunk_of, d_tiles, d_fill_tiles;
+    mutable Layout lay[2];         // 0 identity, 1 spatial
+    DBuf coords, d_fam, d_nodes8, d_grp, d_fill_tiles;
 };
+
+static inline uint32_t spread10(uint32_t v) {
+    v &= 1023u;
+    v = (v | (v << 16)) & 0x030000FFu;
+    v = (v | (v << 8)) & 0x0300F00Fu;
+    v = (v | (v << 4)) & 0x030C30C3u;
+    v = (v | (v << 2)) & 0x09249249u;
+    return v;
+}
+
+static int get_layout(const rvf_mesh* m, bool spatial, const Layout** out) {
+    Layout& L = m->lay[spatial ? 1 : 0];
+    if (!L.built) {
+        int64_t N = m->N, NS = m->NS;
+        std::vector<int32_t> order(N);
+        for (int64_t e = 0; e < N; e++) order[e] = (int32_t)e;
+        if (spatial) {
+            double lo[3] = {1e300, 1e300, 1e300}, hi[3] = {-1e300, -1e300, -1e300};
+            for (int64_t k = 0; k < m->Nn; k++)
+                for (int c = 0; c < 3; c++) { lo[c] = std::min(lo[c], m->hcoords[3 * k + c]); hi[c] = std::max(hi[c], m->hcoords[3 * k + c]); }
+            std::vector<uint64_t> key(N);
+            for (int64_t e = 0; e < N; e++) {
+                int nc = (m->fam[e] == 0 || m->fam[e] == 2) ? 4 : 3;
+                uint32_t q[3];
+                for (int c = 0; c < 3; c++) {
+                    double x = 0;
+                    for (int a = 0; a < nc; a++) x += m->hcoords[3 * (int64_t)(m->nodes8[8 * e + a] - 1) + c];
+                    x /= nc;
+                    double u = hi[c] > lo[c] ? (x - lo[c]) / (hi[c] - lo[c]) : 0.0;
+                    q[c] = (uint32_t)std::min(1023.0, std::max(0.0, u * 1024.0));
+                }
+                uint64_t morton = spread10(q[0]) | (spread10(q[1]) << 1) | (spread10(q[2]) << 2);
+                key[e] = ((uint64_t)m->grp_rank[e] << 32) | morton;
+            }
+            std::stable_sort(order.begin(), order.end(), [&](int32_t a, int32_t b) { return key[a] < key[b]; });
+        }
+        L.slot_orig.assign(NS, -1);
+        L.orig_slot.assign(N, 0);
+        L.slot_grp.assign(NS, 0);
+        for (int64_t k = 0; k < N; k++) {
+            L.slot_orig[k] = order[k];
+            L.orig_slot[order[k]] = (int32_t)k;
+            L.slot_grp[k] = m->grp_rank[order[k]];
+        }
+        // group-pure column chunks of <= RVF_CHUNK slots
+        L.chunk_of.assign(NS, 0);
+        L.chunks.clear();
+        for (int64_t e = 0; e < N;) {
+            int64_t b = e;
+            int g = L.slot_grp[e];
+            while (e < N && e - b < RVF_CHUNK && L.slot_grp[e] == g) e++;
+            Chunk c; c.begin = (int32_t)b; c.end = (int32_t)e; c.grp = g; c.pad = 0;
+            for (int64_t k = b; k < e; k++) L.chunk_of[k] = (int32_t)L.chunks.size();
+            L.chunks.push_back(c);
+        }
+        // tiles (strip s, chunk c) exist iff 64 s < last(c)
+        L.tiles.clear();
+        for (size_t c = 0; c < L.chunks.size(); c++) {
+            int64_t last = L.chunks[c].end - 1;
+            for (int64_t s = 0; s * RVF_STRIP < last; s++) L.tiles.push_back(make_int2((int)s, (int)c));
+        }
+        int rc;
+        if ((rc = upload(L.d_slot_orig, L.slot_orig.data(), sizeof(int32_t) * NS)) ||
+            (rc = upload(L.d_slot_grp, L.slot_grp.data(), sizeof(int32_t) * NS)) ||
+            (rc = upload(L.d_chunks, L.chunks.data(), sizeof(Chunk) * L.chunks.size())) ||
+            (rc = upload(L.d_chunk_of, L.chunk_of.data(), sizeof(int32_t) * NS)) ||
+            (rc = upload(L.d_tiles, L.tiles.data(), sizeof(int2) * L.tiles.size())))
+            return rc;
+        L.built = true;
+    }
+    *out = &L;
+    return 0;
+}
 
 int rvf_mesh_upload(rvf_ctx* ctx, const double* coords, int64_t n_nodes, int64_t n_elems, const int8_t* family,
                     const int32_t* elem_node_idx, const int32_t* elem_group, const int32_t* const nodes[4],
@@ -229,23 +317,8 @@ int rvf_mesh_upload(rvf_ctx* ctx, const double* coords, int64_t n_nodes, int64_t
     m->grp_rank.resize(n_elems);
     for (int64_t e = 0; e < n_elems; e++)
         m->grp_rank[e] = (int32_t)(std::lower_bound(t.begin(), t.end(), elem_group[e]) - t.begin());
-    // plan: strips of 64, group-pure column chunks of <= 16
     m->NS = (int)(((n_elems + RVF_STRIP - 1) / RVF_STRIP) * RVF_STRIP);
-    m->chunk_of.resize(n_elems);
-    for (int64_t e = 0; e < n_elems;) {
-        int64_t b = e;
-        int g = m->grp_rank[e];
-        while (e < n_elems && e - b < RVF_CHUNK && m->grp_rank[e] == g) e++;
-        Chunk c; c.begin = (int32_t)b; c.end = (int32_t)e; c.grp = g; c.pad = 0;
-        for (int64_t k = b; k < e; k++) m->chunk_of[k] = (int32_t)m->chunks.size();
-        m->chunks.push_back(c);
-    }
-    // tiles (strip s, chunk c) exist iff 64 s < last(c); heavy-first order is
-    // irrelevant for uniform work, so order by chunk then strip.
-    for (size_t c = 0; c < m->chunks.size(); c++) {
-        int64_t last = m->chunks[c].end - 1;
-        for (int64_t s = 0; s * RVF_STRIP < last; s++) m->tiles.push_back(make_int2((int)s, (int)c));
-    }
+    m->hcoords.assign(coords, coords + 3 * n_nodes);
     int64_t nb = m->NS / RVF_STRIP;
     for (int64_t bj = 0; bj < nb; bj++)
         for (int64_t bi = 0; bi <= bj; bi++) m->fill_tiles.push_back(make_int2((int)bi, (int)bj));
@@ -254,9 +327,6 @@ int rvf_mesh_upload(rvf_ctx* ctx, const double* coords, int64_t n_nodes, int64_t
         (rc = upload(m->d_fam, m->fam.data(), n_elems)) ||
         (rc = upload(m->d_nodes8, m->nodes8.data(), sizeof(int32_t) * 8 * n_elems)) ||
         (rc = upload(m->d_grp, m->grp_rank.data(), sizeof(int32_t) * n_elems)) ||
-        (rc = upload(m->d_chunks, m->chunks.data(), sizeof(Chunk) * m->chunks.size())) ||
-        (rc = upload(m->d_chunk_of, m->chunk_of.data(), sizeof(int32_t) * n_elems)) ||
-        (rc = upload(m->d_tiles, m->tiles.data(), sizeof(int2) * m->tiles.size())) ||
         (rc = upload(m->d_fill_tiles, m->fill_tiles.data(), sizeof(int2) * m->fill_tiles.size()))) {
         delete m;
         return rc;
@@ -380,7 +450,7 @@ struct rvf_obs {
     std::vector<int32_t> root;   // G*G absolute node index or -1
     int64_t n_nodes = 0, n_tris = 0;
     float Sm = 1.0f;
-    DBuf d_root, d_n64, d_t64, d_n32, d_t32;
+    DBuf d_root, d_n64, d_t64, d_n32, d_t32, d_parent;
 };
 
 static float f32_down(double x) {
@@ -441,6 +511,17 @@ int rvf_obs_build(rvf_ctx* ctx, const rvf_mesh* mesh, const double* soup, const 
             tr.idx = bd.idx; tr.nodes = bd.nodes;
             tr.miss.assign(tr.nodes.size(), 0);
             assign_miss(tr, 1, 0);
+            {   // the wave-cooperative traversal keeps its stack in one entry per lane
+                std::vector<int> depth(tr.nodes.size(), 0);
+                int dmax = 0;
+                for (size_t k = 0; k < tr.nodes.size(); k++)
+                    if (tr.nodes[k].left) {
+                        depth[tr.nodes[k].left - 1] = depth[k] + 1;
+                        depth[tr.nodes[k].right - 1] = depth[k] + 1;
+                        dmax = std::max(dmax, depth[k] + 1);
+                    }
+                if (dmax + 2 > 64) { delete o; return fail(RVF_E_ARG, "rvf_obs_build: BVH deeper than 62 levels"); }
+            }
             tr.node_base = o->n_nodes; tr.tri_base = o->n_tris;
             o->n_nodes += (int64_t)tr.nodes.size();
             o->n_tris += nt;
@@ -454,6 +535,8 @@ int rvf_obs_build(rvf_ctx* ctx, const rvf_mesh* mesh, const double* soup, const 
     std::vector<BvhNode32> n32(std::max<int64_t>(o->n_nodes, 1));
     std::vector<Tri64> t64(std::max<int64_t>(o->n_tris, 1));
     std::vector<Tri32> t32(std::max<int64_t>(o->n_tris, 1));
+    std::vector<int32_t> parent(std::max<int64_t>(o->n_nodes, 1), -1);
+    if (o->n_nodes >= ((int64_t)1 << 27)) { delete o; return fail(RVF_E_ARG, "rvf_obs_build: too many nodes"); }
     const double u = 5.9604644775390625e-08;
     const float pad = (float)(128.0 * u * Sm);
     for (const Tree& tr : o->trees) {
@@ -464,11 +547,15 @@ int rvf_obs_build(rvf_ctx* ctx, const rvf_mesh* mesh, const double* soup, const 
             d.miss = tr.miss[k] ? (int32_t)(tr.node_base + tr.miss[k] - 1) : -1;
             d.start = h.count ? (int32_t)(tr.tri_base + h.start - 1) : 0;
             d.count = h.count;
-            d.pad = 0;
+            d.pad = h.left ? (int32_t)(tr.node_base + h.right - 1) : 0;  // right child
             BvhNode32& f = n32[tr.node_base + k];
             for (int c = 0; c < 3; c++) { f.lo[c] = f32_down(h.lo[c]) - pad; f.hi[c] = f32_up(h.hi[c]) + pad; }
             f.miss = d.miss;
-            f.start_count = (d.start << 4) | d.count;
+            f.start_count = h.count ? ((d.start << 4) | d.count) : (d.pad << 4);
+            if (h.left) {
+                parent[tr.node_base + h.left - 1] = (int32_t)(tr.node_base + k);
+                parent[tr.node_base + h.right - 1] = (int32_t)(tr.node_base + k);
+            }
             if (h.left && h.left != (int)k + 2) { delete o; return fail(RVF_E_STATE, "bvh: non-preorder layout"); }
         }
         int64_t nt = (int64_t)tr.idx.size();
@@ -498,7 +585,8 @@ int rvf_obs_build(rvf_ctx* ctx, const rvf_mesh* mesh, const double* soup, const 
         (rc = upload(o->d_n64, n64.data(), sizeof(BvhNode64) * n64.size())) ||
         (rc = upload(o->d_n32, n32.data(), sizeof(BvhNode32) * n32.size())) ||
         (rc = upload(o->d_t64, t64.data(), sizeof(Tri64) * t64.size())) ||
-        (rc = upload(o->d_t32, t32.data(), sizeof(Tri32) * t32.size()))) {
+        (rc = upload(o->d_t32, t32.data(), sizeof(Tri32) * t32.size())) ||
+        (rc = upload(o->d_parent, parent.data(), sizeof(int32_t) * parent.size()))) {
         delete o;
         return rc;
     }
@@ -566,6 +654,9 @@ struct rvf_plan {
     rvf_opts opts{};
     int nq_quad = 0, nq_tri = 0, maxpts = 0;
     DBuf rule, P64, Q64, P32, Q32, anchor, area, elem_np, bad, F, part, partT, rowsum, W, audit_c, audit_h, counters;
+    DBuf Tseg, Wseg, area_orig;
+    hipGraphExec_t gexec = nullptr;
+    const Layout* L = nullptr;
     // Duffy singular pairs
     int npairs = 0;
     DBuf d_pairs, d_gl01, d_dval, d_rptr, d_rlist;
@@ -584,6 +675,7 @@ int rvf_plan_create(rvf_ctx* ctx, const rvf_mesh* mesh, const rvf_obs* obs, cons
     HIPCHK(hipSetDevice(ctx->device));
     rvf_plan* p = new rvf_plan();
     p->ctx = ctx; p->mesh = mesh; p->obs = obs; p->opts = *opts;
+    int rc0;
     std::vector<double> rule(3 * 64 + 39, 0.0);
     p->nq_quad = quad_rule(opts->nquad, rule.data(), rule.data() + 2 * 64);
     p->nq_tri = tri_rule(opts->nquad, rule.data() + 3 * 64, rule.data() + 3 * 64 + 26);
@@ -593,7 +685,10 @@ int rvf_plan_create(rvf_ctx* ctx, const rvf_mesh* mesh, const rvf_obs* obs, cons
     if (p->maxpts == 0) p->maxpts = 1;
     int64_t N = mesh->N, NS = mesh->NS;
     int G = (int)mesh->tags.size();
-    size_t nch = mesh->chunks.size(), nstrip = NS / RVF_STRIP;
+    // spatially compact strips for occluded quadrature solves (coherent ray bundles)
+    bool spatial = obs && !obs->trees.empty() && opts->method == RVF_METHOD_QUAD;
+    if ((rc0 = get_layout(mesh, spatial, &p->L))) { delete p; return rc0; }
+    size_t nch = p->L->chunks.size(), nstrip = NS / RVF_STRIP;
     int rc;
     if ((rc = upload(p->rule, rule.data(), sizeof(double) * rule.size())) ||
         (rc = p->P64.alloc(sizeof(double) * 8 * p->maxpts * NS)) ||
@@ -601,11 +696,14 @@ int rvf_plan_create(rvf_ctx* ctx, const rvf_mesh* mesh, const rvf_obs* obs, cons
         (rc = p->P32.alloc(sizeof(float) * 8 * p->maxpts * NS)) ||
         (rc = p->Q32.alloc(sizeof(float) * 8 * p->maxpts * NS)) ||
         (rc = p->anchor.alloc(sizeof(double) * 4 * NS)) || (rc = p->area.alloc(sizeof(double) * NS)) ||
+        (rc = p->area_orig.alloc(sizeof(double) * std::max<int64_t>(N, 1))) ||
         (rc = p->elem_np.alloc(sizeof(int32_t) * NS)) || (rc = p->bad.alloc(sizeof(int))) ||
         (rc = p->part.alloc(sizeof(double) * std::max<size_t>(nch, 1) * NS)) ||
         (rc = p->partT.alloc(sizeof(double) * std::max<size_t>(nstrip, 1) * NS)) ||
         (rc = p->rowsum.alloc(sizeof(double) * NS)) || (rc = p->W.alloc(sizeof(double) * std::max(G, 1) * NS)) ||
-        (rc = p->counters.alloc(sizeof(unsigned long long) * 4))) {
+        (rc = p->counters.alloc(sizeof(unsigned long long) * 4)) ||
+        (rc = p->Tseg.alloc(sizeof(double) * RVF_RSEG * NS)) ||
+        (rc = p->Wseg.alloc(sizeof(double) * RVF_RSEG * std::max(G, 1) * NS))) {
         delete p;
         return rc;
     }
@@ -655,9 +753,9 @@ int rvf_plan_create(rvf_ctx* ctx, const rvf_mesh* mesh, const rvf_obs* obs, cons
         int nq = gl1d(opts->nquad, p1, w1);
         for (int k = 0; k < nq; k++) { g01[k] = (p1[k] + 1.0) / 2.0; g01[8 + k] = w1[k] / 2.0; }
         if ((rc = upload(p->d_pairs, pairs.data(), sizeof(DuffyPair) * pairs.size())) ||
-            (rc = upload(p->d_gl01, g01, sizeof g01)) || (rc = p->d_dval.alloc(sizeof(double) * std::max<size_t>(pairs.size(), 1))) ||
+            (rc = upload(p->d_gl01, g01, sizeof g01)) || (rc = p->d_dval.alloc(sizeof(double) * pairs.size())) ||
             (rc = upload(p->d_rptr, rptr.data(), sizeof(int32_t) * rptr.size())) ||
-            (rc = upload(p->d_rlist, rlist.data(), sizeof(int32_t) * std::max<size_t>(rlist.size(), 1)))) {
+            (rc = upload(p->d_rlist, rlist.data(), sizeof(int32_t) * rlist.size()))) {
             delete p;
             return rc;
         }
@@ -668,19 +766,19 @@ int rvf_plan_create(rvf_ctx* ctx, const rvf_mesh* mesh, const rvf_obs* obs, cons
             delete p;
             return rc;
         }
-        HIPCHK(hipMemset(p->audit_c.p, 0, p->audit_c.n));
-        HIPCHK(hipMemset(p->audit_h.p, 0, p->audit_h.n));
+        HIPCHK(hipMemsetAsync(p->audit_c.p, 0, p->audit_c.n, ctx->stream));
+        HIPCHK(hipMemsetAsync(p->audit_h.p, 0, p->audit_h.n, ctx->stream));
     }
     // zero point arrays once (padding lanes / points read but never accumulated)
-    HIPCHK(hipMemset(p->P64.p, 0, p->P64.n));
-    HIPCHK(hipMemset(p->P32.p, 0, p->P32.n));
-    HIPCHK(hipMemset(p->anchor.p, 0, p->anchor.n));
-    HIPCHK(hipMemset(p->area.p, 0, p->area.n));
-    HIPCHK(hipMemset(p->elem_np.p, 0, p->elem_np.n));
+    HIPCHK(hipMemsetAsync(p->P64.p, 0, p->P64.n, ctx->stream));
+    HIPCHK(hipMemsetAsync(p->P32.p, 0, p->P32.n, ctx->stream));
+    HIPCHK(hipMemsetAsync(p->anchor.p, 0, p->anchor.n, ctx->stream));
+    HIPCHK(hipMemsetAsync(p->area.p, 0, p->area.n, ctx->stream));
+    HIPCHK(hipMemsetAsync(p->elem_np.p, 0, p->elem_np.n, ctx->stream));
     for (int k = 0; k < 8; k++) HIPCHK(hipEventCreate(&p->ev[k]));
     p->st.n_elems = N;
     p->st.unique_pairs = N * (N - 1) / 2;
-    p->st.tiles = (int64_t)mesh->tiles.size();
+    p->st.tiles = (int64_t)p->L->tiles.size();
     *out = p;
     return 0;
 }
@@ -688,10 +786,12 @@ int rvf_plan_create(rvf_ctx* ctx, const rvf_mesh* mesh, const rvf_obs* obs, cons
 static PairArgs pair_args(rvf_plan* p) {
     const rvf_mesh* m = p->mesh;
     PairArgs a{};
-    a.tiles = m->d_tiles.as<int2>(); a.ntiles = (int)m->tiles.size();
+    const Layout* L = p->L;
+    a.tiles = L->d_tiles.as<int2>(); a.ntiles = (int)L->tiles.size();
+    a.slot_orig = L->d_slot_orig.as<int32_t>();
     a.NS = m->NS; a.maxpts = p->maxpts; a.N = m->N;
-    a.chunks = m->d_chunks.as<Chunk>();
-    a.elem_np = p->elem_np.as<int32_t>(); a.elem_grp = m->d_grp.as<int32_t>(); a.area = p->area.as<double>();
+    a.chunks = L->d_chunks.as<Chunk>();
+    a.elem_np = p->elem_np.as<int32_t>(); a.elem_grp = L->d_slot_grp.as<int32_t>(); a.area = p->area.as<double>();
     a.P64 = p->P64.as<double>(); a.Q64 = p->Q64.as<double>(); a.P32 = p->P32.as<float>(); a.Q32 = p->Q32.as<float>();
     a.anchor = p->anchor.as<double>();
     a.F = p->F.as<double>(); a.part = p->part.as<double>(); a.partT = p->partT.as<double>();
@@ -700,6 +800,8 @@ static PairArgs pair_args(rvf_plan* p) {
         a.tree_root = p->obs->d_root.as<int32_t>();
         a.nodes64 = p->obs->d_n64.as<BvhNode64>(); a.tris64 = p->obs->d_t64.as<Tri64>();
         a.nodes32 = p->obs->d_n32.as<BvhNode32>(); a.tris32 = p->obs->d_t32.as<Tri32>();
+        a.parent = p->obs->d_parent.as<int32_t>();
+        a.verify = p->obs->soup_read == RVF_SOUP_REFERENCE ? 1 : 0;
     }
     a.audit_count = p->audit_c.as<uint32_t>(); a.audit_hash = p->audit_h.as<uint32_t>();
     a.counters = p->counters.as<unsigned long long>();
@@ -711,16 +813,15 @@ static PairArgs pair_args(rvf_plan* p) {
     return a;
 }
 
-int rvf_plan_run(rvf_plan* p) {
-    if (!p) return fail(RVF_E_ARG, "null plan");
-    rvf_ctx* c = p->ctx;
+// Enqueue one solve on stream st.  With tim, HIP events bracket each stage
+// (not used while capturing a graph).  Zero fills use a kernel, never
+// hipMemsetAsync, so the sequence can be captured and replayed as a graph.
+static int enqueue_solve(rvf_plan* p, hipStream_t st, bool tim) {
     const rvf_mesh* m = p->mesh;
-    hipStream_t st = c->stream;
-    HIPCHK(hipSetDevice(c->device));
-    bool tim = (p->opts.flags & RVF_TIMING) != 0;
     if (tim) HIPCHK(hipEventRecord(p->ev[0], st));
-    HIPCHK(hipMemsetAsync(p->bad.p, 0, sizeof(int), st));
-    HIPCHK(hipMemsetAsync(p->counters.p, 0, p->counters.n, st));
+    int rc;
+    if ((rc = rvf_launch_zero(p->bad.p, sizeof(int), st)) || (rc = rvf_launch_zero(p->counters.p, p->counters.n, st)))
+        return fail(RVF_E_HIP, std::string("zero launch: ") + hipGetErrorString((hipError_t)rc));
     PrecompArgs pa{};
     pa.coords = m->coords.as<double>(); pa.fam = m->d_fam.as<int8_t>(); pa.nodes8 = m->d_nodes8.as<int32_t>();
     pa.N = m->N; pa.NS = m->NS; pa.maxpts = p->maxpts; pa.nq_quad = p->nq_quad; pa.nq_tri = p->nq_tri;
@@ -728,7 +829,9 @@ int rvf_plan_run(rvf_plan* p) {
     pa.P64 = p->P64.as<double>(); pa.Q64 = p->Q64.as<double>(); pa.P32 = p->P32.as<float>(); pa.Q32 = p->Q32.as<float>();
     pa.anchor = p->anchor.as<double>(); pa.area = p->area.as<double>(); pa.elem_np = p->elem_np.as<int32_t>();
     pa.bad = p->bad.as<int>();
-    int rc = rvf_launch_precompute(&pa, st);
+    pa.slot_orig = p->L->d_slot_orig.as<int32_t>();
+    pa.area_orig = p->area_orig.as<double>();
+    rc = rvf_launch_precompute(&pa, st);
     if (rc) return fail(RVF_E_HIP, std::string("precompute launch: ") + hipGetErrorString((hipError_t)rc));
     if (tim) HIPCHK(hipEventRecord(p->ev[1], st));
     PairArgs a = pair_args(p);
@@ -739,7 +842,7 @@ int rvf_plan_run(rvf_plan* p) {
         rc = rvf_launch_pair(&a, p->opts.precision == RVF_MIXED, occ, pc, p->obs ? p->obs->Sm : 1.0f, st);
         if (rc) return fail(RVF_E_HIP, std::string("pair launch: ") + hipGetErrorString((hipError_t)rc));
     } else {
-        rc = rvf_launch_mc(&a, p->area.as<double>(), st);
+        rc = rvf_launch_mc(&a, p->area.as<double>(), p->area_orig.as<double>(), st);
         if (rc) return fail(RVF_E_HIP, std::string("mc launch: ") + hipGetErrorString((hipError_t)rc));
     }
     if (tim) HIPCHK(hipEventRecord(p->ev[2], st));
@@ -749,10 +852,11 @@ int rvf_plan_run(rvf_plan* p) {
         if (rc) return fail(RVF_E_HIP, std::string("duffy launch: ") + hipGetErrorString((hipError_t)rc));
     }
     if (tim) HIPCHK(hipEventRecord(p->ev[3], st));
-    HIPCHK(hipMemsetAsync(p->W.p, 0, p->W.n, st));
-    rc = rvf_launch_reduce(p->part.as<double>(), p->partT.as<double>(), m->d_chunks.as<Chunk>(),
-                           (int)m->chunks.size(), m->d_chunk_of.as<int32_t>(), m->N, m->NS, (int)m->tags.size(),
-                           p->rowsum.as<double>(), p->W.as<double>(), st);
+    if ((rc = rvf_launch_zero(p->Wseg.p, p->Wseg.n, st)))
+        return fail(RVF_E_HIP, std::string("zero launch: ") + hipGetErrorString((hipError_t)rc));
+    rc = rvf_launch_reduce(p->part.as<double>(), p->partT.as<double>(), p->L->d_chunks.as<Chunk>(),
+                           (int)p->L->chunks.size(), p->L->d_chunk_of.as<int32_t>(), m->N, m->NS, (int)m->tags.size(),
+                           p->rowsum.as<double>(), p->W.as<double>(), p->Tseg.as<double>(), p->Wseg.as<double>(), st);
     if (rc) return fail(RVF_E_HIP, std::string("reduce launch: ") + hipGetErrorString((hipError_t)rc));
     if (p->npairs) {
         rc = rvf_launch_duffy(&a, p->d_pairs.as<DuffyPair>(), p->npairs, nullptr, 0, p->d_dval.as<double>(),
@@ -762,11 +866,43 @@ int rvf_plan_run(rvf_plan* p) {
     }
     if (tim) HIPCHK(hipEventRecord(p->ev[4], st));
     if (p->F.p) {
-        rc = rvf_launch_fill(p->F.as<double>(), p->area.as<double>(), m->N, m->d_fill_tiles.as<int2>(),
+        rc = rvf_launch_fill(p->F.as<double>(), p->area_orig.as<double>(), m->N, m->d_fill_tiles.as<int2>(),
                              (int)m->fill_tiles.size(), st);
         if (rc) return fail(RVF_E_HIP, std::string("fill launch: ") + hipGetErrorString((hipError_t)rc));
     }
     if (tim) HIPCHK(hipEventRecord(p->ev[5], st));
+    return 0;
+}
+
+int rvf_plan_run(rvf_plan* p) {
+    if (!p) return fail(RVF_E_ARG, "null plan");
+    rvf_ctx* c = p->ctx;
+    hipStream_t st = c->stream;
+    HIPCHK(hipSetDevice(c->device));
+    bool tim = (p->opts.flags & RVF_TIMING) != 0;
+    if (p->opts.flags & RVF_GRAPH) {
+        // steady-state solves: capture the launch sequence once, replay it
+        if (!p->gexec) {
+            hipGraph_t g;
+            HIPCHK(hipStreamBeginCapture(st, hipStreamCaptureModeThreadLocal));
+            int rc = enqueue_solve(p, st, false);
+            hipError_t e = hipStreamEndCapture(st, &g);
+            if (rc) return rc;
+            if (e != hipSuccess) return fail(RVF_E_HIP, std::string("hipStreamEndCapture: ") + hipGetErrorString(e));
+            e = hipGraphInstantiate(&p->gexec, g, nullptr, nullptr, 0);
+            (void)hipGraphDestroy(g);
+            if (e != hipSuccess) { p->gexec = nullptr; return fail(RVF_E_HIP, std::string("hipGraphInstantiate: ") + hipGetErrorString(e)); }
+        }
+        if (tim) HIPCHK(hipEventRecord(p->ev[0], st));
+        HIPCHK(hipGraphLaunch(p->gexec, st));
+        if (tim) {
+            // per-stage events are not recorded inside the graph: the whole solve is timed
+            for (int k = 1; k <= 5; k++) HIPCHK(hipEventRecord(p->ev[k], st));
+        }
+    } else {
+        int rc = enqueue_solve(p, st, tim);
+        if (rc) return rc;
+    }
     p->ran = true;
     return 0;
 }
@@ -807,22 +943,29 @@ int rvf_plan_fetch(rvf_plan* p, double* F, double* A, double* rowsum, double* F_
         if (!p->F.p) return fail(RVF_E_STATE, "F was not requested (RVF_WANT_F)");
         HIPCHK(hipMemcpy(F, p->F.p, sizeof(double) * N * N, hipMemcpyDeviceToHost));
     }
-    std::vector<double> hA(N), hW((size_t)G * m->NS);
-    HIPCHK(hipMemcpy(hA.data(), p->area.p, sizeof(double) * N, hipMemcpyDeviceToHost));
-    if (A) std::memcpy(A, hA.data(), sizeof(double) * N);
-    if (rowsum) HIPCHK(hipMemcpy(rowsum, p->rowsum.p, sizeof(double) * N, hipMemcpyDeviceToHost));
+    const Layout* L = p->L;
+    std::vector<double> hA(N), hAo(N), hW((size_t)G * m->NS);
+    HIPCHK(hipMemcpy(hA.data(), p->area.p, sizeof(double) * N, hipMemcpyDeviceToHost));      // slot order
+    HIPCHK(hipMemcpy(hAo.data(), p->area_orig.p, sizeof(double) * N, hipMemcpyDeviceToHost)); // element order
+    if (A) std::memcpy(A, hAo.data(), sizeof(double) * N);
+    if (rowsum) {
+        std::vector<double> rs(N);
+        HIPCHK(hipMemcpy(rs.data(), p->rowsum.p, sizeof(double) * N, hipMemcpyDeviceToHost));
+        for (int64_t k = 0; k < N; k++) rowsum[L->slot_orig[k]] = rs[k];
+    }
     if (F_group || A_group) {
         HIPCHK(hipMemcpy(hW.data(), p->W.p, sizeof(double) * G * m->NS, hipMemcpyDeviceToHost));
-        // A_g F_gh = sum_{i in g} A_i sum_{j in h} F_ij (Results.jl:56-82), with the
-        // j < i part from the symmetric raw integral: sum_{j in h} A_j W[g][j]
+        // A_g F_gh = sum_{i in g} A_i sum_{j in h} F_ij (Results.jl:56-82); with W[h][k] =
+        // sum over slots l > k in group h of F, the pairs below the slot diagonal
+        // come from the symmetric raw integral: sum_{l in h} A_l W[g][l]
         std::vector<double> Ag(G, 0.0), R((size_t)G * G, 0.0);
-        for (int64_t i = 0; i < N; i++) Ag[m->grp_rank[i]] += hA[i];
-        for (int64_t i = 0; i < N; i++) {
-            int g = m->grp_rank[i];
+        for (int64_t i = 0; i < N; i++) Ag[m->grp_rank[i]] += hAo[i];
+        for (int64_t k = 0; k < N; k++) {
+            int g = L->slot_grp[k];
             for (int h = 0; h < G; h++) {
-                double v = hA[i] * hW[(size_t)h * m->NS + i];
-                R[(size_t)g * G + h] += v;   // pairs (i in g, j in h, j > i)
-                R[(size_t)h * G + g] += v;   // the same pairs seen from row j
+                double v = hA[k] * hW[(size_t)h * m->NS + k];
+                R[(size_t)g * G + h] += v;
+                R[(size_t)h * G + g] += v;
             }
         }
         if (A_group) for (int g = 0; g < G; g++) A_group[g] = Ag[g];
@@ -847,12 +990,13 @@ int rvf_plan_stats(const rvf_plan* p, rvf_stats* st) {
 
 int rvf_plan_device_ptr(const rvf_plan* p, int which, void** ptr) {
     if (!p || !ptr) return fail(RVF_E_ARG, "null plan");
-    *ptr = which == 0 ? p->F.p : which == 1 ? p->area.p : which == 2 ? p->rowsum.p : nullptr;
+    *ptr = which == 0 ? p->F.p : which == 1 ? p->area_orig.p : which == 2 ? p->rowsum.p : nullptr;
     return 0;
 }
 
 void rvf_plan_free(rvf_plan* p) {
     if (!p) return;
+    if (p->gexec) (void)hipGraphExecDestroy(p->gexec);
     for (int k = 0; k < 8; k++) if (p->ev[k]) (void)hipEventDestroy(p->ev[k]);
     delete p;
 }
@@ -910,6 +1054,24 @@ int rvf_quad_rule(int32_t nquad, int32_t tri, double* pts, double* wts) {
     int n = tri ? tri_rule(nquad, pts, wts) : quad_rule(nquad, pts, wts);
     if (n < 0) return fail(RVF_E_ARG, "nquad out of range");
     return n;
+}
+
+int rvf_copy_probe(rvf_ctx* c, int64_t bytes, int32_t reps, float* ms) {
+    if (!c || bytes < 16 || reps < 1) return fail(RVF_E_ARG, "rvf_copy_probe: bad arguments");
+    HIPCHK(hipSetDevice(c->device));
+    DBuf a, b;
+    int rc;
+    if ((rc = a.alloc(bytes)) || (rc = b.alloc(bytes))) return rc;
+    if ((rc = rvf_launch_zero(a.p, bytes, c->stream))) return fail(RVF_E_HIP, "zero launch failed");
+    HIPCHK(hipEventRecord(c->ev[14], c->stream));
+    for (int k = 0; k < reps; k++)
+        if ((rc = rvf_launch_copy(a.p, b.p, bytes, c->stream))) return fail(RVF_E_HIP, "copy launch failed");
+    HIPCHK(hipEventRecord(c->ev[15], c->stream));
+    HIPCHK(hipEventSynchronize(c->ev[15]));
+    float t = 0;
+    HIPCHK(hipEventElapsedTime(&t, c->ev[14], c->ev[15]));
+    if (ms) *ms = t / reps;
+    return 0;
 }
 
 int rvf_ctx_event_record(rvf_ctx* c, int32_t slot) {

@@ -20,6 +20,8 @@
 // undecided cases are re-tested in FP64 with the reference arithmetic.
 #include "rvf_common.h"
 
+#include <algorithm>
+
 #define PI_D 3.141592653589793
 
 // ---------------------------------------------------------------------------
@@ -126,14 +128,15 @@ __device__ __forceinline__ void eval_point(int fam, const double* __restrict__ c
 
 
 __global__ __launch_bounds__(64) void k_precompute(PrecompArgs a) {
-    int64_t e = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
+    int64_t e = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;  // slot
     if (e >= a.N) return;
-    int f = a.fam[e];
+    const int64_t o = a.slot_orig[e];                             // original element
+    int f = a.fam[o];
     int isq = (f == 0 || f == 2);
     int np = isq ? a.nq_quad : a.nq_tri;
     const double* pts = isq ? a.rule : a.rule + 3 * 64;
     const double* wts = isq ? a.rule + 2 * 64 : a.rule + 3 * 64 + 26;
-    const int32_t* nd = a.nodes8 + 8 * e;
+    const int32_t* nd = a.nodes8 + 8 * o;
     double A = 0.0, cx = 0.0, cy = 0.0, cz = 0.0;
     for (int p = 0; p < np; p++) {
         double x[3], n[3], dA;
@@ -149,6 +152,7 @@ __global__ __launch_bounds__(64) void k_precompute(PrecompArgs a) {
         cx += x[0]; cy += x[1]; cz += x[2];
     }
     a.area[e] = A;
+    a.area_orig[o] = A;
     a.elem_np[e] = np;
     double ax = cx / np, ay = cy / np, az = cz / np;
     a.anchor[e * 4 + 0] = ax; a.anchor[e * 4 + 1] = ay; a.anchor[e * 4 + 2] = az; a.anchor[e * 4 + 3] = 0.0;
@@ -245,12 +249,33 @@ __device__ bool blocked64(const BvhNode64* __restrict__ nodes, const Tri64* __re
     return false;
 }
 
+// One exact slab test (BVH.jl:155-169 semantics; a NaN slab value is a miss).
+__device__ __forceinline__ bool slab64(const BvhNode64& nd, const Ray64& R, double ix, double iy, double iz) {
+    double t1x = (nd.lo[0] - R.ox) * ix, t2x = (nd.hi[0] - R.ox) * ix;
+    double t1y = (nd.lo[1] - R.oy) * iy, t2y = (nd.hi[1] - R.oy) * iy;
+    double t1z = (nd.lo[2] - R.oz) * iz, t2z = (nd.hi[2] - R.oz) * iz;
+    bool nan = (t1x != t1x) | (t2x != t2x) | (t1y != t1y) | (t2y != t2y) | (t1z != t1z) | (t2z != t2z);
+    double tmin = fmax(fmax(fmax(fmin(t1x, t2x), fmin(t1y, t2y)), fmin(t1z, t2z)), 0.0);
+    double tmax = fmin(fmin(fmin(fmax(t1x, t2x), fmax(t1y, t2y)), fmax(t1z, t2z)), R.L);
+    return !nan && tmin <= tmax;
+}
+
+// Per-lane any-hit for every lane with need set; returns the lane mask of
+// blocked rays.  Each lane walks its own ray (independent dependent-load
+// chains keep many L2 requests in flight).
+__device__ __forceinline__ uint64_t occlusion64(const PairArgs& a, bool need, int root, const Ray64& R,
+                                                unsigned long long& naabb, unsigned long long& ntri) {
+    bool b = need && blocked64(a.nodes64, a.tris64, root, R, naabb, ntri);
+    return __ballot(b);
+}
+
 // ---------------------------------------------------------------------------
 // Pair tile prologue shared by all pair kernels
 // ---------------------------------------------------------------------------
 struct TileCtx {
     int s, chunk, lane, jb, je;
-    int64_t i;
+    int64_t i;   // slot of this lane's element
+    int64_t oi;  // its original index
     bool vi;
 };
 
@@ -261,6 +286,7 @@ __device__ __forceinline__ TileCtx tile_ctx(const PairArgs& a) {
     t.lane = threadIdx.x;
     t.i = (int64_t)t.s * RVF_STRIP + t.lane;
     t.vi = t.i < a.N;
+    t.oi = t.vi ? a.slot_orig[t.i] : 0;
     Chunk c = a.chunks[t.chunk];
     t.jb = c.begin; t.je = c.end;
     return t;
@@ -289,10 +315,10 @@ __device__ __forceinline__ bool shares_corner(const Corners& x, const Corners& y
 
 // Per-j epilogue: raw output, row partial (F_ij = raw/A_i) and the
 // transposed partial for row j (sum over lanes of raw/A_j).
-__device__ __forceinline__ void pair_out(const PairArgs& a, const TileCtx& t, int64_t j, bool act, double raw,
-                                         double Ai, double& ls, double& lc) {
+__device__ __forceinline__ void pair_out(const PairArgs& a, const TileCtx& t, int64_t j, int64_t oj, bool act,
+                                         double raw, double Ai, double& ls, double& lc) {
     if (act) {
-        if (a.F) a.F[t.i + a.N * j] = raw;
+        if (a.F) a.F[min(t.oi, oj) + a.N * max(t.oi, oj)] = raw;
         neumaier(ls, lc, raw / Ai);
     }
     double Aj = a.area[j];
@@ -307,7 +333,7 @@ __device__ __forceinline__ void pair_out(const PairArgs& a, const TileCtx& t, in
 template <int PC>
 __global__ __launch_bounds__(64) void k_pair_fp64(PairArgs a) {
     const TileCtx t = tile_ctx(a);
-    const Corners cwi = corners_of(a, t.i, t.vi && a.duffy);
+    const Corners cwi = corners_of(a, t.oi, t.vi && a.duffy);
     const int64_t NS = a.NS;
     const int npi = t.vi ? a.elem_np[t.i] : 0;
     int npmax = npi;
@@ -323,7 +349,8 @@ __global__ __launch_bounds__(64) void k_pair_fp64(PairArgs a) {
             continue;
         }
         bool act = t.vi && t.i < j;
-        if (a.duffy && shares_corner(cwi, corners_of(a, j, true))) act = false;
+        const int64_t oj = a.slot_orig[j];
+        if (a.duffy && shares_corner(cwi, corners_of(a, oj, true))) act = false;
         const int npj = a.elem_np[j];
         const double* __restrict__ qj = a.Q64 + j * a.maxpts * 8;
         double Fij = 0.0;
@@ -366,17 +393,24 @@ __global__ __launch_bounds__(64) void k_pair_fp64(PairArgs a) {
                 }
             }
         }
-        pair_out(a, t, j, act, Fij, Ai, ls, lc);
+        pair_out(a, t, j, oj, act, Fij, Ai, ls, lc);
     }
     a.part[(int64_t)t.chunk * NS + t.i] = ls + lc;
 }
 
 // ---------------------------------------------------------------------------
-// FP64, occluded: the reference's p-outer / q-inner loop, one ray per K != 0.
+// FP64, occluded.  The pair's outer element is the one with the lower
+// original index (the reference loops j > i, Assembly.jl:191-205): its points
+// are the ray origins and the outer sum runs over them.  When the lane's
+// element is the inner one ("rev", possible under the spatial slot layout),
+// the ray starts at the column element's point and the per-outer-point sums
+// are accumulated in LDS (accq[q][lane]) so the summation order is still the
+// reference's.  K is symmetric bit for bit, so one evaluation serves both.
 // ---------------------------------------------------------------------------
 __global__ __launch_bounds__(64) void k_pair_fp64_occ(PairArgs a) {
+    extern __shared__ double accq[];  // [maxpts][64]
     const TileCtx t = tile_ctx(a);
-    const Corners cwi = corners_of(a, t.i, t.vi && a.duffy);
+    const Corners cwi = corners_of(a, t.oi, t.vi && a.duffy);
     const int64_t NS = a.NS;
     const int npi = t.vi ? a.elem_np[t.i] : 0;
     int npmax = npi;
@@ -394,11 +428,14 @@ __global__ __launch_bounds__(64) void k_pair_fp64_occ(PairArgs a) {
             continue;
         }
         bool act = t.vi && t.i < j;
-        if (a.duffy && shares_corner(cwi, corners_of(a, j, true))) act = false;
+        const int64_t oj = a.slot_orig[j];
+        if (a.duffy && shares_corner(cwi, corners_of(a, oj, true))) act = false;
+        const bool rev = t.oi > oj;
         const int npj = a.elem_np[j];
         const int gj = a.elem_grp[j];
         const int root = a.tree_root[gi * a.G + gj];
         const double* __restrict__ qj = a.Q64 + j * a.maxpts * 8;
+        for (int q = 0; q < npj; q++) accq[q * 64 + t.lane] = 0.0;
         double Fij = 0.0;
         uint32_t bc = 0, bh = 0;
         for (int p = 0; p < npmax; p++) {
@@ -407,6 +444,7 @@ __global__ __launch_bounds__(64) void k_pair_fp64_occ(PairArgs a) {
             const double zi = P[(int64_t)(p * 8 + 2) * NS + t.i];
             const double nxi = P[(int64_t)(p * 8 + 3) * NS + t.i], nyi = P[(int64_t)(p * 8 + 4) * NS + t.i];
             const double nzi = P[(int64_t)(p * 8 + 5) * NS + t.i];
+            const double wi = P[(int64_t)(p * 8 + 6) * NS + t.i], dAi = P[(int64_t)(p * 8 + 7) * NS + t.i];
             double inner = 0.0;
             for (int q = 0; q < npj; q++) {
                 const double xj = qj[q * 8 + 0], yj = qj[q * 8 + 1], zj = qj[q * 8 + 2];
@@ -414,26 +452,31 @@ __global__ __launch_bounds__(64) void k_pair_fp64_occ(PairArgs a) {
                 const double wj = qj[q * 8 + 6], dAj = qj[q * 8 + 7];
                 Ray64 R;
                 double K = vf_exact(xi, yi, zi, nxi, nyi, nzi, xj, yj, zj, nxj, nyj, nzj, R);
-                if (pv && K != 0.0 && root >= 0) {
-                    nrays++;
-                    if (blocked64(a.nodes64, a.tris64, root, R, naabb, ntri)) {
-                        K = 0.0;
-                        bc++;
-                        bh += mix32((uint32_t)(p * npj + q) + 0x9e3779b9U);
-                    }
+                if (rev) {  // is_visible(x_outer = x_j, x_inner = x_i): d = x_i - x_j = -(x_j - x_i) exactly
+                    R.ox = xj; R.oy = yj; R.oz = zj;
+                    R.dx = -R.dx; R.dy = -R.dy; R.dz = -R.dz;
+                }
+                const bool need = pv && K != 0.0 && root >= 0;
+                nrays += need;
+                if ((occlusion64(a, need, root, R, naabb, ntri) >> t.lane) & 1) {
+                    K = 0.0;
+                    bc++;
+                    bh += mix32((uint32_t)(rev ? q * npi + p : p * npj + q) + 0x9e3779b9U);
                 }
                 inner += (wj * K) * dAj;
+                if (rev && p < npi) accq[q * 64 + t.lane] += (wi * K) * dAi;
             }
-            if (p < npi) {
-                double wi = P[(int64_t)(p * 8 + 6) * NS + t.i], dAi = P[(int64_t)(p * 8 + 7) * NS + t.i];
-                Fij += (wi * inner) * dAi;
-            }
+            if (p < npi) Fij += (wi * inner) * dAi;
+        }
+        if (rev) {
+            Fij = 0.0;
+            for (int q = 0; q < npj; q++) Fij += (qj[q * 8 + 6] * accq[q * 64 + t.lane]) * qj[q * 8 + 7];
         }
         if (act && a.audit_count) {
-            a.audit_count[t.i + a.N * j] = bc;
-            a.audit_hash[t.i + a.N * j] = bh;
+            a.audit_count[min(t.oi, oj) + a.N * max(t.oi, oj)] = bc;
+            a.audit_hash[min(t.oi, oj) + a.N * max(t.oi, oj)] = bh;
         }
-        pair_out(a, t, j, act, Fij, Ai, ls, lc);
+        pair_out(a, t, j, oj, act, Fij, Ai, ls, lc);
     }
     a.part[(int64_t)t.chunk * NS + t.i] = ls + lc;
     if (a.counters) {
@@ -469,7 +512,7 @@ __global__ __launch_bounds__(64) void k_pair_mixed(PairArgs a) {
     // registers for the whole tile; otherwise chunks of PC points are reloaded
     // per column element (L1/L2 hits).
     const TileCtx t = tile_ctx(a);
-    const Corners cwi = corners_of(a, t.i, t.vi && a.duffy);
+    const Corners cwi = corners_of(a, t.oi, t.vi && a.duffy);
     const int64_t NS = a.NS;
     const int npi = t.vi ? a.elem_np[t.i] : 0;
     int npmax = npi;
@@ -508,7 +551,8 @@ __global__ __launch_bounds__(64) void k_pair_mixed(PairArgs a) {
             continue;
         }
         bool act = t.vi && t.i < j;
-        if (a.duffy && shares_corner(cwi, corners_of(a, j, true))) act = false;
+        const int64_t oj = a.slot_orig[j];
+        if (a.duffy && shares_corner(cwi, corners_of(a, oj, true))) act = false;
         const int npj = a.elem_np[j];
         const float* __restrict__ qj = a.Q32 + j * a.maxpts * 8;
         const float dax = (float)(a.anchor[j * 4 + 0] - aix);
@@ -541,7 +585,7 @@ __global__ __launch_bounds__(64) void k_pair_mixed(PairArgs a) {
             for (int p = 0; p < PC; p++)
                 if (c0 + p < npmax) neumaier(Fs, Fc, WD[p] * (double)inner[p]);
         }
-        pair_out(a, t, j, act, (Fs + Fc) * (1.0 / PI_D), Ai, ls, lc);
+        pair_out(a, t, j, oj, act, (Fs + Fc) * (1.0 / PI_D), Ai, ls, lc);
     }
     a.part[(int64_t)t.chunk * NS + t.i] = ls + lc;
 }
@@ -587,46 +631,67 @@ __device__ __forceinline__ int mt_filter32(const Tri32& T, const Ray32& R, float
 }
 
 struct Exact64Ctx {
-    // lazily built exact FP64 ray for (i, p) -> (j, q)
+    // lazily built exact FP64 ray between (slot i, point p) and (slot j, point q)
     const double* Pi;  // P64 base
     int64_t NS, i;
     int p;
     const double* qjq; // Q64 row of (j, q)
+    bool rev;          // the column element is the outer one: the ray starts at (j, q)
 };
 
+// is_visible's ray (RayCast.jl:27-30) from the outer point to the inner one
 __device__ __forceinline__ void exact_ray(const Exact64Ctx& c, Ray64& R) {
     double xi = c.Pi[(int64_t)(c.p * 8 + 0) * c.NS + c.i];
     double yi = c.Pi[(int64_t)(c.p * 8 + 1) * c.NS + c.i];
     double zi = c.Pi[(int64_t)(c.p * 8 + 2) * c.NS + c.i];
-    double d0 = c.qjq[0] - xi, d1 = c.qjq[1] - yi, d2 = c.qjq[2] - zi;
+    double ox = c.rev ? c.qjq[0] : xi, oy = c.rev ? c.qjq[1] : yi, oz = c.rev ? c.qjq[2] : zi;
+    double tx = c.rev ? xi : c.qjq[0], ty = c.rev ? yi : c.qjq[1], tz = c.rev ? zi : c.qjq[2];
+    double d0 = tx - ox, d1 = ty - oy, d2 = tz - oz;
     double L = sqrt((d0 * d0 + d1 * d1) + d2 * d2);
-    R.ox = xi; R.oy = yi; R.oz = zi;
+    R.ox = ox; R.oy = oy; R.oz = oz;
     R.dx = d0 / L; R.dy = d1 / L; R.dz = d2 / L; R.L = L;
 }
 
-__device__ bool blocked_mixed(const BvhNode32* __restrict__ nodes, const Tri32* __restrict__ tris32,
-                              const Tri64* __restrict__ tris64, int root, const Ray32& R, float Sm, float dd,
-                              const Exact64Ctx& ec, unsigned long long& nretest) {
+// The exact FP64 traversal reaches leaf n iff the FP64 slab test passes at n
+// and at every ancestor; checked by walking the parent links.
+__device__ __forceinline__ bool reaches64(const BvhNode64* __restrict__ nodes, const int32_t* __restrict__ parent,
+                                          int n, const Ray64& R, double ix, double iy, double iz) {
+    for (; n >= 0; n = parent[n])
+        if (!slab64(nodes[n], R, ix, iy, iz)) return false;
+    return true;
+}
+
+// Mixed-precision traversal (per lane): FP32 slab tests on padded bounds (a
+// superset of the FP64 traversal's visits), the FP32 triangle filter, and for
+// every triangle the filter keeps the exact FP64 ray and mt_hit64.  With the
+// reference soup reading (triangles tested need not lie in their boxes) a hit
+// is also confirmed by the leaf's exact FP64 slab chain (reaches64); with the
+// geometric reading a hit triangle lies inside every box on its path, so the
+// chain is only re-tested when `verify` is set.
+__device__ bool blocked_mixed(const PairArgs& a, int root, const Ray32& R, float Sm, float dd, const Exact64Ctx& ec,
+                              bool verify, unsigned long long& nretest) {
     float ix = __builtin_amdgcn_rcpf(R.dx), iy = __builtin_amdgcn_rcpf(R.dy), iz = __builtin_amdgcn_rcpf(R.dz);
     bool have64 = false;
     Ray64 R64;
     int n = root;
     while (n >= 0) {
-        const BvhNode32 nd = nodes[n];
+        const BvhNode32 nd = a.nodes32[n];
         float t1x = (nd.lo[0] - R.ox) * ix, t2x = (nd.hi[0] - R.ox) * ix;
         float t1y = (nd.lo[1] - R.oy) * iy, t2y = (nd.hi[1] - R.oy) * iy;
         float t1z = (nd.lo[2] - R.oz) * iz, t2z = (nd.hi[2] - R.oz) * iz;
         float tmin = fmaxf(fmaxf(fmaxf(fminf(t1x, t2x), fminf(t1y, t2y)), fminf(t1z, t2z)), 0.0f);
         float tmax = fminf(fminf(fminf(fmaxf(t1x, t2x), fmaxf(t1y, t2y)), fmaxf(t1z, t2z)), R.L * 1.0001f);
         if (tmin <= tmax) {
-            int cnt = nd.start_count & 15;
+            const int cnt = nd.start_count & 15;
             if (cnt > 0) {
-                int st = nd.start_count >> 4;
+                const int st = nd.start_count >> 4;
                 for (int k = st; k < st + cnt; k++) {
-                    if (mt_filter32(tris32[k], R, Sm, dd)) {
+                    if (mt_filter32(a.tris32[k], R, Sm, dd)) {
                         if (!have64) { exact_ray(ec, R64); have64 = true; }
                         nretest++;
-                        if (mt_hit64(tris64[k], R64)) return true;
+                        if (mt_hit64(a.tris64[k], R64) &&
+                            (!verify || reaches64(a.nodes64, a.parent, n, R64, 1.0 / R64.dx, 1.0 / R64.dy, 1.0 / R64.dz)))
+                            return true;
                     }
                 }
                 n = nd.miss;
@@ -642,7 +707,7 @@ __device__ bool blocked_mixed(const BvhNode32* __restrict__ nodes, const Tri32* 
 
 __global__ __launch_bounds__(64) void k_pair_mixed_occ(PairArgs a, float Sm) {
     const TileCtx t = tile_ctx(a);
-    const Corners cwi = corners_of(a, t.i, t.vi && a.duffy);
+    const Corners cwi = corners_of(a, t.oi, t.vi && a.duffy);
     const int64_t NS = a.NS;
     const int npi = t.vi ? a.elem_np[t.i] : 0;
     int npmax = npi;
@@ -663,12 +728,15 @@ __global__ __launch_bounds__(64) void k_pair_mixed_occ(PairArgs a, float Sm) {
             continue;
         }
         bool act = t.vi && t.i < j;
-        if (a.duffy && shares_corner(cwi, corners_of(a, j, true))) act = false;
+        const int64_t oj = a.slot_orig[j];
+        if (a.duffy && shares_corner(cwi, corners_of(a, oj, true))) act = false;
         const int npj = a.elem_np[j];
         const int gj = a.elem_grp[j];
         const int root = a.tree_root[gi * a.G + gj];
         const float* __restrict__ qj = a.Q32 + j * a.maxpts * 8;
         const double* __restrict__ qj64 = a.Q64 + j * a.maxpts * 8;
+        const bool rev = t.oi > oj;
+        const float fjx = (float)a.anchor[j * 4 + 0], fjy = (float)a.anchor[j * 4 + 1], fjz = (float)a.anchor[j * 4 + 2];
         const float dax = (float)(a.anchor[j * 4 + 0] - aix);
         const float day = (float)(a.anchor[j * 4 + 1] - aiy);
         const float daz = (float)(a.anchor[j * 4 + 2] - aiz);
@@ -688,20 +756,25 @@ __global__ __launch_bounds__(64) void k_pair_mixed_occ(PairArgs a, float Sm) {
                 const float wdj = qj[q * 8 + 6];
                 float rx, ry, rz, r2;
                 float k = vf_mixed(bx, by, bz, nxi, nyi, nzi, qx, qy, qz, nxj, nyj, nzj, rx, ry, rz, r2);
-                if (pv && k != 0.0f && root >= 0) {
-                    nrays++;
+                const bool need = pv && k != 0.0f && root >= 0;
+                uint64_t pending = __ballot(need);
+                if (pending) {
+                    nrays += need;
                     float rs = __builtin_amdgcn_rsqf(r2);
                     Ray32 R;
-                    R.ox = fax + dxi; R.oy = fay + dyi; R.oz = faz + dzi;
-                    R.dx = rx * rs; R.dy = ry * rs; R.dz = rz * rs; R.L = r2 * rs;
+                    const float sg = rev ? -rs : rs;
+                    R.ox = rev ? fjx + qx : fax + dxi; R.oy = rev ? fjy + qy : fay + dyi;
+                    R.oz = rev ? fjz + qz : faz + dzi;
+                    R.dx = rx * sg; R.dy = ry * sg; R.dz = rz * sg; R.L = r2 * rs;
                     const float uu = 5.9604645e-08f;
                     float dd = 3.0f * uu * (fabsf(dax) + fabsf(day) + fabsf(daz) + fabsf(bx) + fabsf(by) + fabsf(bz) +
                                             fabsf(rx) + fabsf(ry) + fabsf(rz)) * rs + 4.0f * uu;
-                    Exact64Ctx ec{P64, NS, t.i, p, qj64 + q * 8};
-                    if (blocked_mixed(a.nodes32, a.tris32, a.tris64, root, R, Sm, dd, ec, nret)) {
+                    Exact64Ctx ec{P64, NS, t.i, p, qj64 + q * 8, rev};
+                    const bool blk = need && blocked_mixed(a, root, R, Sm, dd, ec, a.verify != 0, nret);
+                    if (blk) {
                         k = 0.0f;
                         bc++;
-                        bh += mix32((uint32_t)(p * npj + q) + 0x9e3779b9U);
+                        bh += mix32((uint32_t)(rev ? q * npi + p : p * npj + q) + 0x9e3779b9U);
                     }
                 }
                 inner = fmaf(wdj, k, inner);
@@ -712,10 +785,10 @@ __global__ __launch_bounds__(64) void k_pair_mixed_occ(PairArgs a, float Sm) {
             }
         }
         if (act && a.audit_count) {
-            a.audit_count[t.i + a.N * j] = bc;
-            a.audit_hash[t.i + a.N * j] = bh;
+            a.audit_count[min(t.oi, oj) + a.N * max(t.oi, oj)] = bc;
+            a.audit_hash[min(t.oi, oj) + a.N * max(t.oi, oj)] = bh;
         }
-        pair_out(a, t, j, act, (Fs + Fc) * (1.0 / PI_D), Ai, ls, lc);
+        pair_out(a, t, j, oj, act, (Fs + Fc) * (1.0 / PI_D), Ai, ls, lc);
     }
     a.part[(int64_t)t.chunk * NS + t.i] = ls + lc;
     if (a.counters) {
@@ -753,8 +826,8 @@ __device__ __forceinline__ void mc_sample(const PairArgs& a, int f, int64_t e, d
 
 // MC area of element e from the stream of the pair (e, e), mirroring the CPU
 // path's area pass element_pair_view_factor_mc(i, i) (Assembly.jl:164-169).
-__global__ __launch_bounds__(64) void k_mc_area(PairArgs a, double* area) {
-    int64_t e = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
+__global__ __launch_bounds__(64) void k_mc_area(PairArgs a, double* area, double* area_orig) {
+    int64_t e = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;  // slot == element (identity layout)
     if (e >= a.N) return;
     int f = a.fam[e];
     uint64_t st = init_rng(a.mc_seed, e * a.N + e + 1);
@@ -778,17 +851,20 @@ __global__ __launch_bounds__(64) void k_mc_area(PairArgs a, double* area) {
     }
     double ri = (f == 0 || f == 2) ? 4.0 : 0.5;
     area[e] = (Ai * ri) / a.mc_nsamp;
+    area_orig[a.slot_orig[e]] = area[e];
 }
 
 __global__ __launch_bounds__(64) void k_pair_mc(PairArgs a) {
     const TileCtx t = tile_ctx(a);
-    const Corners cwi = corners_of(a, t.i, false);
+    const Corners cwi = corners_of(a, t.oi, false);
     (void)cwi;
     const int64_t NS = a.NS;
-    const int fi = t.vi ? a.fam[t.i] : 2;
+    // Monte Carlo solves use the identity slot layout (slot == element), so
+    // i < j is the reference's thread ordering (GPUMCKernels.jl:167).
+    const int fi = t.vi ? a.fam[t.oi] : 2;
     const double Ai = t.vi ? a.area[t.i] : 1.0;
     const int gi = t.vi ? a.elem_grp[t.i] : 0;
-    const int64_t ii = t.vi ? t.i : 0;
+    const int64_t ii = t.vi ? t.oi : 0;
     const int s = a.mc_s;
     double ls = 0.0, lc = 0.0;
     unsigned long long nrays = 0, naabb = 0, ntri = 0;
@@ -798,9 +874,10 @@ __global__ __launch_bounds__(64) void k_pair_mc(PairArgs a) {
             continue;
         }
         const bool act = t.vi && t.i < j;
-        const int fj = a.fam[j];
+        const int64_t oj = a.slot_orig[j];
+        const int fj = a.fam[oj];
         const int root = a.tree_root ? a.tree_root[gi * a.G + a.elem_grp[j]] : -1;
-        uint64_t st = init_rng(a.mc_seed, ii * a.N + j + 1);
+        uint64_t st = init_rng(a.mc_seed, ii * a.N + oj + 1);
         double Ks = 0.0;
         int k = 0;
         double xi[3], ni[3], ai, xj[3], nj[3], aj;
@@ -810,13 +887,12 @@ __global__ __launch_bounds__(64) void k_pair_mc(PairArgs a) {
             bool strat = kk < s * s;
             mc_sample(a, fi, ii, strat ? (si + u1) / s : u1, strat ? (sj + u2) / s : u2, xi, ni, &ai);
             double u3 = xorshift64(st), u4 = xorshift64(st);
-            mc_sample(a, fj, j, strat ? (si + u3) / s : u3, strat ? (sj + u4) / s : u4, xj, nj, &aj);
+            mc_sample(a, fj, oj, strat ? (si + u3) / s : u3, strat ? (sj + u4) / s : u4, xj, nj, &aj);
             Ray64 R;
             double K = vf_exact(xi[0], xi[1], xi[2], ni[0], ni[1], ni[2], xj[0], xj[1], xj[2], nj[0], nj[1], nj[2], R);
-            if (act && K > 0.0 && root >= 0) {
-                nrays++;
-                if (blocked64(a.nodes64, a.tris64, root, R, naabb, ntri)) K = 0.0;
-            }
+            const bool need = act && K > 0.0 && root >= 0;
+            nrays += need;
+            if ((occlusion64(a, need, root, R, naabb, ntri) >> t.lane) & 1) K = 0.0;
             Ks += (K * ai) * aj;
             k++;
         }
@@ -824,7 +900,7 @@ __global__ __launch_bounds__(64) void k_pair_mc(PairArgs a) {
         double ri = (fi == 0 || fi == 2) ? 4.0 : 0.5;
         double rj = (fj == 0 || fj == 2) ? 4.0 : 0.5;
         double raw = ((Ks * ri) * rj) / a.mc_nsamp;
-        pair_out(a, t, j, act, raw, Ai, ls, lc);
+        pair_out(a, t, j, oj, act, raw, Ai, ls, lc);
     }
     a.part[(int64_t)t.chunk * NS + t.i] = ls + lc;
     if (a.counters) {
@@ -1019,18 +1095,20 @@ __global__ __launch_bounds__(256) void k_fill(double* __restrict__ F, const doub
 // ---------------------------------------------------------------------------
 // Row reduction: rowsum[r] and W[h][r] = sum_{j in group h, j > r} F_rj.
 //   upper part: part[c][r] over chunks c with chunk_last(c) > 64*strip(r)
-//   lower part: partT[s][r] over strips s <= (chunk_last(chunk(r)) - 1) / 64
-// Compensated (Neumaier) sums in fixed order.
+//   lower part: partT[s][r] over strips s <= (chunk_end(chunk(r)) - 2) / 64
+// Two stages: RVF_RSEG segments per row (chunk / strip index mod RVF_RSEG)
+// summed with Neumaier into Tseg / Wseg, then a fixed-order final sum.
 // ---------------------------------------------------------------------------
-__global__ __launch_bounds__(256) void k_reduce_rows(const double* __restrict__ part,
-                                                     const double* __restrict__ partT, const Chunk* __restrict__ chunks,
-                                                     int nchunks, const int32_t* __restrict__ chunk_of,
-                                                     int64_t N, int NS, int G, double* __restrict__ rowsum,
-                                                     double* __restrict__ W) {
+#define RVF_RSEG 16
+
+__global__ __launch_bounds__(256) void k_reduce_seg(const double* __restrict__ part, const double* __restrict__ partT,
+                                                    const Chunk* __restrict__ chunks, int nchunks,
+                                                    const int32_t* __restrict__ chunk_of, int64_t N, int NS, int G,
+                                                    double* __restrict__ Tseg, double* __restrict__ Wseg) {
     int64_t r = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
+    const int seg = blockIdx.y;
     if (r >= N) return;
     int64_t sr = r / RVF_STRIP;
-    // first chunk with last index > 64*sr: chunks are sorted; binary search
     int lo = 0, hi = nchunks;
     while (lo < hi) {
         int mid = (lo + hi) >> 1;
@@ -1039,24 +1117,56 @@ __global__ __launch_bounds__(256) void k_reduce_rows(const double* __restrict__ 
     double ts = 0.0, tc = 0.0;
     int curg = -1;
     double gs = 0.0, gc = 0.0;
-    for (int c = lo; c < nchunks; c++) {
+    double* Wr = Wseg + (int64_t)seg * G * NS + r;
+    int c0 = lo + ((seg - lo % RVF_RSEG) + RVF_RSEG) % RVF_RSEG;  // first c >= lo with c % RSEG == seg
+    for (int c = c0; c < nchunks; c += RVF_RSEG) {
         int g = chunks[c].grp;
         if (g != curg) {
-            if (curg >= 0) W[(int64_t)curg * NS + r] += gs + gc;
+            if (curg >= 0) Wr[(int64_t)curg * NS] += gs + gc;
             curg = g; gs = 0.0; gc = 0.0;
         }
         double v = part[(int64_t)c * NS + r];
         neumaier(gs, gc, v);
         neumaier(ts, tc, v);
     }
-    if (curg >= 0) W[(int64_t)curg * NS + r] += gs + gc;
+    if (curg >= 0) Wr[(int64_t)curg * NS] += gs + gc;
     int cr = chunk_of[r];
-    // tile (s, cr) exists iff 64 s < last(cr), i.e. s <= (end - 2) / 64
     int64_t smax = chunks[cr].end >= 2 ? (int64_t)(chunks[cr].end - 2) / RVF_STRIP : -1;
-    double ls = 0.0, lcmp = 0.0;
-    for (int64_t s = 0; s <= smax; s++) neumaier(ls, lcmp, partT[s * NS + r]);
-    neumaier(ts, tc, ls + lcmp);
+    for (int64_t s2 = seg; s2 <= smax; s2 += RVF_RSEG) neumaier(ts, tc, partT[s2 * NS + r]);
+    Tseg[(int64_t)seg * NS + r] = ts + tc;
+}
+
+__global__ __launch_bounds__(256) void k_reduce_fin(const double* __restrict__ Tseg, const double* __restrict__ Wseg,
+                                                    int64_t N, int NS, int G, double* __restrict__ rowsum,
+                                                    double* __restrict__ W) {
+    int64_t r = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
+    if (r >= N) return;
+    double ts = 0.0, tc = 0.0;
+    for (int sg = 0; sg < RVF_RSEG; sg++) neumaier(ts, tc, Tseg[(int64_t)sg * NS + r]);
     rowsum[r] = ts + tc;
+    for (int h = 0; h < G; h++) {
+        double ws = 0.0, wc = 0.0;
+        for (int sg = 0; sg < RVF_RSEG; sg++) neumaier(ws, wc, Wseg[((int64_t)sg * G + h) * NS + r]);
+        W[(int64_t)h * NS + r] = ws + wc;
+    }
+}
+
+// ---------------------------------------------------------------------------
+// Utility kernels: zero fill (graph-capturable replacement for memset) and a
+// plain 16-byte-vector copy (reference point for HBM traffic counters).
+// ---------------------------------------------------------------------------
+__global__ __launch_bounds__(256) void k_zero(uint4* __restrict__ p, int64_t n16, uint32_t* __restrict__ tail,
+                                              int64_t ntail) {
+    int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
+    int64_t stride = (int64_t)gridDim.x * blockDim.x;
+    for (int64_t k = i; k < n16; k += stride) p[k] = make_uint4(0, 0, 0, 0);
+    if (i < ntail) tail[i] = 0;
+}
+
+__global__ __launch_bounds__(256) void k_copy(const uint4* __restrict__ src, uint4* __restrict__ dst, int64_t n16) {
+    int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
+    int64_t stride = (int64_t)gridDim.x * blockDim.x;
+    for (int64_t k = i; k < n16; k += stride) dst[k] = src[k];
 }
 
 // ---------------------------------------------------------------------------
@@ -1074,7 +1184,7 @@ int rvf_launch_pair(const PairArgs* a, int mixed, int occ, int pc, float Sm, hip
     if (a->ntiles == 0) return 0;
     dim3 g((unsigned)a->ntiles), b(64);
     if (!mixed) {
-        if (occ) hipLaunchKernelGGL(k_pair_fp64_occ, g, b, 0, st, *a);
+        if (occ) hipLaunchKernelGGL(k_pair_fp64_occ, g, b, sizeof(double) * 64 * a->maxpts, st, *a);
         else if (pc <= 4) hipLaunchKernelGGL(k_pair_fp64<4>, g, b, 0, st, *a);
         else if (pc <= 9) hipLaunchKernelGGL(k_pair_fp64<9>, g, b, 0, st, *a);
         else hipLaunchKernelGGL(k_pair_fp64<16>, g, b, 0, st, *a);
@@ -1088,8 +1198,9 @@ int rvf_launch_pair(const PairArgs* a, int mixed, int occ, int pc, float Sm, hip
     return (int)hipGetLastError();
 }
 
-int rvf_launch_mc(const PairArgs* a, double* area, hipStream_t st) {
-    if (a->N > 0) hipLaunchKernelGGL(k_mc_area, dim3((unsigned)((a->N + 63) / 64)), dim3(64), 0, st, *a, area);
+int rvf_launch_mc(const PairArgs* a, double* area, double* area_orig, hipStream_t st) {
+    if (a->N > 0)
+        hipLaunchKernelGGL(k_mc_area, dim3((unsigned)((a->N + 63) / 64)), dim3(64), 0, st, *a, area, area_orig);
     if (a->ntiles > 0) hipLaunchKernelGGL(k_pair_mc, dim3((unsigned)a->ntiles), dim3(64), 0, st, *a);
     return (int)hipGetLastError();
 }
@@ -1106,6 +1217,21 @@ int rvf_launch_duffy(const PairArgs* a, const DuffyPair* pairs, int npairs, cons
     return (int)hipGetLastError();
 }
 
+int rvf_launch_zero(void* p, int64_t bytes, hipStream_t st) {
+    if (bytes <= 0) return 0;
+    int64_t n16 = bytes / 16, ntail = (bytes % 16) / 4;
+    unsigned nb = (unsigned)std::min<int64_t>(std::max<int64_t>((n16 + 255) / 256, 1), 4096);
+    hipLaunchKernelGGL(k_zero, dim3(nb), dim3(256), 0, st, (uint4*)p, n16, (uint32_t*)((char*)p + n16 * 16), ntail);
+    return (int)hipGetLastError();
+}
+
+int rvf_launch_copy(const void* src, void* dst, int64_t bytes, hipStream_t st) {
+    int64_t n16 = bytes / 16;
+    if (n16 <= 0) return 0;
+    hipLaunchKernelGGL(k_copy, dim3(4096), dim3(256), 0, st, (const uint4*)src, (uint4*)dst, n16);
+    return (int)hipGetLastError();
+}
+
 int rvf_launch_fill(double* F, const double* area, int64_t N, const int2* tiles, int ntiles, hipStream_t st) {
     if (ntiles == 0) return 0;
     hipLaunchKernelGGL(k_fill, dim3((unsigned)ntiles), dim3(256), 0, st, F, area, N, tiles);
@@ -1114,10 +1240,12 @@ int rvf_launch_fill(double* F, const double* area, int64_t N, const int2* tiles,
 
 int rvf_launch_reduce(const double* part, const double* partT, const Chunk* chunks, int nchunks,
                       const int32_t* chunk_of, int64_t N, int NS, int G, double* rowsum, double* W,
-                      hipStream_t st) {
+                      double* Tseg, double* Wseg, hipStream_t st) {
     if (N == 0) return 0;
-    hipLaunchKernelGGL(k_reduce_rows, dim3((unsigned)((N + 255) / 256)), dim3(256), 0, st, part, partT, chunks,
-                       nchunks, chunk_of, N, NS, G, rowsum, W);
+    unsigned nb = (unsigned)((N + 255) / 256);
+    hipLaunchKernelGGL(k_reduce_seg, dim3(nb, RVF_RSEG), dim3(256), 0, st, part, partT, chunks, nchunks, chunk_of, N,
+                       NS, G, Tseg, Wseg);
+    hipLaunchKernelGGL(k_reduce_fin, dim3(nb), dim3(256), 0, st, Tseg, Wseg, N, NS, G, rowsum, W);
     return (int)hipGetLastError();
 }
 }

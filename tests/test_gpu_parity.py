@@ -217,3 +217,36 @@ def test_fp64_traversal_counts_match_oracle(orc):
     r = orc.compute(m, nquad=2, obstruction_groups=[3], parity_mode="geometric", count=True, want_raw=False)
     assert (g["stats"]["rays"], g["stats"]["aabb_tests"], g["stats"]["tri_tests"]) == \
         (r["rays"], r["aabb_tests"], r["tri_tests"])
+
+
+def test_mixed_cube_with_sphere_flips():
+    # the benchmark geometry at reduced size: cube interior + sphere occluder
+    m = M.unit_cube(12)
+    V, T = M.icosphere(3, 0.3, (0.5, 0.5, 0.5), seed=0x5EED0003)
+    m.group_tri_soup = {7: V[T]}
+    for parity in ("geometric", "reference"):
+        dF, dclose, flips, a, b = mixed_vs_fp64(m, 3, obs=[7], parity=parity)
+        assert a["ac"].sum() > 0
+        assert flips == 0 and dF <= 1e-6 and dclose <= 1e-6, (parity, flips, dF, dclose)
+
+
+def test_graph_replay_matches_eager():
+    # the captured solve is replayed several times, with eager solves of
+    # another plan in between; every replay equals the eager result
+    ctx = api.context(0)
+    m = M.occluded_plates(n=8, subdiv=2)
+    dm = api.DeviceMesh(ctx, m)
+    do = api.DeviceObstruction(ctx, dm, [3], "geometric")
+    eager = api.Plan(ctx, dm, do, 3, "quad", "mixed")
+    eager.run(); eager.sync()
+    ref = eager.fetch()
+    other = api.Plan(ctx, dm, None, 2, "quad", "fp64")
+    g = api.Plan(ctx, dm, do, 3, "quad", "mixed", graph=True)
+    for _ in range(3):
+        g.run(); g.sync()
+        got = g.fetch()
+        assert np.array_equal(got[0], ref[0]) and np.array_equal(got[2], ref[2]) and np.array_equal(got[3], ref[3])
+        other.run(); other.sync()
+    for pl in (eager, other, g):
+        pl.close()
+    do.close(); dm.close()
