@@ -61,8 +61,15 @@ typedef struct rvf_opts {
     int32_t n_samples;  /* Monte Carlo samples per pair */
     uint64_t seed;      /* Monte Carlo stream seed */
     int32_t flags;      /* RVF_WANT_F | RVF_AUDIT | RVF_TIMING */
-    int32_t reserved;
+    int32_t variant;    /* occluded quadrature traversal: RVF_VARIANT_* (0 = library default) */
 } rvf_opts;
+
+/* Occluded-quadrature traversal variants (rvf_opts.variant).  Results do not
+ * depend on the variant: FP64 visibility decisions and sums are identical. */
+#define RVF_VARIANT_DEFAULT 0
+#define RVF_VARIANT_LANE 1        /* each lane traces its own (p, q) rays, one at a time */
+#define RVF_VARIANT_QUEUE(S) (10 + (S))  /* wave-shared ray queue, S rays in flight per lane:
+                                           FP64 S = 1..3, mixed S = 2, 3, 4, 6 */
 
 typedef struct rvf_stats {
     double precompute_ms, pair_ms, duffy_ms, fill_ms, reduce_ms, total_ms;

@@ -26,7 +26,7 @@ class RVFError(RuntimeError):
 
 class Opts(C.Structure):
     _fields_ = [("nquad", C.c_int32), ("method", C.c_int32), ("precision", C.c_int32), ("n_samples", C.c_int32),
-                ("seed", C.c_uint64), ("flags", C.c_int32), ("reserved", C.c_int32)]
+                ("seed", C.c_uint64), ("flags", C.c_int32), ("variant", C.c_int32)]
 
 
 class Stats(C.Structure):

@@ -19,6 +19,19 @@ from . import _lib
 from ._lib import RVF_AUDIT, RVF_TIMING, RVF_WANT_F, RVFError, check, ptr
 from .mesh import MeshData
 
+# Occluded-quadrature traversal variants (rvf_opts.variant, include/rvf.h)
+VARIANT_DEFAULT = 0
+VARIANT_LANE = 1
+
+
+def VARIANT_QUEUE(slots: int) -> int:
+    """Wave-shared ray queue with `slots` rays in flight per lane."""
+    return 10 + slots
+
+
+FP64_VARIANTS = (VARIANT_LANE, VARIANT_QUEUE(1), VARIANT_QUEUE(2), VARIANT_QUEUE(3))
+MIXED_VARIANTS = (VARIANT_LANE, VARIANT_QUEUE(2), VARIANT_QUEUE(3), VARIANT_QUEUE(4), VARIANT_QUEUE(6))
+
 SOUP_READ = {"reference": 0, ("geometric", "xyz_vertex"): 1, ("geometric", "vertex_xyz"): 2}
 
 
@@ -178,7 +191,8 @@ class Plan:
 
     def __init__(self, ctx: Context, dmesh: DeviceMesh, dobs: Optional[DeviceObstruction], nquad: int,
                  method: str = "quad", precision: str = "fp64", n_samples: int = 10000, seed: int = 0,
-                 want_F: bool = True, audit: bool = False, timing: bool = True, graph: bool = False):
+                 want_F: bool = True, audit: bool = False, timing: bool = True, graph: bool = False,
+                 variant: int = 0):
         o = _lib.Opts()
         o.nquad = nquad
         o.method = _lib.METHODS[method]
@@ -187,6 +201,7 @@ class Plan:
         o.seed = seed & ((1 << 64) - 1)
         o.flags = (RVF_WANT_F if want_F else 0) | (RVF_AUDIT if audit else 0) | (RVF_TIMING if timing else 0) | \
             (_lib.RVF_GRAPH if graph else 0)
+        o.variant = variant
         self.opts = o
         self.ctx, self.dmesh, self.dobs = ctx, dmesh, dobs
         h = C.c_void_p()

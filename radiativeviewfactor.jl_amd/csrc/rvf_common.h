@@ -45,6 +45,24 @@ struct __align__(16) Tri32 {
     float pad1;
 };
 
+// Traversal items for the wave-queue kernels: the nodes of every tree
+// (indices [0, M)) followed by the triangles ([M, M+T)) in one array, so one
+// traversal step is one record fetch whatever its kind.
+//   node:     box lo[3], hi[3]; link.x = item after a passed slab test (left
+//             child, or the leaf's first triangle); link.y = miss link.
+//   triangle: v0[3], e1[3], e2[3]; link.x = next item (next triangle of the
+//             leaf, or the leaf's miss link); link.y = the leaf node.
+// Links are -1 at the end of a tree.  FP32 boxes are conservatively padded.
+struct __align__(16) Item32 {
+    float v[9];
+    int32_t link_x, link_y;
+    int32_t pad;
+};
+struct __align__(16) Item64 {
+    double v[9];
+    int32_t link_x, link_y;
+};
+
 // Column chunk: a run of <= RVF_CHUNK consecutive elements of one group.
 struct Chunk {
     int32_t begin;
@@ -83,7 +101,12 @@ struct PairArgs {
     const BvhNode32* nodes32;
     const Tri32* tris32;
     const int32_t* parent;    // per node, absolute; -1 at a root
+    const Item32* items32;    // wave-queue traversal records (nodes then triangles)
+    const Item64* items64;
+    int n_nodes;              // M: items below M are nodes
+    int refill_min;           // wave-queue kernels: refill a slot once this many lanes have it empty
     int verify;               // mixed: confirm FP64 hits by the leaf's exact slab chain
+    float Sm;                 // mixed: scene coordinate magnitude (FP32 error bounds)
     // audit (nullable): per pair blocked-ray count / order-free hash at [i + N*j]
     uint32_t* audit_count;
     uint32_t* audit_hash;

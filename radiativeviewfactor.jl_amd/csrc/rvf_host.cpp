@@ -17,7 +17,10 @@
 // device launch wrappers (rvf_kernels.hip)
 
 extern "C" int rvf_launch_precompute(const PrecompArgs* a, hipStream_t st);
-extern "C" int rvf_launch_pair(const PairArgs* a, int mixed, int occ, int pc, float Sm, hipStream_t st);
+extern "C" int rvf_pair_kind(int mixed, int occ, int pc, int mc);
+extern "C" int rvf_launch_pair(const PairArgs* a, int kind, size_t lds, hipStream_t st);
+extern "C" int rvf_launch_pair_batched(int kind, const PairArgs* reqs, const int2* work, int nwork, int* head,
+                                       int grid, size_t lds, hipStream_t st);
 extern "C" int rvf_launch_mc(const PairArgs* a, double* area, double* area_orig, hipStream_t st);
 extern "C" int rvf_launch_duffy(const PairArgs* a, const DuffyPair* pairs, int npairs, const double* gl01, int nq,
                                 double* val, const int32_t* rptr, const int32_t* rlist, double* rowsum, double* W,
@@ -450,7 +453,7 @@ struct rvf_obs {
     std::vector<int32_t> root;   // G*G absolute node index or -1
     int64_t n_nodes = 0, n_tris = 0;
     float Sm = 1.0f;
-    DBuf d_root, d_n64, d_t64, d_n32, d_t32, d_parent;
+    DBuf d_root, d_n64, d_t64, d_n32, d_t32, d_parent, d_items32, d_items64;
 };
 
 static float f32_down(double x) {
@@ -580,8 +583,41 @@ int rvf_obs_build(rvf_ctx* ctx, const rvf_mesh* mesh, const double* soup, const 
             f.pad0 = f.pad1 = 0;
         }
     }
+    // wave-queue items: nodes [0, M), then triangles [M, M+T)
+    const int64_t M = o->n_nodes, NI = std::max<int64_t>(o->n_nodes + o->n_tris, 1);
+    if (NI >= ((int64_t)1 << 30)) { delete o; return fail(RVF_E_ARG, "rvf_obs_build: too many BVH items"); }
+    std::vector<Item32> i32((size_t)NI);
+    std::vector<Item64> i64((size_t)NI);
+    std::memset(i32.data(), 0, sizeof(Item32) * i32.size());
+    std::memset(i64.data(), 0, sizeof(Item64) * i64.size());
+    for (const Tree& tr : o->trees) {
+        for (size_t k = 0; k < tr.nodes.size(); k++) {
+            const int64_t n = tr.node_base + (int64_t)k;
+            const BvhNode64& d = n64[n];
+            const BvhNode32& f = n32[n];
+            const int32_t pass = d.count ? (int32_t)(M + d.start) : (int32_t)(n + 1);
+            for (int c = 0; c < 3; c++) {
+                i64[n].v[c] = d.lo[c]; i64[n].v[3 + c] = d.hi[c];
+                i32[n].v[c] = f.lo[c]; i32[n].v[3 + c] = f.hi[c];
+            }
+            i64[n].link_x = i32[n].link_x = pass;
+            i64[n].link_y = i32[n].link_y = d.miss;
+            for (int c = 0; c < d.count; c++) {
+                const int64_t t = d.start + c, it = M + t;
+                const bool last = c == d.count - 1;
+                for (int e = 0; e < 3; e++) {
+                    i64[it].v[e] = t64[t].v0[e]; i64[it].v[3 + e] = t64[t].e1[e]; i64[it].v[6 + e] = t64[t].e2[e];
+                    i32[it].v[e] = t32[t].v0[e]; i32[it].v[3 + e] = t32[t].e1[e]; i32[it].v[6 + e] = t32[t].e2[e];
+                }
+                i64[it].link_x = i32[it].link_x = last ? d.miss : (int32_t)(it + 1);
+                i64[it].link_y = i32[it].link_y = (int32_t)n;
+            }
+        }
+    }
     int rc;
-    if ((rc = upload(o->d_root, o->root.data(), sizeof(int32_t) * std::max(1, G * G))) ||
+    if ((rc = upload(o->d_items32, i32.data(), sizeof(Item32) * i32.size())) ||
+        (rc = upload(o->d_items64, i64.data(), sizeof(Item64) * i64.size())) ||
+        (rc = upload(o->d_root, o->root.data(), sizeof(int32_t) * std::max(1, G * G))) ||
         (rc = upload(o->d_n64, n64.data(), sizeof(BvhNode64) * n64.size())) ||
         (rc = upload(o->d_n32, n32.data(), sizeof(BvhNode32) * n32.size())) ||
         (rc = upload(o->d_t64, t64.data(), sizeof(Tri64) * t64.size())) ||
@@ -665,6 +701,8 @@ struct rvf_plan {
     bool ran = false;
 };
 
+static int occ_kind(int variant, bool mixed);
+
 int rvf_plan_create(rvf_ctx* ctx, const rvf_mesh* mesh, const rvf_obs* obs, const rvf_opts* opts, rvf_plan** out) {
     if (!ctx || !mesh || !opts || !out) return fail(RVF_E_ARG, "rvf_plan_create: bad arguments");
     if (opts->nquad < 1 || opts->nquad > 8) return fail(RVF_E_ARG, "nquad must be in 1..8");
@@ -672,6 +710,9 @@ int rvf_plan_create(rvf_ctx* ctx, const rvf_mesh* mesh, const rvf_obs* obs, cons
     if (opts->precision < 0 || opts->precision > 1) return fail(RVF_E_ARG, "unknown precision");
     if (opts->method == RVF_METHOD_MC && opts->n_samples < 1) return fail(RVF_E_ARG, "n_samples must be >= 1");
     if (obs && obs->G != (int)mesh->tags.size()) return fail(RVF_E_ARG, "obstruction set built for another mesh");
+    if (obs && !obs->trees.empty() && opts->method != RVF_METHOD_MC &&
+        occ_kind(opts->variant, opts->precision == RVF_MIXED) < 0)
+        return fail(RVF_E_ARG, "unknown traversal variant for this precision");
     HIPCHK(hipSetDevice(ctx->device));
     rvf_plan* p = new rvf_plan();
     p->ctx = ctx; p->mesh = mesh; p->obs = obs; p->opts = *opts;
@@ -801,7 +842,13 @@ static PairArgs pair_args(rvf_plan* p) {
         a.nodes64 = p->obs->d_n64.as<BvhNode64>(); a.tris64 = p->obs->d_t64.as<Tri64>();
         a.nodes32 = p->obs->d_n32.as<BvhNode32>(); a.tris32 = p->obs->d_t32.as<Tri32>();
         a.parent = p->obs->d_parent.as<int32_t>();
+        a.items32 = p->obs->d_items32.as<Item32>(); a.items64 = p->obs->d_items64.as<Item64>();
+        a.n_nodes = (int)p->obs->n_nodes;
+        a.refill_min = 16;
         a.verify = p->obs->soup_read == RVF_SOUP_REFERENCE ? 1 : 0;
+        a.Sm = p->obs->Sm;
+    } else {
+        a.Sm = 1.0f;
     }
     a.audit_count = p->audit_c.as<uint32_t>(); a.audit_hash = p->audit_h.as<uint32_t>();
     a.counters = p->counters.as<unsigned long long>();
@@ -811,6 +858,43 @@ static PairArgs pair_args(rvf_plan* p) {
     a.mc_s = (int)std::floor(std::sqrt((double)p->opts.n_samples));
     a.mc_seed = p->opts.seed;
     return a;
+}
+
+enum {  // pair-kernel kinds (rvf_kernels.hip)
+    K_FP64_OCC = 3, K_MIX_OCC = 8, K_FP64_Q1 = 10, K_FP64_Q2 = 11, K_FP64_Q3 = 12, K_MIX_Q2 = 13, K_MIX_Q3 = 14,
+    K_MIX_Q4 = 15, K_MIX_Q6 = 16
+};
+#define RVF_QCOMBO 128
+
+static bool plan_occluded_quad(const rvf_plan* p) {
+    return p->obs && !p->obs->trees.empty() && p->opts.method != RVF_METHOD_MC;
+}
+
+// Kernel kind for an occluded quadrature plan's variant, -1 if invalid.
+static int occ_kind(int variant, bool mixed) {
+    if (variant == RVF_VARIANT_DEFAULT) variant = mixed ? RVF_VARIANT_QUEUE(2) : RVF_VARIANT_QUEUE(1);
+    if (variant == RVF_VARIANT_LANE) return mixed ? K_MIX_OCC : K_FP64_OCC;
+    const int S = variant - 10;
+    if (!mixed) return S == 1 ? K_FP64_Q1 : S == 2 ? K_FP64_Q2 : S == 3 ? K_FP64_Q3 : -1;
+    return S == 2 ? K_MIX_Q2 : S == 3 ? K_MIX_Q3 : S == 4 ? K_MIX_Q4 : S == 6 ? K_MIX_Q6 : -1;
+}
+
+// Which pair-kernel variant a plan runs, and the LDS it needs.
+static int pair_kind(const rvf_plan* p) {
+    const bool mixed = p->opts.precision == RVF_MIXED;
+    if (plan_occluded_quad(p)) return occ_kind(p->opts.variant, mixed);
+    int pc = p->maxpts;
+    if (!mixed && pc > 9) pc = 9;  // register chunks of 9 points
+    return rvf_pair_kind(mixed, 0, pc, p->opts.method == RVF_METHOD_MC);
+}
+static size_t pair_lds(const rvf_plan* p) {
+    if (!plan_occluded_quad(p)) return 0;
+    const int kind = pair_kind(p);
+    const bool mixed = p->opts.precision == RVF_MIXED;
+    const size_t accq = mixed ? 0 : sizeof(double) * 64 * (size_t)p->maxpts;
+    if (kind == K_FP64_OCC) return accq;
+    if (kind == K_MIX_OCC) return 0;
+    return accq + RVF_QCOMBO * (sizeof(uint64_t) + 2 * sizeof(uint32_t)) + 64 * sizeof(int32_t);
 }
 
 // Enqueue one solve on stream st.  With tim, HIP events bracket each stage
@@ -835,11 +919,8 @@ static int enqueue_solve(rvf_plan* p, hipStream_t st, bool tim) {
     if (rc) return fail(RVF_E_HIP, std::string("precompute launch: ") + hipGetErrorString((hipError_t)rc));
     if (tim) HIPCHK(hipEventRecord(p->ev[1], st));
     PairArgs a = pair_args(p);
-    bool occ = p->obs && !p->obs->trees.empty();
     if (p->opts.method == RVF_METHOD_QUAD || p->opts.method == RVF_METHOD_DUFFY) {
-        int pc = p->maxpts;
-        if (p->opts.precision == RVF_FP64 && pc > 9) pc = 9;  // register chunks of 9 points
-        rc = rvf_launch_pair(&a, p->opts.precision == RVF_MIXED, occ, pc, p->obs ? p->obs->Sm : 1.0f, st);
+        rc = rvf_launch_pair(&a, pair_kind(p), pair_lds(p), st);
         if (rc) return fail(RVF_E_HIP, std::string("pair launch: ") + hipGetErrorString((hipError_t)rc));
     } else {
         rc = rvf_launch_mc(&a, p->area.as<double>(), p->area_orig.as<double>(), st);

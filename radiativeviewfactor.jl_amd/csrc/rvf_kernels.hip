@@ -21,6 +21,7 @@
 #include "rvf_common.h"
 
 #include <algorithm>
+#include <type_traits>
 
 #define PI_D 3.141592653589793
 
@@ -279,9 +280,9 @@ struct TileCtx {
     bool vi;
 };
 
-__device__ __forceinline__ TileCtx tile_ctx(const PairArgs& a) {
+__device__ __forceinline__ TileCtx tile_ctx(const PairArgs& a, int tile) {
     TileCtx t;
-    int2 tl = a.tiles[blockIdx.x];
+    int2 tl = a.tiles[tile];
     t.s = tl.x; t.chunk = tl.y;
     t.lane = threadIdx.x;
     t.i = (int64_t)t.s * RVF_STRIP + t.lane;
@@ -331,8 +332,8 @@ __device__ __forceinline__ void pair_out(const PairArgs& a, const TileCtx& t, in
 // FP64, unoccluded: i-points in registers, PC per chunk of the p loop.
 // ---------------------------------------------------------------------------
 template <int PC>
-__global__ __launch_bounds__(64) void k_pair_fp64(PairArgs a) {
-    const TileCtx t = tile_ctx(a);
+__device__ __forceinline__ void tile_fp64(const PairArgs& a, int tile) {
+    const TileCtx t = tile_ctx(a, tile);
     const Corners cwi = corners_of(a, t.oi, t.vi && a.duffy);
     const int64_t NS = a.NS;
     const int npi = t.vi ? a.elem_np[t.i] : 0;
@@ -407,9 +408,8 @@ __global__ __launch_bounds__(64) void k_pair_fp64(PairArgs a) {
 // are accumulated in LDS (accq[q][lane]) so the summation order is still the
 // reference's.  K is symmetric bit for bit, so one evaluation serves both.
 // ---------------------------------------------------------------------------
-__global__ __launch_bounds__(64) void k_pair_fp64_occ(PairArgs a) {
-    extern __shared__ double accq[];  // [maxpts][64]
-    const TileCtx t = tile_ctx(a);
+__device__ __forceinline__ void tile_fp64_occ(const PairArgs& a, int tile, double* accq /* [maxpts][64] LDS */) {
+    const TileCtx t = tile_ctx(a, tile);
     const Corners cwi = corners_of(a, t.oi, t.vi && a.duffy);
     const int64_t NS = a.NS;
     const int npi = t.vi ? a.elem_np[t.i] : 0;
@@ -507,11 +507,11 @@ __device__ __forceinline__ float vf_mixed(float bx, float by, float bz, float nx
 }
 
 template <int PC, bool SINGLE>
-__global__ __launch_bounds__(64) void k_pair_mixed(PairArgs a) {
+__device__ __forceinline__ void tile_mixed(const PairArgs& a, int tile) {
     // SINGLE: every element has <= PC points, so the strip's points stay in
     // registers for the whole tile; otherwise chunks of PC points are reloaded
     // per column element (L1/L2 hits).
-    const TileCtx t = tile_ctx(a);
+    const TileCtx t = tile_ctx(a, tile);
     const Corners cwi = corners_of(a, t.oi, t.vi && a.duffy);
     const int64_t NS = a.NS;
     const int npi = t.vi ? a.elem_np[t.i] : 0;
@@ -705,8 +705,9 @@ __device__ bool blocked_mixed(const PairArgs& a, int root, const Ray32& R, float
     return false;
 }
 
-__global__ __launch_bounds__(64) void k_pair_mixed_occ(PairArgs a, float Sm) {
-    const TileCtx t = tile_ctx(a);
+__device__ __forceinline__ void tile_mixed_occ(const PairArgs& a, int tile) {
+    const float Sm = a.Sm;
+    const TileCtx t = tile_ctx(a, tile);
     const Corners cwi = corners_of(a, t.oi, t.vi && a.duffy);
     const int64_t NS = a.NS;
     const int npi = t.vi ? a.elem_np[t.i] : 0;
@@ -798,6 +799,522 @@ __global__ __launch_bounds__(64) void k_pair_mixed_occ(PairArgs a, float Sm) {
     }
 }
 // ---------------------------------------------------------------------------
+// Occluded pair tiles, wave-queue form.
+//
+// Per column element j the (p, q) point combinations are handled in batches
+// of at most RVF_QCOMBO.  Phase A evaluates the kernel for every lane and
+// combination and stores the wave's need mask per combination in LDS.  Phase
+// B drains those rays as one queue shared by the whole wavefront: every lane
+// keeps S rays in flight (independent fetch chains) and refills a finished
+// slot with the next pending ray in queue order, whichever lane's element it
+// starts from, so lanes stay busy until the batch is empty and neighbouring
+// lanes trace neighbouring rays.  A traversal step fetches one item (node or
+// triangle) and applies the slab and triangle tests branch-free.  Blocked
+// rays set a bit per (combination, lane) in LDS.  Phase C re-evaluates the
+// kernel and sums the unblocked terms in each lane's own order, so FP64 sums
+// stay in the reference's order and visibility decisions are per ray as in
+// is_visible.
+// ---------------------------------------------------------------------------
+#define RVF_QCOMBO 128
+
+__device__ __forceinline__ int lane_rank(uint64_t m) {
+    return (int)__builtin_amdgcn_mbcnt_hi((uint32_t)(m >> 32), __builtin_amdgcn_mbcnt_lo((uint32_t)m, 0u));
+}
+__device__ __forceinline__ uint64_t uniform64(uint64_t x) {
+    uint32_t lo = __builtin_amdgcn_readfirstlane((uint32_t)x);
+    uint32_t hi = __builtin_amdgcn_readfirstlane((uint32_t)(x >> 32));
+    return ((uint64_t)hi << 32) | lo;
+}
+
+struct WaveQueue {
+    int c, nc;      // current combination, number of combinations (uniform)
+    uint64_t m;     // pending source lanes of combination c (uniform)
+};
+
+// Hands pending rays, in queue order, to the lanes with `want` set, in lane
+// order.  Must be called with all 64 lanes active.  Returns true in lanes
+// that received a ray, with its combination and source lane.
+__device__ __forceinline__ bool queue_take(uint64_t req, WaveQueue& Q, const uint64_t* need, int lane, int& c_out,
+                                           int& b_out) {
+    bool got = false;
+    while (req != 0 && Q.c < Q.nc) {
+        if (Q.m == 0) {
+            Q.c++;
+            Q.m = Q.c < Q.nc ? uniform64(need[Q.c]) : 0;
+            continue;
+        }
+        const int take = min(__popcll(Q.m), __popcll(req));
+        const bool holder = (Q.m >> lane) & 1;
+        const int hr = lane_rank(Q.m);
+        // forward permute: the holder of rank r sends its lane index to lane r
+        // (non-holders write lane 63, which is read only when every lane holds)
+        const int src = __builtin_amdgcn_ds_permute((holder ? hr : 63) << 2, lane);
+        const bool isreq = (req >> lane) & 1;
+        const int rr = lane_rank(req);
+        const int b = __builtin_amdgcn_ds_bpermute(rr << 2, src);
+        if (isreq && rr < take) { got = true; c_out = Q.c; b_out = b; }
+        Q.m &= ~__ballot(holder && hr < take);
+        req &= ~__ballot(isreq && rr < take);
+    }
+    return got;
+}
+
+// FP32 slab test on a padded box (a superset of the FP64 test's passes).
+__device__ __forceinline__ bool slab32(const float* v, float ox, float oy, float oz, float ix, float iy, float iz,
+                                       float L) {
+    float t1x = (v[0] - ox) * ix, t2x = (v[3] - ox) * ix;
+    float t1y = (v[1] - oy) * iy, t2y = (v[4] - oy) * iy;
+    float t1z = (v[2] - oz) * iz, t2z = (v[5] - oz) * iz;
+    float tmin = fmaxf(fmaxf(fmaxf(fminf(t1x, t2x), fminf(t1y, t2y)), fminf(t1z, t2z)), 0.0f);
+    float tmax = fminf(fminf(fminf(fmaxf(t1x, t2x), fmaxf(t1y, t2y)), fmaxf(t1z, t2z)), L * 1.0001f);
+    return tmin <= tmax;
+}
+
+// mt_filter32 on an item record (same bounds; sa computed from the FP32 edges
+// with a relative margin covering its own rounding).
+__device__ __forceinline__ int mt_filter32_item(const float* v, float ox, float oy, float oz, float dx, float dy,
+                                                float dz, float L, float Sm, float dd) {
+#pragma clang fp contract(fast)
+    const float u = 5.9604645e-08f;
+    float e1x = v[3], e1y = v[4], e1z = v[5];
+    float e2x = v[6], e2y = v[7], e2z = v[8];
+    float hx = dy * e2z - dz * e2y;
+    float hy = dz * e2x - dx * e2z;
+    float hz = dx * e2y - dy * e2x;
+    float av = e1x * hx + e1y * hy + e1z * hz;
+    float E1 = fabsf(e1x) + fabsf(e1y) + fabsf(e1z);
+    float E2 = fabsf(e2x) + fabsf(e2y) + fabsf(e2z);
+    float sa = E1 * E2 * 1.000002f;
+    float KA = 2.0f * sa * (dd + 9.0f * u);
+    float f = __builtin_amdgcn_rcpf(av);
+    float af = fabsf(f) * 1.0000005f;
+    float sx = ox - v[0], sy = oy - v[1], sz = oz - v[2];
+    float ub = f * (sx * hx + sy * hy + sz * hz);
+    float eu = (2.0f * E2 * Sm * (64.0f * u + 6.0f * dd) + 2.0f * KA) * af + 12.0f * u;
+    float qx = sy * e1z - sz * e1y, qy = sz * e1x - sx * e1z, qz = sx * e1y - sy * e1x;
+    float vb = f * (dx * qx + dy * qy + dz * qz);
+    float ev = (2.0f * E1 * Sm * (12.0f * dd + 68.0f * u) + 2.0f * KA) * af + 12.0f * u;
+    float tb = f * (e2x * qx + e2y * qy + e2z * qz);
+    float et = (132.0f * u * sa * Sm + fabsf(tb) * KA) * af + 6.0f * u * fabsf(tb);
+    const bool band = fabsf(av) <= KA + 1e-10f;  // grazing or near the 1e-10 cut: undecided
+    const bool miss = (ub < -eu) | (ub > 1.0f + eu) | (vb < -ev) | (ub + vb > 1.0f + eu + ev) | (tb < -et) |
+                      (tb > L * (1.0f + 2.0f * (dd + 3.0f * u)) + et);
+    return band || !miss;
+}
+
+// Tile-level data one queue batch needs.
+struct QCtx {
+    int64_t sbase;       // first slot of the strip
+    int64_t j, oj;       // column element (slot, original index)
+    int npj, gj, p0, pch;
+    uint32_t magic;      // floor(c / npj) = (c * magic) >> 16 for c < RVF_QCOMBO
+    uint64_t revmask;    // lanes whose element is the pair's inner element
+    const int* lroot;    // LDS: per lane, tree root of (lane's group, column group)
+};
+
+template <bool MIX>
+struct QRay;
+
+template <>
+struct QRay<false> {  // FP64 ray: origin, direction, inverse direction
+    double ox, oy, oz, dx, dy, dz, ix, iy, iz, L;
+};
+template <>
+struct QRay<true> {   // FP32 ray with its direction error bound
+    float ox, oy, oz, dx, dy, dz, ix, iy, iz, L, dd;
+};
+
+__device__ __forceinline__ void combo_pq(const QCtx& X, int c, int& p, int& q) {
+    const int pl = (int)(((uint32_t)c * X.magic) >> 16);
+    p = X.p0 + pl;
+    q = c - pl * X.npj;
+}
+
+// FP64 ray of combination c from lane b's element (is_visible, RayCast.jl:27-30,
+// direction as vf_exact computes it; reversed when the column element is the
+// pair's outer element).  Returns the tree root, -1 if none.
+__device__ __forceinline__ int qray_setup(const PairArgs& a, const QCtx& X, int c, int b, QRay<false>& R) {
+    int p, q;
+    combo_pq(X, c, p, q);
+    const int64_t ib = X.sbase + b;
+    const int64_t NS = a.NS;
+    const double xi = a.P64[(int64_t)(p * 8 + 0) * NS + ib], yi = a.P64[(int64_t)(p * 8 + 1) * NS + ib];
+    const double zi = a.P64[(int64_t)(p * 8 + 2) * NS + ib];
+    const double* qj = a.Q64 + (X.j * a.maxpts + q) * 8;
+    const double xj = qj[0], yj = qj[1], zj = qj[2];
+    const bool rev = (X.revmask >> b) & 1;
+    const double r0 = xj - xi, r1 = yj - yi, r2 = zj - zi;
+    const double rr = (r0 * r0 + r1 * r1) + r2 * r2;
+    const double r = sqrt(rr);
+    double h0 = r0 / r, h1 = r1 / r, h2 = r2 / r;
+    if (rev) { h0 = -h0; h1 = -h1; h2 = -h2; }
+    R.ox = rev ? xj : xi; R.oy = rev ? yj : yi; R.oz = rev ? zj : zi;
+    R.dx = h0; R.dy = h1; R.dz = h2; R.L = r;
+    R.ix = 1.0 / h0; R.iy = 1.0 / h1; R.iz = 1.0 / h2;
+    return X.lroot[b];
+}
+
+// FP32 ray of combination c from lane b's element, from the anchored FP32
+// coordinates exactly as the mixed kernel evaluation forms r.
+__device__ __forceinline__ int qray_setup(const PairArgs& a, const QCtx& X, int c, int b, QRay<true>& R) {
+    int p, q;
+    combo_pq(X, c, p, q);
+    const int64_t ib = X.sbase + b;
+    const int64_t NS = a.NS;
+    const float dxi = a.P32[(int64_t)(p * 8 + 0) * NS + ib], dyi = a.P32[(int64_t)(p * 8 + 1) * NS + ib];
+    const float dzi = a.P32[(int64_t)(p * 8 + 2) * NS + ib];
+    const float* qj = a.Q32 + (X.j * a.maxpts + q) * 8;
+    const float qx = qj[0], qy = qj[1], qz = qj[2];
+    const double aix = a.anchor[ib * 4 + 0], aiy = a.anchor[ib * 4 + 1], aiz = a.anchor[ib * 4 + 2];
+    const double ajx = a.anchor[X.j * 4 + 0], ajy = a.anchor[X.j * 4 + 1], ajz = a.anchor[X.j * 4 + 2];
+    const bool rev = (X.revmask >> b) & 1;
+    const float dax = (float)(ajx - aix), day = (float)(ajy - aiy), daz = (float)(ajz - aiz);
+    const float bx = dax - dxi, by = day - dyi, bz = daz - dzi;
+    const float rx = bx + qx, ry = by + qy, rz = bz + qz;
+    float r2;
+    {
+#pragma clang fp contract(fast)
+        r2 = rx * rx + ry * ry + rz * rz;
+    }
+    const float rs = __builtin_amdgcn_rsqf(r2);
+    const float sg = rev ? -rs : rs;
+    R.ox = rev ? (float)ajx + qx : (float)aix + dxi;
+    R.oy = rev ? (float)ajy + qy : (float)aiy + dyi;
+    R.oz = rev ? (float)ajz + qz : (float)aiz + dzi;
+    R.dx = rx * sg; R.dy = ry * sg; R.dz = rz * sg; R.L = r2 * rs;
+    const float uu = 5.9604645e-08f;
+    R.dd = 3.0f * uu * (fabsf(dax) + fabsf(day) + fabsf(daz) + fabsf(bx) + fabsf(by) + fabsf(bz) + fabsf(rx) +
+                        fabsf(ry) + fabsf(rz)) * rs + 4.0f * uu;
+    R.ix = __builtin_amdgcn_rcpf(R.dx); R.iy = __builtin_amdgcn_rcpf(R.dy); R.iz = __builtin_amdgcn_rcpf(R.dz);
+    return X.lroot[b];
+}
+
+// Exact FP64 decision for a triangle the FP32 filter kept (mixed mode): the
+// reference's ray (is_visible, RayCast.jl:27-30) and _ray_triangle; with the
+// reference soup reading also the leaf's exact slab chain.
+__device__ __forceinline__ bool qexact_hit(const PairArgs& a, const QCtx& X, int c, int b, int tri, int leaf) {
+    int p, q;
+    combo_pq(X, c, p, q);
+    const int64_t ib = X.sbase + b;
+    const int64_t NS = a.NS;
+    const double xi = a.P64[(int64_t)(p * 8 + 0) * NS + ib], yi = a.P64[(int64_t)(p * 8 + 1) * NS + ib];
+    const double zi = a.P64[(int64_t)(p * 8 + 2) * NS + ib];
+    const double* qj = a.Q64 + (X.j * a.maxpts + q) * 8;
+    const double xj = qj[0], yj = qj[1], zj = qj[2];
+    const bool rev = (X.revmask >> b) & 1;
+    const double r0 = xj - xi, r1 = yj - yi, r2 = zj - zi;
+    const double r = sqrt((r0 * r0 + r1 * r1) + r2 * r2);
+    Ray64 R;
+    // one division at a time: this path runs while every slot's ray is live,
+    // so it must not widen the register peak
+    R.dx = r0 / r;
+    __builtin_amdgcn_sched_barrier(0);
+    R.dy = r1 / r;
+    __builtin_amdgcn_sched_barrier(0);
+    R.dz = r2 / r;
+    __builtin_amdgcn_sched_barrier(0);
+    R.L = r;
+    if (rev) { R.dx = -R.dx; R.dy = -R.dy; R.dz = -R.dz; }
+    R.ox = rev ? xj : xi; R.oy = rev ? yj : yi; R.oz = rev ? zj : zi;
+    if (!mt_hit64(a.tris64[tri], R)) return false;
+    return !a.verify || reaches64(a.nodes64, a.parent, leaf, R, 1.0 / R.dx, 1.0 / R.dy, 1.0 / R.dz);
+}
+
+// One traversal step of an FP64 ray: returns the next item, -1 when the ray
+// is done; sets hit when a triangle blocks it.
+__device__ __forceinline__ int qstep(const PairArgs& a, const QCtx&, const Item64& it, int cur, bool act, int,
+                                     const QRay<false>& R, bool& hit, uint32_t& naabb, uint32_t& ntri) {
+    const bool is_node = cur < a.n_nodes;
+    // slab (BVH.jl:155-169, NaN slab values make the box a miss)
+    double t1x = (it.v[0] - R.ox) * R.ix, t2x = (it.v[3] - R.ox) * R.ix;
+    double t1y = (it.v[1] - R.oy) * R.iy, t2y = (it.v[4] - R.oy) * R.iy;
+    double t1z = (it.v[2] - R.oz) * R.iz, t2z = (it.v[5] - R.oz) * R.iz;
+    bool nan = (t1x != t1x) | (t2x != t2x) | (t1y != t1y) | (t2y != t2y) | (t1z != t1z) | (t2z != t2z);
+    double tmin = fmax(fmax(fmax(fmin(t1x, t2x), fmin(t1y, t2y)), fmin(t1z, t2z)), 0.0);
+    double tmax = fmin(fmin(fmin(fmax(t1x, t2x), fmax(t1y, t2y)), fmax(t1z, t2z)), R.L);
+    const bool pass = !nan && tmin <= tmax;
+    // Moller-Trumbore (BVH.jl:181-201) on the same record read as a triangle
+    Tri64 T;
+    T.v0[0] = it.v[0]; T.v0[1] = it.v[1]; T.v0[2] = it.v[2];
+    T.e1[0] = it.v[3]; T.e1[1] = it.v[4]; T.e1[2] = it.v[5];
+    T.e2[0] = it.v[6]; T.e2[1] = it.v[7]; T.e2[2] = it.v[8];
+    Ray64 r64{R.ox, R.oy, R.oz, R.dx, R.dy, R.dz, R.L};
+    const bool th = mt_hit64(T, r64);
+    naabb += act && is_node; ntri += act && !is_node;
+    hit = act && !is_node && th;
+    return (is_node && !pass) ? it.link_y : it.link_x;
+}
+
+// One traversal step of an FP32 ray (mixed mode).  A triangle the FP32
+// filter keeps is reported as a candidate (cand) for the exact FP64 test,
+// which the caller runs after all slots have stepped.
+__device__ __forceinline__ int qstep(const PairArgs& a, const QCtx&, const Item32& it, int cur, bool act, int,
+                                     const QRay<true>& R, bool& cand, uint32_t& nret, uint32_t&) {
+    const bool is_node = cur < a.n_nodes;
+    const bool pass = slab32(it.v, R.ox, R.oy, R.oz, R.ix, R.iy, R.iz, R.L);
+    const int f = mt_filter32_item(it.v, R.ox, R.oy, R.oz, R.dx, R.dy, R.dz, R.L, a.Sm, R.dd);
+    cand = act && !is_node && f;
+    nret += cand;
+    return (is_node && !pass) ? it.link_y : it.link_x;
+}
+
+template <bool MIX>
+using QItem = typename std::conditional<MIX, Item32, Item64>::type;
+
+template <bool MIX>
+__device__ __forceinline__ const QItem<MIX>* qitems(const PairArgs& a);
+template <>
+__device__ __forceinline__ const Item32* qitems<true>(const PairArgs& a) { return a.items32; }
+template <>
+__device__ __forceinline__ const Item64* qitems<false>(const PairArgs& a) { return a.items64; }
+
+// Phase B: drain the batch's rays; blocked rays set blk bits.  A slot is
+// refilled once at least refill_min lanes have it empty (or the wave has no
+// ray in flight), so the refill's cross-lane exchange and ray set-up are
+// amortised over several traversal steps.  The item fetches of all slots are
+// issued before any is used.
+template <bool MIX, int S>
+__device__ __forceinline__ void queue_drain(const PairArgs& a, const QCtx& X, int nc, const uint64_t* need,
+                                            uint32_t* blk, int lane, uint32_t& c0, uint32_t& c1) {
+    const QItem<MIX>* __restrict__ items = qitems<MIX>(a);
+    WaveQueue Q;
+    Q.c = 0; Q.nc = nc; Q.m = uniform64(need[0]);
+    int cur[S], tag[S], ctri[S], cleaf[S];
+    QRay<MIX> R[S];
+#pragma unroll
+    for (int s = 0; s < S; s++) { cur[s] = -1; tag[s] = 0; ctri[s] = 0; cleaf[s] = 0; }
+    for (;;) {
+        if (Q.c < Q.nc) {
+            bool any = false;
+#pragma unroll
+            for (int s = 0; s < S; s++) any |= cur[s] >= 0;
+            const bool idle = __ballot(any) == 0;
+#pragma unroll
+            for (int s = 0; s < S; s++) {
+                const uint64_t req = __ballot(cur[s] < 0);
+                if (Q.c < Q.nc && req != 0 && (idle || __popcll(req) >= a.refill_min)) {
+                    int c = 0, b = 0;
+                    if (queue_take(req, Q, need, lane, c, b)) {
+                        tag[s] = (c << 6) | b;
+                        cur[s] = qray_setup(a, X, c, b, R[s]);
+                    }
+                }
+            }
+        }
+        bool any = false;
+#pragma unroll
+        for (int s = 0; s < S; s++) any |= cur[s] >= 0;
+        if (__ballot(any) == 0) break;
+        bool hit[S];
+        {
+            QItem<MIX> it[S];
+#pragma unroll
+            for (int s = 0; s < S; s++) it[s] = items[max(cur[s], 0)];
+#pragma unroll
+            for (int s = 0; s < S; s++) {
+                const bool act = cur[s] >= 0;
+                const int tri = cur[s] - a.n_nodes;
+                const int nx = qstep(a, X, it[s], max(cur[s], 0), act, tag[s], R[s], hit[s], c0, c1);
+                if (MIX) { ctri[s] = tri; cleaf[s] = it[s].link_y; }
+                cur[s] = act ? nx : -1;
+            }
+        }
+        if (MIX) {  // exact FP64 decisions for the FP32 candidates
+#pragma unroll
+            for (int s = 0; s < S; s++)
+                if (hit[s]) hit[s] = qexact_hit(a, X, tag[s] >> 6, tag[s] & 63, ctri[s], cleaf[s]);
+        }
+#pragma unroll
+        for (int s = 0; s < S; s++) {
+            if (hit[s]) {
+                const int c = tag[s] >> 6, b = tag[s] & 63;
+                atomicOr(&blk[c * 2 + (b >> 5)], 1u << (b & 31));
+                cur[s] = -1;
+            }
+        }
+    }
+}
+
+template <bool MIX, int S>
+__device__ __forceinline__ void tile_occ_queue(const PairArgs& a, int tile, double* lds) {
+    const TileCtx t = tile_ctx(a, tile);
+    const Corners cwi = corners_of(a, t.oi, t.vi && a.duffy);
+    const int64_t NS = a.NS;
+    const int lane = t.lane;
+    const int npi = t.vi ? a.elem_np[t.i] : 0;
+    int npmax = npi;
+#pragma unroll
+    for (int o = 32; o > 0; o >>= 1) npmax = max(npmax, __shfl_xor(npmax, o, 64));
+    npmax = __builtin_amdgcn_readfirstlane(npmax);
+    const double Ai = t.vi ? a.area[t.i] : 1.0;
+    const int gi = t.vi ? a.elem_grp[t.i] : 0;
+    // LDS: [accq: maxpts x 64 doubles (FP64 only)][need: RVF_QCOMBO u64][blk: 2 RVF_QCOMBO u32]
+    double* accq = lds;
+    uint64_t* need = (uint64_t*)(lds + (MIX ? 0 : a.maxpts * 64));
+    uint32_t* blk = (uint32_t*)(need + RVF_QCOMBO);
+    int* lroot = (int*)(blk + 2 * RVF_QCOMBO);
+    const double* __restrict__ P64 = a.P64;
+    const float* __restrict__ P = a.P32;
+    double ls = 0.0, lc = 0.0;
+    uint32_t nrays = 0, c0 = 0, c1 = 0;
+    QCtx X;
+    X.sbase = (int64_t)t.s * RVF_STRIP;
+    X.lroot = lroot;
+    for (int64_t j = t.jb; j < t.je; j++) {
+        if (j <= (int64_t)t.s * RVF_STRIP) {
+            if (lane == 0) a.partT[(int64_t)t.s * a.NS + j] = 0.0;
+            continue;
+        }
+        bool act = t.vi && t.i < j;
+        const int64_t oj = a.slot_orig[j];
+        if (a.duffy && shares_corner(cwi, corners_of(a, oj, true))) act = false;
+        const bool rev = t.oi > oj;
+        const int npj = a.elem_np[j];
+        const int gj = a.elem_grp[j];
+        const int root = a.tree_root[gi * a.G + gj];
+        X.j = j; X.oj = oj; X.npj = npj; X.gj = gj;
+        X.revmask = __ballot(rev);
+        lroot[lane] = root;
+        X.pch = max(1, RVF_QCOMBO / npj);
+        X.magic = (65536u + (uint32_t)npj - 1u) / (uint32_t)npj;
+        const double* __restrict__ qj64 = a.Q64 + j * a.maxpts * 8;
+        const float* __restrict__ qj32 = a.Q32 + j * a.maxpts * 8;
+        if (!MIX)
+            for (int q = 0; q < npj; q++) accq[q * 64 + lane] = 0.0;
+        // mixed mode: anchored offsets of the column element
+        const double aix = a.anchor[t.i * 4 + 0], aiy = a.anchor[t.i * 4 + 1], aiz = a.anchor[t.i * 4 + 2];
+        const float dax = (float)(a.anchor[j * 4 + 0] - aix);
+        const float day = (float)(a.anchor[j * 4 + 1] - aiy);
+        const float daz = (float)(a.anchor[j * 4 + 2] - aiz);
+        double Fij = 0.0, Fs = 0.0, Fc = 0.0;
+        uint32_t bc = 0, bh = 0;
+        for (int p0 = 0; p0 < npmax; p0 += X.pch) {
+            const int p1 = min(npmax, p0 + X.pch);
+            const int nc = (p1 - p0) * npj;
+            X.p0 = p0;
+            // Phase A: need masks
+            for (int p = p0; p < p1; p++) {
+                const bool pv = act && p < npi && root >= 0;
+                if (MIX) {
+                    const float dxi = P[(int64_t)(p * 8 + 0) * NS + t.i], dyi = P[(int64_t)(p * 8 + 1) * NS + t.i];
+                    const float dzi = P[(int64_t)(p * 8 + 2) * NS + t.i];
+                    const float nxi = P[(int64_t)(p * 8 + 3) * NS + t.i], nyi = P[(int64_t)(p * 8 + 4) * NS + t.i];
+                    const float nzi = P[(int64_t)(p * 8 + 5) * NS + t.i];
+                    const float bx = dax - dxi, by = day - dyi, bz = daz - dzi;
+                    for (int q = 0; q < npj; q++) {
+                        float rx, ry, rz, r2;
+                        float k = vf_mixed(bx, by, bz, nxi, nyi, nzi, qj32[q * 8 + 0], qj32[q * 8 + 1], qj32[q * 8 + 2],
+                                           qj32[q * 8 + 3], qj32[q * 8 + 4], qj32[q * 8 + 5], rx, ry, rz, r2);
+                        const bool nd = pv && k != 0.0f;
+                        const uint64_t m = __ballot(nd);
+                        nrays += nd;
+                        const int c = (p - p0) * npj + q;
+                        if (lane == 0) { need[c] = m; blk[2 * c] = 0u; blk[2 * c + 1] = 0u; }
+                    }
+                } else {
+                    const double xi = P64[(int64_t)(p * 8 + 0) * NS + t.i], yi = P64[(int64_t)(p * 8 + 1) * NS + t.i];
+                    const double zi = P64[(int64_t)(p * 8 + 2) * NS + t.i];
+                    const double nxi = P64[(int64_t)(p * 8 + 3) * NS + t.i];
+                    const double nyi = P64[(int64_t)(p * 8 + 4) * NS + t.i];
+                    const double nzi = P64[(int64_t)(p * 8 + 5) * NS + t.i];
+                    for (int q = 0; q < npj; q++) {
+                        Ray64 Rr;
+                        const double K = vf_exact(xi, yi, zi, nxi, nyi, nzi, qj64[q * 8 + 0], qj64[q * 8 + 1],
+                                                  qj64[q * 8 + 2], qj64[q * 8 + 3], qj64[q * 8 + 4], qj64[q * 8 + 5], Rr);
+                        const bool nd = pv && K != 0.0;
+                        const uint64_t m = __ballot(nd);
+                        nrays += nd;
+                        const int c = (p - p0) * npj + q;
+                        if (lane == 0) { need[c] = m; blk[2 * c] = 0u; blk[2 * c + 1] = 0u; }
+                    }
+                }
+            }
+            __syncthreads();
+            // Phase B: trace
+            queue_drain<MIX, S>(a, X, nc, need, blk, lane, c0, c1);
+            __syncthreads();
+            // Phase C: sums in each lane's own order
+            for (int p = p0; p < p1; p++) {
+                if (MIX) {
+                    const float dxi = P[(int64_t)(p * 8 + 0) * NS + t.i], dyi = P[(int64_t)(p * 8 + 1) * NS + t.i];
+                    const float dzi = P[(int64_t)(p * 8 + 2) * NS + t.i];
+                    const float nxi = P[(int64_t)(p * 8 + 3) * NS + t.i], nyi = P[(int64_t)(p * 8 + 4) * NS + t.i];
+                    const float nzi = P[(int64_t)(p * 8 + 5) * NS + t.i];
+                    const float bx = dax - dxi, by = day - dyi, bz = daz - dzi;
+                    float inner = 0.0f;
+                    for (int q = 0; q < npj; q++) {
+                        float rx, ry, rz, r2;
+                        float k = vf_mixed(bx, by, bz, nxi, nyi, nzi, qj32[q * 8 + 0], qj32[q * 8 + 1], qj32[q * 8 + 2],
+                                           qj32[q * 8 + 3], qj32[q * 8 + 4], qj32[q * 8 + 5], rx, ry, rz, r2);
+                        const int c = (p - p0) * npj + q;
+                        if ((blk[2 * c + (lane >> 5)] >> (lane & 31)) & 1u) {
+                            k = 0.0f;
+                            bc++;
+                            bh += mix32((uint32_t)(rev ? q * npi + p : p * npj + q) + 0x9e3779b9U);
+                        }
+                        inner = fmaf(qj32[q * 8 + 6], k, inner);
+                    }
+                    if (p < npi) {
+                        double wd = P64[(int64_t)(p * 8 + 6) * NS + t.i] * P64[(int64_t)(p * 8 + 7) * NS + t.i];
+                        neumaier(Fs, Fc, wd * (double)inner);
+                    }
+                } else {
+                    const double xi = P64[(int64_t)(p * 8 + 0) * NS + t.i], yi = P64[(int64_t)(p * 8 + 1) * NS + t.i];
+                    const double zi = P64[(int64_t)(p * 8 + 2) * NS + t.i];
+                    const double nxi = P64[(int64_t)(p * 8 + 3) * NS + t.i];
+                    const double nyi = P64[(int64_t)(p * 8 + 4) * NS + t.i];
+                    const double nzi = P64[(int64_t)(p * 8 + 5) * NS + t.i];
+                    const double wi = P64[(int64_t)(p * 8 + 6) * NS + t.i], dAi = P64[(int64_t)(p * 8 + 7) * NS + t.i];
+                    double inner = 0.0;
+                    for (int q = 0; q < npj; q++) {
+                        Ray64 Rr;
+                        double K = vf_exact(xi, yi, zi, nxi, nyi, nzi, qj64[q * 8 + 0], qj64[q * 8 + 1], qj64[q * 8 + 2],
+                                            qj64[q * 8 + 3], qj64[q * 8 + 4], qj64[q * 8 + 5], Rr);
+                        const int c = (p - p0) * npj + q;
+                        if ((blk[2 * c + (lane >> 5)] >> (lane & 31)) & 1u) {
+                            K = 0.0;
+                            bc++;
+                            bh += mix32((uint32_t)(rev ? q * npi + p : p * npj + q) + 0x9e3779b9U);
+                        }
+                        inner += (qj64[q * 8 + 6] * K) * qj64[q * 8 + 7];
+                        if (rev && p < npi) accq[q * 64 + lane] += (wi * K) * dAi;
+                    }
+                    if (p < npi) Fij += (wi * inner) * dAi;
+                }
+            }
+            __syncthreads();
+        }
+        double raw;
+        if (MIX) {
+            raw = (Fs + Fc) * (1.0 / PI_D);
+        } else {
+            if (rev) {
+                Fij = 0.0;
+                for (int q = 0; q < npj; q++) Fij += (qj64[q * 8 + 6] * accq[q * 64 + lane]) * qj64[q * 8 + 7];
+            }
+            raw = Fij;
+        }
+        if (act && a.audit_count) {
+            a.audit_count[min(t.oi, oj) + a.N * max(t.oi, oj)] = bc;
+            a.audit_hash[min(t.oi, oj) + a.N * max(t.oi, oj)] = bh;
+        }
+        pair_out(a, t, j, oj, act, raw, Ai, ls, lc);
+    }
+    a.part[(int64_t)t.chunk * NS + t.i] = ls + lc;
+    if (a.counters) {
+        unsigned long long r1 = nrays, r2 = c0, r3 = c1;
+        for (int o = 32; o > 0; o >>= 1) {
+            r1 += __shfl_xor(r1, o, 64); r2 += __shfl_xor(r2, o, 64); r3 += __shfl_xor(r3, o, 64);
+        }
+        if (lane == 0) {
+            atomicAdd(&a.counters[1], r1);
+            if (MIX) atomicAdd(&a.counters[0], r2);
+            else { atomicAdd(&a.counters[2], r2); atomicAdd(&a.counters[3], r3); }
+        }
+    }
+}
+
+// ---------------------------------------------------------------------------
 // Monte Carlo (GPUMCKernels.jl:37-50,146-281): the reference's per-pair
 // stream (splitmix64-seeded xorshift64, thread id (i-1)N + j), stratum k of
 // element i paired with stratum k of element j, FP64 throughout; visibility
@@ -854,8 +1371,8 @@ __global__ __launch_bounds__(64) void k_mc_area(PairArgs a, double* area, double
     area_orig[a.slot_orig[e]] = area[e];
 }
 
-__global__ __launch_bounds__(64) void k_pair_mc(PairArgs a) {
-    const TileCtx t = tile_ctx(a);
+__device__ __forceinline__ void tile_mc(const PairArgs& a, int tile) {
+    const TileCtx t = tile_ctx(a, tile);
     const Corners cwi = corners_of(a, t.oi, false);
     (void)cwi;
     const int64_t NS = a.NS;
@@ -1152,6 +1669,57 @@ __global__ __launch_bounds__(256) void k_reduce_fin(const double* __restrict__ T
 }
 
 // ---------------------------------------------------------------------------
+// Kernel entry points.  Direct launches map one tile per workgroup; the
+// persistent batched form drains a device-resident queue of (request, tile)
+// work items from many meshes in one launch: each wavefront takes the next
+// item with one atomic and exits when the queue is empty.
+// ---------------------------------------------------------------------------
+enum PairKind { PK_FP64_4, PK_FP64_9, PK_FP64_16, PK_FP64_OCC, PK_MIX_4, PK_MIX_9, PK_MIX_16, PK_MIX_16C, PK_MIX_OCC,
+                PK_MC, PK_FP64_Q1, PK_FP64_Q2, PK_FP64_Q3, PK_MIX_Q2, PK_MIX_Q3, PK_MIX_Q4, PK_MIX_Q6, PK_COUNT };
+
+template <int KIND>
+__device__ __forceinline__ void run_tile(const PairArgs& a, int tile, double* lds) {
+    if (KIND == PK_FP64_4) tile_fp64<4>(a, tile);
+    else if (KIND == PK_FP64_9) tile_fp64<9>(a, tile);
+    else if (KIND == PK_FP64_16) tile_fp64<16>(a, tile);
+    else if (KIND == PK_FP64_OCC) tile_fp64_occ(a, tile, lds);
+    else if (KIND == PK_MIX_4) tile_mixed<4, true>(a, tile);
+    else if (KIND == PK_MIX_9) tile_mixed<9, true>(a, tile);
+    else if (KIND == PK_MIX_16) tile_mixed<16, true>(a, tile);
+    else if (KIND == PK_MIX_16C) tile_mixed<16, false>(a, tile);
+    else if (KIND == PK_MIX_OCC) tile_mixed_occ(a, tile);
+    else if (KIND == PK_FP64_Q1) tile_occ_queue<false, 1>(a, tile, lds);
+    else if (KIND == PK_FP64_Q2) tile_occ_queue<false, 2>(a, tile, lds);
+    else if (KIND == PK_FP64_Q3) tile_occ_queue<false, 3>(a, tile, lds);
+    else if (KIND == PK_MIX_Q2) tile_occ_queue<true, 2>(a, tile, lds);
+    else if (KIND == PK_MIX_Q3) tile_occ_queue<true, 3>(a, tile, lds);
+    else if (KIND == PK_MIX_Q4) tile_occ_queue<true, 4>(a, tile, lds);
+    else if (KIND == PK_MIX_Q6) tile_occ_queue<true, 6>(a, tile, lds);
+    else tile_mc(a, tile);
+}
+
+template <int KIND>
+__global__ __launch_bounds__(64) void k_pair(PairArgs a) {
+    extern __shared__ double lds[];
+    run_tile<KIND>(a, blockIdx.x, lds);
+}
+
+template <int KIND>
+__global__ __launch_bounds__(64) void k_pair_batched(const PairArgs* __restrict__ reqs, const int2* __restrict__ work,
+                                                     int nwork, int* __restrict__ head) {
+    extern __shared__ double lds[];
+    for (;;) {
+        int w = 0;
+        if (threadIdx.x == 0) w = atomicAdd(head, 1);
+        w = __shfl(w, 0, 64);
+        if (w >= nwork) break;
+        const int2 it = work[w];
+        const PairArgs& a = reqs[it.x];
+        run_tile<KIND>(a, it.y, lds);
+    }
+}
+
+// ---------------------------------------------------------------------------
 // Utility kernels: zero fill (graph-capturable replacement for memset) and a
 // plain 16-byte-vector copy (reference point for HBM traffic counters).
 // ---------------------------------------------------------------------------
@@ -1172,6 +1740,16 @@ __global__ __launch_bounds__(256) void k_copy(const uint4* __restrict__ src, uin
 // ---------------------------------------------------------------------------
 // Launch wrappers (host side of this translation unit)
 // ---------------------------------------------------------------------------
+template <int KIND>
+static void launch_direct(const PairArgs* a, size_t lds, hipStream_t st) {
+    hipLaunchKernelGGL(k_pair<KIND>, dim3((unsigned)a->ntiles), dim3(64), lds, st, *a);
+}
+template <int KIND>
+static void launch_batched(const PairArgs* reqs, const int2* work, int nwork, int* head, int grid, size_t lds,
+                           hipStream_t st) {
+    hipLaunchKernelGGL(k_pair_batched<KIND>, dim3((unsigned)grid), dim3(64), lds, st, reqs, work, nwork, head);
+}
+
 extern "C" {
 int rvf_launch_precompute(const PrecompArgs* a, hipStream_t st) {
     int64_t nb = (a->N + 63) / 64;
@@ -1180,28 +1758,54 @@ int rvf_launch_precompute(const PrecompArgs* a, hipStream_t st) {
     return (int)hipGetLastError();
 }
 
-int rvf_launch_pair(const PairArgs* a, int mixed, int occ, int pc, float Sm, hipStream_t st) {
-    if (a->ntiles == 0) return 0;
-    dim3 g((unsigned)a->ntiles), b(64);
-    if (!mixed) {
-        if (occ) hipLaunchKernelGGL(k_pair_fp64_occ, g, b, sizeof(double) * 64 * a->maxpts, st, *a);
-        else if (pc <= 4) hipLaunchKernelGGL(k_pair_fp64<4>, g, b, 0, st, *a);
-        else if (pc <= 9) hipLaunchKernelGGL(k_pair_fp64<9>, g, b, 0, st, *a);
-        else hipLaunchKernelGGL(k_pair_fp64<16>, g, b, 0, st, *a);
-    } else {
-        if (occ) hipLaunchKernelGGL(k_pair_mixed_occ, g, b, 0, st, *a, Sm);
-        else if (pc <= 4) hipLaunchKernelGGL((k_pair_mixed<4, true>), g, b, 0, st, *a);
-        else if (pc <= 9) hipLaunchKernelGGL((k_pair_mixed<9, true>), g, b, 0, st, *a);
-        else if (pc <= 16) hipLaunchKernelGGL((k_pair_mixed<16, true>), g, b, 0, st, *a);
-        else hipLaunchKernelGGL((k_pair_mixed<16, false>), g, b, 0, st, *a);
+int rvf_pair_kind(int mixed, int occ, int pc, int mc) {
+    if (mc) return PK_MC;
+    if (!mixed) return occ ? PK_FP64_OCC : pc <= 4 ? PK_FP64_4 : pc <= 9 ? PK_FP64_9 : PK_FP64_16;
+    return occ ? PK_MIX_OCC : pc <= 4 ? PK_MIX_4 : pc <= 9 ? PK_MIX_9 : pc <= 16 ? PK_MIX_16 : PK_MIX_16C;
+}
+
+#define RVF_KIND_SWITCH(kind, CALL)                 \
+    switch (kind) {                                 \
+    case PK_FP64_4: CALL(PK_FP64_4); break;         \
+    case PK_FP64_9: CALL(PK_FP64_9); break;         \
+    case PK_FP64_16: CALL(PK_FP64_16); break;       \
+    case PK_FP64_OCC: CALL(PK_FP64_OCC); break;     \
+    case PK_MIX_4: CALL(PK_MIX_4); break;           \
+    case PK_MIX_9: CALL(PK_MIX_9); break;           \
+    case PK_MIX_16: CALL(PK_MIX_16); break;         \
+    case PK_MIX_16C: CALL(PK_MIX_16C); break;       \
+    case PK_MIX_OCC: CALL(PK_MIX_OCC); break;       \
+    case PK_FP64_Q1: CALL(PK_FP64_Q1); break;       \
+    case PK_FP64_Q2: CALL(PK_FP64_Q2); break;       \
+    case PK_FP64_Q3: CALL(PK_FP64_Q3); break;       \
+    case PK_MIX_Q2: CALL(PK_MIX_Q2); break;         \
+    case PK_MIX_Q3: CALL(PK_MIX_Q3); break;         \
+    case PK_MIX_Q4: CALL(PK_MIX_Q4); break;         \
+    case PK_MIX_Q6: CALL(PK_MIX_Q6); break;         \
+    default: CALL(PK_MC); break;                    \
     }
+
+int rvf_launch_pair(const PairArgs* a, int kind, size_t lds, hipStream_t st) {
+    if (a->ntiles == 0) return 0;
+#define DIRECT(K) launch_direct<K>(a, lds, st)
+    RVF_KIND_SWITCH(kind, DIRECT)
+#undef DIRECT
+    return (int)hipGetLastError();
+}
+
+int rvf_launch_pair_batched(int kind, const PairArgs* reqs, const int2* work, int nwork, int* head, int grid,
+                            size_t lds, hipStream_t st) {
+    if (nwork == 0) return 0;
+#define BATCHED(K) launch_batched<K>(reqs, work, nwork, head, grid, lds, st)
+    RVF_KIND_SWITCH(kind, BATCHED)
+#undef BATCHED
     return (int)hipGetLastError();
 }
 
 int rvf_launch_mc(const PairArgs* a, double* area, double* area_orig, hipStream_t st) {
     if (a->N > 0)
         hipLaunchKernelGGL(k_mc_area, dim3((unsigned)((a->N + 63) / 64)), dim3(64), 0, st, *a, area, area_orig);
-    if (a->ntiles > 0) hipLaunchKernelGGL(k_pair_mc, dim3((unsigned)a->ntiles), dim3(64), 0, st, *a);
+    if (a->ntiles > 0) launch_direct<PK_MC>(a, 0, st);
     return (int)hipGetLastError();
 }
 

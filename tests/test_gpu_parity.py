@@ -17,11 +17,11 @@ from rvf_amd import mesh as M
 pytestmark = pytest.mark.gpu
 
 
-def gpu(mesh, nquad, obs=(), precision="fp64", parity="reference", audit=False, want_F=True):
+def gpu(mesh, nquad, obs=(), precision="fp64", parity="reference", audit=False, want_F=True, variant=0):
     ctx = api.context(0)
     dm = api.DeviceMesh(ctx, mesh)
     do = api.DeviceObstruction(ctx, dm, obs, parity) if len(obs) else None
-    plan = api.Plan(ctx, dm, do, nquad, "quad", precision, want_F=want_F, audit=audit)
+    plan = api.Plan(ctx, dm, do, nquad, "quad", precision, want_F=want_F, audit=audit, variant=variant)
     plan.run(); plan.sync()
     F, A, rs, Fg, Ag, tags, ac, ah = plan.fetch()
     st = plan.stats()
@@ -37,8 +37,8 @@ def oracle(orc, mesh, nquad, obs=(), parity="reference", audit=False, rows=None)
     return r, F
 
 
-def check_fp64(orc, mesh, nquad, obs=(), parity="reference"):
-    g = gpu(mesh, nquad, obs, "fp64", parity, audit=len(obs) > 0)
+def check_fp64(orc, mesh, nquad, obs=(), parity="reference", variant=0):
+    g = gpu(mesh, nquad, obs, "fp64", parity, audit=len(obs) > 0, variant=variant)
     r, F = oracle(orc, mesh, nquad, obs, parity, audit=len(obs) > 0)
     assert np.array_equal(g["A"], r["A"]), "areas differ"
     assert np.array_equal(g["F"], F), f"F differs: max {np.abs(g['F'] - F).max()}"
@@ -88,6 +88,30 @@ def test_occluded_fp64_visibility_bitexact(orc, parity):
     m = M.occluded_plates(n=8, subdiv=2)
     g, r, F = check_fp64(orc, m, 3, obs=[3], parity=parity)
     assert g["ac"].sum() > 0  # the sphere blocks something
+
+
+@pytest.mark.parametrize("variant", api.FP64_VARIANTS)
+@pytest.mark.parametrize("parity", ["reference", "geometric"])
+def test_occluded_fp64_variants_bitexact(orc, parity, variant):
+    # every traversal variant gives the oracle's F and visibility bit for bit
+    m = M.occluded_plates(n=8, subdiv=2)
+    g, r, F = check_fp64(orc, m, 3, obs=[3], parity=parity, variant=variant)
+    assert g["ac"].sum() > 0
+
+
+@pytest.mark.parametrize("variant", [api.VARIANT_QUEUE(2), api.VARIANT_LANE])
+def test_occluded_fp64_random_boxes_nquad5(orc, variant):
+    # Quad8/Tri3/Quad4 mixes with box occluders; nquad 5 makes the (p, q)
+    # combinations of a Quad8 pair (625) span several queue batches
+    rng = np.random.default_rng(0x5EED0007)
+    done = 0
+    while done < 2:
+        mesh, obs, nq = M.random_box_enclosure(rng, int(rng.integers(80, 160)), max_occluders=2,
+                                               max_occluder_tris=300)
+        if not obs:
+            continue
+        check_fp64(orc, mesh, 5, obs=obs, parity="geometric", variant=variant)
+        done += 1
 
 
 def test_obstruction_group_that_radiates(orc):
@@ -209,14 +233,29 @@ def test_cube48_full_size(orc):
     assert float(np.abs(a["rs"] - b["rs"]).max()) <= 1e-6
 
 
-def test_fp64_traversal_counts_match_oracle(orc):
-    # the FP64 kernel walks the canonical stackless order (GPUBVH.jl:202-285):
+@pytest.mark.parametrize("variant", api.FP64_VARIANTS)
+def test_fp64_traversal_counts_match_oracle(orc, variant):
+    # the FP64 kernels walk the canonical stackless order (GPUBVH.jl:202-285):
     # rays, box tests and triangle tests equal the oracle's counts exactly
     m = M.occluded_plates(n=6, subdiv=2)
-    g = gpu(m, 2, [3], "fp64", "geometric", want_F=False)
+    g = gpu(m, 2, [3], "fp64", "geometric", want_F=False, variant=variant)
     r = orc.compute(m, nquad=2, obstruction_groups=[3], parity_mode="geometric", count=True, want_raw=False)
     assert (g["stats"]["rays"], g["stats"]["aabb_tests"], g["stats"]["tri_tests"]) == \
         (r["rays"], r["aabb_tests"], r["tri_tests"])
+
+
+@pytest.mark.parametrize("variant", api.MIXED_VARIANTS)
+def test_mixed_variants_agree(variant):
+    # mixed-precision decisions are the exact FP64 test on the FP32 traversal's
+    # candidates, so every variant reports the same blocked rays
+    m = M.unit_cube(8)
+    V, T = M.icosphere(3, 0.3, (0.5, 0.5, 0.5), seed=0x5EED0003)
+    m.group_tri_soup = {7: V[T]}
+    ref = gpu(m, 3, [7], "fp64", "geometric", audit=True)
+    got = gpu(m, 3, [7], "mixed", "geometric", audit=True, variant=variant)
+    iu = np.triu_indices(m.n_elems, 1)
+    assert np.array_equal(got["ac"][iu], ref["ac"][iu]) and np.array_equal(got["ah"][iu], ref["ah"][iu])
+    assert float(np.abs(got["F"] - ref["F"]).max()) <= 1e-6
 
 
 def test_mixed_cube_with_sphere_flips():
