@@ -70,6 +70,8 @@ typedef struct rvf_opts {
 #define RVF_VARIANT_LANE 1        /* each lane traces its own (p, q) rays, one at a time */
 #define RVF_VARIANT_QUEUE(S) (10 + (S))  /* wave-shared ray queue, S rays in flight per lane:
                                            FP64 S = 1..3, mixed S = 2, 3, 4, 6 */
+#define RVF_VARIANT_SPLIT(S) (20 + (S))  /* as QUEUE, slots walk nodes only; triangle tests run
+                                           wave-wide from an LDS task ring: S = 1..3 */
 
 typedef struct rvf_stats {
     double precompute_ms, pair_ms, duffy_ms, fill_ms, reduce_ms, total_ms;
@@ -136,6 +138,11 @@ int rvf_plan_fetch(rvf_plan* plan, double* F, double* A, double* rowsum, double*
 int rvf_plan_stats(const rvf_plan* plan, rvf_stats* st);
 /* device pointer of an output (0 = F, 1 = A, 2 = rowsum) for device-side use */
 int rvf_plan_device_ptr(const rvf_plan* plan, int which, void** ptr);
+/* Tuning knobs of an existing plan (no counterpart in the reference; used by
+ * the autotuner): traversal variant (RVF_VARIANT_*, 0 = default) and the
+ * wave-queue refill threshold (lanes with an empty slot before a refill,
+ * 1..64; 0 = default 16).  Results do not depend on either. */
+int rvf_plan_tune(rvf_plan* plan, int32_t variant, int32_t refill_min);
 void rvf_plan_free(rvf_plan* plan);
 
 /* One-shot: the body of the Julia hook (compute_view_factors_gpu,

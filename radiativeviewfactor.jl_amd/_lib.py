@@ -71,6 +71,7 @@ def lib():
         "rvf_plan_fetch": ([P, P, P, P, P, P, P, P, P], C.c_int),
         "rvf_plan_stats": ([P, P], C.c_int),
         "rvf_plan_device_ptr": ([P, C.c_int, PP], C.c_int),
+        "rvf_plan_tune": ([P, C.c_int32, C.c_int32], C.c_int),
         "rvf_plan_free": ([P], None),
         "rvf_compute_view_factors": ([P, P, P, P, P, P, P, P, P, P], C.c_int),
         "rvf_quad_rule": ([I32, I32, P, P], C.c_int),

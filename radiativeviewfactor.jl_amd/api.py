@@ -29,8 +29,18 @@ def VARIANT_QUEUE(slots: int) -> int:
     return 10 + slots
 
 
-FP64_VARIANTS = (VARIANT_LANE, VARIANT_QUEUE(1), VARIANT_QUEUE(2), VARIANT_QUEUE(3))
-MIXED_VARIANTS = (VARIANT_LANE, VARIANT_QUEUE(2), VARIANT_QUEUE(3), VARIANT_QUEUE(4), VARIANT_QUEUE(6))
+def VARIANT_SPLIT(slots: int) -> int:
+    """Ray slots walk nodes only; triangle tests run wave-wide from a task ring."""
+    return 20 + slots
+
+
+FP64_VARIANTS = (VARIANT_LANE, VARIANT_QUEUE(1), VARIANT_QUEUE(2), VARIANT_QUEUE(3), VARIANT_SPLIT(1),
+                 VARIANT_SPLIT(2), VARIANT_SPLIT(3))
+MIXED_VARIANTS = (VARIANT_LANE, VARIANT_QUEUE(2), VARIANT_QUEUE(3), VARIANT_QUEUE(4), VARIANT_QUEUE(6),
+                  VARIANT_SPLIT(1), VARIANT_SPLIT(2), VARIANT_SPLIT(3))
+# variants whose FP64 traversal stops at the first blocking triangle, so their
+# box / triangle test counters equal the canonical stackless counts
+EXACT_COUNT_VARIANTS = (VARIANT_LANE, VARIANT_QUEUE(1), VARIANT_QUEUE(2), VARIANT_QUEUE(3))
 
 SOUP_READ = {"reference": 0, ("geometric", "xyz_vertex"): 1, ("geometric", "vertex_xyz"): 2}
 
@@ -209,6 +219,11 @@ class Plan:
                                          C.byref(h)))
         self.h = h
         self.want_F, self.audit = want_F, audit and dobs is not None
+
+    def tune(self, variant: int = 0, refill_min: int = 0):
+        """Switch the traversal variant / wave-queue refill threshold (results unchanged)."""
+        check(_lib.lib().rvf_plan_tune(self.h, variant, refill_min))
+        self.opts.variant = variant
 
     def run(self):
         check(_lib.lib().rvf_plan_run(self.h))

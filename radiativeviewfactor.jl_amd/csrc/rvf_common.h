@@ -35,14 +35,14 @@ struct __align__(16) Tri64 {
     double pad;
 };
 
-// FP32 triangle (48 B): v0, e1, e2 and the determinant band scale.
+// FP32 triangle (48 B): v0, e1, e2 and edge magnitudes for error bounds.
 struct __align__(16) Tri32 {
     float v0[3];
     float sa;        // |e1|_1 * |e2|_1, scales the |a| < 1e-10 band
     float e1[3];
-    float pad0;
+    float E1;        // |e1|_1 of the FP64 edge, rounded up
     float e2[3];
-    float pad1;
+    float E2;        // |e2|_1 of the FP64 edge, rounded up
 };
 
 // Traversal items for the wave-queue kernels: the nodes of every tree

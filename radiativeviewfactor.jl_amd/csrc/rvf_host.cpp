@@ -580,7 +580,8 @@ int rvf_obs_build(rvf_ctx* ctx, const rvf_mesh* mesh, const double* soup, const 
                 E1 += std::fabs(d.e1[c]); E2 += std::fabs(d.e2[c]);
             }
             f.sa = (float)(E1 * E2 * (1 + 1e-6));
-            f.pad0 = f.pad1 = 0;
+            f.E1 = f32_up(E1 * (1 + 1e-6));
+            f.E2 = f32_up(E2 * (1 + 1e-6));
         }
     }
     // wave-queue items: nodes [0, M), then triangles [M, M+T)
@@ -699,6 +700,7 @@ struct rvf_plan {
     hipEvent_t ev[8] = {};
     rvf_stats st{};
     bool ran = false;
+    int refill_min = 16;
 };
 
 static int occ_kind(int variant, bool mixed);
@@ -844,7 +846,7 @@ static PairArgs pair_args(rvf_plan* p) {
         a.parent = p->obs->d_parent.as<int32_t>();
         a.items32 = p->obs->d_items32.as<Item32>(); a.items64 = p->obs->d_items64.as<Item64>();
         a.n_nodes = (int)p->obs->n_nodes;
-        a.refill_min = 16;
+        a.refill_min = p->refill_min;
         a.verify = p->obs->soup_read == RVF_SOUP_REFERENCE ? 1 : 0;
         a.Sm = p->obs->Sm;
     } else {
@@ -862,8 +864,10 @@ static PairArgs pair_args(rvf_plan* p) {
 
 enum {  // pair-kernel kinds (rvf_kernels.hip)
     K_FP64_OCC = 3, K_MIX_OCC = 8, K_FP64_Q1 = 10, K_FP64_Q2 = 11, K_FP64_Q3 = 12, K_MIX_Q2 = 13, K_MIX_Q3 = 14,
-    K_MIX_Q4 = 15, K_MIX_Q6 = 16
+    K_MIX_Q4 = 15, K_MIX_Q6 = 16, K_FP64_T1 = 17, K_FP64_T2 = 18, K_FP64_T4 = 19, K_MIX_T2 = 20, K_MIX_T4 = 21,
+    K_MIX_T6 = 22
 };
+#define RVF_TQCAP 2048
 #define RVF_QCOMBO 128
 
 static bool plan_occluded_quad(const rvf_plan* p) {
@@ -872,8 +876,13 @@ static bool plan_occluded_quad(const rvf_plan* p) {
 
 // Kernel kind for an occluded quadrature plan's variant, -1 if invalid.
 static int occ_kind(int variant, bool mixed) {
-    if (variant == RVF_VARIANT_DEFAULT) variant = mixed ? RVF_VARIANT_QUEUE(2) : RVF_VARIANT_QUEUE(1);
+    if (variant == RVF_VARIANT_DEFAULT) variant = mixed ? RVF_VARIANT_SPLIT(3) : RVF_VARIANT_QUEUE(1);
     if (variant == RVF_VARIANT_LANE) return mixed ? K_MIX_OCC : K_FP64_OCC;
+    if (variant > 20) {  // split node / triangle-task form
+        const int S = variant - 20;
+        if (!mixed) return S == 1 ? K_FP64_T1 : S == 2 ? K_FP64_T2 : S == 3 ? K_FP64_T4 : -1;
+        return S == 1 ? K_MIX_T6 : S == 2 ? K_MIX_T2 : S == 3 ? K_MIX_T4 : -1;
+    }
     const int S = variant - 10;
     if (!mixed) return S == 1 ? K_FP64_Q1 : S == 2 ? K_FP64_Q2 : S == 3 ? K_FP64_Q3 : -1;
     return S == 2 ? K_MIX_Q2 : S == 3 ? K_MIX_Q3 : S == 4 ? K_MIX_Q4 : S == 6 ? K_MIX_Q6 : -1;
@@ -894,7 +903,10 @@ static size_t pair_lds(const rvf_plan* p) {
     const size_t accq = mixed ? 0 : sizeof(double) * 64 * (size_t)p->maxpts;
     if (kind == K_FP64_OCC) return accq;
     if (kind == K_MIX_OCC) return 0;
-    return accq + RVF_QCOMBO * (sizeof(uint64_t) + 2 * sizeof(uint32_t)) + 64 * sizeof(int32_t);
+    const size_t ring =
+        kind >= K_FP64_T1 ? RVF_TQCAP * sizeof(uint32_t) + (mixed ? 128 * sizeof(uint32_t) + 6 * 64 * sizeof(float) : 0)
+                          : 0;
+    return accq + RVF_QCOMBO * (sizeof(uint64_t) + 2 * sizeof(uint32_t)) + 64 * sizeof(int32_t) + ring;
 }
 
 // Enqueue one solve on stream st.  With tim, HIP events bracket each stage
@@ -952,6 +964,20 @@ static int enqueue_solve(rvf_plan* p, hipStream_t st, bool tim) {
         if (rc) return fail(RVF_E_HIP, std::string("fill launch: ") + hipGetErrorString((hipError_t)rc));
     }
     if (tim) HIPCHK(hipEventRecord(p->ev[5], st));
+    return 0;
+}
+
+int rvf_plan_tune(rvf_plan* p, int32_t variant, int32_t refill_min) {
+    if (!p) return fail(RVF_E_ARG, "rvf_plan_tune: null plan");
+    if (refill_min < 0 || refill_min > 64) return fail(RVF_E_ARG, "rvf_plan_tune: refill_min must be in 0..64");
+    if (p->obs && !p->obs->trees.empty() && p->opts.method != RVF_METHOD_MC &&
+        occ_kind(variant, p->opts.precision == RVF_MIXED) < 0)
+        return fail(RVF_E_ARG, "rvf_plan_tune: unknown traversal variant for this precision");
+    HIPCHK(hipSetDevice(p->ctx->device));
+    HIPCHK(hipStreamSynchronize(p->ctx->stream));
+    p->opts.variant = variant;
+    p->refill_min = refill_min ? refill_min : 16;
+    if (p->gexec) { (void)hipGraphExecDestroy(p->gexec); p->gexec = nullptr; }  // re-captured on the next run
     return 0;
 }
 

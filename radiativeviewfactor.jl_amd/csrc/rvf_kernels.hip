@@ -19,6 +19,7 @@
 // box traversal on padded bounds with an FP32 ray-triangle filter whose
 // undecided cases are re-tested in FP64 with the reference arithmetic.
 #include "rvf_common.h"
+#include "rvf_filter.h"
 
 #include <algorithm>
 #include <type_traits>
@@ -910,6 +911,8 @@ struct QCtx {
     uint32_t magic;      // floor(c / npj) = (c * magic) >> 16 for c < RVF_QCOMBO
     uint64_t revmask;    // lanes whose element is the pair's inner element
     const int* lroot;    // LDS: per lane, tree root of (lane's group, column group)
+    const float* stage;  // LDS (mixed, split form): per lane [dax day daz fax fay faz] x 64
+    float fjx, fjy, fjz; // column element's anchor as FP32
 };
 
 template <bool MIX>
@@ -1135,7 +1138,255 @@ __device__ __forceinline__ void queue_drain(const PairArgs& a, const QCtx& X, in
     }
 }
 
+// ---------------------------------------------------------------------------
+// Split form of the wave-queue traversal: the S ray slots of a lane only walk
+// nodes (a slot holds origin, inverse direction and length); each triangle of
+// a leaf a ray reaches becomes a task in an LDS ring, and the whole wavefront
+// runs those tasks 64 at a time, so the triangle test runs at full width
+// instead of riding along with every node step.  A ray keeps walking past a
+// leaf whose tasks are pending; a task that finds a hit sets the ray's blocked
+// bit, and the slot stops at its next leaf once it sees the bit.  The blocked
+// decision is the OR over every triangle of every leaf the traversal reaches,
+// which is the stackless any-hit result (BVH.jl:213-250).
+// ---------------------------------------------------------------------------
+#define RVF_TQCAP 2048  // triangle tasks per wavefront ring (power of two): one iteration of S <= 3
+                        // slots pushes at most S x 64 lanes x 8 triangles per leaf
+#define RVF_TQ_TRI_BITS 19  // a task packs (tag << 19) | triangle
+#define RVF_CQCAP 128       // mixed mode: candidates for the exact FP64 test (>= 64 + 63)
+
+template <bool MIX>
+struct NRay;
+template <>
+struct NRay<false> { double ox, oy, oz, ix, iy, iz, L; };
+template <>
+struct NRay<true> { float ox, oy, oz, ix, iy, iz, L; };
+
+__device__ __forceinline__ int nray_setup(const PairArgs& a, const QCtx& X, int c, int b, NRay<false>& N) {
+    QRay<false> R;
+    const int root = qray_setup(a, X, c, b, R);
+    N.ox = R.ox; N.oy = R.oy; N.oz = R.oz; N.ix = R.ix; N.iy = R.iy; N.iz = R.iz; N.L = R.L;
+    return root;
+}
+__device__ __forceinline__ int nray_setup(const PairArgs& a, const QCtx& X, int c, int b, NRay<true>& N) {
+    QRay<true> R;
+    const int root = qray_setup(a, X, c, b, R);
+    N.ox = R.ox; N.oy = R.oy; N.oz = R.oz; N.ix = R.ix; N.iy = R.iy; N.iz = R.iz; N.L = R.L;
+    return root;
+}
+
+// Node step: returns the next node (-1 = done); a passed leaf reports its
+// first triangle and count.
+__device__ __forceinline__ int nstep(const BvhNode64& nd, int cur, const NRay<false>& N, int& first, int& cnt) {
+    double t1x = (nd.lo[0] - N.ox) * N.ix, t2x = (nd.hi[0] - N.ox) * N.ix;
+    double t1y = (nd.lo[1] - N.oy) * N.iy, t2y = (nd.hi[1] - N.oy) * N.iy;
+    double t1z = (nd.lo[2] - N.oz) * N.iz, t2z = (nd.hi[2] - N.oz) * N.iz;
+    bool nan = (t1x != t1x) | (t2x != t2x) | (t1y != t1y) | (t2y != t2y) | (t1z != t1z) | (t2z != t2z);
+    double tmin = fmax(fmax(fmax(fmin(t1x, t2x), fmin(t1y, t2y)), fmin(t1z, t2z)), 0.0);
+    double tmax = fmin(fmin(fmin(fmax(t1x, t2x), fmax(t1y, t2y)), fmax(t1z, t2z)), N.L);
+    const bool pass = !nan && tmin <= tmax;
+    const bool leaf = nd.count > 0;
+    first = nd.start;
+    cnt = (pass && leaf) ? nd.count : 0;
+    return (pass && !leaf) ? cur + 1 : nd.miss;
+}
+__device__ __forceinline__ int nstep(const BvhNode32& nd, int cur, const NRay<true>& N, int& first, int& cnt) {
+    const float v[6] = {nd.lo[0], nd.lo[1], nd.lo[2], nd.hi[0], nd.hi[1], nd.hi[2]};
+    const bool pass = slab32(v, N.ox, N.oy, N.oz, N.ix, N.iy, N.iz, N.L);
+    const int c = nd.start_count & 15;
+    first = nd.start_count >> 4;
+    cnt = (pass && c > 0) ? c : 0;
+    return (pass && c == 0) ? cur + 1 : nd.miss;
+}
+
+template <bool MIX>
+__device__ __forceinline__ const typename std::conditional<MIX, BvhNode32, BvhNode64>::type* nnodes(const PairArgs& a);
+template <>
+__device__ __forceinline__ const BvhNode32* nnodes<true>(const PairArgs& a) { return a.nodes32; }
+template <>
+__device__ __forceinline__ const BvhNode64* nnodes<false>(const PairArgs& a) { return a.nodes64; }
+
+// One triangle task: the ray of (combination, lane) against one triangle,
+// with is_visible's origin and direction (RayCast.jl:27-30).
+__device__ __forceinline__ bool ttest(const PairArgs& a, const QCtx& X, int c, int b, int tri, uint32_t&) {
+    int p, q;
+    combo_pq(X, c, p, q);
+    const int64_t ib = X.sbase + b;
+    const int64_t NS = a.NS;
+    const double xi = a.P64[(int64_t)(p * 8 + 0) * NS + ib], yi = a.P64[(int64_t)(p * 8 + 1) * NS + ib];
+    const double zi = a.P64[(int64_t)(p * 8 + 2) * NS + ib];
+    const double* qj = a.Q64 + (X.j * a.maxpts + q) * 8;
+    const double xj = qj[0], yj = qj[1], zj = qj[2];
+    const bool rev = (X.revmask >> b) & 1;
+    const double r0 = xj - xi, r1 = yj - yi, r2 = zj - zi;
+    const double r = sqrt((r0 * r0 + r1 * r1) + r2 * r2);
+    Ray64 R64;
+    R64.dx = r0 / r; R64.dy = r1 / r; R64.dz = r2 / r; R64.L = r;
+    if (rev) { R64.dx = -R64.dx; R64.dy = -R64.dy; R64.dz = -R64.dz; }
+    R64.ox = rev ? xj : xi; R64.oy = rev ? yj : yi; R64.oz = rev ? zj : zi;
+    return mt_hit64(a.tris64[tri], R64);
+}
+
+// Mixed mode: the FP32 ray from the staged anchored offsets (as the kernel
+// evaluation forms r) and the conservative filter; true = candidate for the
+// exact FP64 test.
+__device__ __forceinline__ bool ttest_mixed(const PairArgs& a, const QCtx& X, int c, int b, int tri) {
+    int p, q;
+    combo_pq(X, c, p, q);
+    const int64_t ib = X.sbase + b;
+    const int64_t NS = a.NS;
+    const float dxi = a.P32[(int64_t)(p * 8 + 0) * NS + ib], dyi = a.P32[(int64_t)(p * 8 + 1) * NS + ib];
+    const float dzi = a.P32[(int64_t)(p * 8 + 2) * NS + ib];
+    const float* qj = a.Q32 + (X.j * a.maxpts + q) * 8;
+    const float qx = qj[0], qy = qj[1], qz = qj[2];
+    const float dax = X.stage[0 * 64 + b], day = X.stage[1 * 64 + b], daz = X.stage[2 * 64 + b];
+    const float fax = X.stage[3 * 64 + b], fay = X.stage[4 * 64 + b], faz = X.stage[5 * 64 + b];
+    const Tri32 T = a.tris32[tri];
+    const bool rev = (X.revmask >> b) & 1;
+    const float bx = dax - dxi, by = day - dyi, bz = daz - dzi;
+    const float rx = bx + qx, ry = by + qy, rz = bz + qz;
+    float r2;
+    {
+#pragma clang fp contract(fast)
+        r2 = rx * rx + ry * ry + rz * rz;
+    }
+    const float rs = __builtin_amdgcn_rsqf(r2);
+    const float sg = rev ? -rs : rs;
+    RvfFRay R;
+    R.ox = rev ? X.fjx + qx : fax + dxi;
+    R.oy = rev ? X.fjy + qy : fay + dyi;
+    R.oz = rev ? X.fjz + qz : faz + dzi;
+    R.dx = rx * sg; R.dy = ry * sg; R.dz = rz * sg; R.L = r2 * rs;
+    const float uu = 5.9604645e-08f;
+    const float dd = 3.0f * uu * (fabsf(dax) + fabsf(day) + fabsf(daz) + fabsf(bx) + fabsf(by) + fabsf(bz) +
+                                  fabsf(rx) + fabsf(ry) + fabsf(rz)) * rs + 4.0f * uu;
+    rvf_fray_margins(R, dd, a.Sm);
+    return rvf_tri_filter(T.v0, T.e1, T.e2, T.E1, T.E2, R, 80.0f * uu * a.Sm);
+}
+
 template <bool MIX, int S>
+__device__ __forceinline__ void split_drain(const PairArgs& a, const QCtx& X, int nc, const uint64_t* need,
+                                            uint32_t* blk, int lane, uint32_t* tq, uint32_t& c0, uint32_t& c1) {
+    static_assert(S * 64 * 8 + 64 <= RVF_TQCAP, "task ring too small for S slots");
+    const auto* __restrict__ nodes = nnodes<MIX>(a);
+    WaveQueue Q;
+    Q.c = 0; Q.nc = nc; Q.m = uniform64(need[0]);
+    uint32_t head = 0, tail = 0;   // uniform ring cursors
+    uint32_t chead = 0, ctail = 0; // candidate ring (mixed)
+    uint32_t* cq = tq + RVF_TQCAP;
+    int cur[S], tag[S];
+    NRay<MIX> R[S];
+#pragma unroll
+    for (int s = 0; s < S; s++) { cur[s] = -1; tag[s] = 0; }
+    for (;;) {
+        if (Q.c < Q.nc) {
+            bool any = false;
+#pragma unroll
+            for (int s = 0; s < S; s++) any |= cur[s] >= 0;
+            const bool idle = __ballot(any) == 0;
+#pragma unroll
+            for (int s = 0; s < S; s++) {
+                const uint64_t req = __ballot(cur[s] < 0);
+                if (Q.c < Q.nc && req != 0 && (idle || __popcll(req) >= a.refill_min)) {
+                    int c = 0, b = 0;
+                    if (queue_take(req, Q, need, lane, c, b)) {
+                        tag[s] = (c << 6) | b;
+                        cur[s] = nray_setup(a, X, c, b, R[s]);
+                    }
+                }
+            }
+        }
+        // node steps of every slot; passed leaves queue their triangles.  A ray
+        // whose blocked bit a task has set stops here.
+        int first[S], cnt[S];
+        uint32_t total = 0;
+        {
+            typename std::remove_cv<typename std::remove_reference<decltype(nodes[0])>::type>::type nd[S];
+            uint32_t bw[S];
+#pragma unroll
+            for (int s = 0; s < S; s++) {
+                nd[s] = nodes[max(cur[s], 0)];
+                const int c = tag[s] >> 6, b = tag[s] & 63;
+                bw[s] = blk[c * 2 + (b >> 5)] >> (b & 31);
+            }
+#pragma unroll
+            for (int s = 0; s < S; s++) {
+                const bool act = cur[s] >= 0 && !(bw[s] & 1u);
+                const int nx = nstep(nd[s], max(cur[s], 0), R[s], first[s], cnt[s]);
+                if (!MIX) c0 += act;
+                cnt[s] = act ? cnt[s] : 0;
+                total += (uint32_t)cnt[s];
+                cur[s] = act ? nx : -1;
+            }
+        }
+        // ring offsets: exclusive prefix of the lane totals (<= 8 S < 32) over the wave
+        uint32_t off = 0, tot = 0;
+#pragma unroll
+        for (int bit = 0; bit < 5; bit++) {
+            const uint64_t bm = __ballot((total >> bit) & 1u);
+            off += (uint32_t)lane_rank(bm) << bit;
+            tot += (uint32_t)__popcll(bm) << bit;
+        }
+        if (tot) {
+#pragma unroll
+            for (int s = 0; s < S; s++) {
+                for (int k = 0; k < cnt[s]; k++)
+                    tq[(tail + off + k) & (RVF_TQCAP - 1)] =
+                        ((uint32_t)tag[s] << RVF_TQ_TRI_BITS) | (uint32_t)(first[s] + k);
+                off += (uint32_t)cnt[s];
+            }
+            tail += tot;
+        }
+        __builtin_amdgcn_wave_barrier();
+        bool any = false;
+#pragma unroll
+        for (int s = 0; s < S; s++) any |= cur[s] >= 0;
+        const bool wany = __ballot(any) != 0;
+        // triangle tasks, 64 at a time; everything once no ray is in flight.
+        // Mixed mode: the filter's candidates go to a second ring whose exact
+        // FP64 tests also run 64 (or, at the end, all remaining) at a time.
+        while (tail - head >= 64 || (!wany && (tail != head || ctail != chead))) {
+            const uint32_t n = min(64u, tail - head);
+            bool keep = false;
+            uint32_t e = 0;
+            if ((uint32_t)lane < n) {
+                e = tq[(head + lane) & (RVF_TQCAP - 1)];
+                const int tg = (int)(e >> RVF_TQ_TRI_BITS), tri = (int)(e & ((1u << RVF_TQ_TRI_BITS) - 1));
+                const int c = tg >> 6, b = tg & 63;
+                if (MIX) {
+                    keep = ttest_mixed(a, X, c, b, tri);
+                } else {
+                    c1++;
+                    if (ttest(a, X, c, b, tri, c1)) atomicOr(&blk[c * 2 + (b >> 5)], 1u << (b & 31));
+                }
+            }
+            head += n;
+            if (MIX) {
+                const uint64_t km = __ballot(keep);
+                if (keep) cq[(ctail + (uint32_t)lane_rank(km)) & (RVF_CQCAP - 1)] = e;
+                ctail += (uint32_t)__popcll(km);
+            }
+            __builtin_amdgcn_wave_barrier();
+            if (MIX) {
+                while (ctail - chead >= 64 || (!wany && tail == head && ctail != chead)) {
+                    const uint32_t m = min(64u, ctail - chead);
+                    if ((uint32_t)lane < m) {
+                        const uint32_t ce = cq[(chead + lane) & (RVF_CQCAP - 1)];
+                        const int tg = (int)(ce >> RVF_TQ_TRI_BITS), tri = (int)(ce & ((1u << RVF_TQ_TRI_BITS) - 1));
+                        const int c = tg >> 6, b = tg & 63;
+                        c0++;
+                        if (qexact_hit(a, X, c, b, tri, a.items32[a.n_nodes + tri].link_y))
+                            atomicOr(&blk[c * 2 + (b >> 5)], 1u << (b & 31));
+                    }
+                    chead += m;
+                    __builtin_amdgcn_wave_barrier();
+                }
+            }
+        }
+        if (!wany && Q.c >= Q.nc) break;
+    }
+}
+
+template <bool MIX, int S, bool SPLIT>
 __device__ __forceinline__ void tile_occ_queue(const PairArgs& a, int tile, double* lds) {
     const TileCtx t = tile_ctx(a, tile);
     const Corners cwi = corners_of(a, t.oi, t.vi && a.duffy);
@@ -1153,6 +1404,8 @@ __device__ __forceinline__ void tile_occ_queue(const PairArgs& a, int tile, doub
     uint64_t* need = (uint64_t*)(lds + (MIX ? 0 : a.maxpts * 64));
     uint32_t* blk = (uint32_t*)(need + RVF_QCOMBO);
     int* lroot = (int*)(blk + 2 * RVF_QCOMBO);
+    uint32_t* tq = (uint32_t*)(lroot + 64);
+    float* stage = (float*)(tq + (SPLIT ? RVF_TQCAP + (MIX ? RVF_CQCAP : 0) : 0));
     const double* __restrict__ P64 = a.P64;
     const float* __restrict__ P = a.P32;
     double ls = 0.0, lc = 0.0;
@@ -1160,6 +1413,7 @@ __device__ __forceinline__ void tile_occ_queue(const PairArgs& a, int tile, doub
     QCtx X;
     X.sbase = (int64_t)t.s * RVF_STRIP;
     X.lroot = lroot;
+    X.stage = stage;
     for (int64_t j = t.jb; j < t.je; j++) {
         if (j <= (int64_t)t.s * RVF_STRIP) {
             if (lane == 0) a.partT[(int64_t)t.s * a.NS + j] = 0.0;
@@ -1186,6 +1440,11 @@ __device__ __forceinline__ void tile_occ_queue(const PairArgs& a, int tile, doub
         const float dax = (float)(a.anchor[j * 4 + 0] - aix);
         const float day = (float)(a.anchor[j * 4 + 1] - aiy);
         const float daz = (float)(a.anchor[j * 4 + 2] - aiz);
+        if (MIX && SPLIT) {
+            stage[0 * 64 + lane] = dax; stage[1 * 64 + lane] = day; stage[2 * 64 + lane] = daz;
+            stage[3 * 64 + lane] = (float)aix; stage[4 * 64 + lane] = (float)aiy; stage[5 * 64 + lane] = (float)aiz;
+            X.fjx = (float)a.anchor[j * 4 + 0]; X.fjy = (float)a.anchor[j * 4 + 1]; X.fjz = (float)a.anchor[j * 4 + 2];
+        }
         double Fij = 0.0, Fs = 0.0, Fc = 0.0;
         uint32_t bc = 0, bh = 0;
         for (int p0 = 0; p0 < npmax; p0 += X.pch) {
@@ -1231,7 +1490,8 @@ __device__ __forceinline__ void tile_occ_queue(const PairArgs& a, int tile, doub
             }
             __syncthreads();
             // Phase B: trace
-            queue_drain<MIX, S>(a, X, nc, need, blk, lane, c0, c1);
+            if constexpr (SPLIT) split_drain<MIX, S>(a, X, nc, need, blk, lane, tq, c0, c1);
+            else queue_drain<MIX, S>(a, X, nc, need, blk, lane, c0, c1);
             __syncthreads();
             // Phase C: sums in each lane's own order
             for (int p = p0; p < p1; p++) {
@@ -1675,7 +1935,8 @@ __global__ __launch_bounds__(256) void k_reduce_fin(const double* __restrict__ T
 // item with one atomic and exits when the queue is empty.
 // ---------------------------------------------------------------------------
 enum PairKind { PK_FP64_4, PK_FP64_9, PK_FP64_16, PK_FP64_OCC, PK_MIX_4, PK_MIX_9, PK_MIX_16, PK_MIX_16C, PK_MIX_OCC,
-                PK_MC, PK_FP64_Q1, PK_FP64_Q2, PK_FP64_Q3, PK_MIX_Q2, PK_MIX_Q3, PK_MIX_Q4, PK_MIX_Q6, PK_COUNT };
+                PK_MC, PK_FP64_Q1, PK_FP64_Q2, PK_FP64_Q3, PK_MIX_Q2, PK_MIX_Q3, PK_MIX_Q4, PK_MIX_Q6,
+                PK_FP64_T1, PK_FP64_T2, PK_FP64_T4, PK_MIX_T2, PK_MIX_T4, PK_MIX_T6, PK_COUNT };
 
 template <int KIND>
 __device__ __forceinline__ void run_tile(const PairArgs& a, int tile, double* lds) {
@@ -1688,13 +1949,19 @@ __device__ __forceinline__ void run_tile(const PairArgs& a, int tile, double* ld
     else if (KIND == PK_MIX_16) tile_mixed<16, true>(a, tile);
     else if (KIND == PK_MIX_16C) tile_mixed<16, false>(a, tile);
     else if (KIND == PK_MIX_OCC) tile_mixed_occ(a, tile);
-    else if (KIND == PK_FP64_Q1) tile_occ_queue<false, 1>(a, tile, lds);
-    else if (KIND == PK_FP64_Q2) tile_occ_queue<false, 2>(a, tile, lds);
-    else if (KIND == PK_FP64_Q3) tile_occ_queue<false, 3>(a, tile, lds);
-    else if (KIND == PK_MIX_Q2) tile_occ_queue<true, 2>(a, tile, lds);
-    else if (KIND == PK_MIX_Q3) tile_occ_queue<true, 3>(a, tile, lds);
-    else if (KIND == PK_MIX_Q4) tile_occ_queue<true, 4>(a, tile, lds);
-    else if (KIND == PK_MIX_Q6) tile_occ_queue<true, 6>(a, tile, lds);
+    else if (KIND == PK_FP64_Q1) tile_occ_queue<false, 1, false>(a, tile, lds);
+    else if (KIND == PK_FP64_Q2) tile_occ_queue<false, 2, false>(a, tile, lds);
+    else if (KIND == PK_FP64_Q3) tile_occ_queue<false, 3, false>(a, tile, lds);
+    else if (KIND == PK_MIX_Q2) tile_occ_queue<true, 2, false>(a, tile, lds);
+    else if (KIND == PK_MIX_Q3) tile_occ_queue<true, 3, false>(a, tile, lds);
+    else if (KIND == PK_MIX_Q4) tile_occ_queue<true, 4, false>(a, tile, lds);
+    else if (KIND == PK_MIX_Q6) tile_occ_queue<true, 6, false>(a, tile, lds);
+    else if (KIND == PK_FP64_T1) tile_occ_queue<false, 1, true>(a, tile, lds);
+    else if (KIND == PK_FP64_T2) tile_occ_queue<false, 2, true>(a, tile, lds);
+    else if (KIND == PK_FP64_T4) tile_occ_queue<false, 3, true>(a, tile, lds);
+    else if (KIND == PK_MIX_T2) tile_occ_queue<true, 2, true>(a, tile, lds);
+    else if (KIND == PK_MIX_T4) tile_occ_queue<true, 3, true>(a, tile, lds);
+    else if (KIND == PK_MIX_T6) tile_occ_queue<true, 1, true>(a, tile, lds);
     else tile_mc(a, tile);
 }
 
@@ -1782,6 +2049,12 @@ int rvf_pair_kind(int mixed, int occ, int pc, int mc) {
     case PK_MIX_Q3: CALL(PK_MIX_Q3); break;         \
     case PK_MIX_Q4: CALL(PK_MIX_Q4); break;         \
     case PK_MIX_Q6: CALL(PK_MIX_Q6); break;         \
+    case PK_FP64_T1: CALL(PK_FP64_T1); break;       \
+    case PK_FP64_T2: CALL(PK_FP64_T2); break;       \
+    case PK_FP64_T4: CALL(PK_FP64_T4); break;       \
+    case PK_MIX_T2: CALL(PK_MIX_T2); break;         \
+    case PK_MIX_T4: CALL(PK_MIX_T4); break;         \
+    case PK_MIX_T6: CALL(PK_MIX_T6); break;         \
     default: CALL(PK_MC); break;                    \
     }
 

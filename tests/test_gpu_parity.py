@@ -233,7 +233,7 @@ def test_cube48_full_size(orc):
     assert float(np.abs(a["rs"] - b["rs"]).max()) <= 1e-6
 
 
-@pytest.mark.parametrize("variant", api.FP64_VARIANTS)
+@pytest.mark.parametrize("variant", api.EXACT_COUNT_VARIANTS)
 def test_fp64_traversal_counts_match_oracle(orc, variant):
     # the FP64 kernels walk the canonical stackless order (GPUBVH.jl:202-285):
     # rays, box tests and triangle tests equal the oracle's counts exactly
