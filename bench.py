@@ -26,6 +26,7 @@ sys.path.insert(0, ROOT)
 
 VALU_PEAK_FP32 = 157.3e12   # MI355X FP32 vector (AMD datasheet)
 VALU_PEAK_FP64 = 78.6e12    # MI355X FP64 vector
+TRAFFIC_COPY_BYTES = 256 << 20  # calibration copy: reads and writes this many bytes once
 
 
 def build_workload(name):
@@ -75,6 +76,20 @@ def run_mode(api, ctx, dm, do, nquad, precision, steps, warmup, audit=False):
     return plan, total_ms / steps, np.array(lat), np.array(pair_ms)
 
 
+def measured_traffic(workload, kernel):
+    """HBM bytes per launch of the pair kernel from the committed rocprofv3
+    FETCH_SIZE / WRITE_SIZE summary, calibrated against the copy kernel
+    (profiles/traffic_<workload>.json); None when no summary matches."""
+    path = os.path.join(ROOT, "profiles", f"traffic_{workload}.json")
+    if not os.path.exists(path):
+        return None
+    with open(path) as f:
+        t = json.load(f)
+    if t.get("kernel") != kernel:
+        return None
+    return t["bytes_per_launch"]
+
+
 def cpu_baseline(mesh, nquad, obs, parity, budget_s):
     sys.path.insert(0, os.path.join(ROOT, "oracle"))
     import oracle as orc
@@ -110,6 +125,9 @@ def main():
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-verify", action="store_true")
     ap.add_argument("--no-secondary", dest="secondary", action="store_false")
+    ap.add_argument("--traffic-probe", action="store_true",
+                    help="one copy of known size, then one solve (for rocprofv3 --pmc runs)")
+    ap.add_argument("--probe-precision", default="mixed", choices=["mixed", "fp64"])
     args = ap.parse_args()
     if args.gpus != 1:
         print("bench.py: this build runs on one GPU; multi-GPU sharding of pair tiles is designed "
@@ -125,6 +143,14 @@ def main():
     dm = api.DeviceMesh(ctx, mesh)
     do = api.DeviceObstruction(ctx, dm, obs, parity) if obs else None
     npts = nquad * nquad
+
+    if args.traffic_probe:
+        ctx.copy_probe(TRAFFIC_COPY_BYTES, 1)
+        plan = api.Plan(ctx, dm, do, nquad, "quad", args.probe_precision, want_F=True, timing=True)
+        plan.run(); plan.sync()
+        print(json.dumps({"copy_bytes": TRAFFIC_COPY_BYTES, "kernel": plan.kernel_name(),
+                          "pair_ms": plan.stats()["pair_ms"]}))
+        return
 
     plan_m, ms_m, lat_m, pair_m = run_mode(api, ctx, dm, do, nquad, "mixed", args.steps, args.warmup)
     st_m = plan_m.stats()
@@ -164,8 +190,8 @@ def main():
     pair_avg_s = float(pair_m.mean()) * 1e-3
     achieved = flops / pair_avg_s
     roof = {"bound": "valu", "achieved": achieved / 1e12, "peak": VALU_PEAK_FP32 / 1e12, "unit": "TFLOP/s",
-            "frac": achieved / VALU_PEAK_FP32, "traffic": None,
-            "kernel": "k_pair_mixed_occ" if do is not None else "k_pair_mixed<9>",
+            "frac": achieved / VALU_PEAK_FP32, "traffic": measured_traffic(args.workload, plan_m.kernel_name()),
+            "kernel": plan_m.kernel_name(),
             "flops_per_launch": flops, "avg_launch_ms": float(pair_m.mean())}
     cpu = None
     if not args.no_cpu:
@@ -185,6 +211,9 @@ def main():
         "kernel_ms": {"mixed": {k: st_m[k] for k in ("precompute_ms", "pair_ms", "fill_ms", "reduce_ms")},
                       "fp64": {k: st_d[k] for k in ("precompute_ms", "pair_ms", "fill_ms", "reduce_ms")}},
         "rays": st_m.get("rays"), "fp64_retests": st_m.get("retests"),
+        "traversal": {"mixed_kernel": plan_m.kernel_name(), "fp64_kernel": plan_d.kernel_name(),
+                      "fp64_box_tests": st_d.get("aabb_tests"), "fp64_triangle_tests": st_d.get("tri_tests"),
+                      "mixed_slot_steps": st_m.get("slot_steps"), "mixed_task_batches": st_m.get("task_batches")},
         "verification": ver, "roofline": roof, "cpu_baseline": cpu,
     }
     if cpu:

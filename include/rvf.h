@@ -81,6 +81,10 @@ typedef struct rvf_stats {
     int64_t singular_pairs;
     int64_t aabb_tests; /* FP64 mode: box tests in the stackless order */
     int64_t tri_tests;  /* FP64 mode: triangle tests in the stackless order */
+    /* wave-queue traversal (occluded quadrature): loop iterations per wavefront
+     * summed, slot steps that advanced a ray, triangle-task and exact-test
+     * batches (split variants) */
+    int64_t trav_iters, slot_steps, task_batches, exact_batches;
 } rvf_stats;
 
 /* --- lifecycle (no reference counterpart: the reference is one Julia process) */
@@ -143,6 +147,9 @@ int rvf_plan_device_ptr(const rvf_plan* plan, int which, void** ptr);
  * wave-queue refill threshold (lanes with an empty slot before a refill,
  * 1..64; 0 = default 16).  Results do not depend on either. */
 int rvf_plan_tune(rvf_plan* plan, int32_t variant, int32_t refill_min);
+/* Name of the pair kernel the plan launches (as profilers show it), e.g.
+ * "k_pair<mixed_occ_split3>" (no reference counterpart; for measurement). */
+int rvf_plan_kernel(const rvf_plan* plan, char* name, int32_t cap);
 void rvf_plan_free(rvf_plan* plan);
 
 /* One-shot: the body of the Julia hook (compute_view_factors_gpu,

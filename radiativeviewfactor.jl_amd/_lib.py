@@ -34,7 +34,8 @@ class Stats(C.Structure):
                 ("fill_ms", C.c_double), ("reduce_ms", C.c_double), ("total_ms", C.c_double),
                 ("n_elems", C.c_int64), ("unique_pairs", C.c_int64), ("tiles", C.c_int64), ("rays", C.c_int64),
                 ("retests", C.c_int64), ("singular_pairs", C.c_int64), ("aabb_tests", C.c_int64),
-                ("tri_tests", C.c_int64)]
+                ("tri_tests", C.c_int64), ("trav_iters", C.c_int64), ("slot_steps", C.c_int64),
+                ("task_batches", C.c_int64), ("exact_batches", C.c_int64)]
 
     def as_dict(self):
         return {k: getattr(self, k) for k, _ in self._fields_}
@@ -72,6 +73,7 @@ def lib():
         "rvf_plan_stats": ([P, P], C.c_int),
         "rvf_plan_device_ptr": ([P, C.c_int, PP], C.c_int),
         "rvf_plan_tune": ([P, C.c_int32, C.c_int32], C.c_int),
+        "rvf_plan_kernel": ([P, C.c_char_p, C.c_int32], C.c_int),
         "rvf_plan_free": ([P], None),
         "rvf_compute_view_factors": ([P, P, P, P, P, P, P, P, P, P], C.c_int),
         "rvf_quad_rule": ([I32, I32, P, P], C.c_int),

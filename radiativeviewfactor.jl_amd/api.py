@@ -106,6 +106,13 @@ class Context:
         check(_lib.lib().rvf_ctx_event_elapsed(self.h, a, b, C.byref(ms)))
         return ms.value
 
+    def copy_probe(self, nbytes: int, reps: int = 1) -> float:
+        """Device-to-device copy of nbytes with the library's copy kernel
+        (a kernel of known traffic, for calibrating HBM counters); avg ms."""
+        ms = C.c_float()
+        check(_lib.lib().rvf_copy_probe(self.h, nbytes, reps, C.byref(ms)))
+        return ms.value
+
     def sync(self):
         check(_lib.lib().rvf_ctx_sync(self.h))
 
@@ -224,6 +231,12 @@ class Plan:
         """Switch the traversal variant / wave-queue refill threshold (results unchanged)."""
         check(_lib.lib().rvf_plan_tune(self.h, variant, refill_min))
         self.opts.variant = variant
+
+    def kernel_name(self) -> str:
+        """The pair kernel this plan launches, as profilers name it."""
+        buf = C.create_string_buffer(128)
+        check(_lib.lib().rvf_plan_kernel(self.h, buf, 128))
+        return buf.value.decode()
 
     def run(self):
         check(_lib.lib().rvf_plan_run(self.h))

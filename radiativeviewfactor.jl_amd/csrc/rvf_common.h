@@ -63,6 +63,27 @@ struct __align__(16) Item64 {
     int32_t link_x, link_y;
 };
 
+// Pair-kernel kinds: (enumerator, kernel tag).  Each kind is a separate
+// kernel k_pair<tag>, so profiles name the variant that ran.
+#define RVF_PAIR_KINDS(X)                                                                                    \
+    X(FP64_PC4, fp64_pc4) X(FP64_PC9, fp64_pc9) X(FP64_PC16, fp64_pc16) X(FP64_OCC_LANE, fp64_occ_lane)     \
+    X(MIX_PC4, mixed_pc4) X(MIX_PC9, mixed_pc9) X(MIX_PC16, mixed_pc16) X(MIX_PC16C, mixed_pc16_chunked)    \
+    X(MIX_OCC_LANE, mixed_occ_lane) X(MC, monte_carlo)                                                      \
+    X(FP64_QUEUE1, fp64_occ_queue1) X(FP64_QUEUE2, fp64_occ_queue2) X(FP64_QUEUE3, fp64_occ_queue3)        \
+    X(MIX_QUEUE2, mixed_occ_queue2) X(MIX_QUEUE3, mixed_occ_queue3) X(MIX_QUEUE4, mixed_occ_queue4)        \
+    X(MIX_QUEUE6, mixed_occ_queue6) X(FP64_SPLIT1, fp64_occ_split1) X(FP64_SPLIT2, fp64_occ_split2)         \
+    X(FP64_SPLIT3, fp64_occ_split3) X(MIX_SPLIT1, mixed_occ_split1) X(MIX_SPLIT2, mixed_occ_split2)         \
+    X(MIX_SPLIT3, mixed_occ_split3)
+enum RvfPairKind {
+#define RVF_KIND_ENUM(e, t) RVF_K_##e,
+    RVF_PAIR_KINDS(RVF_KIND_ENUM)
+#undef RVF_KIND_ENUM
+    RVF_K_COUNT
+};
+#define RVF_QCOMBO 128   // (p, q) combinations per wave-queue batch
+#define RVF_TQCAP 2048   // triangle tasks per wavefront ring (split variants)
+#define RVF_CQCAP 128    // mixed split variants: candidates for the exact FP64 test
+
 // Column chunk: a run of <= RVF_CHUNK consecutive elements of one group.
 struct Chunk {
     int32_t begin;

@@ -704,6 +704,7 @@ struct rvf_plan {
 };
 
 static int occ_kind(int variant, bool mixed);
+static int pair_kind(const rvf_plan* p);
 
 int rvf_plan_create(rvf_ctx* ctx, const rvf_mesh* mesh, const rvf_obs* obs, const rvf_opts* opts, rvf_plan** out) {
     if (!ctx || !mesh || !opts || !out) return fail(RVF_E_ARG, "rvf_plan_create: bad arguments");
@@ -744,7 +745,7 @@ int rvf_plan_create(rvf_ctx* ctx, const rvf_mesh* mesh, const rvf_obs* obs, cons
         (rc = p->part.alloc(sizeof(double) * std::max<size_t>(nch, 1) * NS)) ||
         (rc = p->partT.alloc(sizeof(double) * std::max<size_t>(nstrip, 1) * NS)) ||
         (rc = p->rowsum.alloc(sizeof(double) * NS)) || (rc = p->W.alloc(sizeof(double) * std::max(G, 1) * NS)) ||
-        (rc = p->counters.alloc(sizeof(unsigned long long) * 4)) ||
+        (rc = p->counters.alloc(sizeof(unsigned long long) * 8)) ||
         (rc = p->Tseg.alloc(sizeof(double) * RVF_RSEG * NS)) ||
         (rc = p->Wseg.alloc(sizeof(double) * RVF_RSEG * std::max(G, 1) * NS))) {
         delete p;
@@ -862,14 +863,6 @@ static PairArgs pair_args(rvf_plan* p) {
     return a;
 }
 
-enum {  // pair-kernel kinds (rvf_kernels.hip)
-    K_FP64_OCC = 3, K_MIX_OCC = 8, K_FP64_Q1 = 10, K_FP64_Q2 = 11, K_FP64_Q3 = 12, K_MIX_Q2 = 13, K_MIX_Q3 = 14,
-    K_MIX_Q4 = 15, K_MIX_Q6 = 16, K_FP64_T1 = 17, K_FP64_T2 = 18, K_FP64_T4 = 19, K_MIX_T2 = 20, K_MIX_T4 = 21,
-    K_MIX_T6 = 22
-};
-#define RVF_TQCAP 2048
-#define RVF_QCOMBO 128
-
 static bool plan_occluded_quad(const rvf_plan* p) {
     return p->obs && !p->obs->trees.empty() && p->opts.method != RVF_METHOD_MC;
 }
@@ -877,15 +870,16 @@ static bool plan_occluded_quad(const rvf_plan* p) {
 // Kernel kind for an occluded quadrature plan's variant, -1 if invalid.
 static int occ_kind(int variant, bool mixed) {
     if (variant == RVF_VARIANT_DEFAULT) variant = mixed ? RVF_VARIANT_SPLIT(3) : RVF_VARIANT_QUEUE(1);
-    if (variant == RVF_VARIANT_LANE) return mixed ? K_MIX_OCC : K_FP64_OCC;
+    if (variant == RVF_VARIANT_LANE) return mixed ? RVF_K_MIX_OCC_LANE : RVF_K_FP64_OCC_LANE;
     if (variant > 20) {  // split node / triangle-task form
         const int S = variant - 20;
-        if (!mixed) return S == 1 ? K_FP64_T1 : S == 2 ? K_FP64_T2 : S == 3 ? K_FP64_T4 : -1;
-        return S == 1 ? K_MIX_T6 : S == 2 ? K_MIX_T2 : S == 3 ? K_MIX_T4 : -1;
+        if (!mixed) return S == 1 ? RVF_K_FP64_SPLIT1 : S == 2 ? RVF_K_FP64_SPLIT2 : S == 3 ? RVF_K_FP64_SPLIT3 : -1;
+        return S == 1 ? RVF_K_MIX_SPLIT1 : S == 2 ? RVF_K_MIX_SPLIT2 : S == 3 ? RVF_K_MIX_SPLIT3 : -1;
     }
     const int S = variant - 10;
-    if (!mixed) return S == 1 ? K_FP64_Q1 : S == 2 ? K_FP64_Q2 : S == 3 ? K_FP64_Q3 : -1;
-    return S == 2 ? K_MIX_Q2 : S == 3 ? K_MIX_Q3 : S == 4 ? K_MIX_Q4 : S == 6 ? K_MIX_Q6 : -1;
+    if (!mixed) return S == 1 ? RVF_K_FP64_QUEUE1 : S == 2 ? RVF_K_FP64_QUEUE2 : S == 3 ? RVF_K_FP64_QUEUE3 : -1;
+    return S == 2 ? RVF_K_MIX_QUEUE2 : S == 3 ? RVF_K_MIX_QUEUE3 : S == 4 ? RVF_K_MIX_QUEUE4
+         : S == 6 ? RVF_K_MIX_QUEUE6 : -1;
 }
 
 // Which pair-kernel variant a plan runs, and the LDS it needs.
@@ -901,11 +895,12 @@ static size_t pair_lds(const rvf_plan* p) {
     const int kind = pair_kind(p);
     const bool mixed = p->opts.precision == RVF_MIXED;
     const size_t accq = mixed ? 0 : sizeof(double) * 64 * (size_t)p->maxpts;
-    if (kind == K_FP64_OCC) return accq;
-    if (kind == K_MIX_OCC) return 0;
+    if (kind == RVF_K_FP64_OCC_LANE) return accq;
+    if (kind == RVF_K_MIX_OCC_LANE) return 0;
     const size_t ring =
-        kind >= K_FP64_T1 ? RVF_TQCAP * sizeof(uint32_t) + (mixed ? 128 * sizeof(uint32_t) + 6 * 64 * sizeof(float) : 0)
-                          : 0;
+        kind >= RVF_K_FP64_SPLIT1 ? RVF_TQCAP * sizeof(uint32_t) +
+                                        (mixed ? RVF_CQCAP * sizeof(uint32_t) + 6 * 64 * sizeof(float) : 0)
+                                  : 0;
     return accq + RVF_QCOMBO * (sizeof(uint64_t) + 2 * sizeof(uint32_t)) + 64 * sizeof(int32_t) + ring;
 }
 
@@ -967,6 +962,19 @@ static int enqueue_solve(rvf_plan* p, hipStream_t st, bool tim) {
     return 0;
 }
 
+int rvf_plan_kernel(const rvf_plan* p, char* name, int32_t cap) {
+    if (!p || !name || cap < 1) return fail(RVF_E_ARG, "rvf_plan_kernel: bad arguments");
+    static const char* const names[] = {
+#define RVF_KIND_NAME(e, t) "k_pair<" #t ">",
+        RVF_PAIR_KINDS(RVF_KIND_NAME)
+#undef RVF_KIND_NAME
+    };
+    const int k = pair_kind(p);
+    if (k < 0 || k >= RVF_K_COUNT) return fail(RVF_E_STATE, "rvf_plan_kernel: no pair kernel");
+    std::snprintf(name, (size_t)cap, "%s", names[k]);
+    return 0;
+}
+
 int rvf_plan_tune(rvf_plan* p, int32_t variant, int32_t refill_min) {
     if (!p) return fail(RVF_E_ARG, "rvf_plan_tune: null plan");
     if (refill_min < 0 || refill_min > 64) return fail(RVF_E_ARG, "rvf_plan_tune: refill_min must be in 0..64");
@@ -1021,12 +1029,16 @@ int rvf_plan_sync(rvf_plan* p) {
     int bad = 0;
     HIPCHK(hipMemcpy(&bad, p->bad.p, sizeof(int), hipMemcpyDeviceToHost));
     if (bad) return fail(RVF_E_ARG, "degenerate element: zero or non-finite area element at a quadrature point");
-    unsigned long long cnt[4];
+    unsigned long long cnt[8];
     HIPCHK(hipMemcpy(cnt, p->counters.p, sizeof cnt, hipMemcpyDeviceToHost));
     p->st.retests = (int64_t)cnt[0];
     p->st.rays = (int64_t)cnt[1];
     p->st.aabb_tests = (int64_t)cnt[2];
     p->st.tri_tests = (int64_t)cnt[3];
+    p->st.trav_iters = (int64_t)cnt[4];
+    p->st.slot_steps = (int64_t)cnt[5];
+    p->st.task_batches = (int64_t)cnt[6];
+    p->st.exact_batches = (int64_t)cnt[7];
     if (p->opts.flags & RVF_TIMING) {
         float ms;
         HIPCHK(hipEventElapsedTime(&ms, p->ev[0], p->ev[1])); p->st.precompute_ms = ms;

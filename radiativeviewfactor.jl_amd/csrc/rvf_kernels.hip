@@ -816,7 +816,6 @@ __device__ __forceinline__ void tile_mixed_occ(const PairArgs& a, int tile) {
 // stay in the reference's order and visibility decisions are per ray as in
 // is_visible.
 // ---------------------------------------------------------------------------
-#define RVF_QCOMBO 128
 
 __device__ __forceinline__ int lane_rank(uint64_t m) {
     return (int)__builtin_amdgcn_mbcnt_hi((uint32_t)(m >> 32), __builtin_amdgcn_mbcnt_lo((uint32_t)m, 0u));
@@ -913,6 +912,15 @@ struct QCtx {
     const int* lroot;    // LDS: per lane, tree root of (lane's group, column group)
     const float* stage;  // LDS (mixed, split form): per lane [dax day daz fax fay faz] x 64
     float fjx, fjy, fjz; // column element's anchor as FP32
+};
+
+// Per-lane traversal counters of the wave-queue kernels.
+struct QCount {
+    uint32_t c0 = 0, c1 = 0;     // FP64: box tests, triangle tests; mixed: exact re-tests
+    uint32_t iters = 0;          // traversal loop iterations (counted by lane 0)
+    uint32_t steps = 0;          // slot steps that advanced a ray
+    uint32_t tbatch = 0;         // triangle-task batches (lane 0)
+    uint32_t ebatch = 0;         // exact-test batches (lane 0)
 };
 
 template <bool MIX>
@@ -1078,7 +1086,7 @@ __device__ __forceinline__ const Item64* qitems<false>(const PairArgs& a) { retu
 // issued before any is used.
 template <bool MIX, int S>
 __device__ __forceinline__ void queue_drain(const PairArgs& a, const QCtx& X, int nc, const uint64_t* need,
-                                            uint32_t* blk, int lane, uint32_t& c0, uint32_t& c1) {
+                                            uint32_t* blk, int lane, QCount& K) {
     const QItem<MIX>* __restrict__ items = qitems<MIX>(a);
     WaveQueue Q;
     Q.c = 0; Q.nc = nc; Q.m = uniform64(need[0]);
@@ -1108,6 +1116,7 @@ __device__ __forceinline__ void queue_drain(const PairArgs& a, const QCtx& X, in
 #pragma unroll
         for (int s = 0; s < S; s++) any |= cur[s] >= 0;
         if (__ballot(any) == 0) break;
+        K.iters += lane == 0;
         bool hit[S];
         {
             QItem<MIX> it[S];
@@ -1117,7 +1126,8 @@ __device__ __forceinline__ void queue_drain(const PairArgs& a, const QCtx& X, in
             for (int s = 0; s < S; s++) {
                 const bool act = cur[s] >= 0;
                 const int tri = cur[s] - a.n_nodes;
-                const int nx = qstep(a, X, it[s], max(cur[s], 0), act, tag[s], R[s], hit[s], c0, c1);
+                const int nx = qstep(a, X, it[s], max(cur[s], 0), act, tag[s], R[s], hit[s], K.c0, K.c1);
+                K.steps += act;
                 if (MIX) { ctri[s] = tri; cleaf[s] = it[s].link_y; }
                 cur[s] = act ? nx : -1;
             }
@@ -1149,10 +1159,9 @@ __device__ __forceinline__ void queue_drain(const PairArgs& a, const QCtx& X, in
 // decision is the OR over every triangle of every leaf the traversal reaches,
 // which is the stackless any-hit result (BVH.jl:213-250).
 // ---------------------------------------------------------------------------
-#define RVF_TQCAP 2048  // triangle tasks per wavefront ring (power of two): one iteration of S <= 3
-                        // slots pushes at most S x 64 lanes x 8 triangles per leaf
+// RVF_TQCAP (a power of two): one iteration of S <= 3 slots pushes at most
+// S x 64 lanes x 8 triangles per leaf; RVF_CQCAP >= 64 + 63.
 #define RVF_TQ_TRI_BITS 19  // a task packs (tag << 19) | triangle
-#define RVF_CQCAP 128       // mixed mode: candidates for the exact FP64 test (>= 64 + 63)
 
 template <bool MIX>
 struct NRay;
@@ -1265,7 +1274,7 @@ __device__ __forceinline__ bool ttest_mixed(const PairArgs& a, const QCtx& X, in
 
 template <bool MIX, int S>
 __device__ __forceinline__ void split_drain(const PairArgs& a, const QCtx& X, int nc, const uint64_t* need,
-                                            uint32_t* blk, int lane, uint32_t* tq, uint32_t& c0, uint32_t& c1) {
+                                            uint32_t* blk, int lane, uint32_t* tq, QCount& K) {
     static_assert(S * 64 * 8 + 64 <= RVF_TQCAP, "task ring too small for S slots");
     const auto* __restrict__ nodes = nnodes<MIX>(a);
     WaveQueue Q;
@@ -1297,6 +1306,7 @@ __device__ __forceinline__ void split_drain(const PairArgs& a, const QCtx& X, in
         }
         // node steps of every slot; passed leaves queue their triangles.  A ray
         // whose blocked bit a task has set stops here.
+        K.iters += lane == 0;
         int first[S], cnt[S];
         uint32_t total = 0;
         {
@@ -1312,7 +1322,8 @@ __device__ __forceinline__ void split_drain(const PairArgs& a, const QCtx& X, in
             for (int s = 0; s < S; s++) {
                 const bool act = cur[s] >= 0 && !(bw[s] & 1u);
                 const int nx = nstep(nd[s], max(cur[s], 0), R[s], first[s], cnt[s]);
-                if (!MIX) c0 += act;
+                if (!MIX) K.c0 += act;
+                K.steps += act;
                 cnt[s] = act ? cnt[s] : 0;
                 total += (uint32_t)cnt[s];
                 cur[s] = act ? nx : -1;
@@ -1355,10 +1366,11 @@ __device__ __forceinline__ void split_drain(const PairArgs& a, const QCtx& X, in
                 if (MIX) {
                     keep = ttest_mixed(a, X, c, b, tri);
                 } else {
-                    c1++;
-                    if (ttest(a, X, c, b, tri, c1)) atomicOr(&blk[c * 2 + (b >> 5)], 1u << (b & 31));
+                    K.c1++;
+                    if (ttest(a, X, c, b, tri, K.c1)) atomicOr(&blk[c * 2 + (b >> 5)], 1u << (b & 31));
                 }
             }
+            K.tbatch += lane == 0 && n > 0;
             head += n;
             if (MIX) {
                 const uint64_t km = __ballot(keep);
@@ -1373,10 +1385,11 @@ __device__ __forceinline__ void split_drain(const PairArgs& a, const QCtx& X, in
                         const uint32_t ce = cq[(chead + lane) & (RVF_CQCAP - 1)];
                         const int tg = (int)(ce >> RVF_TQ_TRI_BITS), tri = (int)(ce & ((1u << RVF_TQ_TRI_BITS) - 1));
                         const int c = tg >> 6, b = tg & 63;
-                        c0++;
+                        K.c0++;
                         if (qexact_hit(a, X, c, b, tri, a.items32[a.n_nodes + tri].link_y))
                             atomicOr(&blk[c * 2 + (b >> 5)], 1u << (b & 31));
                     }
+                    K.ebatch += lane == 0;
                     chead += m;
                     __builtin_amdgcn_wave_barrier();
                 }
@@ -1409,7 +1422,8 @@ __device__ __forceinline__ void tile_occ_queue(const PairArgs& a, int tile, doub
     const double* __restrict__ P64 = a.P64;
     const float* __restrict__ P = a.P32;
     double ls = 0.0, lc = 0.0;
-    uint32_t nrays = 0, c0 = 0, c1 = 0;
+    uint32_t nrays = 0;
+    QCount K;
     QCtx X;
     X.sbase = (int64_t)t.s * RVF_STRIP;
     X.lroot = lroot;
@@ -1490,8 +1504,8 @@ __device__ __forceinline__ void tile_occ_queue(const PairArgs& a, int tile, doub
             }
             __syncthreads();
             // Phase B: trace
-            if constexpr (SPLIT) split_drain<MIX, S>(a, X, nc, need, blk, lane, tq, c0, c1);
-            else queue_drain<MIX, S>(a, X, nc, need, blk, lane, c0, c1);
+            if constexpr (SPLIT) split_drain<MIX, S>(a, X, nc, need, blk, lane, tq, K);
+            else queue_drain<MIX, S>(a, X, nc, need, blk, lane, K);
             __syncthreads();
             // Phase C: sums in each lane's own order
             for (int p = p0; p < p1; p++) {
@@ -1562,14 +1576,16 @@ __device__ __forceinline__ void tile_occ_queue(const PairArgs& a, int tile, doub
     }
     a.part[(int64_t)t.chunk * NS + t.i] = ls + lc;
     if (a.counters) {
-        unsigned long long r1 = nrays, r2 = c0, r3 = c1;
-        for (int o = 32; o > 0; o >>= 1) {
-            r1 += __shfl_xor(r1, o, 64); r2 += __shfl_xor(r2, o, 64); r3 += __shfl_xor(r3, o, 64);
-        }
+        unsigned long long v[7] = {nrays, K.c0, K.c1, K.iters, K.steps, K.tbatch, K.ebatch};
+#pragma unroll
+        for (int o = 32; o > 0; o >>= 1)
+#pragma unroll
+            for (int e = 0; e < 7; e++) v[e] += __shfl_xor(v[e], o, 64);
         if (lane == 0) {
-            atomicAdd(&a.counters[1], r1);
-            if (MIX) atomicAdd(&a.counters[0], r2);
-            else { atomicAdd(&a.counters[2], r2); atomicAdd(&a.counters[3], r3); }
+            atomicAdd(&a.counters[1], v[0]);
+            if (MIX) atomicAdd(&a.counters[0], v[1]);
+            else { atomicAdd(&a.counters[2], v[1]); atomicAdd(&a.counters[3], v[2]); }
+            for (int e = 3; e < 7; e++) atomicAdd(&a.counters[e + 1], v[e]);
         }
     }
 }
@@ -1934,44 +1950,45 @@ __global__ __launch_bounds__(256) void k_reduce_fin(const double* __restrict__ T
 // work items from many meshes in one launch: each wavefront takes the next
 // item with one atomic and exits when the queue is empty.
 // ---------------------------------------------------------------------------
-enum PairKind { PK_FP64_4, PK_FP64_9, PK_FP64_16, PK_FP64_OCC, PK_MIX_4, PK_MIX_9, PK_MIX_16, PK_MIX_16C, PK_MIX_OCC,
-                PK_MC, PK_FP64_Q1, PK_FP64_Q2, PK_FP64_Q3, PK_MIX_Q2, PK_MIX_Q3, PK_MIX_Q4, PK_MIX_Q6,
-                PK_FP64_T1, PK_FP64_T2, PK_FP64_T4, PK_MIX_T2, PK_MIX_T4, PK_MIX_T6, PK_COUNT };
-
 template <int KIND>
 __device__ __forceinline__ void run_tile(const PairArgs& a, int tile, double* lds) {
-    if (KIND == PK_FP64_4) tile_fp64<4>(a, tile);
-    else if (KIND == PK_FP64_9) tile_fp64<9>(a, tile);
-    else if (KIND == PK_FP64_16) tile_fp64<16>(a, tile);
-    else if (KIND == PK_FP64_OCC) tile_fp64_occ(a, tile, lds);
-    else if (KIND == PK_MIX_4) tile_mixed<4, true>(a, tile);
-    else if (KIND == PK_MIX_9) tile_mixed<9, true>(a, tile);
-    else if (KIND == PK_MIX_16) tile_mixed<16, true>(a, tile);
-    else if (KIND == PK_MIX_16C) tile_mixed<16, false>(a, tile);
-    else if (KIND == PK_MIX_OCC) tile_mixed_occ(a, tile);
-    else if (KIND == PK_FP64_Q1) tile_occ_queue<false, 1, false>(a, tile, lds);
-    else if (KIND == PK_FP64_Q2) tile_occ_queue<false, 2, false>(a, tile, lds);
-    else if (KIND == PK_FP64_Q3) tile_occ_queue<false, 3, false>(a, tile, lds);
-    else if (KIND == PK_MIX_Q2) tile_occ_queue<true, 2, false>(a, tile, lds);
-    else if (KIND == PK_MIX_Q3) tile_occ_queue<true, 3, false>(a, tile, lds);
-    else if (KIND == PK_MIX_Q4) tile_occ_queue<true, 4, false>(a, tile, lds);
-    else if (KIND == PK_MIX_Q6) tile_occ_queue<true, 6, false>(a, tile, lds);
-    else if (KIND == PK_FP64_T1) tile_occ_queue<false, 1, true>(a, tile, lds);
-    else if (KIND == PK_FP64_T2) tile_occ_queue<false, 2, true>(a, tile, lds);
-    else if (KIND == PK_FP64_T4) tile_occ_queue<false, 3, true>(a, tile, lds);
-    else if (KIND == PK_MIX_T2) tile_occ_queue<true, 2, true>(a, tile, lds);
-    else if (KIND == PK_MIX_T4) tile_occ_queue<true, 3, true>(a, tile, lds);
-    else if (KIND == PK_MIX_T6) tile_occ_queue<true, 1, true>(a, tile, lds);
+    if (KIND == RVF_K_FP64_PC4) tile_fp64<4>(a, tile);
+    else if (KIND == RVF_K_FP64_PC9) tile_fp64<9>(a, tile);
+    else if (KIND == RVF_K_FP64_PC16) tile_fp64<16>(a, tile);
+    else if (KIND == RVF_K_FP64_OCC_LANE) tile_fp64_occ(a, tile, lds);
+    else if (KIND == RVF_K_MIX_PC4) tile_mixed<4, true>(a, tile);
+    else if (KIND == RVF_K_MIX_PC9) tile_mixed<9, true>(a, tile);
+    else if (KIND == RVF_K_MIX_PC16) tile_mixed<16, true>(a, tile);
+    else if (KIND == RVF_K_MIX_PC16C) tile_mixed<16, false>(a, tile);
+    else if (KIND == RVF_K_MIX_OCC_LANE) tile_mixed_occ(a, tile);
+    else if (KIND == RVF_K_FP64_QUEUE1) tile_occ_queue<false, 1, false>(a, tile, lds);
+    else if (KIND == RVF_K_FP64_QUEUE2) tile_occ_queue<false, 2, false>(a, tile, lds);
+    else if (KIND == RVF_K_FP64_QUEUE3) tile_occ_queue<false, 3, false>(a, tile, lds);
+    else if (KIND == RVF_K_MIX_QUEUE2) tile_occ_queue<true, 2, false>(a, tile, lds);
+    else if (KIND == RVF_K_MIX_QUEUE3) tile_occ_queue<true, 3, false>(a, tile, lds);
+    else if (KIND == RVF_K_MIX_QUEUE4) tile_occ_queue<true, 4, false>(a, tile, lds);
+    else if (KIND == RVF_K_MIX_QUEUE6) tile_occ_queue<true, 6, false>(a, tile, lds);
+    else if (KIND == RVF_K_FP64_SPLIT1) tile_occ_queue<false, 1, true>(a, tile, lds);
+    else if (KIND == RVF_K_FP64_SPLIT2) tile_occ_queue<false, 2, true>(a, tile, lds);
+    else if (KIND == RVF_K_FP64_SPLIT3) tile_occ_queue<false, 3, true>(a, tile, lds);
+    else if (KIND == RVF_K_MIX_SPLIT1) tile_occ_queue<true, 1, true>(a, tile, lds);
+    else if (KIND == RVF_K_MIX_SPLIT2) tile_occ_queue<true, 2, true>(a, tile, lds);
+    else if (KIND == RVF_K_MIX_SPLIT3) tile_occ_queue<true, 3, true>(a, tile, lds);
     else tile_mc(a, tile);
 }
 
-template <int KIND>
+// kernel tags: k_pair<tag> names the variant in profiles
+#define RVF_KIND_TAG(e, t) struct t { static constexpr int kind = RVF_K_##e; };
+RVF_PAIR_KINDS(RVF_KIND_TAG)
+#undef RVF_KIND_TAG
+
+template <class TAG>
 __global__ __launch_bounds__(64) void k_pair(PairArgs a) {
     extern __shared__ double lds[];
-    run_tile<KIND>(a, blockIdx.x, lds);
+    run_tile<TAG::kind>(a, blockIdx.x, lds);
 }
 
-template <int KIND>
+template <class TAG>
 __global__ __launch_bounds__(64) void k_pair_batched(const PairArgs* __restrict__ reqs, const int2* __restrict__ work,
                                                      int nwork, int* __restrict__ head) {
     extern __shared__ double lds[];
@@ -1982,7 +1999,7 @@ __global__ __launch_bounds__(64) void k_pair_batched(const PairArgs* __restrict_
         if (w >= nwork) break;
         const int2 it = work[w];
         const PairArgs& a = reqs[it.x];
-        run_tile<KIND>(a, it.y, lds);
+        run_tile<TAG::kind>(a, it.y, lds);
     }
 }
 
@@ -2007,14 +2024,14 @@ __global__ __launch_bounds__(256) void k_copy(const uint4* __restrict__ src, uin
 // ---------------------------------------------------------------------------
 // Launch wrappers (host side of this translation unit)
 // ---------------------------------------------------------------------------
-template <int KIND>
+template <class TAG>
 static void launch_direct(const PairArgs* a, size_t lds, hipStream_t st) {
-    hipLaunchKernelGGL(k_pair<KIND>, dim3((unsigned)a->ntiles), dim3(64), lds, st, *a);
+    hipLaunchKernelGGL(k_pair<TAG>, dim3((unsigned)a->ntiles), dim3(64), lds, st, *a);
 }
-template <int KIND>
+template <class TAG>
 static void launch_batched(const PairArgs* reqs, const int2* work, int nwork, int* head, int grid, size_t lds,
                            hipStream_t st) {
-    hipLaunchKernelGGL(k_pair_batched<KIND>, dim3((unsigned)grid), dim3(64), lds, st, reqs, work, nwork, head);
+    hipLaunchKernelGGL(k_pair_batched<TAG>, dim3((unsigned)grid), dim3(64), lds, st, reqs, work, nwork, head);
 }
 
 extern "C" {
@@ -2026,59 +2043,37 @@ int rvf_launch_precompute(const PrecompArgs* a, hipStream_t st) {
 }
 
 int rvf_pair_kind(int mixed, int occ, int pc, int mc) {
-    if (mc) return PK_MC;
-    if (!mixed) return occ ? PK_FP64_OCC : pc <= 4 ? PK_FP64_4 : pc <= 9 ? PK_FP64_9 : PK_FP64_16;
-    return occ ? PK_MIX_OCC : pc <= 4 ? PK_MIX_4 : pc <= 9 ? PK_MIX_9 : pc <= 16 ? PK_MIX_16 : PK_MIX_16C;
+    if (mc) return RVF_K_MC;
+    if (!mixed) return occ ? RVF_K_FP64_OCC_LANE : pc <= 4 ? RVF_K_FP64_PC4 : pc <= 9 ? RVF_K_FP64_PC9 : RVF_K_FP64_PC16;
+    return occ ? RVF_K_MIX_OCC_LANE
+               : pc <= 4 ? RVF_K_MIX_PC4 : pc <= 9 ? RVF_K_MIX_PC9 : pc <= 16 ? RVF_K_MIX_PC16 : RVF_K_MIX_PC16C;
 }
 
-#define RVF_KIND_SWITCH(kind, CALL)                 \
-    switch (kind) {                                 \
-    case PK_FP64_4: CALL(PK_FP64_4); break;         \
-    case PK_FP64_9: CALL(PK_FP64_9); break;         \
-    case PK_FP64_16: CALL(PK_FP64_16); break;       \
-    case PK_FP64_OCC: CALL(PK_FP64_OCC); break;     \
-    case PK_MIX_4: CALL(PK_MIX_4); break;           \
-    case PK_MIX_9: CALL(PK_MIX_9); break;           \
-    case PK_MIX_16: CALL(PK_MIX_16); break;         \
-    case PK_MIX_16C: CALL(PK_MIX_16C); break;       \
-    case PK_MIX_OCC: CALL(PK_MIX_OCC); break;       \
-    case PK_FP64_Q1: CALL(PK_FP64_Q1); break;       \
-    case PK_FP64_Q2: CALL(PK_FP64_Q2); break;       \
-    case PK_FP64_Q3: CALL(PK_FP64_Q3); break;       \
-    case PK_MIX_Q2: CALL(PK_MIX_Q2); break;         \
-    case PK_MIX_Q3: CALL(PK_MIX_Q3); break;         \
-    case PK_MIX_Q4: CALL(PK_MIX_Q4); break;         \
-    case PK_MIX_Q6: CALL(PK_MIX_Q6); break;         \
-    case PK_FP64_T1: CALL(PK_FP64_T1); break;       \
-    case PK_FP64_T2: CALL(PK_FP64_T2); break;       \
-    case PK_FP64_T4: CALL(PK_FP64_T4); break;       \
-    case PK_MIX_T2: CALL(PK_MIX_T2); break;         \
-    case PK_MIX_T4: CALL(PK_MIX_T4); break;         \
-    case PK_MIX_T6: CALL(PK_MIX_T6); break;         \
-    default: CALL(PK_MC); break;                    \
-    }
+#define RVF_KIND_CASE(e, t) case RVF_K_##e: RVF_KIND_CALL(t); break;
 
 int rvf_launch_pair(const PairArgs* a, int kind, size_t lds, hipStream_t st) {
     if (a->ntiles == 0) return 0;
-#define DIRECT(K) launch_direct<K>(a, lds, st)
-    RVF_KIND_SWITCH(kind, DIRECT)
-#undef DIRECT
+    if (kind < 0 || kind >= RVF_K_COUNT) return (int)hipErrorInvalidValue;
+#define RVF_KIND_CALL(t) launch_direct<t>(a, lds, st)
+    switch (kind) { RVF_PAIR_KINDS(RVF_KIND_CASE) }
+#undef RVF_KIND_CALL
     return (int)hipGetLastError();
 }
 
 int rvf_launch_pair_batched(int kind, const PairArgs* reqs, const int2* work, int nwork, int* head, int grid,
                             size_t lds, hipStream_t st) {
     if (nwork == 0) return 0;
-#define BATCHED(K) launch_batched<K>(reqs, work, nwork, head, grid, lds, st)
-    RVF_KIND_SWITCH(kind, BATCHED)
-#undef BATCHED
+    if (kind < 0 || kind >= RVF_K_COUNT) return (int)hipErrorInvalidValue;
+#define RVF_KIND_CALL(t) launch_batched<t>(reqs, work, nwork, head, grid, lds, st)
+    switch (kind) { RVF_PAIR_KINDS(RVF_KIND_CASE) }
+#undef RVF_KIND_CALL
     return (int)hipGetLastError();
 }
 
 int rvf_launch_mc(const PairArgs* a, double* area, double* area_orig, hipStream_t st) {
     if (a->N > 0)
         hipLaunchKernelGGL(k_mc_area, dim3((unsigned)((a->N + 63) / 64)), dim3(64), 0, st, *a, area, area_orig);
-    if (a->ntiles > 0) launch_direct<PK_MC>(a, 0, st);
+    if (a->ntiles > 0) launch_direct<monte_carlo>(a, 0, st);
     return (int)hipGetLastError();
 }
 
