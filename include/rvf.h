@@ -167,6 +167,52 @@ int rvf_ctx_event_record(rvf_ctx* ctx, int32_t slot);
 int rvf_ctx_event_elapsed(rvf_ctx* ctx, int32_t slot_a, int32_t slot_b, float* ms);
 int rvf_ctx_sync(rvf_ctx* ctx);
 
+/* --- request server: continuous batching of solves on one GPU (no reference
+ * counterpart: the reference solves one mesh per call, src/Assembly.jl:64-96).
+ * Clients on any thread submit solves; a worker thread runs rounds in which one
+ * persistent batched kernel per kernel kind drains (request, tile) items from
+ * every running request, and admits pending requests in arrival order while
+ * their estimated device footprint fits hbm_budget.  A request's result is
+ * identical to its standalone rvf_compute_view_factors result. */
+typedef struct rvf_server rvf_server;
+typedef struct rvf_server_opts {
+    int64_t hbm_budget;   /* device bytes for admitted requests (0 = 64 GiB) */
+    int32_t max_active;   /* requests admitted at once (0 = 64) */
+    int32_t round_tiles;  /* pair tiles per request per round (0 = 4096) */
+} rvf_server_opts;
+typedef struct rvf_request_stats {
+    double t_submit_ms, t_admit_ms, t_start_ms, t_done_ms;  /* since the server started */
+    int64_t footprint_bytes, tiles;
+} rvf_request_stats;
+typedef struct rvf_server_counters {
+    int64_t submitted, completed, failed, cancelled, rounds, tiles;
+    int64_t peak_bytes, used_bytes, pending, active;
+} rvf_server_counters;
+/* request states returned by rvf_server_poll / rvf_server_wait */
+#define RVF_REQ_PENDING 0
+#define RVF_REQ_RUNNING 1
+#define RVF_REQ_DONE 2
+#define RVF_REQ_FAILED 3
+#define RVF_REQ_CANCELLED 4
+int rvf_server_create(rvf_ctx* ctx, const rvf_server_opts* opts /*nullable*/, rvf_server** out);
+void rvf_server_destroy(rvf_server* server);  /* cancels pending requests, finishes running ones */
+/* mesh arguments as rvf_mesh_upload, obstruction arguments as rvf_obs_build
+ * (soup nullable: no obstruction); copied, so the caller may free them at once.
+ * opts.flags RVF_WANT_F keeps F for rvf_server_result. */
+int rvf_server_submit(rvf_server* server, const double* coords, int64_t n_nodes, int64_t n_elems,
+                      const int8_t* family, const int32_t* elem_node_idx, const int32_t* elem_group,
+                      const int32_t* const nodes[4], const int64_t n_per_family[4], const double* soup,
+                      const int32_t* tri_group, int64_t T, const int32_t* obs_groups, int32_t n_obs,
+                      int32_t soup_read, const rvf_opts* opts, int64_t* ticket);
+int rvf_server_poll(rvf_server* server, int64_t ticket);                     /* RVF_REQ_* */
+int rvf_server_wait(rvf_server* server, int64_t ticket, int32_t timeout_ms); /* -1 = no limit */
+/* results as rvf_plan_fetch (each pointer nullable) plus the request's timeline */
+int rvf_server_result(rvf_server* server, int64_t ticket, double* F, double* A, double* rowsum, double* F_group,
+                      double* A_group, int32_t* group_tags, rvf_request_stats* stats);
+int rvf_server_cancel(rvf_server* server, int64_t ticket);
+int rvf_server_release(rvf_server* server, int64_t ticket);  /* forget a finished request */
+int rvf_server_stats(rvf_server* server, rvf_server_counters* stats);
+
 /* measurement aid (no reference counterpart): copy `bytes` device-to-device
  * `reps` times with the library's 16-byte-vector copy kernel; average ms.
  * Used to calibrate HBM traffic counters against a kernel of known size. */

@@ -29,6 +29,26 @@ class Opts(C.Structure):
                 ("seed", C.c_uint64), ("flags", C.c_int32), ("variant", C.c_int32)]
 
 
+class ServerOpts(C.Structure):
+    _fields_ = [("hbm_budget", C.c_int64), ("max_active", C.c_int32), ("round_tiles", C.c_int32)]
+
+
+class RequestStats(C.Structure):
+    _fields_ = [("t_submit_ms", C.c_double), ("t_admit_ms", C.c_double), ("t_start_ms", C.c_double),
+                ("t_done_ms", C.c_double), ("footprint_bytes", C.c_int64), ("tiles", C.c_int64)]
+
+    def as_dict(self):
+        return {k: getattr(self, k) for k, _ in self._fields_}
+
+
+class ServerCounters(C.Structure):
+    _fields_ = [(k, C.c_int64) for k in ("submitted", "completed", "failed", "cancelled", "rounds", "tiles",
+                                          "peak_bytes", "used_bytes", "pending", "active")]
+
+    def as_dict(self):
+        return {k: getattr(self, k) for k, _ in self._fields_}
+
+
 class Stats(C.Structure):
     _fields_ = [("precompute_ms", C.c_double), ("pair_ms", C.c_double), ("duffy_ms", C.c_double),
                 ("fill_ms", C.c_double), ("reduce_ms", C.c_double), ("total_ms", C.c_double),
@@ -74,6 +94,15 @@ def lib():
         "rvf_plan_device_ptr": ([P, C.c_int, PP], C.c_int),
         "rvf_plan_tune": ([P, C.c_int32, C.c_int32], C.c_int),
         "rvf_plan_kernel": ([P, C.c_char_p, C.c_int32], C.c_int),
+        "rvf_server_create": ([P, P, PP], C.c_int),
+        "rvf_server_destroy": ([P], None),
+        "rvf_server_submit": ([P, P, I64, I64, P, P, P, P, P, P, P, I64, P, I32, I32, P, P], C.c_int),
+        "rvf_server_poll": ([P, I64], C.c_int),
+        "rvf_server_wait": ([P, I64, I32], C.c_int),
+        "rvf_server_result": ([P, I64, P, P, P, P, P, P, P], C.c_int),
+        "rvf_server_cancel": ([P, I64], C.c_int),
+        "rvf_server_release": ([P, I64], C.c_int),
+        "rvf_server_stats": ([P, P], C.c_int),
         "rvf_plan_free": ([P], None),
         "rvf_compute_view_factors": ([P, P, P, P, P, P, P, P, P, P], C.c_int),
         "rvf_quad_rule": ([I32, I32, P, P], C.c_int),

@@ -203,6 +203,93 @@ class DeviceObstruction:
             pass
 
 
+REQ_PENDING, REQ_RUNNING, REQ_DONE, REQ_FAILED, REQ_CANCELLED = range(5)
+
+
+class Server:
+    """Request server (include/rvf.h rvf_server_*): solves submitted from any
+    thread are admitted by device-memory footprint and batched with the other
+    running solves; each result equals the standalone solve's."""
+
+    def __init__(self, ctx: Context, hbm_budget: int = 0, max_active: int = 0, round_tiles: int = 0):
+        o = _lib.ServerOpts(hbm_budget, max_active, round_tiles)
+        h = C.c_void_p()
+        check(_lib.lib().rvf_server_create(ctx.h, C.byref(o), C.byref(h)))
+        self.ctx, self.h = ctx, h
+        self._meta = {}
+
+    def submit(self, mesh: MeshData, nquad: int, obstruction_groups: Sequence[int] = (), parity: str = "reference",
+               precision: str = "fp64", want_F: bool = False, variant: int = 0) -> int:
+        mats, counts, idx = family_matrices(mesh)
+        arr = (C.c_void_p * 4)(*[m.ctypes.data if m.size else None for m in mats])
+        tags = sorted(set(int(g) for g in obstruction_groups if int(g) in mesh.group_tri_soup))
+        soup = tg = None
+        T = 0
+        if tags:
+            soup = np.ascontiguousarray(np.concatenate([mesh.group_tri_soup[t] for t in tags]), np.float64)
+            tg = np.concatenate([np.full(mesh.group_tri_soup[t].shape[0], t, np.int32) for t in tags])
+            T = soup.shape[0]
+        obs = np.ascontiguousarray(list(obstruction_groups), np.int32)
+        mode = SOUP_READ["reference"] if parity == "reference" else SOUP_READ[("geometric", mesh.soup_layout)]
+        o = _lib.Opts()
+        o.nquad = nquad
+        o.method = _lib.METHODS["quad"]
+        o.precision = _lib.PRECISIONS[precision]
+        o.flags = RVF_WANT_F if want_F else 0
+        o.variant = variant
+        t = C.c_int64()
+        check(_lib.lib().rvf_server_submit(self.h, ptr(mesh.coords), mesh.n_nodes, mesh.n_elems, ptr(mesh.family),
+                                           ptr(idx), ptr(mesh.group), arr, ptr(counts), ptr(soup), ptr(tg), T,
+                                           ptr(obs), len(obs), mode, C.byref(o), C.byref(t)))
+        self._meta[t.value] = (mesh.n_elems, len(set(int(g) for g in mesh.group)), want_F)
+        return t.value
+
+    def poll(self, ticket: int) -> int:
+        return check(_lib.lib().rvf_server_poll(self.h, ticket))
+
+    def wait(self, ticket: int, timeout_ms: int = -1) -> int:
+        return check(_lib.lib().rvf_server_wait(self.h, ticket, timeout_ms))
+
+    def result(self, ticket: int):
+        """(F or None, A, rowsum, F_group, A_group, tags, request stats dict)."""
+        N, G, want_F = self._meta[ticket]
+        F = np.empty((N, N), order="F") if want_F else None
+        A = np.empty(N); rs = np.empty(N)
+        Fg = np.empty((G, G), order="F"); Ag = np.empty(G); tags = np.empty(G, np.int32)
+        st = _lib.RequestStats()
+        check(_lib.lib().rvf_server_result(self.h, ticket, ptr(F), ptr(A), ptr(rs), ptr(Fg), ptr(Ag), ptr(tags),
+                                           C.byref(st)))
+        return F, A, rs, Fg, Ag, [int(x) for x in tags], st.as_dict()
+
+    def request_stats(self, ticket: int):
+        st = _lib.RequestStats()
+        _lib.lib().rvf_server_result(self.h, ticket, None, None, None, None, None, None, C.byref(st))
+        return st.as_dict()
+
+    def cancel(self, ticket: int):
+        check(_lib.lib().rvf_server_cancel(self.h, ticket))
+
+    def release(self, ticket: int):
+        check(_lib.lib().rvf_server_release(self.h, ticket))
+        self._meta.pop(ticket, None)
+
+    def stats(self):
+        c = _lib.ServerCounters()
+        check(_lib.lib().rvf_server_stats(self.h, C.byref(c)))
+        return c.as_dict()
+
+    def close(self):
+        if self.h:
+            _lib.lib().rvf_server_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
 class Plan:
     """Device buffers + launch sequence for one (mesh, obstruction, options)."""
 

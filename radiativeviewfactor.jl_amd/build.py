@@ -15,7 +15,7 @@ FLAGS = [
     "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops",
     "-Wno-unused-command-line-argument",
 ]
-SOURCES = [os.path.join(CSRC, "rvf_kernels.hip"), os.path.join(CSRC, "rvf_host.cpp")]
+SOURCES = [os.path.join(CSRC, "rvf_kernels.hip"), os.path.join(CSRC, "rvf_host.cpp"), os.path.join(CSRC, "rvf_server.cpp")]
 
 
 def build_library(verbose=False):

@@ -19,9 +19,8 @@
 extern "C" int rvf_launch_precompute(const PrecompArgs* a, hipStream_t st);
 extern "C" int rvf_pair_kind(int mixed, int occ, int pc, int mc);
 extern "C" int rvf_launch_pair(const PairArgs* a, int kind, size_t lds, hipStream_t st);
-extern "C" int rvf_launch_pair_batched(int kind, const PairArgs* reqs, const int2* work, int nwork, int* head,
-                                       int grid, size_t lds, hipStream_t st);
-extern "C" int rvf_launch_mc(const PairArgs* a, double* area, double* area_orig, hipStream_t st);
+
+extern "C" int rvf_launch_mc_area(const PairArgs* a, double* area, double* area_orig, hipStream_t st);
 extern "C" int rvf_launch_duffy(const PairArgs* a, const DuffyPair* pairs, int npairs, const double* gl01, int nq,
                                 double* val, const int32_t* rptr, const int32_t* rlist, double* rowsum, double* W,
                                 int phase, hipStream_t st);
@@ -827,7 +826,7 @@ int rvf_plan_create(rvf_ctx* ctx, const rvf_mesh* mesh, const rvf_obs* obs, cons
     return 0;
 }
 
-static PairArgs pair_args(rvf_plan* p) {
+static PairArgs pair_args(const rvf_plan* p) {
     const rvf_mesh* m = p->mesh;
     PairArgs a{};
     const Layout* L = p->L;
@@ -904,10 +903,13 @@ static size_t pair_lds(const rvf_plan* p) {
     return accq + RVF_QCOMBO * (sizeof(uint64_t) + 2 * sizeof(uint32_t)) + 64 * sizeof(int32_t) + ring;
 }
 
-// Enqueue one solve on stream st.  With tim, HIP events bracket each stage
-// (not used while capturing a graph).  Zero fills use a kernel, never
+// A solve is three stages on one stream: the prologue (zero fills, point
+// precompute, Monte Carlo areas), the pair tiles (one launch here; the server
+// batches the tiles of many plans instead), and the epilogue (Duffy pairs,
+// row/group reductions, the F fill).  Zero fills use a kernel, never
 // hipMemsetAsync, so the sequence can be captured and replayed as a graph.
-static int enqueue_solve(rvf_plan* p, hipStream_t st, bool tim) {
+// With tim, HIP events bracket each stage (not used while capturing).
+static int enqueue_prologue(rvf_plan* p, hipStream_t st, bool tim) {
     const rvf_mesh* m = p->mesh;
     if (tim) HIPCHK(hipEventRecord(p->ev[0], st));
     int rc;
@@ -924,16 +926,27 @@ static int enqueue_solve(rvf_plan* p, hipStream_t st, bool tim) {
     pa.area_orig = p->area_orig.as<double>();
     rc = rvf_launch_precompute(&pa, st);
     if (rc) return fail(RVF_E_HIP, std::string("precompute launch: ") + hipGetErrorString((hipError_t)rc));
-    if (tim) HIPCHK(hipEventRecord(p->ev[1], st));
-    PairArgs a = pair_args(p);
-    if (p->opts.method == RVF_METHOD_QUAD || p->opts.method == RVF_METHOD_DUFFY) {
-        rc = rvf_launch_pair(&a, pair_kind(p), pair_lds(p), st);
-        if (rc) return fail(RVF_E_HIP, std::string("pair launch: ") + hipGetErrorString((hipError_t)rc));
-    } else {
-        rc = rvf_launch_mc(&a, p->area.as<double>(), p->area_orig.as<double>(), st);
-        if (rc) return fail(RVF_E_HIP, std::string("mc launch: ") + hipGetErrorString((hipError_t)rc));
+    if (p->opts.method == RVF_METHOD_MC) {
+        PairArgs a = pair_args(p);
+        rc = rvf_launch_mc_area(&a, p->area.as<double>(), p->area_orig.as<double>(), st);
+        if (rc) return fail(RVF_E_HIP, std::string("mc area launch: ") + hipGetErrorString((hipError_t)rc));
     }
+    if (tim) HIPCHK(hipEventRecord(p->ev[1], st));
+    return 0;
+}
+
+static int enqueue_pairs(rvf_plan* p, hipStream_t st) {
+    PairArgs a = pair_args(p);
+    const int rc = rvf_launch_pair(&a, pair_kind(p), pair_lds(p), st);
+    if (rc) return fail(RVF_E_HIP, std::string("pair launch: ") + hipGetErrorString((hipError_t)rc));
+    return 0;
+}
+
+static int enqueue_epilogue(rvf_plan* p, hipStream_t st, bool tim) {
+    const rvf_mesh* m = p->mesh;
+    int rc;
     if (tim) HIPCHK(hipEventRecord(p->ev[2], st));
+    PairArgs a = pair_args(p);
     if (p->npairs) {
         rc = rvf_launch_duffy(&a, p->d_pairs.as<DuffyPair>(), p->npairs, p->d_gl01.as<double>(), p->opts.nquad,
                               p->d_dval.as<double>(), nullptr, nullptr, nullptr, nullptr, 0, st);
@@ -961,6 +974,40 @@ static int enqueue_solve(rvf_plan* p, hipStream_t st, bool tim) {
     if (tim) HIPCHK(hipEventRecord(p->ev[5], st));
     return 0;
 }
+
+
+static int enqueue_solve(rvf_plan* p, hipStream_t st, bool tim) {
+    int rc;
+    if ((rc = enqueue_prologue(p, st, tim)) || (rc = enqueue_pairs(p, st)) || (rc = enqueue_epilogue(p, st, tim)))
+        return rc;
+    return 0;
+}
+
+// --- internal interface for the request server (rvf_server.cpp) ---
+int rvfi_plan_prologue(rvf_plan* p, hipStream_t st) { return enqueue_prologue(p, st, false); }
+int rvfi_plan_epilogue(rvf_plan* p, hipStream_t st) {
+    const int rc = enqueue_epilogue(p, st, false);
+    if (!rc) p->ran = true;  // results can be fetched once the stream has drained
+    return rc;
+}
+int rvfi_plan_batch(const rvf_plan* p, PairArgs* a, int* kind, size_t* lds) {
+    *a = pair_args(p);
+    *kind = pair_kind(p);
+    *lds = pair_lds(p);
+    return 0;
+}
+size_t rvfi_plan_bytes(const rvf_plan* p) {
+    const DBuf* bufs[] = {&p->rule, &p->P64, &p->Q64, &p->P32, &p->Q32, &p->anchor, &p->area, &p->elem_np, &p->bad,
+                          &p->F, &p->part, &p->partT, &p->rowsum, &p->W, &p->audit_c, &p->audit_h, &p->counters,
+                          &p->Tseg, &p->Wseg, &p->area_orig, &p->d_pairs, &p->d_gl01, &p->d_dval, &p->d_rptr,
+                          &p->d_rlist};
+    size_t s = 0;
+    for (const DBuf* b : bufs) s += b->n;
+    return s;
+}
+hipStream_t rvfi_ctx_stream(const rvf_ctx* c) { return c->stream; }
+int rvfi_ctx_device(const rvf_ctx* c) { return c->device; }
+int rvfi_fail(int code, const std::string& msg) { return fail(code, msg); }
 
 int rvf_plan_kernel(const rvf_plan* p, char* name, int32_t cap) {
     if (!p || !name || cap < 1) return fail(RVF_E_ARG, "rvf_plan_kernel: bad arguments");

@@ -1990,12 +1990,17 @@ __global__ __launch_bounds__(64) void k_pair(PairArgs a) {
 
 template <class TAG>
 __global__ __launch_bounds__(64) void k_pair_batched(const PairArgs* __restrict__ reqs, const int2* __restrict__ work,
-                                                     int nwork, int* __restrict__ head) {
+                                                     const int* __restrict__ ctl) {
+    // ctl[0]: queue head (zeroed before the launch), ctl[1]: number of items.
+    // The claim is made by every lane without a lane-dependent branch (lane 0
+    // adds 1, the others 0) and read back with readfirstlane, so the item index
+    // and the loop exit are wave-uniform: the loop never runs with part of the
+    // wave, which the tile code's cross-lane operations require.
     extern __shared__ double lds[];
+    int* head = const_cast<int*>(ctl);
+    const int nwork = ctl[1];
     for (;;) {
-        int w = 0;
-        if (threadIdx.x == 0) w = atomicAdd(head, 1);
-        w = __shfl(w, 0, 64);
+        const int w = __builtin_amdgcn_readfirstlane(atomicAdd(head, threadIdx.x == 0 ? 1 : 0));
         if (w >= nwork) break;
         const int2 it = work[w];
         const PairArgs& a = reqs[it.x];
@@ -2029,9 +2034,15 @@ static void launch_direct(const PairArgs* a, size_t lds, hipStream_t st) {
     hipLaunchKernelGGL(k_pair<TAG>, dim3((unsigned)a->ntiles), dim3(64), lds, st, *a);
 }
 template <class TAG>
-static void launch_batched(const PairArgs* reqs, const int2* work, int nwork, int* head, int grid, size_t lds,
+static void launch_batched(const PairArgs* reqs, const int2* work, const int* ctl, int grid, size_t lds,
                            hipStream_t st) {
-    hipLaunchKernelGGL(k_pair_batched<TAG>, dim3((unsigned)grid), dim3(64), lds, st, reqs, work, nwork, head);
+    hipLaunchKernelGGL(k_pair_batched<TAG>, dim3((unsigned)grid), dim3(64), lds, st, reqs, work, ctl);
+}
+template <class TAG>
+static int batched_blocks(size_t lds) {
+    int nb = 0;
+    if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&nb, k_pair_batched<TAG>, 64, lds) != hipSuccess) return 0;
+    return nb;
 }
 
 extern "C" {
@@ -2060,20 +2071,31 @@ int rvf_launch_pair(const PairArgs* a, int kind, size_t lds, hipStream_t st) {
     return (int)hipGetLastError();
 }
 
-int rvf_launch_pair_batched(int kind, const PairArgs* reqs, const int2* work, int nwork, int* head, int grid,
-                            size_t lds, hipStream_t st) {
-    if (nwork == 0) return 0;
+// Persistent batched pair launch: `grid` workgroups drain the (request, tile)
+// items of `work`; ctl = {head, count} in device memory (head zeroed first).
+int rvf_launch_pair_batched(int kind, const PairArgs* reqs, const int2* work, const int* ctl, int grid, size_t lds,
+                            hipStream_t st) {
+    if (grid <= 0) return 0;
     if (kind < 0 || kind >= RVF_K_COUNT) return (int)hipErrorInvalidValue;
-#define RVF_KIND_CALL(t) launch_batched<t>(reqs, work, nwork, head, grid, lds, st)
+#define RVF_KIND_CALL(t) launch_batched<t>(reqs, work, ctl, grid, lds, st)
     switch (kind) { RVF_PAIR_KINDS(RVF_KIND_CASE) }
 #undef RVF_KIND_CALL
     return (int)hipGetLastError();
 }
 
-int rvf_launch_mc(const PairArgs* a, double* area, double* area_orig, hipStream_t st) {
+// Resident workgroups per CU of the batched kernel of `kind` with `lds` bytes.
+int rvf_batched_blocks_per_cu(int kind, size_t lds) {
+    if (kind < 0 || kind >= RVF_K_COUNT) return 0;
+    int nb = 0;
+#define RVF_KIND_CALL(t) nb = batched_blocks<t>(lds)
+    switch (kind) { RVF_PAIR_KINDS(RVF_KIND_CASE) }
+#undef RVF_KIND_CALL
+    return nb;
+}
+
+int rvf_launch_mc_area(const PairArgs* a, double* area, double* area_orig, hipStream_t st) {
     if (a->N > 0)
         hipLaunchKernelGGL(k_mc_area, dim3((unsigned)((a->N + 63) / 64)), dim3(64), 0, st, *a, area, area_orig);
-    if (a->ntiles > 0) launch_direct<monte_carlo>(a, 0, st);
     return (int)hipGetLastError();
 }
 
