@@ -125,6 +125,9 @@ def main():
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-verify", action="store_true")
     ap.add_argument("--no-secondary", dest="secondary", action="store_false")
+    ap.add_argument("--serving-requests", type=int, default=24)
+    ap.add_argument("--serving-rate", type=float, default=8.0, help="Poisson arrival rate (requests/s)")
+    ap.add_argument("--no-serving", dest="serving", action="store_false")
     ap.add_argument("--traffic-probe", action="store_true",
                     help="one copy of known size, then one solve (for rocprofv3 --pmc runs)")
     ap.add_argument("--probe-precision", default="mixed", choices=["mixed", "fp64"])
@@ -221,7 +224,68 @@ def main():
     plan_m.close(); plan_d.close()
     if args.secondary and args.workload != "cube":
         out["unoccluded_cube"] = secondary_cube(api, ctx, args.steps)
+    if args.serving and args.serving_requests > 0:
+        out["serving"] = serving_stream(api, ctx, args.serving_requests, args.serving_rate)
     print(json.dumps(out))
+
+
+def serving_stream(api, ctx, nreq, rate, seed=0x5EED0005):
+    """The serving stream: seeded Poisson arrivals of random closed box
+    enclosures (sides U[0.5, 2], 500-20,000 elements log-uniform, per-face
+    Quad4/Tri3/Quad8, 0-3 box occluders of <= 8k Tri3 each, nquad U{2..5}),
+    submitted to the request server at their arrival times.  Per precision:
+    throughput (ordered element pairs of all requests / makespan) and latency
+    percentiles (submission to result on the server's clock); mixed against
+    FP64 per request: max |row-sum difference| and max |F_group difference|."""
+    import threading
+    sys.path.insert(0, ROOT)
+    from rvf_amd import mesh as M
+    rng = np.random.default_rng(seed)
+    reqs = []
+    for _ in range(nreq):
+        n = int(round(np.exp(rng.uniform(np.log(500), np.log(20000)))))
+        mesh, obs, nq = M.random_box_enclosure(rng, n, max_occluders=3, max_occluder_tris=8000)
+        reqs.append((mesh, obs, int(nq)))
+    arrivals = np.cumsum(rng.exponential(1.0 / rate, size=nreq))
+    res = {"requests": nreq, "arrival_rate_per_s": rate, "seed": seed,
+           "elements": [int(m.n_elems) for m, _, _ in reqs], "nquad": [nq for _, _, nq in reqs],
+           "occluders": [len(o) for _, o, _ in reqs]}
+    outs = {}
+    for prec in ("mixed", "fp64"):
+        srv = api.Server(ctx)
+        tickets = [None] * nreq
+        t0 = time.perf_counter()
+
+        def submitter():
+            for k, (mesh, obs, nq) in enumerate(reqs):
+                dt = arrivals[k] - (time.perf_counter() - t0)
+                if dt > 0:
+                    time.sleep(dt)
+                tickets[k] = srv.submit(mesh, nq, obs, "geometric", prec)
+
+        th = threading.Thread(target=submitter)
+        th.start()
+        th.join()
+        for t in tickets:
+            if srv.wait(t, 600000) != api.REQ_DONE:
+                raise RuntimeError("serving request failed")
+        got = [srv.result(t) for t in tickets]
+        st = [g[6] for g in got]
+        lat = np.array([s["t_done_ms"] - s["t_submit_ms"] for s in st])
+        span_s = (max(s["t_done_ms"] for s in st) - min(s["t_submit_ms"] for s in st)) * 1e-3
+        pairs = float(sum(float(m.n_elems) ** 2 for m, _, _ in reqs))
+        c = srv.stats()
+        srv.close()
+        outs[prec] = got
+        res[prec] = {"pairs_per_s": pairs / span_s, "makespan_s": span_s,
+                     "latency_ms": {"p50": float(np.percentile(lat, 50)), "p99": float(np.percentile(lat, 99)),
+                                    "max": float(lat.max())},
+                     "rounds": c["rounds"], "peak_device_bytes": c["peak_bytes"],
+                     "worker_ms": {k: c[k] for k in ("launch_ms", "admit_ms", "wait_ms", "retire_ms")}}
+    res["speedup_mixed_vs_fp64"] = res["fp64"]["makespan_s"] / res["mixed"]["makespan_s"]
+    res["closure_err"] = float(max(np.abs(a[2] - b[2]).max() for a, b in zip(outs["mixed"], outs["fp64"])))
+    res["max_abs_dF_group"] = float(max(np.abs(a[3] - b[3]).max() for a, b in zip(outs["mixed"], outs["fp64"])))
+    return res
 
 
 def secondary_cube(api, ctx, steps):

@@ -179,6 +179,9 @@ typedef struct rvf_server_opts {
     int64_t hbm_budget;   /* device bytes for admitted requests (0 = 64 GiB) */
     int32_t max_active;   /* requests admitted at once (0 = 64) */
     int32_t round_tiles;  /* pair tiles per request per round (0 = 4096) */
+    int32_t direct_tiles; /* requests with at least this many tiles get direct launches of their
+                             tile ranges, smaller ones the batched kernel (0 = 512, -1 = never) */
+    int32_t reserved;
 } rvf_server_opts;
 typedef struct rvf_request_stats {
     double t_submit_ms, t_admit_ms, t_start_ms, t_done_ms;  /* since the server started */
@@ -187,6 +190,9 @@ typedef struct rvf_request_stats {
 typedef struct rvf_server_counters {
     int64_t submitted, completed, failed, cancelled, rounds, tiles;
     int64_t peak_bytes, used_bytes, pending, active;
+    /* worker time: launching rounds, admitting (upload, BVH, plan), waiting
+     * for rounds, retiring (result copies, frees) */
+    double launch_ms, admit_ms, wait_ms, retire_ms;
 } rvf_server_counters;
 /* request states returned by rvf_server_poll / rvf_server_wait */
 #define RVF_REQ_PENDING 0

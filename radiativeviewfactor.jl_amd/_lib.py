@@ -30,7 +30,8 @@ class Opts(C.Structure):
 
 
 class ServerOpts(C.Structure):
-    _fields_ = [("hbm_budget", C.c_int64), ("max_active", C.c_int32), ("round_tiles", C.c_int32)]
+    _fields_ = [("hbm_budget", C.c_int64), ("max_active", C.c_int32), ("round_tiles", C.c_int32),
+                ("direct_tiles", C.c_int32), ("reserved", C.c_int32)]
 
 
 class RequestStats(C.Structure):
@@ -43,7 +44,8 @@ class RequestStats(C.Structure):
 
 class ServerCounters(C.Structure):
     _fields_ = [(k, C.c_int64) for k in ("submitted", "completed", "failed", "cancelled", "rounds", "tiles",
-                                          "peak_bytes", "used_bytes", "pending", "active")]
+                                          "peak_bytes", "used_bytes", "pending", "active")] + \
+               [(k, C.c_double) for k in ("launch_ms", "admit_ms", "wait_ms", "retire_ms")]
 
     def as_dict(self):
         return {k: getattr(self, k) for k, _ in self._fields_}

@@ -211,8 +211,9 @@ class Server:
     thread are admitted by device-memory footprint and batched with the other
     running solves; each result equals the standalone solve's."""
 
-    def __init__(self, ctx: Context, hbm_budget: int = 0, max_active: int = 0, round_tiles: int = 0):
-        o = _lib.ServerOpts(hbm_budget, max_active, round_tiles)
+    def __init__(self, ctx: Context, hbm_budget: int = 0, max_active: int = 0, round_tiles: int = 0,
+                 direct_tiles: int = 0):
+        o = _lib.ServerOpts(hbm_budget, max_active, round_tiles, direct_tiles, 0)
         h = C.c_void_p()
         check(_lib.lib().rvf_server_create(ctx.h, C.byref(o), C.byref(h)))
         self.ctx, self.h = ctx, h

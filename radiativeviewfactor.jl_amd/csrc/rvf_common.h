@@ -94,7 +94,8 @@ struct Chunk {
 
 struct PairArgs {
     const int2* tiles;        // (strip, chunk)
-    int ntiles;
+    int ntiles;               // tiles of this launch (grid of a direct launch)
+    int tile0;                // first tile of a direct launch (tile = tile0 + block)
     int NS;                   // slots = N rounded up to RVF_STRIP
     int maxpts;               // per-element point stride in Q arrays
     int64_t N;                // elements

@@ -990,6 +990,10 @@ int rvfi_plan_epilogue(rvf_plan* p, hipStream_t st) {
     if (!rc) p->ran = true;  // results can be fetched once the stream has drained
     return rc;
 }
+static int plan_collect(rvf_plan* p);
+// the server calls this once an event after the request's epilogue has
+// completed (the context stream may still be busy with other requests)
+int rvfi_plan_collect(rvf_plan* p) { return plan_collect(p); }
 int rvfi_plan_batch(const rvf_plan* p, PairArgs* a, int* kind, size_t* lds) {
     *a = pair_args(p);
     *kind = pair_kind(p);
@@ -1069,10 +1073,9 @@ int rvf_plan_run(rvf_plan* p) {
     return 0;
 }
 
-int rvf_plan_sync(rvf_plan* p) {
-    if (!p) return fail(RVF_E_ARG, "null plan");
-    HIPCHK(hipSetDevice(p->ctx->device));
-    HIPCHK(hipStreamSynchronize(p->ctx->stream));
+// Reads the degenerate-element flag and the traversal counters of a solve
+// whose work has completed.
+static int plan_collect(rvf_plan* p) {
     int bad = 0;
     HIPCHK(hipMemcpy(&bad, p->bad.p, sizeof(int), hipMemcpyDeviceToHost));
     if (bad) return fail(RVF_E_ARG, "degenerate element: zero or non-finite area element at a quadrature point");
@@ -1086,6 +1089,15 @@ int rvf_plan_sync(rvf_plan* p) {
     p->st.slot_steps = (int64_t)cnt[5];
     p->st.task_batches = (int64_t)cnt[6];
     p->st.exact_batches = (int64_t)cnt[7];
+    return 0;
+}
+
+int rvf_plan_sync(rvf_plan* p) {
+    if (!p) return fail(RVF_E_ARG, "null plan");
+    HIPCHK(hipSetDevice(p->ctx->device));
+    HIPCHK(hipStreamSynchronize(p->ctx->stream));
+    int rc = plan_collect(p);
+    if (rc) return rc;
     if (p->opts.flags & RVF_TIMING) {
         float ms;
         HIPCHK(hipEventElapsedTime(&ms, p->ev[0], p->ev[1])); p->st.precompute_ms = ms;

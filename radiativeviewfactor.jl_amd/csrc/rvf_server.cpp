@@ -1,18 +1,23 @@
 // Request server: continuous batching of view-factor solves on one GPU.
 //
 // Clients submit solves (mesh, optional obstruction soup, options) from any
-// thread; one worker thread owns the device stream.  Each scheduling round
-//   1. launches, per pair-kernel kind, one persistent batched kernel that
-//      drains a device list of (request, tile) items drawn from every running
-//      request (at most round_tiles tiles per request, interleaved in chunks
-//      so that the requests' work overlaps on the device),
-//   2. while the round runs, admits pending requests in arrival order as long
-//      as their estimated HBM footprint fits the budget and a slot is free:
-//      uploads the mesh, builds the BVHs, creates the plan and enqueues its
-//      prologue (these follow the round on the stream),
-//   3. after the round, enqueues the epilogue of requests whose tiles are all
-//      done, copies their results to host memory and frees their device
-//      memory, which lets further requests in.
+// thread; one worker thread owns the device streams.  The worker loops:
+//   1. admits pending requests in arrival order while their estimated HBM
+//      footprint fits the budget and a slot is free: uploads the mesh, builds
+//      the BVHs, creates the plan and enqueues its prologue (point precompute)
+//      on the context stream;
+//   2. a large request (at least direct_tiles tiles) gets its whole tile range
+//      as a direct launch, then its epilogue, on its slot's stream, ordered
+//      after its prologue by an event; the slot streams run concurrently, so
+//      requests that arrive while others run start at once and share the
+//      device at workgroup granularity;
+//   3. small requests share rounds: per pair-kernel kind, one persistent
+//      batched kernel drains a device list of up to round_tiles (request,
+//      tile) items per request in one launch on the context stream, followed
+//      by the epilogues of the requests it finished;
+//   4. retires requests whose epilogue event has completed: copies their
+//      results to host memory and frees their device memory, which lets
+//      further requests in.
 // A request's tiles are the ones its standalone plan launches, computed by
 // the same device code, and its reductions run per request, so a result is
 // bit-identical whether it was solved alone or in a mix.
@@ -38,11 +43,13 @@
 // internal interface of rvf_host.cpp
 int rvfi_plan_prologue(rvf_plan* p, hipStream_t st);
 int rvfi_plan_epilogue(rvf_plan* p, hipStream_t st);
+int rvfi_plan_collect(rvf_plan* p);
 int rvfi_plan_batch(const rvf_plan* p, PairArgs* a, int* kind, size_t* lds);
 size_t rvfi_plan_bytes(const rvf_plan* p);
 hipStream_t rvfi_ctx_stream(const rvf_ctx* c);
 int rvfi_ctx_device(const rvf_ctx* c);
 int rvfi_fail(int code, const std::string& msg);
+extern "C" int rvf_launch_pair(const PairArgs* a, int kind, size_t lds, hipStream_t st);
 extern "C" int rvf_launch_pair_batched(int kind, const PairArgs* reqs, const int2* work, const int* ctl, int grid,
                                        size_t lds, hipStream_t st);
 extern "C" int rvf_batched_blocks_per_cu(int kind, size_t lds);
@@ -75,7 +82,9 @@ struct Req {
     rvf_plan* plan = nullptr;
     PairArgs args{};
     int kind = 0, slot = -1, ntiles = 0, next_tile = 0;
-    bool epilogue = false;             // epilogue enqueued: retire after the round
+    bool epilogue = false;             // epilogue enqueued: retire once done_ev completes
+    hipEvent_t ready_ev = nullptr;     // prologue done (context stream)
+    hipEvent_t done_ev = nullptr;      // epilogue done
     size_t lds = 0, est_bytes = 0;
     // results
     std::vector<double> F, A, rowsum, Fg, Ag;
@@ -101,12 +110,22 @@ struct rvf_server {
     std::map<int64_t, Req*> all;
     size_t used = 0;                   // estimated bytes of admitted requests
     std::vector<int> free_slots;
+    // Device objects of finished requests wait here for the reaper thread:
+    // freeing device memory waits for all work queued on the device, which
+    // would stall the worker behind other requests' kernels.
+    struct Dead { rvf_plan* plan; rvf_obs* obs; rvf_mesh* mesh; size_t bytes; };
+    std::deque<Dead> graveyard;
+    std::condition_variable cv_reap;
+    bool reap_stop = false;
+    std::thread reaper;
     // device batch buffers
     PairArgs* d_args = nullptr;        // [max_active]
     int2* d_work = nullptr;            // [work_cap]
     int* d_ctl = nullptr;              // [2 * RVF_K_COUNT]: head, count per kind
     int2* h_work = nullptr;            // pinned staging
     int* h_ctl = nullptr;
+    static constexpr int kStreams = 4;
+    hipStream_t streams[kStreams] = {};
     int64_t work_cap = 0;
     int ncu = 0;
     std::chrono::steady_clock::time_point t0;
@@ -138,14 +157,22 @@ size_t estimate_bytes(const Req& r) {
     return b + b / 8 + (1u << 20);
 }
 
+// Ends a request: its device work must have completed.  Hands its device
+// objects to the reaper (which returns the memory to the budget) and frees
+// its slot.
 void finish(rvf_server* s, Req* r, int state) {
-    if (r->plan) { rvf_plan_free(r->plan); r->plan = nullptr; }
-    if (r->obs) { rvf_obs_free(r->obs); r->obs = nullptr; }
-    if (r->mesh) { rvf_mesh_free(r->mesh); r->mesh = nullptr; }
+    if (r->ready_ev) { (void)hipEventDestroy(r->ready_ev); r->ready_ev = nullptr; }
+    if (r->done_ev) { (void)hipEventDestroy(r->done_ev); r->done_ev = nullptr; }
     std::lock_guard<std::mutex> g(s->mu);
+    if (r->plan || r->obs || r->mesh) {
+        s->graveyard.push_back({r->plan, r->obs, r->mesh, r->est_bytes});
+        s->cv_reap.notify_all();
+    } else {
+        s->used -= std::min(s->used, r->est_bytes);
+    }
+    r->plan = nullptr; r->obs = nullptr; r->mesh = nullptr;
     if (r->slot >= 0) s->free_slots.push_back(r->slot);
     r->slot = -1;
-    s->used -= std::min(s->used, r->est_bytes);
     r->state = state;
     r->st.t_done_ms = s->now_ms();
     if (state == REQ_DONE) s->st.completed++;
@@ -155,7 +182,16 @@ void finish(rvf_server* s, Req* r, int state) {
     s->cv_done.notify_all();
 }
 
-// Upload, build and enqueue the prologue of an admitted request.
+hipStream_t req_stream(const rvf_server* s, const Req* r) { return s->streams[r->slot % rvf_server::kStreams]; }
+
+// Large requests get direct launches of their tile ranges; smaller ones are
+// drained by the batched kernel.
+bool is_direct(const rvf_server* s, const Req* r) { return s->o.direct_tiles > 0 && r->ntiles >= s->o.direct_tiles; }
+
+// Upload, build and enqueue the prologue of an admitted request.  The
+// prologue goes on the context stream, where plan creation queued its zero
+// fills of the same buffers; ready_ev, recorded after it, orders it before
+// the request's tiles on a slot stream.
 int admit(rvf_server* s, Req* r) {
     hipStream_t st = rvfi_ctx_stream(s->ctx);
     const int32_t* nodes[4] = {r->nodes[0].data(), r->nodes[1].data(), r->nodes[2].data(), r->nodes[3].data()};
@@ -167,6 +203,12 @@ int admit(rvf_server* s, Req* r) {
     if (!rc) rc = rvf_plan_create(s->ctx, r->mesh, r->obs, &r->opts, &r->plan);
     if (!rc) rc = rvfi_plan_batch(r->plan, &r->args, &r->kind, &r->lds);
     if (!rc) rc = rvfi_plan_prologue(r->plan, st);
+    if (!rc) {
+        hipError_t e = hipEventCreateWithFlags(&r->ready_ev, hipEventDisableTiming);
+        if (e == hipSuccess) e = hipEventCreateWithFlags(&r->done_ev, hipEventDisableTiming);
+        if (e == hipSuccess) e = hipEventRecord(r->ready_ev, st);
+        if (e != hipSuccess) rc = rvfi_fail(RVF_E_HIP, std::string("server: events: ") + hipGetErrorString(e));
+    }
     if (!rc) {
         r->ntiles = r->args.ntiles;
         hipError_t e = hipMemcpyAsync(s->d_args + r->slot, &r->args, sizeof(PairArgs), hipMemcpyHostToDevice, st);
@@ -184,72 +226,39 @@ int admit(rvf_server* s, Req* r) {
     return 0;
 }
 
+void reaper_loop(rvf_server* s) {
+    (void)hipSetDevice(rvfi_ctx_device(s->ctx));
+    for (;;) {
+        rvf_server::Dead d;
+        {
+            std::unique_lock<std::mutex> g(s->mu);
+            s->cv_reap.wait(g, [&] { return s->reap_stop || !s->graveyard.empty(); });
+            if (s->graveyard.empty()) break;
+            d = s->graveyard.front();
+            s->graveyard.pop_front();
+        }
+        rvf_plan_free(d.plan);
+        rvf_obs_free(d.obs);
+        rvf_mesh_free(d.mesh);
+        {
+            std::lock_guard<std::mutex> g(s->mu);
+            s->used -= std::min(s->used, d.bytes);
+        }
+        s->cv_work.notify_all();
+    }
+}
+
 void worker_loop(rvf_server* s) {
     (void)hipSetDevice(rvfi_ctx_device(s->ctx));
     hipStream_t st = rvfi_ctx_stream(s->ctx);
-    std::vector<Req*> running;  // requests with tiles or an epilogue outstanding
+    std::vector<Req*> running;  // admitted, not yet retired
+    hipEvent_t round_ev = nullptr;
+    (void)hipEventCreateWithFlags(&round_ev, hipEventDisableTiming);
+    bool round_open = false;    // a batched round is in flight
     for (;;) {
-        // ---- build and launch one round from the running requests ----
-        struct Chunk { Req* r; int t0, t1; };
-        std::vector<Chunk> chunks[RVF_K_COUNT];
-        std::vector<Req*> completing;
-        int64_t nitems = 0;
-        for (Req* r : running) {
-            if (r->cancel) continue;
-            if (r->next_tile < r->ntiles) {
-                const int n = std::min(r->ntiles - r->next_tile, s->o.round_tiles);
-                for (int t = r->next_tile; t < r->next_tile + n; t += 32)
-                    chunks[r->kind].push_back({r, t, std::min(t + 32, r->next_tile + n)});
-                r->next_tile += n;
-                nitems += n;
-                if (r->st.t_start_ms == 0) r->st.t_start_ms = s->now_ms();
-            }
-            if (r->next_tile >= r->ntiles && !r->epilogue) {
-                completing.push_back(r);
-                r->epilogue = true;
-            }
-        }
+        const double t0 = s->now_ms();
         int rc = 0;
-        if (nitems > 0 || !completing.empty()) {
-            // interleave the chunks of a kind round-robin over its requests
-            int64_t off = 0;
-            for (int k = 0; k < RVF_K_COUNT && !rc; k++) {
-                if (chunks[k].empty()) continue;
-                std::map<Req*, std::vector<Chunk>> byreq;
-                for (const Chunk& c : chunks[k]) byreq[c.r].push_back(c);
-                const int64_t start = off;
-                size_t lds = 0;
-                for (bool more = true; more;) {
-                    more = false;
-                    for (auto& kv : byreq) {
-                        if (kv.second.empty()) continue;
-                        const Chunk c = kv.second.front();
-                        kv.second.erase(kv.second.begin());
-                        for (int t = c.t0; t < c.t1; t++) s->h_work[off++] = make_int2(c.r->slot, t);
-                        lds = std::max(lds, c.r->lds);
-                        more = true;
-                    }
-                }
-                const int cnt = (int)(off - start);
-                s->h_ctl[2 * k] = 0;
-                s->h_ctl[2 * k + 1] = cnt;
-                hipError_t e = hipMemcpyAsync(s->d_work + start, s->h_work + start, sizeof(int2) * cnt,
-                                              hipMemcpyHostToDevice, st);
-                if (e == hipSuccess)
-                    e = hipMemcpyAsync(s->d_ctl + 2 * k, s->h_ctl + 2 * k, 2 * sizeof(int), hipMemcpyHostToDevice, st);
-                if (e != hipSuccess) { rc = rvfi_fail(RVF_E_HIP, std::string("server: work upload: ") + hipGetErrorString(e)); break; }
-                const int per_cu = std::max(1, rvf_batched_blocks_per_cu(k, lds));
-                const int grid = (int)std::min<int64_t>(cnt, (int64_t)s->ncu * per_cu);
-                rc = rvf_launch_pair_batched(k, s->d_args, s->d_work + start, s->d_ctl + 2 * k, grid, lds, st);
-                if (rc) rc = rvfi_fail(RVF_E_HIP, std::string("server: batched launch: ") + hipGetErrorString((hipError_t)rc));
-            }
-            for (Req* r : completing)
-                if (!rc && !r->cancel) rc = rvfi_plan_epilogue(r->plan, st);
-            std::lock_guard<std::mutex> g(s->mu);
-            s->st.rounds++;
-            s->st.tiles += nitems;
-        }
-        // ---- admissions (their prologues follow the round on the stream) ----
+        // ---- 1. admissions ----
         std::vector<Req*> admitted;
         {
             std::unique_lock<std::mutex> g(s->mu);
@@ -284,34 +293,149 @@ void worker_loop(rvf_server* s) {
                 s->active.push_back(r);
                 admitted.push_back(r);
             }
-            if (nitems == 0 && completing.empty() && admitted.empty() && running.empty()) {
+            if (admitted.empty() && running.empty()) {
                 if (s->stop && s->pending.empty()) break;
                 s->cv_work.wait(g);
                 continue;
             }
         }
         for (Req* r : admitted) {
-            if (admit(s, r)) { finish(s, r, REQ_FAILED); continue; }
+            if (admit(s, r)) {
+                (void)hipStreamSynchronize(st);  // a partial prologue may still use its buffers
+                finish(s, r, REQ_FAILED);
+                continue;
+            }
             r->st.tiles = r->ntiles;
             running.push_back(r);
         }
-        // ---- wait for the round, then retire finished requests ----
-        hipError_t e = hipStreamSynchronize(st);
-        if (e != hipSuccess && !rc) rc = rvfi_fail(RVF_E_HIP, std::string("server: round: ") + hipGetErrorString(e));
-        if (rc) {  // a device failure: fail every running request
+        const double t1 = s->now_ms();
+        // ---- 2. large requests: whole tile range + epilogue on the slot stream ----
+        for (Req* r : running) {
+            if (rc || r->cancel || r->epilogue || !is_direct(s, r)) continue;
+            hipStream_t rs = req_stream(s, r);
+            hipError_t e = hipStreamWaitEvent(rs, r->ready_ev, 0);
+            if (e != hipSuccess) { rc = rvfi_fail(RVF_E_HIP, std::string("server: wait event: ") + hipGetErrorString(e)); break; }
+            PairArgs a = r->args;
+            a.tile0 = 0;
+            a.ntiles = r->ntiles;
+            rc = rvf_launch_pair(&a, r->kind, r->lds, rs);
+            if (rc) { rc = rvfi_fail(RVF_E_HIP, std::string("server: pair launch: ") + hipGetErrorString((hipError_t)rc)); break; }
+            r->next_tile = r->ntiles;
+            r->st.t_start_ms = s->now_ms();
+            if ((rc = rvfi_plan_epilogue(r->plan, rs))) break;
+            e = hipEventRecord(r->done_ev, rs);
+            if (e != hipSuccess) { rc = rvfi_fail(RVF_E_HIP, std::string("server: record: ") + hipGetErrorString(e)); break; }
+            r->epilogue = true;
+            std::lock_guard<std::mutex> g(s->mu);
+            s->st.tiles += r->ntiles;
+        }
+        // ---- 3. small requests: one batched round at a time ----
+        if (!rc && !round_open) {
+            struct Chunk { Req* r; int t0, t1; };
+            std::vector<Chunk> chunks[RVF_K_COUNT];
+            std::vector<Req*> completing;
+            int64_t nitems = 0;
+            for (Req* r : running) {
+                if (r->cancel || r->epilogue || is_direct(s, r)) continue;
+                if (r->next_tile < r->ntiles) {
+                    const int n = std::min(r->ntiles - r->next_tile, s->o.round_tiles);
+                    for (int t = r->next_tile; t < r->next_tile + n; t += 32)
+                        chunks[r->kind].push_back({r, t, std::min(t + 32, r->next_tile + n)});
+                    r->next_tile += n;
+                    nitems += n;
+                    if (r->st.t_start_ms == 0) r->st.t_start_ms = s->now_ms();
+                }
+                if (r->next_tile >= r->ntiles) completing.push_back(r);
+            }
+            if (nitems > 0 || !completing.empty()) {
+                // this round's requests were all admitted before: their prologues
+                // precede it on the context stream
+                int64_t off = 0;
+                for (int k = 0; k < RVF_K_COUNT && !rc; k++) {
+                    if (chunks[k].empty()) continue;
+                    std::map<Req*, std::vector<Chunk>> byreq;
+                    for (const Chunk& c : chunks[k]) byreq[c.r].push_back(c);
+                    const int64_t start = off;
+                    size_t lds = 0;
+                    for (bool more = true; more;) {  // interleave the requests' chunks
+                        more = false;
+                        for (auto& kv : byreq) {
+                            if (kv.second.empty()) continue;
+                            const Chunk c = kv.second.front();
+                            kv.second.erase(kv.second.begin());
+                            for (int t = c.t0; t < c.t1; t++) s->h_work[off++] = make_int2(c.r->slot, t);
+                            lds = std::max(lds, c.r->lds);
+                            more = true;
+                        }
+                    }
+                    const int cnt = (int)(off - start);
+                    s->h_ctl[2 * k] = 0;
+                    s->h_ctl[2 * k + 1] = cnt;
+                    hipError_t e = hipMemcpyAsync(s->d_work + start, s->h_work + start, sizeof(int2) * cnt,
+                                                  hipMemcpyHostToDevice, st);
+                    if (e == hipSuccess)
+                        e = hipMemcpyAsync(s->d_ctl + 2 * k, s->h_ctl + 2 * k, 2 * sizeof(int), hipMemcpyHostToDevice, st);
+                    if (e != hipSuccess) { rc = rvfi_fail(RVF_E_HIP, std::string("server: work upload: ") + hipGetErrorString(e)); break; }
+                    const int per_cu = std::max(1, rvf_batched_blocks_per_cu(k, lds));
+                    const int grid = (int)std::min<int64_t>(cnt, (int64_t)s->ncu * per_cu);
+                    rc = rvf_launch_pair_batched(k, s->d_args, s->d_work + start, s->d_ctl + 2 * k, grid, lds, st);
+                    if (rc) rc = rvfi_fail(RVF_E_HIP, std::string("server: batched launch: ") + hipGetErrorString((hipError_t)rc));
+                }
+                for (Req* r : completing) {
+                    if (rc) break;
+                    if ((rc = rvfi_plan_epilogue(r->plan, st))) break;
+                    hipError_t e = hipEventRecord(r->done_ev, st);
+                    if (e != hipSuccess) { rc = rvfi_fail(RVF_E_HIP, std::string("server: record: ") + hipGetErrorString(e)); break; }
+                    r->epilogue = true;
+                }
+                if (!rc) {
+                    hipError_t e = hipEventRecord(round_ev, st);
+                    if (e != hipSuccess) rc = rvfi_fail(RVF_E_HIP, std::string("server: record: ") + hipGetErrorString(e));
+                    round_open = true;
+                }
+                std::lock_guard<std::mutex> g(s->mu);
+                s->st.rounds++;
+                s->st.tiles += nitems;
+            }
+        }
+        const double t2 = s->now_ms();
+        if (rc) {  // a launch failed: fail every running request
             const std::string msg = rvf_last_error();
+            (void)hipDeviceSynchronize();
             for (Req* r : running) { r->err = msg; finish(s, r, REQ_FAILED); }
             running.clear();
+            round_open = false;
             continue;
         }
+        // ---- 4. wait for progress: the open round, or any request's epilogue ----
+        bool progressed = false;
+        while (!progressed) {
+            if (round_open && hipEventQuery(round_ev) == hipSuccess) { round_open = false; progressed = true; }
+            for (Req* r : running)
+                if (r->epilogue && hipEventQuery(r->done_ev) == hipSuccess) progressed = true;
+            if (progressed) break;
+            {
+                std::lock_guard<std::mutex> g(s->mu);
+                if (!s->pending.empty() && !s->free_slots.empty() &&
+                    s->used + s->pending.front()->est_bytes <= (size_t)s->o.hbm_budget)
+                    break;  // a new request can be admitted now
+                if (s->stop && running.empty()) break;
+            }
+            bool any_open = round_open;
+            for (Req* r : running) any_open |= r->epilogue;
+            if (!any_open) break;  // nothing in flight: go launch
+            std::this_thread::sleep_for(std::chrono::microseconds(200));
+        }
+        const double t3 = s->now_ms();
+        // ---- retire finished requests ----
         std::vector<Req*> keep;
         for (Req* r : running) {
-            if (r->cancel) { finish(s, r, REQ_CANCELLED); continue; }
-            if (!r->epilogue) {
+            if (r->cancel && !r->epilogue && !round_open) { finish(s, r, REQ_CANCELLED); continue; }
+            if (!r->epilogue || hipEventQuery(r->done_ev) != hipSuccess) {
                 keep.push_back(r);
                 continue;
             }
-            int frc = rvf_plan_sync(r->plan);
+            int frc = rvfi_plan_collect(r->plan);
             if (!frc) {
                 const int64_t N = r->N;
                 int32_t G = 0;
@@ -326,7 +450,13 @@ void worker_loop(rvf_server* s) {
             finish(s, r, frc ? REQ_FAILED : REQ_DONE);
         }
         running.swap(keep);
+        std::lock_guard<std::mutex> g(s->mu);
+        s->st.admit_ms += t1 - t0;
+        s->st.launch_ms += t2 - t1;
+        s->st.wait_ms += t3 - t2;
+        s->st.retire_ms += s->now_ms() - t3;
     }
+    if (round_ev) (void)hipEventDestroy(round_ev);
 }
 
 }  // namespace
@@ -341,6 +471,7 @@ int rvf_server_create(rvf_ctx* ctx, const rvf_server_opts* opts, rvf_server** ou
     if (s->o.hbm_budget <= 0) s->o.hbm_budget = (int64_t)64 << 30;
     if (s->o.max_active <= 0) s->o.max_active = 64;
     if (s->o.round_tiles <= 0) s->o.round_tiles = 4096;
+    if (s->o.direct_tiles == 0) s->o.direct_tiles = 512;
     s->o.max_active = std::min(s->o.max_active, 1024);
     hipError_t e = hipSetDevice(rvfi_ctx_device(ctx));
     hipDeviceProp_t prop;
@@ -352,12 +483,15 @@ int rvf_server_create(rvf_ctx* ctx, const rvf_server_opts* opts, rvf_server** ou
     if (e == hipSuccess) e = hipMalloc((void**)&s->d_ctl, sizeof(int) * 2 * RVF_K_COUNT);
     if (e == hipSuccess) e = hipHostMalloc((void**)&s->h_work, sizeof(int2) * s->work_cap, hipHostMallocDefault);
     if (e == hipSuccess) e = hipHostMalloc((void**)&s->h_ctl, sizeof(int) * 2 * RVF_K_COUNT, hipHostMallocDefault);
+    for (int k = 0; k < rvf_server::kStreams && e == hipSuccess; k++)
+        e = hipStreamCreateWithFlags(&s->streams[k], hipStreamNonBlocking);
     if (e != hipSuccess) {
         rvf_server_destroy(s);
         return rvfi_fail(RVF_E_HIP, std::string("rvf_server_create: ") + hipGetErrorString(e));
     }
     for (int k = s->o.max_active - 1; k >= 0; k--) s->free_slots.push_back(k);
     s->t0 = std::chrono::steady_clock::now();
+    s->reaper = std::thread(reaper_loop, s);
     s->worker = std::thread(worker_loop, s);
     *out = s;
     return 0;
@@ -374,12 +508,22 @@ void rvf_server_destroy(rvf_server* s) {
         s->cv_work.notify_all();
         s->worker.join();
     }
+    if (s->reaper.joinable()) {
+        {
+            std::lock_guard<std::mutex> g(s->mu);
+            s->reap_stop = true;
+        }
+        s->cv_reap.notify_all();
+        s->reaper.join();
+    }
     for (auto& kv : s->all) delete kv.second;
     if (s->d_args) (void)hipFree(s->d_args);
     if (s->d_work) (void)hipFree(s->d_work);
     if (s->d_ctl) (void)hipFree(s->d_ctl);
     if (s->h_work) (void)hipHostFree(s->h_work);
     if (s->h_ctl) (void)hipHostFree(s->h_ctl);
+    for (int k = 0; k < rvf_server::kStreams; k++)
+        if (s->streams[k]) (void)hipStreamDestroy(s->streams[k]);
     delete s;
 }
 

@@ -36,12 +36,15 @@ def _alone(ctx, mesh, obs, nq, prec):
     return F, A, rs, Fg, Ag, tags
 
 
-def test_results_identical_alone_and_in_mix():
+@pytest.mark.parametrize("direct_tiles", [-1, 1, 0])
+def test_results_identical_alone_and_in_mix(direct_tiles):
+    # -1: every request through the persistent batched kernel; 1: every request
+    # through direct tile-range launches on the slot streams; 0: the default split
     ctx = api.context(0)
     reqs = _requests(0x5EED0C01, 10)
     ref = [_alone(ctx, *r) for r in reqs]
     # small rounds and a small slot count force interleaving and admission waves
-    srv = api.Server(ctx, max_active=4, round_tiles=8)
+    srv = api.Server(ctx, max_active=4, round_tiles=8, direct_tiles=direct_tiles)
     tickets = [srv.submit(m, nq, obs, "geometric", prec, want_F=True) for m, obs, nq, prec in reqs]
     for t in tickets:
         assert srv.wait(t, 120000) == api.REQ_DONE
@@ -51,7 +54,11 @@ def test_results_identical_alone_and_in_mix():
         assert np.array_equal(Fg, r[3]) and np.array_equal(Ag, r[4]) and tags == r[5]
         assert st["t_submit_ms"] <= st["t_admit_ms"] <= st["t_start_ms"] <= st["t_done_ms"]
     c = srv.stats()
-    assert c["completed"] == len(reqs) and c["failed"] == 0 and c["rounds"] > len(reqs)
+    assert c["completed"] == len(reqs) and c["failed"] == 0
+    if direct_tiles == -1:
+        assert c["rounds"] > len(reqs)  # round_tiles=8 splits every request over rounds
+    elif direct_tiles == 1:
+        assert c["rounds"] == 0         # whole tile ranges on the slot streams, no rounds
     srv.close()
 
 
