@@ -59,8 +59,15 @@ def flop_model(N, npts, rays=0, aabb=0, tri=0):
     return pairs * (npts * npts * 28 + npts * 3) + 15 * rays + 24 * aabb + 52 * tri
 
 
-def run_mode(api, ctx, dm, do, nquad, precision, steps, warmup, audit=False):
+def run_mode(api, ctx, dm, do, nquad, precision, steps, warmup, audit=False, tune=None):
     plan = api.Plan(ctx, dm, do, nquad, "quad", precision, want_F=True, audit=audit, timing=True)
+    if tune is not None and do is not None:
+        # autotuner (setup, outside the timed region): time each traversal
+        # configuration over a tile sample and keep the fastest
+        t0 = time.perf_counter()
+        v, r, meas = plan.autotune(sample_tiles=tune["sample_tiles"])
+        tune.update(variant=v, refill_min=r, kernel=plan.kernel_name(), seconds=time.perf_counter() - t0,
+                    measurements=meas)
     for _ in range(warmup):
         plan.run(); plan.sync()
     lat, pair_ms = [], []
@@ -128,6 +135,9 @@ def main():
     ap.add_argument("--serving-requests", type=int, default=24)
     ap.add_argument("--serving-rate", type=float, default=8.0, help="Poisson arrival rate (requests/s)")
     ap.add_argument("--no-serving", dest="serving", action="store_false")
+    ap.add_argument("--no-autotune", dest="autotune", action="store_false",
+                    help="keep the default traversal configuration instead of autotuning it")
+    ap.add_argument("--tune-tiles", type=int, default=8192, help="tiles the autotuner times per configuration")
     ap.add_argument("--traffic-probe", action="store_true",
                     help="one copy of known size, then one solve (for rocprofv3 --pmc runs)")
     ap.add_argument("--probe-precision", default="mixed", choices=["mixed", "fp64"])
@@ -155,10 +165,12 @@ def main():
                           "pair_ms": plan.stats()["pair_ms"]}))
         return
 
-    plan_m, ms_m, lat_m, pair_m = run_mode(api, ctx, dm, do, nquad, "mixed", args.steps, args.warmup)
+    tune_m = {"sample_tiles": args.tune_tiles} if args.autotune else None
+    tune_d = {"sample_tiles": args.tune_tiles} if args.autotune else None
+    plan_m, ms_m, lat_m, pair_m = run_mode(api, ctx, dm, do, nquad, "mixed", args.steps, args.warmup, tune=tune_m)
     st_m = plan_m.stats()
     plan_d, ms_d, lat_d, pair_d = run_mode(api, ctx, dm, do, nquad, "fp64", max(2, args.steps // 2),
-                                           min(args.warmup, 1))
+                                           min(args.warmup, 1), tune=tune_d)
     st_d = plan_d.stats()
 
     # verification of mixed against FP64 on the same inputs (outside the timed region)
@@ -219,6 +231,8 @@ def main():
                       "mixed_slot_steps": st_m.get("slot_steps"), "mixed_task_batches": st_m.get("task_batches")},
         "verification": ver, "roofline": roof, "cpu_baseline": cpu,
     }
+    if args.autotune:
+        out["autotune"] = {"mixed": tune_m, "fp64": tune_d}
     if cpu:
         out["gpu_vs_cpu_unique_pairs"] = (N * (N - 1) / 2) / (ms_m * 1e-3) / cpu["value"]
     plan_m.close(); plan_d.close()

@@ -150,6 +150,25 @@ int rvf_plan_tune(rvf_plan* plan, int32_t variant, int32_t refill_min);
 /* Name of the pair kernel the plan launches (as profilers show it), e.g.
  * "k_pair<mixed_occ_split3>" (no reference counterpart; for measurement). */
 int rvf_plan_kernel(const rvf_plan* plan, char* name, int32_t cap);
+/* One measured configuration of rvf_plan_autotune. */
+typedef struct {
+    int32_t variant;     /* RVF_VARIANT_* */
+    int32_t refill_min;  /* wave-queue refill threshold (0 where it does not apply) */
+    int64_t tiles;       /* tiles timed */
+    double ms;           /* pair-kernel time over those tiles */
+} rvf_tune_entry;
+/* Autotuner (no reference counterpart): runs the plan's prologue, then times
+ * its pair kernel under every traversal variant valid for its precision over
+ * sample_tiles tiles (0 = 8192) taken at a fixed stride through the tile
+ * list, re-times the three fastest over the whole problem (or four times the
+ * sample when the problem is larger), tries the refill thresholds 8 and 32
+ * for the winner, and keeps the fastest configuration as rvf_plan_tune would
+ * (*variant, *refill_min).  Writes up to cap measurements to entries and
+ * their count to *n.  Plans without obstruction trees have a single pair
+ * kernel: *n = 0 and nothing changes.  Results of later runs are the same
+ * whichever configuration is kept. */
+int rvf_plan_autotune(rvf_plan* plan, int64_t sample_tiles, rvf_tune_entry* entries, int32_t cap, int32_t* n,
+                      int32_t* variant, int32_t* refill_min);
 void rvf_plan_free(rvf_plan* plan);
 
 /* One-shot: the body of the Julia hook (compute_view_factors_gpu,

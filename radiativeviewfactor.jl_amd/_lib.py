@@ -51,6 +51,10 @@ class ServerCounters(C.Structure):
         return {k: getattr(self, k) for k, _ in self._fields_}
 
 
+class TuneEntry(C.Structure):
+    _fields_ = [("variant", C.c_int32), ("refill_min", C.c_int32), ("tiles", C.c_int64), ("ms", C.c_double)]
+
+
 class Stats(C.Structure):
     _fields_ = [("precompute_ms", C.c_double), ("pair_ms", C.c_double), ("duffy_ms", C.c_double),
                 ("fill_ms", C.c_double), ("reduce_ms", C.c_double), ("total_ms", C.c_double),
@@ -96,6 +100,8 @@ def lib():
         "rvf_plan_device_ptr": ([P, C.c_int, PP], C.c_int),
         "rvf_plan_tune": ([P, C.c_int32, C.c_int32], C.c_int),
         "rvf_plan_kernel": ([P, C.c_char_p, C.c_int32], C.c_int),
+        "rvf_plan_autotune": ([P, I64, P, I32, C.POINTER(C.c_int32), C.POINTER(C.c_int32), C.POINTER(C.c_int32)],
+                              C.c_int),
         "rvf_server_create": ([P, P, PP], C.c_int),
         "rvf_server_destroy": ([P], None),
         "rvf_server_submit": ([P, P, I64, I64, P, P, P, P, P, P, P, I64, P, I32, I32, P, P], C.c_int),

@@ -10,6 +10,8 @@ Duffy, Monte Carlo seed, device; src/Assembly.jl:77-78,243-252).
 from __future__ import annotations
 
 import ctypes as C
+import json
+import os
 from dataclasses import dataclass, field
 from typing import Dict, List, Optional, Sequence
 
@@ -60,6 +62,10 @@ class MI355XBackend:
     device: int = 0
     parity: str = "reference"
     audit: bool = False
+    # autotuner table (JSON path): occluded solves take the traversal
+    # configuration stored for their problem class, measuring and storing it
+    # on a miss; None keeps the default configuration
+    tune_table: Optional[str] = None
 
 
 @dataclass
@@ -203,6 +209,49 @@ class DeviceObstruction:
             pass
 
 
+class TuneTable:
+    """Persisted autotuner choices: a JSON file mapping a problem class
+    (precision, method, nquad, element count and occluder triangle count
+    rounded to powers of two) to the fastest (variant, refill_min) measured
+    for it on this machine."""
+
+    def __init__(self, path: str):
+        self.path = path
+        self.entries = {}
+        if os.path.exists(path):
+            with open(path) as f:
+                self.entries = {k: tuple(v) for k, v in json.load(f).items()}
+
+    @staticmethod
+    def key(plan: "Plan") -> Optional[str]:
+        if plan.dobs is None or plan.dobs.n_tris == 0:
+            return None
+
+        def b2(x):
+            return 1 << max(0, int(x) - 1).bit_length()
+        o = plan.opts
+        prec = [k for k, v in _lib.PRECISIONS.items() if v == o.precision][0]
+        meth = [k for k, v in _lib.METHODS.items() if v == o.method][0]
+        return f"v{_lib.lib().rvf_version()}|{prec}|{meth}|nq{o.nquad}|N{b2(plan.dmesh.mesh.n_elems)}|T{b2(plan.dobs.n_tris)}"
+
+    def __contains__(self, key):
+        return key in self.entries
+
+    def __getitem__(self, key):
+        return self.entries[key]
+
+    def __setitem__(self, key, value):
+        self.entries[key] = (int(value[0]), int(value[1]))
+
+    def save(self):
+        d = os.path.dirname(os.path.abspath(self.path))
+        os.makedirs(d, exist_ok=True)
+        tmp = self.path + ".tmp"
+        with open(tmp, "w") as f:
+            json.dump({k: list(v) for k, v in sorted(self.entries.items())}, f, indent=1)
+        os.replace(tmp, self.path)
+
+
 REQ_PENDING, REQ_RUNNING, REQ_DONE, REQ_FAILED, REQ_CANCELLED = range(5)
 
 
@@ -320,6 +369,33 @@ class Plan:
         check(_lib.lib().rvf_plan_tune(self.h, variant, refill_min))
         self.opts.variant = variant
 
+    def autotune(self, sample_tiles: int = 0, table: Optional["TuneTable"] = None):
+        """Pick the fastest traversal configuration for this plan.
+
+        With a table holding an entry for this plan's problem class, that
+        entry is applied without measuring; otherwise the native autotuner
+        times every variant over a tile sample (rvf_plan_autotune) and the
+        winner is stored in the table.  Returns (variant, refill_min,
+        measurements), measurements being [] for a table hit.  Results of
+        later runs do not depend on the choice."""
+        key = table.key(self) if table is not None else None
+        if key is not None and key in table:
+            v, r = table[key]
+            self.tune(v, r)
+            return v, r, []
+        ents = (_lib.TuneEntry * 32)()
+        n, v, r = C.c_int32(), C.c_int32(), C.c_int32()
+        check(_lib.lib().rvf_plan_autotune(self.h, sample_tiles, ents, 32, C.byref(n), C.byref(v), C.byref(r)))
+        meas = [{"variant": e.variant, "refill_min": e.refill_min, "tiles": e.tiles, "ms": e.ms}
+                for e in ents[:n.value]]
+        if not meas:
+            return v.value, r.value, []
+        self.opts.variant = v.value
+        if table is not None and key is not None:
+            table[key] = (v.value, r.value)
+            table.save()
+        return v.value, r.value, meas
+
     def kernel_name(self) -> str:
         """The pair kernel this plan launches, as profilers name it."""
         buf = C.create_string_buffer(128)
@@ -406,6 +482,8 @@ def compute_view_factors(mesh: MeshData, nquad: int = 4, obstruction_groups: Seq
             dobs = None
     plan = Plan(ctx, dmesh, dobs, nquad, method, backend.precision, n_samples, backend.seed, want_F=True,
                 audit=backend.audit)
+    if backend.tune_table and dobs is not None and method != "mc":
+        plan.autotune(table=TuneTable(backend.tune_table))
     plan.run()
     plan.sync()
     F, A, rs, Fg, Ag, tags, ac, ah = plan.fetch()

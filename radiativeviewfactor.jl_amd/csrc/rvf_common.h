@@ -95,7 +95,9 @@ struct Chunk {
 struct PairArgs {
     const int2* tiles;        // (strip, chunk)
     int ntiles;               // tiles of this launch (grid of a direct launch)
-    int tile0;                // first tile of a direct launch (tile = tile0 + block)
+    int tile0;                // first tile of a direct launch (tile = tile0 + block * tile_step)
+    int tile_step;            // tile stride of a direct launch (1 but for sampled timing)
+    int pad_ts;
     int NS;                   // slots = N rounded up to RVF_STRIP
     int maxpts;               // per-element point stride in Q arrays
     int64_t N;                // elements

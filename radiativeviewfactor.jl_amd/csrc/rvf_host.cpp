@@ -830,7 +830,7 @@ static PairArgs pair_args(const rvf_plan* p) {
     const rvf_mesh* m = p->mesh;
     PairArgs a{};
     const Layout* L = p->L;
-    a.tiles = L->d_tiles.as<int2>(); a.ntiles = (int)L->tiles.size();
+    a.tiles = L->d_tiles.as<int2>(); a.ntiles = (int)L->tiles.size(); a.tile_step = 1;
     a.slot_orig = L->d_slot_orig.as<int32_t>();
     a.NS = m->NS; a.maxpts = p->maxpts; a.N = m->N;
     a.chunks = L->d_chunks.as<Chunk>();
@@ -1036,6 +1036,98 @@ int rvf_plan_tune(rvf_plan* p, int32_t variant, int32_t refill_min) {
     HIPCHK(hipStreamSynchronize(p->ctx->stream));
     p->opts.variant = variant;
     p->refill_min = refill_min ? refill_min : 16;
+    if (p->gexec) { (void)hipGraphExecDestroy(p->gexec); p->gexec = nullptr; }  // re-captured on the next run
+    return 0;
+}
+
+int rvf_plan_autotune(rvf_plan* p, int64_t sample_tiles, rvf_tune_entry* ent, int32_t cap, int32_t* n,
+                      int32_t* variant, int32_t* refill_min) {
+    if (!p || sample_tiles < 0 || cap < 0 || (cap > 0 && !ent) || !n)
+        return fail(RVF_E_ARG, "rvf_plan_autotune: bad arguments");
+    *n = 0;
+    if (variant) *variant = p->opts.variant;
+    if (refill_min) *refill_min = p->refill_min;
+    if (!plan_occluded_quad(p)) return 0;
+    rvf_ctx* c = p->ctx;
+    hipStream_t st = c->stream;
+    HIPCHK(hipSetDevice(c->device));
+    int rc = enqueue_prologue(p, st, false);  // the point tables the pair kernel reads
+    if (rc) return rc;
+    const int64_t nt = (int64_t)p->L->tiles.size();
+    if (nt == 0) return 0;
+    const int64_t base = std::min<int64_t>(nt, sample_tiles ? sample_tiles : 8192);
+    const bool mixed = p->opts.precision == RVF_MIXED;
+    const int v0 = p->opts.variant, r0 = p->refill_min;
+    // Times the pair kernel over `want` tiles taken at a fixed stride through
+    // the tile list (all tiles when want >= nt).  Tile costs vary with where
+    // the pair sits relative to the occluders, so a strided sample follows the
+    // whole problem's mix where contiguous ranges do not.
+    auto measure = [&](int v, int refill, int64_t want, float* ms) -> int {
+        p->opts.variant = v;
+        p->refill_min = refill;
+        PairArgs a = pair_args(p);
+        const int kind = pair_kind(p);
+        const size_t lds = pair_lds(p);
+        const int64_t step = want >= nt ? 1 : nt / want;
+        const int64_t cnt = want >= nt ? nt : want;  // (cnt - 1) * step < nt
+        a.tile0 = 0;
+        a.tile_step = (int)step;
+        a.ntiles = (int)std::min<int64_t>(cnt, 256);  // first launch of a kernel loads its code
+        int e = rvf_launch_pair(&a, kind, lds, st);
+        HIPCHK(hipEventRecord(p->ev[6], st));
+        a.ntiles = (int)cnt;
+        if (!e) e = rvf_launch_pair(&a, kind, lds, st);
+        if (e) return fail(RVF_E_HIP, std::string("autotune launch: ") + hipGetErrorString((hipError_t)e));
+        HIPCHK(hipEventRecord(p->ev[7], st));
+        HIPCHK(hipEventSynchronize(p->ev[7]));
+        HIPCHK(hipEventElapsedTime(ms, p->ev[6], p->ev[7]));
+        return 0;
+    };
+    std::vector<int> cands = {RVF_VARIANT_LANE};
+    for (int S = 1; S <= 6; S++)
+        if (occ_kind(RVF_VARIANT_QUEUE(S), mixed) >= 0) cands.push_back(RVF_VARIANT_QUEUE(S));
+    for (int S = 1; S <= 3; S++) cands.push_back(RVF_VARIANT_SPLIT(S));
+    auto add = [&](int v, int refill, int64_t want, float ms) {
+        if (*n < cap) ent[*n] = rvf_tune_entry{v, v == RVF_VARIANT_LANE ? 0 : refill, std::min(want, nt), (double)ms};
+        ++*n;
+    };
+    // 1. rank every variant on the sample
+    std::vector<std::pair<float, int>> rank;
+    for (int v : cands) {
+        float ms = 0;
+        if ((rc = measure(v, 16, base, &ms))) break;
+        add(v, 16, base, ms);
+        rank.push_back({ms, v});
+    }
+    // 2. re-time the three fastest on the whole problem (or a four times
+    //    larger sample), then the refill thresholds of the winner
+    int best_v = v0, best_r = r0;
+    float best = 3.4e38f;
+    const int64_t big = std::min<int64_t>(nt, 4 * base);
+    if (!rc) {
+        std::sort(rank.begin(), rank.end());
+        for (size_t k = 0; k < rank.size() && k < 3; k++) {
+            float ms = 0;
+            if ((rc = measure(rank[k].second, 16, big, &ms))) break;
+            add(rank[k].second, 16, big, ms);
+            if (ms < best) { best = ms; best_v = rank[k].second; best_r = 16; }
+        }
+    }
+    if (!rc && best_v != RVF_VARIANT_LANE) {
+        const int bv = best_v;
+        for (int refill : {8, 32}) {
+            float ms = 0;
+            if ((rc = measure(bv, refill, big, &ms))) break;
+            add(bv, refill, big, ms);
+            if (ms < best) { best = ms; best_r = refill; }
+        }
+    }
+    if (rc) { p->opts.variant = v0; p->refill_min = r0; return rc; }
+    if (*n > cap) *n = cap;
+    p->opts.variant = best_v;
+    p->refill_min = best_r;
+    if (variant) *variant = best_v;
+    if (refill_min) *refill_min = best_v == RVF_VARIANT_LANE ? 0 : best_r;
     if (p->gexec) { (void)hipGraphExecDestroy(p->gexec); p->gexec = nullptr; }  // re-captured on the next run
     return 0;
 }

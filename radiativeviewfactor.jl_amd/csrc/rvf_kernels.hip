@@ -1985,7 +1985,7 @@ RVF_PAIR_KINDS(RVF_KIND_TAG)
 template <class TAG>
 __global__ __launch_bounds__(64) void k_pair(PairArgs a) {
     extern __shared__ double lds[];
-    run_tile<TAG::kind>(a, a.tile0 + (int)blockIdx.x, lds);
+    run_tile<TAG::kind>(a, a.tile0 + (int)blockIdx.x * a.tile_step, lds);
 }
 
 template <class TAG>

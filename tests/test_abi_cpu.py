@@ -71,3 +71,23 @@ def test_family_matrices_layout():
         f = int(m.family[e])
         assert list(mats[f][idx[e] - 1]) == list(m.nodes[e, :(8, 6, 4, 3)[f]])
     assert counts.sum() == m.n_elems
+
+
+def test_tune_table_round_trip(tmp_path):
+    # the autotuner's persisted table: keys bucket sizes to powers of two,
+    # entries survive a reload, plans without occluders have no key
+    from types import SimpleNamespace as NS
+    from rvf_amd import _lib as L
+    o = L.Opts(); o.nquad = 3; o.precision = L.PRECISIONS["mixed"]; o.method = L.METHODS["quad"]
+    plan = NS(opts=o, dmesh=NS(mesh=NS(n_elems=13824)), dobs=NS(n_tris=20480))
+    k = api.TuneTable.key(plan)
+    assert k.endswith("|mixed|quad|nq3|N16384|T32768")
+    near = NS(opts=o, dmesh=NS(mesh=NS(n_elems=9000)), dobs=NS(n_tris=17000))
+    assert api.TuneTable.key(near) == k
+    assert api.TuneTable.key(NS(opts=o, dmesh=plan.dmesh, dobs=None)) is None
+    t = api.TuneTable(str(tmp_path / "sub" / "t.json"))
+    assert k not in t
+    t[k] = (api.VARIANT_SPLIT(3), 16)
+    t.save()
+    t2 = api.TuneTable(str(tmp_path / "sub" / "t.json"))
+    assert k in t2 and t2[k] == (api.VARIANT_SPLIT(3), 16)

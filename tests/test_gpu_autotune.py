@@ -1,0 +1,74 @@
+"""Autotuner (rvf_plan_autotune, api.TuneTable): it measures every traversal
+variant valid for the plan's precision, keeps the fastest, and the solve that
+follows gives the same result as the default configuration (bit for bit in
+FP64 and in mixed precision, whose blocked rays come from exact FP64 tests);
+a persisted table entry is applied without measuring."""
+import numpy as np
+import pytest
+
+from rvf_amd import api
+from rvf_amd import mesh as M
+
+pytestmark = pytest.mark.gpu
+
+
+def _scene():
+    m = M.unit_cube(10)
+    V, T = M.icosphere(3, 0.3, (0.5, 0.5, 0.5), seed=0x5EED0003)
+    m.group_tri_soup = {7: V[T]}
+    return m
+
+
+def _solve(ctx, dm, do, prec, tune=None, **kw):
+    plan = api.Plan(ctx, dm, do, 3, "quad", prec, want_F=True, audit=True)
+    out = None
+    if tune is not None:
+        out = plan.autotune(**tune)
+    plan.run(); plan.sync()
+    res = plan.fetch()
+    name = plan.kernel_name()
+    plan.close()
+    return res, out, name
+
+
+@pytest.mark.parametrize("prec", ["fp64", "mixed"])
+def test_autotune_measures_all_variants_and_keeps_results(prec, tmp_path):
+    ctx = api.context(0)
+    m = _scene()
+    dm = api.DeviceMesh(ctx, m)
+    do = api.DeviceObstruction(ctx, dm, [7], "geometric")
+    ref, _, _ = _solve(ctx, dm, do, prec)
+    table = api.TuneTable(str(tmp_path / "tune.json"))
+    got, (v, r, meas), name = _solve(ctx, dm, do, prec, tune=dict(sample_tiles=512, table=table))
+    variants = api.FP64_VARIANTS if prec == "fp64" else api.MIXED_VARIANTS
+    measured = {e["variant"] for e in meas}
+    assert set(variants) <= measured, (sorted(measured), variants)
+    assert all(e["ms"] > 0 for e in meas)
+    # the first len(variants) entries rank the sample; the choice is the
+    # fastest of the re-timed ones that follow
+    ranked = sorted(meas[:len(variants)], key=lambda e: e["ms"])
+    again = meas[len(variants):]
+    assert [e["variant"] for e in again[:3]] == [e["variant"] for e in ranked[:3]]
+    best = min(again, key=lambda e: e["ms"])
+    assert (v, r) == (best["variant"], best["refill_min"]) and v in variants
+    for a, b in zip(got, ref):
+        if isinstance(a, np.ndarray):
+            assert np.array_equal(a, b)
+        else:
+            assert a == b
+    # the stored choice is applied by a new plan without measuring
+    table2 = api.TuneTable(str(tmp_path / "tune.json"))
+    got2, (v2, r2, meas2), name2 = _solve(ctx, dm, do, prec, tune=dict(table=table2))
+    assert meas2 == [] and (v2, r2) == (v, r) and name2 == name
+    assert np.array_equal(got2[0], ref[0])
+    do.close(); dm.close()
+
+
+def test_autotune_unoccluded_plan_is_a_no_op():
+    ctx = api.context(0)
+    dm = api.DeviceMesh(ctx, M.unit_cube(6))
+    plan = api.Plan(ctx, dm, None, 3, "quad", "fp64")
+    name = plan.kernel_name()
+    v, r, meas = plan.autotune()
+    assert meas == [] and plan.kernel_name() == name
+    plan.close(); dm.close()
