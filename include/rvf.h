@@ -85,6 +85,9 @@ typedef struct rvf_stats {
      * summed, slot steps that advanced a ray, triangle-task and exact-test
      * batches (split variants) */
     int64_t trav_iters, slot_steps, task_batches, exact_batches;
+    /* mixed mode, split traversal: triangles the FP32 classifier proved hit
+     * (no FP64 re-test needed) */
+    int64_t sure_hits;
 } rvf_stats;
 
 /* --- lifecycle (no reference counterpart: the reference is one Julia process) */

@@ -39,10 +39,19 @@ RVF_HD inline void rvf_fray_margins(RvfFRay& R, float dd, float Sm) {
     R.C = 2.0f * Sm * (12.0f * dd + 64.0f * u);
 }
 
+// Classifies a ray against a triangle from FP32 data:
+//   0  the FP64 test certainly misses,
+//   1  undecided (the exact FP64 test decides),
+//   2  the FP64 test certainly hits.
 // v0, e1, e2: the triangle as FP32; E1, E2: |e1|_1, |e2|_1 of the FP64 edges
-// rounded up; K = 80u Sm.
-RVF_HD inline int rvf_tri_filter(const float* v0, const float* e1, const float* e2, float E1, float E2,
-                                 const RvfFRay& R, float K) {
+// rounded up; K = 80u Sm.  A miss needs one comparison to fail by more than
+// its margin.  A hit needs every comparison to hold by twice its margin: the
+// exact value then holds by at least 1.5 margins, which covers the FP64
+// test's own rounding (each margin is at least 16u relative to the scale of
+// what it bounds, that rounding below 2^-50) and the FP32 rounding of the
+// comparison itself (a few u relative).
+RVF_HD inline int rvf_tri_classify(const float* v0, const float* e1, const float* e2, float E1, float E2,
+                                   const RvfFRay& R, float K) {
     const float hx = R.dy * e2[2] - R.dz * e2[1];
     const float hy = R.dz * e2[0] - R.dx * e2[2];
     const float hz = R.dx * e2[1] - R.dy * e2[0];
@@ -63,5 +72,16 @@ RVF_HD inline int rvf_tri_filter(const float* v0, const float* e1, const float* 
     const bool undecided = aa <= da + 1e-10f;
     const bool miss = (ua < -dU) | (ua > top + dU) | (va < -dV) | (ua + va > top + dU + dV) | (ta < -dT) |
                       (ta > R.L * top * (1.0f + R.A) + dT);
-    return (undecided || !miss) ? 1 : 0;
+    if (!undecided && miss) return 0;
+    const float low = aa - 2.0f * da;  // exact |a| exceeds this
+    const bool sure = (low > 2e-10f) & (ua >= 2.0f * dU) & (va >= 2.0f * dV) &
+                      (ua + va + 2.0f * (dU + dV) <= low) & (ta >= 2.0f * dT) &
+                      (ta + 2.0f * dT <= R.L * low * (1.0f - 2.0f * R.A));
+    return sure ? 2 : 1;
+}
+
+// 0 when the FP64 test certainly misses, 1 when it may hit.
+RVF_HD inline int rvf_tri_filter(const float* v0, const float* e1, const float* e2, float E1, float E2,
+                                 const RvfFRay& R, float K) {
+    return rvf_tri_classify(v0, e1, e2, E1, E2, R, K) != 0;
 }

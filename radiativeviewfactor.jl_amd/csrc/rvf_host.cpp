@@ -744,7 +744,7 @@ int rvf_plan_create(rvf_ctx* ctx, const rvf_mesh* mesh, const rvf_obs* obs, cons
         (rc = p->part.alloc(sizeof(double) * std::max<size_t>(nch, 1) * NS)) ||
         (rc = p->partT.alloc(sizeof(double) * std::max<size_t>(nstrip, 1) * NS)) ||
         (rc = p->rowsum.alloc(sizeof(double) * NS)) || (rc = p->W.alloc(sizeof(double) * std::max(G, 1) * NS)) ||
-        (rc = p->counters.alloc(sizeof(unsigned long long) * 8)) ||
+        (rc = p->counters.alloc(sizeof(unsigned long long) * 16)) ||
         (rc = p->Tseg.alloc(sizeof(double) * RVF_RSEG * NS)) ||
         (rc = p->Wseg.alloc(sizeof(double) * RVF_RSEG * std::max(G, 1) * NS))) {
         delete p;
@@ -1171,7 +1171,7 @@ static int plan_collect(rvf_plan* p) {
     int bad = 0;
     HIPCHK(hipMemcpy(&bad, p->bad.p, sizeof(int), hipMemcpyDeviceToHost));
     if (bad) return fail(RVF_E_ARG, "degenerate element: zero or non-finite area element at a quadrature point");
-    unsigned long long cnt[8];
+    unsigned long long cnt[16];
     HIPCHK(hipMemcpy(cnt, p->counters.p, sizeof cnt, hipMemcpyDeviceToHost));
     p->st.retests = (int64_t)cnt[0];
     p->st.rays = (int64_t)cnt[1];
@@ -1181,6 +1181,7 @@ static int plan_collect(rvf_plan* p) {
     p->st.slot_steps = (int64_t)cnt[5];
     p->st.task_batches = (int64_t)cnt[6];
     p->st.exact_batches = (int64_t)cnt[7];
+    p->st.sure_hits = (int64_t)cnt[8];
     return 0;
 }
 

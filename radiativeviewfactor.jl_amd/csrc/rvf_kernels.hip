@@ -916,7 +916,7 @@ struct QCtx {
 
 // Per-lane traversal counters of the wave-queue kernels.
 struct QCount {
-    uint32_t c0 = 0, c1 = 0;     // FP64: box tests, triangle tests; mixed: exact re-tests
+    uint32_t c0 = 0, c1 = 0;     // FP64: box tests, triangle tests; mixed: exact re-tests, certain hits
     uint32_t iters = 0;          // traversal loop iterations (counted by lane 0)
     uint32_t steps = 0;          // slot steps that advanced a ray
     uint32_t tbatch = 0;         // triangle-task batches (lane 0)
@@ -1236,9 +1236,9 @@ __device__ __forceinline__ bool ttest(const PairArgs& a, const QCtx& X, int c, i
 }
 
 // Mixed mode: the FP32 ray from the staged anchored offsets (as the kernel
-// evaluation forms r) and the conservative filter; true = candidate for the
-// exact FP64 test.
-__device__ __forceinline__ bool ttest_mixed(const PairArgs& a, const QCtx& X, int c, int b, int tri) {
+// evaluation forms r) and the conservative classifier: 0 certain miss,
+// 1 candidate for the exact FP64 test, 2 certain hit.
+__device__ __forceinline__ int ttest_mixed(const PairArgs& a, const QCtx& X, int c, int b, int tri) {
     int p, q;
     combo_pq(X, c, p, q);
     const int64_t ib = X.sbase + b;
@@ -1269,7 +1269,7 @@ __device__ __forceinline__ bool ttest_mixed(const PairArgs& a, const QCtx& X, in
     const float dd = 3.0f * uu * (fabsf(dax) + fabsf(day) + fabsf(daz) + fabsf(bx) + fabsf(by) + fabsf(bz) +
                                   fabsf(rx) + fabsf(ry) + fabsf(rz)) * rs + 4.0f * uu;
     rvf_fray_margins(R, dd, a.Sm);
-    return rvf_tri_filter(T.v0, T.e1, T.e2, T.E1, T.E2, R, 80.0f * uu * a.Sm);
+    return rvf_tri_classify(T.v0, T.e1, T.e2, T.E1, T.E2, R, 80.0f * uu * a.Sm);
 }
 
 template <bool MIX, int S>
@@ -1364,7 +1364,15 @@ __device__ __forceinline__ void split_drain(const PairArgs& a, const QCtx& X, in
                 const int tg = (int)(e >> RVF_TQ_TRI_BITS), tri = (int)(e & ((1u << RVF_TQ_TRI_BITS) - 1));
                 const int c = tg >> 6, b = tg & 63;
                 if (MIX) {
-                    keep = ttest_mixed(a, X, c, b, tri);
+                    // a certain hit blocks the ray at once; with the reference
+                    // soup reading the exact test also checks the leaf's box
+                    // chain, so every kept triangle goes to it
+                    const int cls = ttest_mixed(a, X, c, b, tri);
+                    keep = cls == 1 || (cls == 2 && a.verify);
+                    if (cls == 2 && !a.verify) {
+                        K.c1++;
+                        atomicOr(&blk[c * 2 + (b >> 5)], 1u << (b & 31));
+                    }
                 } else {
                     K.c1++;
                     if (ttest(a, X, c, b, tri, K.c1)) atomicOr(&blk[c * 2 + (b >> 5)], 1u << (b & 31));
@@ -1583,7 +1591,7 @@ __device__ __forceinline__ void tile_occ_queue(const PairArgs& a, int tile, doub
             for (int e = 0; e < 7; e++) v[e] += __shfl_xor(v[e], o, 64);
         if (lane == 0) {
             atomicAdd(&a.counters[1], v[0]);
-            if (MIX) atomicAdd(&a.counters[0], v[1]);
+            if (MIX) { atomicAdd(&a.counters[0], v[1]); atomicAdd(&a.counters[8], v[2]); }
             else { atomicAdd(&a.counters[2], v[1]); atomicAdd(&a.counters[3], v[2]); }
             for (int e = 3; e < 7; e++) atomicAdd(&a.counters[e + 1], v[e]);
         }

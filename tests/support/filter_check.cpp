@@ -1,7 +1,8 @@
-// CPU check of the conservative FP32 ray-triangle filter (rvf_filter.h):
-// random and adversarial rays whose FP64 Moller-Trumbore test (the
-// reference's _ray_triangle semantics, BVH.jl:181-201) hits must never be
-// rejected by the filter.  Built and driven by tests/test_filter_cpu.py.
+// CPU check of the conservative FP32 ray-triangle classifier (rvf_filter.h):
+// over random and adversarial rays, one whose FP64 Moller-Trumbore test (the
+// reference's _ray_triangle semantics, BVH.jl:181-201) hits is never
+// classified a certain miss, and one it misses is never classified a certain
+// hit.  Built and driven by tests/test_filter_cpu.py.
 #define RVF_HD
 #include <cmath>
 #include <cstdint>
@@ -31,12 +32,15 @@ static float up(double x) {
     return f;
 }
 
-// mode 0: the filter as built; mode 1: margins forced to zero (must fail)
-extern "C" int64_t filter_check(uint64_t seed, int64_t n, int mode, int64_t* hits_out) {
+// mode 0: the classifier as built; mode 1: margins forced to zero (must fail).
+// Returns the FP64 hits classified as certain misses; *sure_out counts the
+// certain hits, *false_sure_out those among them the FP64 test misses.
+extern "C" int64_t filter_check(uint64_t seed, int64_t n, int mode, int64_t* hits_out, int64_t* sure_out,
+                                int64_t* false_sure_out) {
     std::mt19937_64 g(seed);
     std::uniform_real_distribution<double> U01(0.0, 1.0);
     const float uu = 5.9604645e-08f;
-    int64_t bad = 0, hits = 0;
+    int64_t bad = 0, hits = 0, sure = 0, false_sure = 0;
     for (int64_t k = 0; k < n; k++) {
         const double Sm = 1.0 + 3.0 * U01(g);
         double v[3][3];
@@ -50,7 +54,7 @@ extern "C" int64_t filter_check(uint64_t seed, int64_t n, int mode, int64_t* hit
         for (int c = 0; c < 3; c++) { e1[c] = v[1][c] - v[0][c]; e2[c] = v[2][c] - v[0][c]; }
         // target point on or near the triangle: barycentrics at the edges/vertices often
         double bu = U01(g), bv = U01(g);
-        const int kind = (int)(g() % 6);
+        const int kind = (int)(g() % 8);  // 6, 7: generic rays (no boundary nearby)
         const double tiny = std::ldexp(1.0, -(int)(20 + g() % 30));
         if (kind == 0) bu = (g() & 1) ? tiny : -tiny;
         if (kind == 1) bv = (g() & 1) ? tiny : -tiny;
@@ -94,9 +98,13 @@ extern "C" int64_t filter_check(uint64_t seed, int64_t n, int mode, int64_t* hit
             fv0[c] = (float)v[0][c]; fe1[c] = (float)e1[c]; fe2[c] = (float)e2[c];
             E1 += std::fabs(e1[c]); E2 += std::fabs(e2[c]);
         }
-        const int keep = rvf_tri_filter(fv0, fe1, fe2, up(E1 * (1 + 1e-6)), up(E2 * (1 + 1e-6)), R, K);
-        if (hit && !keep) bad++;
+        const int cls = rvf_tri_classify(fv0, fe1, fe2, up(E1 * (1 + 1e-6)), up(E2 * (1 + 1e-6)), R, K);
+        if (hit && cls == 0) bad++;
+        sure += cls == 2;
+        false_sure += cls == 2 && !hit;
     }
     if (hits_out) *hits_out = hits;
+    if (sure_out) *sure_out = sure;
+    if (false_sure_out) *false_sure_out = false_sure;
     return bad;
 }
