@@ -23,7 +23,7 @@ struct __align__(16) BvhNode32 {
     float lo[3];
     int32_t miss;
     float hi[3];
-    int32_t start_count;  // leaves: (start << 4) | count (count <= 8); interior: right child << 4
+    int32_t start_count;  // leaves: (start << 4) | count (count <= 15); interior: right child << 4
 };
 
 // Triangle for Moller-Trumbore, FP64: v0, e1 = v1 - v0, e2 = v2 - v0 as the
@@ -127,6 +127,13 @@ struct PairArgs {
     const int32_t* parent;    // per node, absolute; -1 at a root
     const Item32* items32;    // wave-queue traversal records (nodes then triangles)
     const Item64* items64;
+    // mixed split traversal: a tree of its own over the same triangles (binned
+    // SAH, small leaves), FP32 padded boxes; triangles in its leaf order and
+    // their indices in the reference trees' order (exact test, leaf check)
+    const int32_t* ttree_root; // [G*G] root node or -1
+    const BvhNode32* tnodes32;
+    const Tri32* ttris32;
+    const int32_t* tmap;
     int n_nodes;              // M: items below M are nodes
     int refill_min;           // wave-queue kernels: refill a slot once this many lanes have it empty
     int verify;               // mixed: confirm FP64 hits by the leaf's exact slab chain

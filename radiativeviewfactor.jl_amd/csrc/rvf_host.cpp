@@ -444,15 +444,124 @@ static void assign_miss(Tree& t, int node, int next) {  // GPUBVH.jl:69-82
     }
 }
 
+// Traversal tree of the mixed split kernels (no reference counterpart: the
+// mixed mode's decisions come from exact FP64 triangle tests, so its FP32
+// traversal may use any tree whose padded boxes bound the triangles).  Binned
+// surface-area heuristic over the triangles as Moller-Trumbore reads them,
+// leaves of at most 4: a triangle task costs several node steps, so small
+// leaves pay.  Preorder layout with miss links, like the reference trees.
+struct SahBuilder {
+    static constexpr int kMaxLeaf = 4, kBins = 16;
+    static constexpr double kStep = 1.0, kTri = 3.0;  // relative costs
+    std::vector<double> lo, hi, cen;  // [3*t]
+    std::vector<int> idx;             // local triangle order
+    std::vector<HostNode> nodes;      // 1-based links; leaves: start (1-based), count
+    static double area(const double* l, const double* h) {
+        const double x = h[0] - l[0], y = h[1] - l[1], z = h[2] - l[2];
+        return (x < 0 || y < 0 || z < 0) ? 0.0 : 2.0 * (x * y + y * z + z * x);
+    }
+    int build(int s, int e, int depth) {  // idx[s, e)
+        double blo[3], bhi[3], clo[3], chi[3];
+        for (int c = 0; c < 3; c++) { blo[c] = clo[c] = INFINITY; bhi[c] = chi[c] = -INFINITY; }
+        for (int k = s; k < e; k++) {
+            const int t = idx[k];
+            for (int c = 0; c < 3; c++) {
+                blo[c] = std::min(blo[c], lo[3 * t + c]); bhi[c] = std::max(bhi[c], hi[3 * t + c]);
+                clo[c] = std::min(clo[c], cen[3 * t + c]); chi[c] = std::max(chi[c], cen[3 * t + c]);
+            }
+        }
+        HostNode nd;
+        std::memcpy(nd.lo, blo, sizeof blo); std::memcpy(nd.hi, bhi, sizeof bhi);
+        nd.left = nd.right = 0; nd.start = s + 1; nd.count = e - s;
+        nodes.push_back(nd);
+        const int me = (int)nodes.size();
+        const int n = e - s;
+        if (n <= 1) return me;
+        int bax = -1, bbin = -1;
+        double best = INFINITY;
+        if (depth < 40) {  // deeper: object median, which bounds the depth
+            const double An = std::max(area(blo, bhi), 1e-300);
+            for (int ax = 0; ax < 3; ax++) {
+                const double ext = chi[ax] - clo[ax];
+                if (!(ext > 0)) continue;
+                int cnt[kBins] = {0};
+                double bl[kBins][3], bh[kBins][3];
+                for (int b = 0; b < kBins; b++)
+                    for (int c = 0; c < 3; c++) { bl[b][c] = INFINITY; bh[b][c] = -INFINITY; }
+                for (int k = s; k < e; k++) {
+                    const int t = idx[k];
+                    const int b = std::min(kBins - 1, (int)((cen[3 * t + ax] - clo[ax]) / ext * kBins));
+                    cnt[b]++;
+                    for (int c = 0; c < 3; c++) {
+                        bl[b][c] = std::min(bl[b][c], lo[3 * t + c]); bh[b][c] = std::max(bh[b][c], hi[3 * t + c]);
+                    }
+                }
+                double rA[kBins];
+                int rN[kBins];
+                double al[3] = {INFINITY, INFINITY, INFINITY}, ah[3] = {-INFINITY, -INFINITY, -INFINITY};
+                int acc = 0;
+                for (int b = kBins - 1; b > 0; b--) {
+                    acc += cnt[b];
+                    for (int c = 0; c < 3; c++) { al[c] = std::min(al[c], bl[b][c]); ah[c] = std::max(ah[c], bh[b][c]); }
+                    rA[b] = area(al, ah); rN[b] = acc;
+                }
+                for (int c = 0; c < 3; c++) { al[c] = INFINITY; ah[c] = -INFINITY; }
+                acc = 0;
+                for (int b = 1; b < kBins; b++) {  // split before bin b
+                    acc += cnt[b - 1];
+                    for (int c = 0; c < 3; c++) {
+                        al[c] = std::min(al[c], bl[b - 1][c]); ah[c] = std::max(ah[c], bh[b - 1][c]);
+                    }
+                    if (acc == 0 || rN[b] == 0) continue;
+                    const double cost = kStep + kTri * (area(al, ah) * acc + rA[b] * rN[b]) / An;
+                    if (cost < best) { best = cost; bax = ax; bbin = b; }
+                }
+            }
+            if (n <= kMaxLeaf && !(best < kTri * n)) return me;  // a leaf is cheaper
+        }
+        int mid;
+        if (bax >= 0) {
+            const double ext = chi[bax] - clo[bax];
+            auto it = std::partition(idx.begin() + s, idx.begin() + e, [&](int t) {
+                return std::min(kBins - 1, (int)((cen[3 * t + bax] - clo[bax]) / ext * kBins)) < bbin;
+            });
+            mid = (int)(it - idx.begin());
+        } else {
+            int ax = 0;
+            for (int c = 1; c < 3; c++) if (chi[c] - clo[c] > chi[ax] - clo[ax]) ax = c;
+            mid = s + n / 2;
+            std::nth_element(idx.begin() + s, idx.begin() + mid, idx.begin() + e,
+                             [&](int x, int y) { return cen[3 * x + ax] < cen[3 * y + ax]; });
+        }
+        if (mid <= s || mid >= e) mid = s + n / 2;
+        nodes[me - 1].start = 0; nodes[me - 1].count = 0;
+        const int l = build(s, mid, depth + 1);
+        const int r = build(mid, e, depth + 1);
+        nodes[me - 1].left = l; nodes[me - 1].right = r;
+        return me;
+    }
+};
+
+static void assign_miss_nodes(const std::vector<HostNode>& nodes, std::vector<int>& miss, int node, int next) {
+    miss[node - 1] = next;
+    const HostNode& nd = nodes[node - 1];
+    if (nd.left != 0) {
+        assign_miss_nodes(nodes, miss, nd.left, nd.right);
+        assign_miss_nodes(nodes, miss, nd.right, next);
+    }
+}
+
 struct rvf_obs {
     rvf_ctx* ctx = nullptr;
     int G = 0;
     int soup_read = 0;
     std::vector<Tree> trees;
     std::vector<int32_t> root;   // G*G absolute node index or -1
-    int64_t n_nodes = 0, n_tris = 0;
+    std::vector<int32_t> troot;  // G*G traversal-tree root or -1
+    int64_t n_nodes = 0, n_tris = 0, n_tnodes = 0;
     float Sm = 1.0f;
     DBuf d_root, d_n64, d_t64, d_n32, d_t32, d_parent, d_items32, d_items64;
+    DBuf d_troot, d_tn32, d_tt32, d_tmap;
 };
 
 static float f32_down(double x) {
@@ -583,6 +692,52 @@ int rvf_obs_build(rvf_ctx* ctx, const rvf_mesh* mesh, const double* soup, const 
             f.E2 = f32_up(E2 * (1 + 1e-6));
         }
     }
+    // traversal trees of the mixed split kernels, one per reference tree
+    std::vector<BvhNode32> tn32;
+    std::vector<Tri32> tt32(std::max<int64_t>(o->n_tris, 1));
+    std::vector<int32_t> tmap(std::max<int64_t>(o->n_tris, 1), 0);
+    std::map<int64_t, int32_t> troot_of;  // reference root -> traversal root
+    for (const Tree& tr : o->trees) {
+        const int64_t nt = (int64_t)tr.idx.size();
+        SahBuilder sb;
+        sb.lo.resize(3 * nt); sb.hi.resize(3 * nt); sb.cen.resize(3 * nt);
+        for (int64_t k = 0; k < nt; k++) {
+            const Tri64& d = t64[tr.tri_base + k];
+            for (int c = 0; c < 3; c++) {
+                const double a = d.v0[c], b = d.v0[c] + d.e1[c], e = d.v0[c] + d.e2[c];
+                sb.lo[3 * k + c] = std::min(a, std::min(b, e));
+                sb.hi[3 * k + c] = std::max(a, std::max(b, e));
+                sb.cen[3 * k + c] = 0.5 * (sb.lo[3 * k + c] + sb.hi[3 * k + c]);
+            }
+        }
+        sb.idx.resize(nt);
+        for (int64_t k = 0; k < nt; k++) sb.idx[k] = (int)k;
+        sb.build(0, (int)nt, 0);
+        std::vector<int> miss(sb.nodes.size(), 0);
+        assign_miss_nodes(sb.nodes, miss, 1, 0);
+        const int64_t base = (int64_t)tn32.size();
+        if (base + (int64_t)sb.nodes.size() >= ((int64_t)1 << 27)) { delete o; return fail(RVF_E_ARG, "rvf_obs_build: too many nodes"); }
+        for (size_t k = 0; k < sb.nodes.size(); k++) {
+            const HostNode& h = sb.nodes[k];
+            BvhNode32 f;
+            for (int c = 0; c < 3; c++) { f.lo[c] = f32_down(h.lo[c]) - pad; f.hi[c] = f32_up(h.hi[c]) + pad; }
+            f.miss = miss[k] ? (int32_t)(base + miss[k] - 1) : -1;
+            f.start_count = h.count ? (int32_t)(((tr.tri_base + h.start - 1) << 4) | h.count)
+                                    : (int32_t)((base + h.right - 1) << 4);
+            if (h.left && h.left != (int)k + 2) { delete o; return fail(RVF_E_STATE, "bvh: non-preorder layout"); }
+            tn32.push_back(f);
+        }
+        for (int64_t m = 0; m < nt; m++) {
+            tt32[tr.tri_base + m] = t32[tr.tri_base + sb.idx[m]];
+            tmap[tr.tri_base + m] = (int32_t)(tr.tri_base + sb.idx[m]);
+        }
+        troot_of[tr.node_base] = (int32_t)base;
+    }
+    o->n_tnodes = (int64_t)tn32.size();
+    if (tn32.empty()) tn32.resize(1);
+    o->troot.assign((size_t)G * G, -1);
+    for (size_t k = 0; k < o->root.size(); k++)
+        if (o->root[k] >= 0) o->troot[k] = troot_of.at(o->root[k]);
     // wave-queue items: nodes [0, M), then triangles [M, M+T)
     const int64_t M = o->n_nodes, NI = std::max<int64_t>(o->n_nodes + o->n_tris, 1);
     if (NI >= ((int64_t)1 << 30)) { delete o; return fail(RVF_E_ARG, "rvf_obs_build: too many BVH items"); }
@@ -622,7 +777,11 @@ int rvf_obs_build(rvf_ctx* ctx, const rvf_mesh* mesh, const double* soup, const 
         (rc = upload(o->d_n32, n32.data(), sizeof(BvhNode32) * n32.size())) ||
         (rc = upload(o->d_t64, t64.data(), sizeof(Tri64) * t64.size())) ||
         (rc = upload(o->d_t32, t32.data(), sizeof(Tri32) * t32.size())) ||
-        (rc = upload(o->d_parent, parent.data(), sizeof(int32_t) * parent.size()))) {
+        (rc = upload(o->d_parent, parent.data(), sizeof(int32_t) * parent.size())) ||
+        (rc = upload(o->d_troot, o->troot.data(), sizeof(int32_t) * std::max(1, G * G))) ||
+        (rc = upload(o->d_tn32, tn32.data(), sizeof(BvhNode32) * tn32.size())) ||
+        (rc = upload(o->d_tt32, tt32.data(), sizeof(Tri32) * tt32.size())) ||
+        (rc = upload(o->d_tmap, tmap.data(), sizeof(int32_t) * tmap.size()))) {
         delete o;
         return rc;
     }
@@ -845,6 +1004,8 @@ static PairArgs pair_args(const rvf_plan* p) {
         a.nodes32 = p->obs->d_n32.as<BvhNode32>(); a.tris32 = p->obs->d_t32.as<Tri32>();
         a.parent = p->obs->d_parent.as<int32_t>();
         a.items32 = p->obs->d_items32.as<Item32>(); a.items64 = p->obs->d_items64.as<Item64>();
+        a.ttree_root = p->obs->d_troot.as<int32_t>(); a.tnodes32 = p->obs->d_tn32.as<BvhNode32>();
+        a.ttris32 = p->obs->d_tt32.as<Tri32>(); a.tmap = p->obs->d_tmap.as<int32_t>();
         a.n_nodes = (int)p->obs->n_nodes;
         a.refill_min = p->refill_min;
         a.verify = p->obs->soup_read == RVF_SOUP_REFERENCE ? 1 : 0;

@@ -1210,7 +1210,7 @@ __device__ __forceinline__ int nstep(const BvhNode32& nd, int cur, const NRay<tr
 template <bool MIX>
 __device__ __forceinline__ const typename std::conditional<MIX, BvhNode32, BvhNode64>::type* nnodes(const PairArgs& a);
 template <>
-__device__ __forceinline__ const BvhNode32* nnodes<true>(const PairArgs& a) { return a.nodes32; }
+__device__ __forceinline__ const BvhNode32* nnodes<true>(const PairArgs& a) { return a.tnodes32; }  // traversal tree
 template <>
 __device__ __forceinline__ const BvhNode64* nnodes<false>(const PairArgs& a) { return a.nodes64; }
 
@@ -1249,7 +1249,7 @@ __device__ __forceinline__ int ttest_mixed(const PairArgs& a, const QCtx& X, int
     const float qx = qj[0], qy = qj[1], qz = qj[2];
     const float dax = X.stage[0 * 64 + b], day = X.stage[1 * 64 + b], daz = X.stage[2 * 64 + b];
     const float fax = X.stage[3 * 64 + b], fay = X.stage[4 * 64 + b], faz = X.stage[5 * 64 + b];
-    const Tri32 T = a.tris32[tri];
+    const Tri32 T = a.ttris32[tri];  // traversal-tree order
     const bool rev = (X.revmask >> b) & 1;
     const float bx = dax - dxi, by = day - dyi, bz = daz - dzi;
     const float rx = bx + qx, ry = by + qy, rz = bz + qz;
@@ -1394,7 +1394,8 @@ __device__ __forceinline__ void split_drain(const PairArgs& a, const QCtx& X, in
                         const int tg = (int)(ce >> RVF_TQ_TRI_BITS), tri = (int)(ce & ((1u << RVF_TQ_TRI_BITS) - 1));
                         const int c = tg >> 6, b = tg & 63;
                         K.c0++;
-                        if (qexact_hit(a, X, c, b, tri, a.items32[a.n_nodes + tri].link_y))
+                        const int rt = a.tmap[tri];  // the triangle in the reference trees' order
+                        if (qexact_hit(a, X, c, b, rt, a.items32[a.n_nodes + rt].link_y))
                             atomicOr(&blk[c * 2 + (b >> 5)], 1u << (b & 31));
                     }
                     K.ebatch += lane == 0;
@@ -1447,7 +1448,8 @@ __device__ __forceinline__ void tile_occ_queue(const PairArgs& a, int tile, doub
         const bool rev = t.oi > oj;
         const int npj = a.elem_np[j];
         const int gj = a.elem_grp[j];
-        const int root = a.tree_root[gi * a.G + gj];
+        // mixed split traversal walks its own tree over the same triangles
+        const int root = (MIX && SPLIT ? a.ttree_root : a.tree_root)[gi * a.G + gj];
         X.j = j; X.oj = oj; X.npj = npj; X.gj = gj;
         X.revmask = __ballot(rev);
         lroot[lane] = root;
