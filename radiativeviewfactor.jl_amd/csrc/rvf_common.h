@@ -81,7 +81,9 @@ enum RvfPairKind {
     RVF_K_COUNT
 };
 #define RVF_QCOMBO 128   // (p, q) combinations per wave-queue batch
-#define RVF_TQCAP 2048   // triangle tasks per wavefront ring (split variants)
+#define RVF_TQCAP 2048   // triangle tasks per wavefront ring (FP64 split variants)
+#define RVF_TQCAP_MIX 1024  // the same for the mixed split variants (small leaves)
+#define RVF_TLEAF_MAX 4  // triangles per leaf of the mixed traversal trees
 #define RVF_CQCAP 128    // mixed split variants: candidates for the exact FP64 test
 
 // Column chunk: a run of <= RVF_CHUNK consecutive elements of one group.

@@ -451,7 +451,7 @@ static void assign_miss(Tree& t, int node, int next) {  // GPUBVH.jl:69-82
 // leaves of at most 4: a triangle task costs several node steps, so small
 // leaves pay.  Preorder layout with miss links, like the reference trees.
 struct SahBuilder {
-    static constexpr int kMaxLeaf = 4, kBins = 16;
+    static constexpr int kMaxLeaf = RVF_TLEAF_MAX, kBins = 16;
     static constexpr double kStep = 1.0, kTri = 3.0;  // relative costs
     std::vector<double> lo, hi, cen;  // [3*t]
     std::vector<int> idx;             // local triangle order
@@ -1058,9 +1058,10 @@ static size_t pair_lds(const rvf_plan* p) {
     if (kind == RVF_K_FP64_OCC_LANE) return accq;
     if (kind == RVF_K_MIX_OCC_LANE) return 0;
     const size_t ring =
-        kind >= RVF_K_FP64_SPLIT1 ? RVF_TQCAP * sizeof(uint32_t) +
-                                        (mixed ? RVF_CQCAP * sizeof(uint32_t) + 6 * 64 * sizeof(float) : 0)
-                                  : 0;
+        kind >= RVF_K_FP64_SPLIT1
+            ? (mixed ? (RVF_TQCAP_MIX + RVF_CQCAP) * sizeof(uint32_t) + 6 * 64 * sizeof(float)
+                     : RVF_TQCAP * sizeof(uint32_t))
+            : 0;
     return accq + RVF_QCOMBO * (sizeof(uint64_t) + 2 * sizeof(uint32_t)) + 64 * sizeof(int32_t) + ring;
 }
 

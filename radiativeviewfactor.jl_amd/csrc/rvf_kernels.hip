@@ -1155,12 +1155,16 @@ __device__ __forceinline__ void queue_drain(const PairArgs& a, const QCtx& X, in
 // runs those tasks 64 at a time, so the triangle test runs at full width
 // instead of riding along with every node step.  A ray keeps walking past a
 // leaf whose tasks are pending; a task that finds a hit sets the ray's blocked
-// bit, and the slot stops at its next leaf once it sees the bit.  The blocked
+// bit, and the slot stops at the next iteration's check.  The blocked
 // decision is the OR over every triangle of every leaf the traversal reaches,
-// which is the stackless any-hit result (BVH.jl:213-250).
+// which is the stackless any-hit result (BVH.jl:213-250).  The FP64 variants
+// walk the reference trees; the mixed ones their own traversal trees (small
+// leaves), whose decisions are the same because every triangle an exact test
+// could hit lies inside the padded boxes of its leaf and their ancestors.
 // ---------------------------------------------------------------------------
-// RVF_TQCAP (a power of two): one iteration of S <= 3 slots pushes at most
-// S x 64 lanes x 8 triangles per leaf; RVF_CQCAP >= 64 + 63.
+// RVF_TQCAP, RVF_TQCAP_MIX (powers of two): one iteration of S <= 3 slots
+// pushes at most S x 64 lanes x the leaf size (8 in the reference trees,
+// RVF_TLEAF_MAX in the traversal trees); RVF_CQCAP >= 64 + 63.
 #define RVF_TQ_TRI_BITS 19  // a task packs (tag << 19) | triangle
 
 template <bool MIX>
@@ -1168,7 +1172,9 @@ struct NRay;
 template <>
 struct NRay<false> { double ox, oy, oz, ix, iy, iz, L; };
 template <>
-struct NRay<true> { float ox, oy, oz, ix, iy, iz, L; };
+struct NRay<true> {  // slab form t = fma(bound, i, n): n = -o * i; L with the slab test's slack
+    float nx, ny, nz, ix, iy, iz, L;
+};
 
 __device__ __forceinline__ int nray_setup(const PairArgs& a, const QCtx& X, int c, int b, NRay<false>& N) {
     QRay<false> R;
@@ -1179,7 +1185,16 @@ __device__ __forceinline__ int nray_setup(const PairArgs& a, const QCtx& X, int 
 __device__ __forceinline__ int nray_setup(const PairArgs& a, const QCtx& X, int c, int b, NRay<true>& N) {
     QRay<true> R;
     const int root = qray_setup(a, X, c, b, R);
-    N.ox = R.ox; N.oy = R.oy; N.oz = R.oz; N.ix = R.ix; N.iy = R.iy; N.iz = R.iz; N.L = R.L;
+    // A zero direction component gives an infinite inverse; clamped to 1e30 the
+    // slab of that axis is still (-huge, +huge) inside the box's extent and
+    // empty outside, and fma(bound, i, -o*i) stays finite (|bound|, |o| <= Sm).
+    // The FMA form's rounding, about u |o i|, is far inside the boxes' padding
+    // of 128 u Sm (128 u Sm |i| in t).
+    N.ix = fminf(fmaxf(R.ix, -1e30f), 1e30f);
+    N.iy = fminf(fmaxf(R.iy, -1e30f), 1e30f);
+    N.iz = fminf(fmaxf(R.iz, -1e30f), 1e30f);
+    N.nx = -R.ox * N.ix; N.ny = -R.oy * N.iy; N.nz = -R.oz * N.iz;
+    N.L = R.L * 1.0001f;
     return root;
 }
 
@@ -1199,8 +1214,12 @@ __device__ __forceinline__ int nstep(const BvhNode64& nd, int cur, const NRay<fa
     return (pass && !leaf) ? cur + 1 : nd.miss;
 }
 __device__ __forceinline__ int nstep(const BvhNode32& nd, int cur, const NRay<true>& N, int& first, int& cnt) {
-    const float v[6] = {nd.lo[0], nd.lo[1], nd.lo[2], nd.hi[0], nd.hi[1], nd.hi[2]};
-    const bool pass = slab32(v, N.ox, N.oy, N.oz, N.ix, N.iy, N.iz, N.L);
+    const float t1x = fmaf(nd.lo[0], N.ix, N.nx), t2x = fmaf(nd.hi[0], N.ix, N.nx);
+    const float t1y = fmaf(nd.lo[1], N.iy, N.ny), t2y = fmaf(nd.hi[1], N.iy, N.ny);
+    const float t1z = fmaf(nd.lo[2], N.iz, N.nz), t2z = fmaf(nd.hi[2], N.iz, N.nz);
+    const float tmin = fmaxf(fmaxf(fmaxf(fminf(t1x, t2x), fminf(t1y, t2y)), fminf(t1z, t2z)), 0.0f);
+    const float tmax = fminf(fminf(fminf(fmaxf(t1x, t2x), fmaxf(t1y, t2y)), fmaxf(t1z, t2z)), N.L);
+    const bool pass = tmin <= tmax;
     const int c = nd.start_count & 15;
     first = nd.start_count >> 4;
     cnt = (pass && c > 0) ? c : 0;
@@ -1275,18 +1294,33 @@ __device__ __forceinline__ int ttest_mixed(const PairArgs& a, const QCtx& X, int
 template <bool MIX, int S>
 __device__ __forceinline__ void split_drain(const PairArgs& a, const QCtx& X, int nc, const uint64_t* need,
                                             uint32_t* blk, int lane, uint32_t* tq, QCount& K) {
-    static_assert(S * 64 * 8 + 64 <= RVF_TQCAP, "task ring too small for S slots");
+    // ring capacity: one iteration pushes at most S x 64 leaves' triangles on
+    // top of fewer than 64 pending tasks
+    constexpr uint32_t TQC = MIX ? RVF_TQCAP_MIX : RVF_TQCAP;
+    static_assert(S * 64 * (MIX ? RVF_TLEAF_MAX : 8) + 64 <= TQC, "task ring too small for S slots");
     const auto* __restrict__ nodes = nnodes<MIX>(a);
     WaveQueue Q;
     Q.c = 0; Q.nc = nc; Q.m = uniform64(need[0]);
     uint32_t head = 0, tail = 0;   // uniform ring cursors
     uint32_t chead = 0, ctail = 0; // candidate ring (mixed)
-    uint32_t* cq = tq + RVF_TQCAP;
+    bool bdirty = false;           // a batch may have set blocked bits (uniform)
+    uint32_t* cq = tq + TQC;
     int cur[S], tag[S];
     NRay<MIX> R[S];
 #pragma unroll
     for (int s = 0; s < S; s++) { cur[s] = -1; tag[s] = 0; }
     for (;;) {
+        // A ray whose blocked bit a task has set stops here, and its slot is
+        // refilled below.  Blocked bits change only in task and exact batches,
+        // so they are read after one ran.
+        if (bdirty) {
+#pragma unroll
+            for (int s = 0; s < S; s++) {
+                const int c = tag[s] >> 6, b = tag[s] & 63;
+                if (cur[s] >= 0 && ((blk[c * 2 + (b >> 5)] >> (b & 31)) & 1u)) cur[s] = -1;
+            }
+            bdirty = false;
+        }
         if (Q.c < Q.nc) {
             bool any = false;
 #pragma unroll
@@ -1304,23 +1338,18 @@ __device__ __forceinline__ void split_drain(const PairArgs& a, const QCtx& X, in
                 }
             }
         }
-        // node steps of every slot; passed leaves queue their triangles.  A ray
-        // whose blocked bit a task has set stops here.
+        // node steps of every slot; passed leaves queue their triangles
         K.iters += lane == 0;
         int first[S], cnt[S];
         uint32_t total = 0;
         {
             typename std::remove_cv<typename std::remove_reference<decltype(nodes[0])>::type>::type nd[S];
-            uint32_t bw[S];
+#pragma unroll
+            for (int s = 0; s < S; s++)
+                nd[s] = *(decltype(&nodes[0]))((const char*)nodes + (uint32_t)max(cur[s], 0) * (uint32_t)sizeof(nd[0]));
 #pragma unroll
             for (int s = 0; s < S; s++) {
-                nd[s] = nodes[max(cur[s], 0)];
-                const int c = tag[s] >> 6, b = tag[s] & 63;
-                bw[s] = blk[c * 2 + (b >> 5)] >> (b & 31);
-            }
-#pragma unroll
-            for (int s = 0; s < S; s++) {
-                const bool act = cur[s] >= 0 && !(bw[s] & 1u);
+                const bool act = cur[s] >= 0;
                 const int nx = nstep(nd[s], max(cur[s], 0), R[s], first[s], cnt[s]);
                 if (!MIX) K.c0 += act;
                 K.steps += act;
@@ -1341,7 +1370,7 @@ __device__ __forceinline__ void split_drain(const PairArgs& a, const QCtx& X, in
 #pragma unroll
             for (int s = 0; s < S; s++) {
                 for (int k = 0; k < cnt[s]; k++)
-                    tq[(tail + off + k) & (RVF_TQCAP - 1)] =
+                    tq[(tail + off + k) & (TQC - 1)] =
                         ((uint32_t)tag[s] << RVF_TQ_TRI_BITS) | (uint32_t)(first[s] + k);
                 off += (uint32_t)cnt[s];
             }
@@ -1360,7 +1389,7 @@ __device__ __forceinline__ void split_drain(const PairArgs& a, const QCtx& X, in
             bool keep = false;
             uint32_t e = 0;
             if ((uint32_t)lane < n) {
-                e = tq[(head + lane) & (RVF_TQCAP - 1)];
+                e = tq[(head + lane) & (TQC - 1)];
                 const int tg = (int)(e >> RVF_TQ_TRI_BITS), tri = (int)(e & ((1u << RVF_TQ_TRI_BITS) - 1));
                 const int c = tg >> 6, b = tg & 63;
                 if (MIX) {
@@ -1380,6 +1409,7 @@ __device__ __forceinline__ void split_drain(const PairArgs& a, const QCtx& X, in
             }
             K.tbatch += lane == 0 && n > 0;
             head += n;
+            bdirty = true;
             if (MIX) {
                 const uint64_t km = __ballot(keep);
                 if (keep) cq[(ctail + (uint32_t)lane_rank(km)) & (RVF_CQCAP - 1)] = e;
@@ -1427,7 +1457,7 @@ __device__ __forceinline__ void tile_occ_queue(const PairArgs& a, int tile, doub
     uint32_t* blk = (uint32_t*)(need + RVF_QCOMBO);
     int* lroot = (int*)(blk + 2 * RVF_QCOMBO);
     uint32_t* tq = (uint32_t*)(lroot + 64);
-    float* stage = (float*)(tq + (SPLIT ? RVF_TQCAP + (MIX ? RVF_CQCAP : 0) : 0));
+    float* stage = (float*)(tq + (SPLIT ? (MIX ? RVF_TQCAP_MIX + RVF_CQCAP : RVF_TQCAP) : 0));
     const double* __restrict__ P64 = a.P64;
     const float* __restrict__ P = a.P32;
     double ls = 0.0, lc = 0.0;
@@ -1992,8 +2022,14 @@ __device__ __forceinline__ void run_tile(const PairArgs& a, int tile, double* ld
 RVF_PAIR_KINDS(RVF_KIND_TAG)
 #undef RVF_KIND_TAG
 
+// Resident waves per SIMD the compiler must allow (1 = no constraint).  The
+// mixed split traversal waits on node fetches more than on arithmetic; at
+// 128 VGPRs four of its waves share a SIMD instead of three (its few spills
+// are in the per-column setup, outside the traversal loop).
+constexpr int k_pair_waves(int kind) { return (kind == RVF_K_MIX_SPLIT3 || kind == RVF_K_MIX_SPLIT2) ? 4 : 1; }
+
 template <class TAG>
-__global__ __launch_bounds__(64) void k_pair(PairArgs a) {
+__global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(k_pair_waves(TAG::kind)))) void k_pair(PairArgs a) {
     extern __shared__ double lds[];
     run_tile<TAG::kind>(a, a.tile0 + (int)blockIdx.x * a.tile_step, lds);
 }
