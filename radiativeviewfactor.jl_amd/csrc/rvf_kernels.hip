@@ -1182,9 +1182,14 @@ __device__ __forceinline__ int nray_setup(const PairArgs& a, const QCtx& X, int 
     N.ox = R.ox; N.oy = R.oy; N.oz = R.oz; N.ix = R.ix; N.iy = R.iy; N.iz = R.iz; N.L = R.L;
     return root;
 }
+struct SRay { float ox, oy, oz, dx, dy, dz, L, dd; };  // FP32 ray and its direction error bound
+__device__ __forceinline__ SRay sray(const PairArgs& a, const QCtx& X, int c, int b);
 __device__ __forceinline__ int nray_setup(const PairArgs& a, const QCtx& X, int c, int b, NRay<true>& N) {
-    QRay<true> R;
-    const int root = qray_setup(a, X, c, b, R);
+    const SRay S = sray(a, X, c, b);  // the same ray as the triangle tasks'
+    struct { float ox, oy, oz, ix, iy, iz, L; } R;
+    R.ox = S.ox; R.oy = S.oy; R.oz = S.oz; R.L = S.L;
+    R.ix = __builtin_amdgcn_rcpf(S.dx); R.iy = __builtin_amdgcn_rcpf(S.dy); R.iz = __builtin_amdgcn_rcpf(S.dz);
+    const int root = X.lroot[b];
     // A zero direction component gives an infinite inverse; clamped to 1e30 the
     // slab of that axis is still (-huge, +huge) inside the box's extent and
     // empty outside, and fma(bound, i, -o*i) stays finite (|bound|, |o| <= Sm).
@@ -1254,10 +1259,10 @@ __device__ __forceinline__ bool ttest(const PairArgs& a, const QCtx& X, int c, i
     return mt_hit64(a.tris64[tri], R64);
 }
 
-// Mixed mode: the FP32 ray from the staged anchored offsets (as the kernel
-// evaluation forms r) and the conservative classifier: 0 certain miss,
-// 1 candidate for the exact FP64 test, 2 certain hit.
-__device__ __forceinline__ int ttest_mixed(const PairArgs& a, const QCtx& X, int c, int b, int tri) {
+// Mixed split traversal: the FP32 ray of combination c and lane b from the
+// staged anchored offsets, as the kernel evaluation forms r.  dd bounds the
+// direction's error (the classifier's margins).
+__device__ __forceinline__ SRay sray(const PairArgs& a, const QCtx& X, int c, int b) {
     int p, q;
     combo_pq(X, c, p, q);
     const int64_t ib = X.sbase + b;
@@ -1268,7 +1273,6 @@ __device__ __forceinline__ int ttest_mixed(const PairArgs& a, const QCtx& X, int
     const float qx = qj[0], qy = qj[1], qz = qj[2];
     const float dax = X.stage[0 * 64 + b], day = X.stage[1 * 64 + b], daz = X.stage[2 * 64 + b];
     const float fax = X.stage[3 * 64 + b], fay = X.stage[4 * 64 + b], faz = X.stage[5 * 64 + b];
-    const Tri32 T = a.ttris32[tri];  // traversal-tree order
     const bool rev = (X.revmask >> b) & 1;
     const float bx = dax - dxi, by = day - dyi, bz = daz - dzi;
     const float rx = bx + qx, ry = by + qy, rz = bz + qz;
@@ -1279,15 +1283,26 @@ __device__ __forceinline__ int ttest_mixed(const PairArgs& a, const QCtx& X, int
     }
     const float rs = __builtin_amdgcn_rsqf(r2);
     const float sg = rev ? -rs : rs;
-    RvfFRay R;
+    SRay R;
     R.ox = rev ? X.fjx + qx : fax + dxi;
     R.oy = rev ? X.fjy + qy : fay + dyi;
     R.oz = rev ? X.fjz + qz : faz + dzi;
     R.dx = rx * sg; R.dy = ry * sg; R.dz = rz * sg; R.L = r2 * rs;
     const float uu = 5.9604645e-08f;
-    const float dd = 3.0f * uu * (fabsf(dax) + fabsf(day) + fabsf(daz) + fabsf(bx) + fabsf(by) + fabsf(bz) +
-                                  fabsf(rx) + fabsf(ry) + fabsf(rz)) * rs + 4.0f * uu;
-    rvf_fray_margins(R, dd, a.Sm);
+    R.dd = 3.0f * uu * (fabsf(dax) + fabsf(day) + fabsf(daz) + fabsf(bx) + fabsf(by) + fabsf(bz) +
+                        fabsf(rx) + fabsf(ry) + fabsf(rz)) * rs + 4.0f * uu;
+    return R;
+}
+
+// Mixed mode triangle task: the conservative classifier on that ray:
+// 0 certain miss, 1 candidate for the exact FP64 test, 2 certain hit.
+__device__ __forceinline__ int ttest_mixed(const PairArgs& a, const QCtx& X, int c, int b, int tri) {
+    const SRay S = sray(a, X, c, b);
+    const Tri32 T = a.ttris32[tri];  // traversal-tree order
+    RvfFRay R;
+    R.ox = S.ox; R.oy = S.oy; R.oz = S.oz; R.dx = S.dx; R.dy = S.dy; R.dz = S.dz; R.L = S.L;
+    rvf_fray_margins(R, S.dd, a.Sm);
+    const float uu = 5.9604645e-08f;
     return rvf_tri_classify(T.v0, T.e1, T.e2, T.E1, T.E2, R, 80.0f * uu * a.Sm);
 }
 
