@@ -200,9 +200,10 @@ typedef struct rvf_server rvf_server;
 typedef struct rvf_server_opts {
     int64_t hbm_budget;   /* device bytes for admitted requests (0 = 64 GiB) */
     int32_t max_active;   /* requests admitted at once (0 = 64) */
-    int32_t round_tiles;  /* pair tiles per request per round (0 = 4096) */
-    int32_t direct_tiles; /* requests with at least this many tiles get direct launches of their
-                             tile ranges, smaller ones the batched kernel (0 = 512, -1 = never) */
+    int32_t round_tiles;  /* pair tiles per request per round of the batched kernel (0 = 4096) */
+    int32_t direct_tiles; /* requests with at least this many tiles get one launch of their whole
+                             tile range on a stream of their own, smaller ones rounds of the
+                             batched kernel (0 = 512, -1 = never) */
     int32_t reserved;
 } rvf_server_opts;
 typedef struct rvf_request_stats {
@@ -212,8 +213,8 @@ typedef struct rvf_request_stats {
 typedef struct rvf_server_counters {
     int64_t submitted, completed, failed, cancelled, rounds, tiles;
     int64_t peak_bytes, used_bytes, pending, active;
-    /* worker time: launching rounds, admitting (upload, BVH, plan), waiting
-     * for rounds, retiring (result copies, frees) */
+    /* worker time: launching (direct ranges and batched rounds), admitting
+     * (upload, BVH, plan), waiting for the device, retiring (result copies) */
     double launch_ms, admit_ms, wait_ms, retire_ms;
 } rvf_server_counters;
 /* request states returned by rvf_server_poll / rvf_server_wait */
