@@ -196,6 +196,31 @@ int rvf_ctx_sync(rvf_ctx* ctx);
  * every running request, and admits pending requests in arrival order while
  * their estimated device footprint fits hbm_budget.  A request's result is
  * identical to its standalone rvf_compute_view_factors result. */
+/* --- mesh files (host only; no GPU) -------------------------------------- */
+/* Reads a mesh file into host arrays, as load_mesh (src/MeshIO.jl:94-145) and
+ * load_vtu (ext/RadiativeViewFactorReadVTKExt.jl:71-157) do: Gmsh .msh in
+ * format 2.2 or 4.1 (ASCII or binary), or an XML VTK unstructured grid (.vtu,
+ * or .vtk in XML form; ascii, base64 or appended data, raw or zlib).
+ * Elements of the supported surface families in every physical group of
+ * dimension 2 (Quad9 read as Quad8); without physical groups a single
+ * "default" group over every surface entity; VTK groups from a per-cell
+ * integer array (group_field, or NULL to try CellEntityIds, gmsh:physical,
+ * RegionId, MaterialIds, region, group).  reverse_normals swaps nodes as
+ * _reverse_all_normals! (src/MeshIO.jl:545-562).  surface_dim must be 2. */
+typedef struct rvf_meshfile rvf_meshfile;
+int rvf_meshfile_read(const char* path, int32_t surface_dim, int32_t reverse_normals, const char* group_field,
+                      rvf_meshfile** out);
+/* flags: 1 = default group synthesised, 2 = Quad9 centre nodes dropped,
+ * 4 = the requested VTK group array was absent (single default group) */
+int rvf_meshfile_info(const rvf_meshfile* mf, int64_t* n_nodes, int64_t* n_elems, int32_t* n_groups,
+                      int32_t* flags);
+/* coords 3 x Nn column-major; family N (0 Quad8, 1 Tri6, 2 Quad4, 3 Tri3);
+ * nodes 8 x N column-major, 1-based, unused slots 0; group N (tags) */
+int rvf_meshfile_arrays(const rvf_meshfile* mf, double* coords, int8_t* family, int32_t* nodes, int32_t* group);
+/* group k (ascending tag order): tag and name; returns the name's length */
+int rvf_meshfile_group(const rvf_meshfile* mf, int32_t k, int32_t* tag, char* name, int32_t cap);
+void rvf_meshfile_free(rvf_meshfile* mf);
+
 typedef struct rvf_server rvf_server;
 typedef struct rvf_server_opts {
     int64_t hbm_budget;   /* device bytes for admitted requests (0 = 64 GiB) */

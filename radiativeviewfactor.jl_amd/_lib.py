@@ -102,6 +102,11 @@ def lib():
         "rvf_plan_kernel": ([P, C.c_char_p, C.c_int32], C.c_int),
         "rvf_plan_autotune": ([P, I64, P, I32, C.POINTER(C.c_int32), C.POINTER(C.c_int32), C.POINTER(C.c_int32)],
                               C.c_int),
+        "rvf_meshfile_read": ([C.c_char_p, I32, I32, C.c_char_p, PP], C.c_int),
+        "rvf_meshfile_info": ([P, P, P, P, P], C.c_int),
+        "rvf_meshfile_arrays": ([P, P, P, P, P], C.c_int),
+        "rvf_meshfile_group": ([P, I32, P, C.c_char_p, I32], C.c_int),
+        "rvf_meshfile_free": ([P], None),
         "rvf_server_create": ([P, P, PP], C.c_int),
         "rvf_server_destroy": ([P], None),
         "rvf_server_submit": ([P, P, I64, I64, P, P, P, P, P, P, P, I64, P, I32, I32, P, P], C.c_int),

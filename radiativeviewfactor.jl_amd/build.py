@@ -15,11 +15,13 @@ FLAGS = [
     "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops",
     "-Wno-unused-command-line-argument",
 ]
-SOURCES = [os.path.join(CSRC, "rvf_kernels.hip"), os.path.join(CSRC, "rvf_host.cpp"), os.path.join(CSRC, "rvf_server.cpp")]
+SOURCES = [os.path.join(CSRC, "rvf_kernels.hip"), os.path.join(CSRC, "rvf_host.cpp"), os.path.join(CSRC, "rvf_server.cpp"),
+           os.path.join(CSRC, "rvf_meshio.cpp")]
+LIBS = ["-lz"]  # compressed VTK data
 
 
 def build_library(verbose=False):
-    cmd = [HIPCC] + FLAGS + ["-x", "hip"] + SOURCES + ["-o", LIB + ".tmp"]
+    cmd = [HIPCC] + FLAGS + ["-x", "hip"] + SOURCES + ["-o", LIB + ".tmp"] + LIBS
     r = subprocess.run(cmd, capture_output=True, text=True)
     if r.returncode != 0:
         raise RuntimeError("hipcc failed:\n" + r.stdout + r.stderr)

@@ -113,20 +113,79 @@ class MeshData:
 def build_group_obs_soups(mesh: MeshData, groups: Optional[Sequence[int]] = None) -> Dict[int, np.ndarray]:
     """Per-group obstruction triangle soups from element corners, restating
     src/MeshIO.jl:648-685 (Tri -> (c1,c2,c3); quads -> (c1,c2,c3) + (c1,c3,c4)),
-    written in the loader's ``[xyz, vertex, tri]`` layout."""
+    written in the loader's ``[xyz, vertex, tri]`` layout, element by element."""
     soups = {}
     tags = groups if groups is not None else sorted(mesh.group_tags)
+    X = mesh.coords
     for g in tags:
         idx = np.nonzero(mesh.group == g)[0]
-        tris = []
-        for e in idx:
-            c = mesh.nodes[e] - 1
-            v1, v2, v3 = mesh.coords[c[0]], mesh.coords[c[1]], mesh.coords[c[2]]
-            tris.append((v1, v2, v3))
-            if mesh.family[e] in (0, 2):
-                tris.append((v1, v3, mesh.coords[c[3]]))
-        soups[int(g)] = np.array(tris, np.float64).reshape(-1, 3, 3)
+        c = mesh.nodes[idx, :4].astype(np.int64) - 1
+        quad = np.isin(mesh.family[idx], (0, 2))
+        first = np.concatenate([[0], np.cumsum(1 + quad.astype(np.int64))[:-1]]).astype(np.int64)
+        out = np.empty((len(idx) + int(quad.sum()), 3, 3), np.float64)
+        out[first] = np.stack([X[c[:, 0]], X[c[:, 1]], X[c[:, 2]]], axis=1)
+        q = np.nonzero(quad)[0]
+        out[first[q] + 1] = np.stack([X[c[q, 0]], X[c[q, 2]], X[c[q, 3]]], axis=1)
+        soups[int(g)] = out
     return soups
+
+
+# ----------------------------------------------------------------------------
+# Mesh files
+# ----------------------------------------------------------------------------
+def load_mesh(filename: str, surface_dim: int = 2, reverse_normals: bool = False, verbose: bool = True,
+              group_field: Optional[str] = None) -> MeshData:
+    """Mirror of load_mesh (src/MeshIO.jl:94-145): a Gmsh .msh (format 2.2 or
+    4.1, ASCII or binary) or an XML VTK unstructured grid, read by the
+    library's native reader (rvf_meshfile_read), with the obstruction soups
+    of every group built as _build_group_obs_soups does.  Only surface meshes
+    (surface_dim 2) are read; curve meshes are CPU-only in the reference."""
+    import ctypes as C
+    from . import _lib
+    L = _lib.lib()
+    h = C.c_void_p()
+    _lib.check(L.rvf_meshfile_read(str(filename).encode(), surface_dim, 1 if reverse_normals else 0,
+                                   group_field.encode() if group_field else None, C.byref(h)))
+    try:
+        nn, ne, ng, fl = C.c_int64(), C.c_int64(), C.c_int32(), C.c_int32()
+        _lib.check(L.rvf_meshfile_info(h, C.byref(nn), C.byref(ne), C.byref(ng), C.byref(fl)))
+        coords = np.zeros((nn.value, 3), np.float64)
+        fam = np.zeros(ne.value, np.int8)
+        nodes = np.zeros((ne.value, 8), np.int32)
+        grp = np.zeros(ne.value, np.int32)
+        _lib.check(L.rvf_meshfile_arrays(h, _lib.ptr(coords), _lib.ptr(fam), _lib.ptr(nodes), _lib.ptr(grp)))
+        tags = {}
+        buf = C.create_string_buffer(4096)
+        for k in range(ng.value):
+            t = C.c_int32()
+            _lib.check(L.rvf_meshfile_group(h, k, C.byref(t), buf, 4096))
+            tags[int(t.value)] = buf.value.decode()
+        flags = fl.value
+    finally:
+        L.rvf_meshfile_free(h)
+    m = MeshData(coords, fam, nodes, grp, tags)
+    m.group_tri_soup = build_group_obs_soups(m)
+    if verbose:
+        counts = {n: int(np.sum(fam == c)) for n, c in (("quad8", 0), ("quad4", 2), ("tri6", 1), ("tri3", 3))}
+        if flags & 1:
+            print('No physical groups of dimension 2 found; created a synthetic "default" group.')
+        if flags & 2:
+            print("Quad9 (type 10) found — centre node dropped, treated as Quad8.")
+        if flags & 4:
+            print(f'Requested group_field "{group_field}" not found as a per-cell array; using a single '
+                  '"default" group.')
+        print(f"Loaded {m.n_elems} surface elements (quad8: {counts['quad8']}, quad4: {counts['quad4']}, "
+              f"tri6: {counts['tri6']}, tri3: {counts['tri3']}) in {len(tags)} physical group(s).")
+        if reverse_normals:
+            print("  All normals reversed.")
+    return m
+
+
+def load_vtu(filename: str, surface_dim: int = 2, reverse_normals: bool = False, verbose: bool = True,
+             group_field: Optional[str] = None) -> MeshData:
+    """Mirror of load_vtu (src/MeshIO.jl:190-206): XML VTK through the same
+    native reader."""
+    return load_mesh(filename, surface_dim, reverse_normals, verbose, group_field)
 
 
 # ----------------------------------------------------------------------------

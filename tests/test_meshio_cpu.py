@@ -1,0 +1,188 @@
+"""Mesh-file ingestion (rvf_meshfile_read through mesh.load_mesh), on the CPU.
+
+Pins, from the reference's load_mesh tests (test/runtests.jl:303-395), whose
+meshes Gmsh generates; here the same geometries are written by an independent
+writer (tests/support/meshfiles.py) in every supported encoding:
+  * a unit plate at element order 1 gives only Tri3/Quad4 elements and order
+    2 only Tri6/Quad8, each of total area 1 (atol 1e-9);
+  * without physical groups the only group is "default";
+  * two unstructured plates one apart ("bottom", "top") give
+    F(bottom -> top) = 0.19982 within 5e-3 and pass check_reciprocity.
+Beyond those: the elements read (family, node coordinates, group) equal the
+written ones in every format; Quad9 keeps its first 8 nodes; reverse_normals
+swaps nodes as _reverse_all_normals! (src/MeshIO.jl:545-562); unnamed groups
+are named group_<tag>; an entity in two groups contributes to both; bad
+files fail with a message.  Areas and view factors come from the oracle.
+"""
+import os
+
+import numpy as np
+import pytest
+
+from rvf_amd import api
+from rvf_amd import mesh as M
+from rvf_amd._lib import RVFError
+
+from support import meshfiles as MF
+
+MSH_FORMATS = [("2.2", False), ("2.2", True), ("4.1", False), ("4.1", True)]
+VTU_FORMATS = [dict(encoding="ascii"), dict(encoding="binary"), dict(encoding="binary", compress=True),
+               dict(encoding="appended-raw"), dict(encoding="appended-raw", compress=True, header64=True),
+               dict(encoding="appended-base64"), dict(encoding="appended-base64", compress=True),
+               dict(encoding="binary", header64=True, leading_zero=True), dict(encoding="binary", joint=True)]
+
+
+def _elements(m):
+    out = []
+    for e in range(m.n_elems):
+        f = int(m.family[e])
+        nn = M.FAMILY_NNODES[f]
+        out.append((f, m.coords[m.nodes[e, :nn] - 1], int(m.group[e])))
+    return out
+
+
+def _same(read, model, default_tag=None):
+    want = MF.model_elements(model)
+    assert len(read) == len(want)
+    for (f, x, g), (wf, wx, wg) in zip(read, want):
+        assert f == wf and np.array_equal(x, wx) and g == (wg if wg is not None else default_tag)
+
+
+def _area(orc, m):
+    r = orc.compute(m, nquad=4)
+    return float(np.sum(r["A"]))
+
+
+@pytest.mark.parametrize("version,binary", MSH_FORMATS)
+@pytest.mark.parametrize("order", [1, 2])
+def test_plate_orders_families_and_area(orc, tmp_path, version, binary, order):
+    model = MF.plate_model(order)
+    p = str(tmp_path / "plate.msh")
+    MF.write_msh(model, p, version, binary)
+    m = M.load_mesh(p, verbose=False)
+    fams = {M.FAMILY_NAME[int(f)] for f in m.family}
+    assert fams == ({"tri3", "quad4"} if order == 1 else {"tri", "quad"})
+    assert m.group_tags == {7: "plate"}
+    _same(_elements(m), model)
+    assert abs(_area(orc, m) - 1.0) <= 1e-9
+    # obstruction soups as _build_group_obs_soups builds them
+    assert m.group_tri_soup[7].shape == (int(np.sum(m.family == 3) + np.sum(m.family == 1)) +
+                                         2 * int(np.sum(m.family == 2) + np.sum(m.family == 0)), 3, 3)
+
+
+@pytest.mark.parametrize("version,binary", MSH_FORMATS)
+def test_no_physical_groups_gives_default(tmp_path, version, binary):
+    model = MF.plate_model(1, physical=False)
+    p = str(tmp_path / "plate.msh")
+    MF.write_msh(model, p, version, binary)
+    m = M.load_mesh(p, verbose=False)
+    assert list(m.group_tags.values()) == ["default"]
+    tag = next(iter(m.group_tags))
+    _same(_elements(m), model, default_tag=tag)
+
+
+@pytest.mark.parametrize("version,binary", [("4.1", False), ("2.2", True)])
+@pytest.mark.parametrize("order", [1, 2])
+def test_two_unstructured_plates_view_factor(orc, tmp_path, version, binary, order):
+    model = MF.two_plates_model(order)
+    p = str(tmp_path / "plates.msh")
+    MF.write_msh(model, p, version, binary)
+    m = M.load_mesh(p, verbose=False)
+    assert {M.FAMILY_NAME[int(f)] for f in m.family} == ({"tri3"} if order == 1 else {"tri"})
+    assert len(m.group_tags) == 2
+    r = orc.compute(m, nquad=4)
+    F = r["raw"] / r["A"][:, None]
+    tags, names, Fg, Ag = orc.aggregate(m, F, r["A"])
+    b, t = names.index("bottom"), names.index("top")
+    assert abs(Fg[b, t] - 0.19982) <= 5e-3
+    res = api.ViewFactorResult(F, r["A"], Fg, Ag, tags, names)
+    assert api.check_reciprocity(res)
+
+
+@pytest.mark.parametrize("fmt", VTU_FORMATS, ids=lambda d: "-".join(f"{k}={v}" for k, v in d.items()))
+def test_vtu_encodings(orc, tmp_path, fmt):
+    model = MF.plate_model(2)
+    read = M.load_mesh(_write_msh(tmp_path, model), verbose=False)
+    cells = []
+    for e in range(read.n_elems):
+        f = int(read.family[e])
+        cells.append((MF.VTK_TYPE[f], list(read.nodes[e, :M.FAMILY_NNODES[f]] - 1)))
+    cells.append((3, [0, 1]))  # a line cell: skipped for surface meshes
+    groups = np.concatenate([read.group, [99]])
+    p = str(tmp_path / "plate.vtu")
+    MF.write_vtu(p, read.coords, cells, {"CellEntityIds": groups}, **fmt)
+    v = M.load_mesh(p, verbose=False)
+    assert np.array_equal(v.coords, read.coords)
+    assert np.array_equal(v.family, read.family) and np.array_equal(v.nodes, read.nodes)
+    assert np.array_equal(v.group, read.group) and v.group_tags == {7: "group_7"}
+    assert abs(_area(orc, v) - 1.0) <= 1e-9
+
+
+def _write_msh(tmp_path, model, name="m.msh"):
+    p = str(tmp_path / name)
+    MF.write_msh(model, p, "4.1", False)
+    return p
+
+
+def test_vtu_group_field_and_default(tmp_path):
+    xyz = np.array([[0, 0, 0], [1, 0, 0], [1, 1, 0], [0, 1, 0]], np.float64)
+    cells = [(5, [0, 1, 2]), (5, [0, 2, 3])]
+    p = str(tmp_path / "t.vtu")
+    MF.write_vtu(p, xyz, cells, {"zone": np.array([3, 4])})
+    v = M.load_mesh(p, verbose=False)  # "zone" is not a candidate name
+    assert v.group_tags == {1: "default"} and list(v.group) == [1, 1]
+    v = M.load_mesh(p, verbose=False, group_field="zone")
+    assert v.group_tags == {3: "group_3", 4: "group_4"} and list(v.group) == [3, 4]
+
+
+def test_quad9_unnamed_and_shared_entity(tmp_path):
+    # one Quad9 element on entity 5 (in physical groups 2 and 3, group 3
+    # unnamed) and one Tri3 on entity 6 (group 2)
+    xyz = np.array([[0, 0, 0], [1, 0, 0], [1, 1, 0], [0, 1, 0], [.5, 0, 0], [1, .5, 0], [.5, 1, 0], [0, .5, 0],
+                    [.5, .5, 0], [2, 0, 0]], np.float64)
+    tags = np.arange(1, 11, dtype=np.int64)
+    e5 = MF.Entity(2, 5, [2, 3], [(10, np.array([tags[:9]]))], tags[:9])
+    e6 = MF.Entity(2, 6, [2], [(2, np.array([[2, 10, 3]]))], tags[9:])
+    model = MF.GmshModel(tags, xyz, [e5, e6], {(2, 2): "walls"})
+    for version, binary in MSH_FORMATS:
+        p = str(tmp_path / f"q9_{version}_{binary}.msh")
+        MF.write_msh(model, p, version, binary)
+        m = M.load_mesh(p, verbose=False)
+        assert m.group_tags == {2: "walls", 3: "group_3"}
+        assert list(m.family) == [0, 3, 0] and list(m.group) == [2, 2, 3]
+        assert list(m.nodes[0]) == list(range(1, 9)) and list(m.nodes[1, :3]) == [2, 10, 3]
+
+
+def test_reverse_normals(tmp_path):
+    p = _write_msh(tmp_path, MF.plate_model(2))
+    a = M.load_mesh(p, verbose=False)
+    b = M.load_mesh(p, verbose=False, reverse_normals=True)
+    for e in range(a.n_elems):
+        n = list(a.nodes[e])
+        if a.family[e] == 0:      # Quad8: 1<->3, 5<->7
+            n[0], n[2], n[4], n[6] = n[2], n[0], n[6], n[4]
+        elif a.family[e] == 1:    # Tri6: 1<->3, 4<->6
+            n[0], n[2], n[3], n[5] = n[2], n[0], n[5], n[3]
+        else:
+            n[0], n[2] = n[2], n[0]
+        assert list(b.nodes[e]) == n
+
+
+def test_errors(tmp_path):
+    with pytest.raises(RVFError, match="not found"):
+        M.load_mesh(str(tmp_path / "missing.msh"), verbose=False)
+    p = tmp_path / "v40.msh"
+    p.write_text("$MeshFormat\n4.0 0 8\n$EndMeshFormat\n")
+    with pytest.raises(RVFError, match="4.0"):
+        M.load_mesh(str(p), verbose=False)
+    p = tmp_path / "lines.msh"
+    p.write_text("$MeshFormat\n2.2 0 8\n$EndMeshFormat\n$Nodes\n2\n1 0 0 0\n2 1 0 0\n$EndNodes\n"
+                 "$Elements\n1\n1 1 2 4 1 1 2\n$EndElements\n")
+    with pytest.raises(RVFError, match="No supported elements|No entities"):
+        M.load_mesh(str(p), verbose=False)
+    p = tmp_path / "trunc.msh"
+    p.write_text("$MeshFormat\n2.2 0 8\n$EndMeshFormat\n$Nodes\n3\n1 0 0 0\n")
+    with pytest.raises(RVFError):
+        M.load_mesh(str(p), verbose=False)
+    with pytest.raises(RVFError, match="surface_dim"):
+        M.load_mesh(_write_msh(tmp_path, MF.plate_model(1)), surface_dim=1, verbose=False)
