@@ -157,7 +157,17 @@ __global__ __launch_bounds__(64) void k_precompute(PrecompArgs a) {
     a.area_orig[o] = A;
     a.elem_np[e] = np;
     double ax = cx / np, ay = cy / np, az = cz / np;
-    a.anchor[e * 4 + 0] = ax; a.anchor[e * 4 + 1] = ay; a.anchor[e * 4 + 2] = az; a.anchor[e * 4 + 3] = 0.0;
+    // anchor[3]: radius of the element's points about the anchor (occlusion
+    // culling of whole pairs, tile_occ_queue), rounded up
+    double r2 = 0.0;
+    for (int p = 0; p < np; p++) {
+        const double dx = a.P64[(int64_t)(p * 8 + 0) * a.NS + e] - ax;
+        const double dy = a.P64[(int64_t)(p * 8 + 1) * a.NS + e] - ay;
+        const double dz = a.P64[(int64_t)(p * 8 + 2) * a.NS + e] - az;
+        r2 = fmax(r2, dx * dx + dy * dy + dz * dz);
+    }
+    a.anchor[e * 4 + 0] = ax; a.anchor[e * 4 + 1] = ay; a.anchor[e * 4 + 2] = az;
+    a.anchor[e * 4 + 3] = sqrt(r2) * (1.0 + 1e-9);
     for (int p = 0; p < np; p++) {
         const double* P = a.P64;
         int64_t NS = a.NS;
@@ -1453,6 +1463,31 @@ __device__ __forceinline__ void split_drain(const PairArgs& a, const QCtx& X, in
     }
 }
 
+// Can a ray between a point of element i and a point of element j reach the
+// box?  Such rays lie in the convex hull of the two elements' points, inside
+// the capsule of radius max(R_i, R_j) around the anchors' segment, hence
+// inside that segment swept by the cube of that half-width: the test is the
+// segment against the box grown by the radius.  The box is a tree's padded
+// FP32 root box (128 u Sm beyond the triangles), so this FP64 test's own
+// rounding cannot drop a ray that could touch a triangle, nor one whose exact
+// FP64 slab test at the root (BVH.jl:155-169) could pass.
+__device__ __forceinline__ bool capsule_meets_box(const double* ai, const double* aj, double R, const BvhNode32& b) {
+    double tmin = 0.0, tmax = 1.0;
+    for (int c = 0; c < 3; c++) {
+        const double lo = (double)b.lo[c] - R, hi = (double)b.hi[c] + R;
+        const double o = ai[c], d = aj[c] - ai[c];
+        if (d == 0.0) {
+            if (o < lo || o > hi) return false;
+            continue;
+        }
+        double t1 = (lo - o) / d, t2 = (hi - o) / d;
+        if (t1 > t2) { const double t = t1; t1 = t2; t2 = t; }
+        tmin = fmax(tmin, t1);
+        tmax = fmin(tmax, t2);
+    }
+    return tmin <= tmax;
+}
+
 template <bool MIX, int S, bool SPLIT>
 __device__ __forceinline__ void tile_occ_queue(const PairArgs& a, int tile, double* lds) {
     const TileCtx t = tile_ctx(a, tile);
@@ -1476,7 +1511,7 @@ __device__ __forceinline__ void tile_occ_queue(const PairArgs& a, int tile, doub
     const double* __restrict__ P64 = a.P64;
     const float* __restrict__ P = a.P32;
     double ls = 0.0, lc = 0.0;
-    uint32_t nrays = 0;
+    uint32_t nrays = 0, culled = 0;  // rays cast; of those, rays of pairs culled at the root box
     QCount K;
     QCtx X;
     X.sbase = (int64_t)t.s * RVF_STRIP;
@@ -1495,6 +1530,15 @@ __device__ __forceinline__ void tile_occ_queue(const PairArgs& a, int tile, doub
         const int gj = a.elem_grp[j];
         // mixed split traversal walks its own tree over the same triangles
         const int root = (MIX && SPLIT ? a.ttree_root : a.tree_root)[gi * a.G + gj];
+        // pairs whose rays cannot reach the occluders' root box trace nothing
+        // (their rays count as cast with one box test, as the reference's)
+        bool cap = false;
+        if (act && root >= 0) {
+            const double ai[3] = {a.anchor[t.i * 4 + 0], a.anchor[t.i * 4 + 1], a.anchor[t.i * 4 + 2]};
+            const double aj[3] = {a.anchor[j * 4 + 0], a.anchor[j * 4 + 1], a.anchor[j * 4 + 2]};
+            const double R = fmax(a.anchor[t.i * 4 + 3], a.anchor[j * 4 + 3]);
+            cap = capsule_meets_box(ai, aj, R, a.nodes32[a.tree_root[gi * a.G + gj]]);
+        }
         X.j = j; X.oj = oj; X.npj = npj; X.gj = gj;
         X.revmask = __ballot(rev);
         lroot[lane] = root;
@@ -1534,7 +1578,7 @@ __device__ __forceinline__ void tile_occ_queue(const PairArgs& a, int tile, doub
                         float k = vf_mixed(bx, by, bz, nxi, nyi, nzi, qj32[q * 8 + 0], qj32[q * 8 + 1], qj32[q * 8 + 2],
                                            qj32[q * 8 + 3], qj32[q * 8 + 4], qj32[q * 8 + 5], rx, ry, rz, r2);
                         const bool nd = pv && k != 0.0f;
-                        const uint64_t m = __ballot(nd);
+                        const uint64_t m = __ballot(nd && cap);
                         nrays += nd;
                         const int c = (p - p0) * npj + q;
                         if (lane == 0) { need[c] = m; blk[2 * c] = 0u; blk[2 * c + 1] = 0u; }
@@ -1550,8 +1594,9 @@ __device__ __forceinline__ void tile_occ_queue(const PairArgs& a, int tile, doub
                         const double K = vf_exact(xi, yi, zi, nxi, nyi, nzi, qj64[q * 8 + 0], qj64[q * 8 + 1],
                                                   qj64[q * 8 + 2], qj64[q * 8 + 3], qj64[q * 8 + 4], qj64[q * 8 + 5], Rr);
                         const bool nd = pv && K != 0.0;
-                        const uint64_t m = __ballot(nd);
+                        const uint64_t m = __ballot(nd && cap);
                         nrays += nd;
+                        culled += nd && !cap;
                         const int c = (p - p0) * npj + q;
                         if (lane == 0) { need[c] = m; blk[2 * c] = 0u; blk[2 * c + 1] = 0u; }
                     }
@@ -1631,6 +1676,7 @@ __device__ __forceinline__ void tile_occ_queue(const PairArgs& a, int tile, doub
     }
     a.part[(int64_t)t.chunk * NS + t.i] = ls + lc;
     if (a.counters) {
+        if (!MIX) K.c0 += culled;  // FP64 box tests: a culled ray's root test
         unsigned long long v[7] = {nrays, K.c0, K.c1, K.iters, K.steps, K.tbatch, K.ebatch};
 #pragma unroll
         for (int o = 32; o > 0; o >>= 1)
