@@ -7,8 +7,9 @@
 //      the BVHs, creates the plan and enqueues its prologue (point precompute)
 //      on the context stream;
 //   2. a large request (at least direct_tiles tiles) gets its whole tile range
-//      as a direct launch, then its epilogue, on its slot's stream, ordered
-//      after its prologue by an event; the slot streams run concurrently, so
+//      as a direct launch, then its epilogue, on the slot stream with the
+//      fewest tiles in flight, ordered after its prologue by an event; the
+//      slot streams run concurrently, so
 //      requests that arrive while others run start at once and share the
 //      device at workgroup granularity;
 //   3. small requests share rounds: per pair-kernel kind, one persistent
@@ -82,6 +83,7 @@ struct Req {
     rvf_plan* plan = nullptr;
     PairArgs args{};
     int kind = 0, slot = -1, ntiles = 0, next_tile = 0;
+    int stream = -1;                   // slot stream of a direct request
     bool epilogue = false;             // epilogue enqueued: retire once done_ev completes
     hipEvent_t ready_ev = nullptr;     // prologue done (context stream)
     hipEvent_t done_ev = nullptr;      // epilogue done
@@ -110,11 +112,15 @@ struct rvf_server {
     std::map<int64_t, Req*> all;
     size_t used = 0;                   // estimated bytes of admitted requests
     std::vector<int> free_slots;
-    // Device objects of finished requests wait here for the reaper thread:
-    // freeing device memory waits for all work queued on the device, which
-    // would stall the worker behind other requests' kernels.
+    // Device objects of finished requests wait here for the reaper thread.
+    // Freeing device memory waits for all work queued on the device and holds
+    // up the worker's launches meanwhile, so the reaper frees only when the
+    // memory is wanted: an admission waits on the budget, the retained bytes
+    // pass a quarter of the budget, or the server stops.
     struct Dead { rvf_plan* plan; rvf_obs* obs; rvf_mesh* mesh; size_t bytes; };
     std::deque<Dead> graveyard;
+    size_t dead_bytes = 0;
+    bool mem_wanted = false;           // an admission waits for memory
     std::condition_variable cv_reap;
     bool reap_stop = false;
     std::thread reaper;
@@ -124,8 +130,11 @@ struct rvf_server {
     int* d_ctl = nullptr;              // [2 * RVF_K_COUNT]: head, count per kind
     int2* h_work = nullptr;            // pinned staging
     int* h_ctl = nullptr;
+    PairArgs* h_args = nullptr;        // pinned staging of d_args (a slot's entry is rewritten only after
+                                       // its previous request retired)
     static constexpr int kStreams = 4;
     hipStream_t streams[kStreams] = {};
+    int64_t stream_tiles[kStreams] = {};  // tiles launched on each stream and not yet retired (worker only)
     int64_t work_cap = 0;
     int ncu = 0;
     std::chrono::steady_clock::time_point t0;
@@ -161,11 +170,13 @@ size_t estimate_bytes(const Req& r) {
 // objects to the reaper (which returns the memory to the budget) and frees
 // its slot.
 void finish(rvf_server* s, Req* r, int state) {
+    if (r->stream >= 0) { s->stream_tiles[r->stream] -= r->ntiles; r->stream = -1; }
     if (r->ready_ev) { (void)hipEventDestroy(r->ready_ev); r->ready_ev = nullptr; }
     if (r->done_ev) { (void)hipEventDestroy(r->done_ev); r->done_ev = nullptr; }
     std::lock_guard<std::mutex> g(s->mu);
     if (r->plan || r->obs || r->mesh) {
         s->graveyard.push_back({r->plan, r->obs, r->mesh, r->est_bytes});
+        s->dead_bytes += r->est_bytes;
         s->cv_reap.notify_all();
     } else {
         s->used -= std::min(s->used, r->est_bytes);
@@ -182,7 +193,14 @@ void finish(rvf_server* s, Req* r, int state) {
     s->cv_done.notify_all();
 }
 
-hipStream_t req_stream(const rvf_server* s, const Req* r) { return s->streams[r->slot % rvf_server::kStreams]; }
+// A direct request goes to the slot stream with the fewest tiles in flight,
+// so a short request does not queue behind a long one.
+int pick_stream(const rvf_server* s) {
+    int best = 0;
+    for (int k = 1; k < rvf_server::kStreams; k++)
+        if (s->stream_tiles[k] < s->stream_tiles[best]) best = k;
+    return best;
+}
 
 // Large requests get direct launches of their tile ranges; smaller ones are
 // drained by the batched kernel.
@@ -211,9 +229,10 @@ int admit(rvf_server* s, Req* r) {
     }
     if (!rc) {
         r->ntiles = r->args.ntiles;
-        hipError_t e = hipMemcpyAsync(s->d_args + r->slot, &r->args, sizeof(PairArgs), hipMemcpyHostToDevice, st);
+        s->h_args[r->slot] = r->args;  // pinned: the copy is asynchronous
+        hipError_t e = hipMemcpyAsync(s->d_args + r->slot, s->h_args + r->slot, sizeof(PairArgs),
+                                      hipMemcpyHostToDevice, st);
         if (e != hipSuccess) rc = rvfi_fail(RVF_E_HIP, std::string("server: args upload: ") + hipGetErrorString(e));
-        // the copy reads r->args asynchronously; the request outlives the round
     }
     if (rc) {
         r->err = rvf_last_error();
@@ -232,10 +251,14 @@ void reaper_loop(rvf_server* s) {
         rvf_server::Dead d;
         {
             std::unique_lock<std::mutex> g(s->mu);
-            s->cv_reap.wait(g, [&] { return s->reap_stop || !s->graveyard.empty(); });
+            s->cv_reap.wait(g, [&] {
+                return s->reap_stop || (!s->graveyard.empty() &&
+                                        (s->mem_wanted || s->dead_bytes > (size_t)s->o.hbm_budget / 4));
+            });
             if (s->graveyard.empty()) break;
             d = s->graveyard.front();
             s->graveyard.pop_front();
+            s->dead_bytes -= std::min(s->dead_bytes, d.bytes);
         }
         rvf_plan_free(d.plan);
         rvf_obs_free(d.obs);
@@ -282,7 +305,12 @@ void worker_loop(rvf_server* s) {
                     s->cv_done.notify_all();
                     continue;
                 }
-                if (s->used + r->est_bytes > (size_t)s->o.hbm_budget) break;  // wait for memory
+                if (s->used + r->est_bytes > (size_t)s->o.hbm_budget) {  // wait for memory
+                    s->mem_wanted = true;
+                    s->cv_reap.notify_all();
+                    break;
+                }
+                s->mem_wanted = false;
                 s->pending.pop_front();
                 r->slot = s->free_slots.back();
                 s->free_slots.pop_back();
@@ -312,7 +340,9 @@ void worker_loop(rvf_server* s) {
         // ---- 2. large requests: whole tile range + epilogue on the slot stream ----
         for (Req* r : running) {
             if (rc || r->cancel || r->epilogue || !is_direct(s, r)) continue;
-            hipStream_t rs = req_stream(s, r);
+            r->stream = pick_stream(s);
+            s->stream_tiles[r->stream] += r->ntiles;
+            hipStream_t rs = s->streams[r->stream];
             hipError_t e = hipStreamWaitEvent(rs, r->ready_ev, 0);
             if (e != hipSuccess) { rc = rvfi_fail(RVF_E_HIP, std::string("server: wait event: ") + hipGetErrorString(e)); break; }
             PairArgs a = r->args;
@@ -483,6 +513,7 @@ int rvf_server_create(rvf_ctx* ctx, const rvf_server_opts* opts, rvf_server** ou
     if (e == hipSuccess) e = hipMalloc((void**)&s->d_ctl, sizeof(int) * 2 * RVF_K_COUNT);
     if (e == hipSuccess) e = hipHostMalloc((void**)&s->h_work, sizeof(int2) * s->work_cap, hipHostMallocDefault);
     if (e == hipSuccess) e = hipHostMalloc((void**)&s->h_ctl, sizeof(int) * 2 * RVF_K_COUNT, hipHostMallocDefault);
+    if (e == hipSuccess) e = hipHostMalloc((void**)&s->h_args, sizeof(PairArgs) * s->o.max_active, hipHostMallocDefault);
     for (int k = 0; k < rvf_server::kStreams && e == hipSuccess; k++)
         e = hipStreamCreateWithFlags(&s->streams[k], hipStreamNonBlocking);
     if (e != hipSuccess) {
@@ -521,6 +552,7 @@ void rvf_server_destroy(rvf_server* s) {
     if (s->d_work) (void)hipFree(s->d_work);
     if (s->d_ctl) (void)hipFree(s->d_ctl);
     if (s->h_work) (void)hipHostFree(s->h_work);
+    if (s->h_args) (void)hipHostFree(s->h_args);
     if (s->h_ctl) (void)hipHostFree(s->h_ctl);
     for (int k = 0; k < rvf_server::kStreams; k++)
         if (s->streams[k]) (void)hipStreamDestroy(s->streams[k]);
