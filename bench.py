@@ -59,8 +59,8 @@ def flop_model(N, npts, rays=0, aabb=0, tri=0):
     return pairs * (npts * npts * 28 + npts * 3) + 15 * rays + 24 * aabb + 52 * tri
 
 
-def run_mode(api, ctx, dm, do, nquad, precision, steps, warmup, audit=False, tune=None):
-    plan = api.Plan(ctx, dm, do, nquad, "quad", precision, want_F=True, audit=audit, timing=True)
+def run_mode(api, ctx, dm, do, nquad, precision, steps, warmup, audit=False, tune=None, self_vf=False):
+    plan = api.Plan(ctx, dm, do, nquad, "quad", precision, want_F=True, audit=audit, timing=True, self_vf=self_vf)
     if tune is not None and do is not None:
         # autotuner (setup, outside the timed region): time each traversal
         # configuration over a tile sample and keep the fastest
@@ -239,6 +239,8 @@ def main():
     plan_m.close(); plan_d.close()
     if args.secondary and args.workload != "cube":
         out["unoccluded_cube"] = secondary_cube(api, ctx, args.steps)
+    if args.secondary:
+        out["self_vf_sphere"] = secondary_self_vf(api, ctx, args.steps)
     if args.serving and args.serving_requests > 0:
         out["serving"] = serving_stream(api, ctx, args.serving_requests, args.serving_rate)
     print(json.dumps(out))
@@ -335,6 +337,40 @@ def secondary_cube(api, ctx, steps):
     res["max_abs_dF"] = float(np.abs(Fs["mixed"][0] - Fs["fp64"][0]).max())
     res["closure_err"] = float(np.abs(Fs["mixed"][1] - Fs["fp64"][1]).max())
     res["rowsum_minus_1_max_fp64"] = float(np.abs(Fs["fp64"][1] - 1.0).max())
+    dm.close()
+    return res
+
+
+def secondary_self_vf(api, ctx, steps, n=48):
+    """Self view factors on a concave surface: the interior of a unit sphere
+    as 6 n^2 curved Quad8 elements (13,824 at n = 48), nquad 3, unoccluded.
+    Mixed and FP64 solve times with and without the pairs (i, i), and the
+    closure the diagonal restores (inside a sphere every F_ij, the diagonal
+    included, is A_j / 4 pi up to the quadrature's missing p = q terms)."""
+    from rvf_amd import mesh as M
+    mesh = M.sphere_interior(n, 1.0, 2)
+    N = mesh.n_elems
+    dm = api.DeviceMesh(ctx, mesh)
+    res = {"workload": f"sphere interior, cubed sphere 6x{n}x{n} Quad8 ({N:,} elements), nquad=3"}
+    out = {}
+    for prec in ("mixed", "fp64"):
+        for self_vf in (False, True):
+            plan, ms, lat, pair = run_mode(api, ctx, dm, None, 3, prec, steps, 1, self_vf=self_vf)
+            F, A, rs, *_ = plan.fetch()
+            out[prec, self_vf] = (F if self_vf else None, rs)
+            res[f"{prec}{'_self_vf' if self_vf else ''}"] = {
+                "ms_per_step": ms, "pair_kernel_ms": float(pair.mean()),
+                "pairs_per_s": (N * (N + 1) / 2 if self_vf else N * (N - 1) / 2) / (ms * 1e-3)}
+            plan.close()
+    for prec in ("mixed", "fp64"):
+        res[f"{prec}_self_vf_overhead"] = res[f"{prec}_self_vf"]["ms_per_step"] / res[prec]["ms_per_step"] - 1
+    Fm, rsm = out["mixed", True]
+    Fd, rsd = out["fp64", True]
+    res["max_abs_dF_mixed_vs_fp64"] = float(np.abs(Fm - Fd).max())
+    res["max_abs_dFii_mixed_vs_fp64"] = float(np.abs(np.diag(Fm) - np.diag(Fd)).max())
+    res["closure_err_mixed_vs_fp64"] = float(np.abs(rsm - rsd).max())
+    res["rowsum_minus_1_max_fp64"] = {"self_vf": float(np.abs(rsd - 1).max()),
+                                      "without": float(np.abs(out["fp64", False][1] - 1).max())}
     dm.close()
     return res
 

@@ -48,6 +48,7 @@ extern "C" {
 #define RVF_AUDIT 2      /* per-pair blocked-ray count + hash (occluded solves) */
 #define RVF_TIMING 4     /* record per-kernel HIP event timings */
 #define RVF_GRAPH 8      /* capture the solve's launch sequence once and replay it as a hipGraph */
+#define RVF_SELF_VF 16   /* self view factors: integrate the pairs (i, i) too (src/Assembly.jl:173,193) */
 
 typedef struct rvf_ctx rvf_ctx;
 typedef struct rvf_mesh rvf_mesh;
@@ -60,7 +61,7 @@ typedef struct rvf_opts {
     int32_t precision;  /* RVF_FP64 / RVF_MIXED */
     int32_t n_samples;  /* Monte Carlo samples per pair */
     uint64_t seed;      /* Monte Carlo stream seed */
-    int32_t flags;      /* RVF_WANT_F | RVF_AUDIT | RVF_TIMING */
+    int32_t flags;      /* RVF_WANT_F | RVF_AUDIT | RVF_TIMING | RVF_GRAPH | RVF_SELF_VF */
     int32_t variant;    /* occluded quadrature traversal: RVF_VARIANT_* (0 = library default) */
 } rvf_opts;
 

@@ -52,7 +52,7 @@ def lib():
         L.orc_singularity.argtypes = [P, C.c_int, P, C.c_int, P, P]
         L.orc_compute.argtypes = [P, C.c_int64, C.c_int64, P, P, P, C.c_int, P, P, C.c_int64, P, C.c_int,
                                   C.c_int, C.c_int, C.c_int, C.c_uint64, C.c_int, C.c_int, C.c_int64,
-                                  C.c_int64, P, P, P, P, C.c_int, P]
+                                  C.c_int64, P, P, P, P, C.c_int, P, C.c_int]
         _lib = L
     return _lib
 
@@ -150,10 +150,12 @@ def xorshift_stream(seed, tid, n):
 
 
 def compute(mesh, nquad=4, obstruction_groups=(), parity_mode="reference", method="quad", n_samples=256,
-            seed=0, nthreads=0, schedule="static", rows=None, want_raw=True, audit=False, count=False):
+            seed=0, nthreads=0, schedule="static", rows=None, want_raw=True, audit=False, count=False,
+            self_vf=False):
     """Restatement of ``_compute_cpu`` (src/Assembly.jl:102-225).  Returns a
     dict with ``raw`` (N x N, Fortran-ordered like Julia; rows outside
-    ``rows`` stay zero), ``A``, optional audit arrays and test counters."""
+    ``rows`` stay zero), ``A``, optional audit arrays and test counters.
+    ``self_vf`` integrates the diagonal pairs (i, i) too (Assembly.jl:173,193)."""
     N = mesh.n_elems
     soup, sg = mesh.merged_soup()
     obs = np.ascontiguousarray(list(obstruction_groups), np.int32)
@@ -168,7 +170,7 @@ def compute(mesh, nquad=4, obstruction_groups=(), parity_mode="reference", metho
     rc = lib().orc_compute(_p(mesh.coords), mesh.n_nodes, N, _p(mesh.family), _p(mesh.nodes), _p(mesh.group),
                            nquad, _p(soup), _p(sg), soup.shape[0], _p(obs), len(obs), mode, meth, n_samples,
                            C.c_uint64(seed), nthreads, 0 if schedule == "static" else 1, r0, r1, _p(raw), _p(A),
-                           _p(ac), _p(ah), 1 if count else 0, _p(cnt))
+                           _p(ac), _p(ah), 1 if count else 0, _p(cnt), 1 if self_vf else 0)
     assert rc == 0
     return {"raw": raw, "A": A, "audit_count": ac, "audit_hash": ah,
             "rays": int(cnt[0]), "aabb_tests": int(cnt[1]), "tri_tests": int(cnt[2])}

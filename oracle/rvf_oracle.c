@@ -824,7 +824,7 @@ int orc_compute(const double* coords, int64_t Nn, int64_t N, const int8_t* fam, 
                 const int32_t* group, int nquad, const double* soup, const int32_t* soup_group, int64_t T,
                 const int32_t* obs, int n_obs, int soup_mode, int method, int n_samples, uint64_t seed,
                 int nthreads, int schedule, int64_t row_begin, int64_t row_end, double* raw, double* A,
-                uint32_t* audit_count, uint32_t* audit_hash, int count, int64_t* counters) {
+                uint32_t* audit_count, uint32_t* audit_hash, int count, int64_t* counters, int self_vf) {
     (void)Nn;
     OrcMesh m;
     memset(&m, 0, sizeof m);
@@ -930,7 +930,8 @@ int orc_compute(const double* coords, int64_t Nn, int64_t N, const int8_t* fam, 
                 int g = group[i];
                 for (int k = 0; k < G; k++) if (gtags[k] == g) { a = k; break; }
             }
-            for (int64_t j = i + 1; j < N; j++) {
+            /* self_vf: the pair (i, i) too (Assembly.jl:173,193) */
+            for (int64_t j = self_vf ? i : i + 1; j < N; j++) {
                 const OrcBVH* bvh = NULL;
                 if (sets) {
                     int g = group[j], b = 0;

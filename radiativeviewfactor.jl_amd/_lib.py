@@ -11,7 +11,7 @@ import os
 HERE = os.path.dirname(os.path.abspath(__file__))
 LIB_PATH = os.path.join(HERE, "librvf_hip.so")
 
-RVF_WANT_F, RVF_AUDIT, RVF_TIMING, RVF_GRAPH = 1, 2, 4, 8
+RVF_WANT_F, RVF_AUDIT, RVF_TIMING, RVF_GRAPH, RVF_SELF_VF = 1, 2, 4, 8, 16
 METHODS = {"quad": 0, "mc": 1, "duffy": 2}
 PRECISIONS = {"fp64": 0, "mixed": 1}
 

@@ -269,7 +269,7 @@ class Server:
         self._meta = {}
 
     def submit(self, mesh: MeshData, nquad: int, obstruction_groups: Sequence[int] = (), parity: str = "reference",
-               precision: str = "fp64", want_F: bool = False, variant: int = 0) -> int:
+               precision: str = "fp64", want_F: bool = False, variant: int = 0, self_vf: bool = False) -> int:
         mats, counts, idx = family_matrices(mesh)
         arr = (C.c_void_p * 4)(*[m.ctypes.data if m.size else None for m in mats])
         tags = sorted(set(int(g) for g in obstruction_groups if int(g) in mesh.group_tri_soup))
@@ -285,7 +285,7 @@ class Server:
         o.nquad = nquad
         o.method = _lib.METHODS["quad"]
         o.precision = _lib.PRECISIONS[precision]
-        o.flags = RVF_WANT_F if want_F else 0
+        o.flags = (RVF_WANT_F if want_F else 0) | (_lib.RVF_SELF_VF if self_vf else 0)
         o.variant = variant
         t = C.c_int64()
         check(_lib.lib().rvf_server_submit(self.h, ptr(mesh.coords), mesh.n_nodes, mesh.n_elems, ptr(mesh.family),
@@ -346,7 +346,7 @@ class Plan:
     def __init__(self, ctx: Context, dmesh: DeviceMesh, dobs: Optional[DeviceObstruction], nquad: int,
                  method: str = "quad", precision: str = "fp64", n_samples: int = 10000, seed: int = 0,
                  want_F: bool = True, audit: bool = False, timing: bool = True, graph: bool = False,
-                 variant: int = 0):
+                 variant: int = 0, self_vf: bool = False):
         o = _lib.Opts()
         o.nquad = nquad
         o.method = _lib.METHODS[method]
@@ -354,7 +354,7 @@ class Plan:
         o.n_samples = n_samples
         o.seed = seed & ((1 << 64) - 1)
         o.flags = (RVF_WANT_F if want_F else 0) | (RVF_AUDIT if audit else 0) | (RVF_TIMING if timing else 0) | \
-            (_lib.RVF_GRAPH if graph else 0)
+            (_lib.RVF_GRAPH if graph else 0) | (_lib.RVF_SELF_VF if self_vf else 0)
         o.variant = variant
         self.opts = o
         self.ctx, self.dmesh, self.dobs = ctx, dmesh, dobs
@@ -454,7 +454,8 @@ def compute_view_factors(mesh: MeshData, nquad: int = 4, obstruction_groups: Seq
                          backend: Optional[MI355XBackend] = None, self_vf: bool = False,
                          monte_carlo: bool = False, n_samples: int = 10000, rng=None, use_duffy: bool = False,
                          verbose: bool = True) -> ViewFactorResult:
-    """GPU branch of compute_view_factors (src/Assembly.jl:64-96) on MI355X."""
+    """GPU branch of compute_view_factors (src/Assembly.jl:64-96) on MI355X.  ``self_vf`` adds the
+    pairs (i, i) (src/Assembly.jl:173,193), which the reference computes on the CPU only."""
     if backend is None:
         backend = MI355XBackend()
     if isinstance(backend, type):
@@ -464,8 +465,6 @@ def compute_view_factors(mesh: MeshData, nquad: int = 4, obstruction_groups: Seq
     duffy = use_duffy or backend.use_duffy
     if duffy and monte_carlo:
         raise ValueError("use_duffy and monte_carlo cannot both be true.")
-    if self_vf and verbose:
-        print("self_vf is CPU-only; ignored for GPU backends.")
     method = "mc" if monte_carlo else ("duffy" if duffy else "quad")
     ctx = context(backend.device)
     N = mesh.n_elems
@@ -481,7 +480,7 @@ def compute_view_factors(mesh: MeshData, nquad: int = 4, obstruction_groups: Seq
                   "proceeding without obstruction checking.")
             dobs = None
     plan = Plan(ctx, dmesh, dobs, nquad, method, backend.precision, n_samples, backend.seed, want_F=True,
-                audit=backend.audit)
+                audit=backend.audit, self_vf=self_vf)
     if backend.tune_table and dobs is not None and method != "mc":
         plan.autotune(table=TuneTable(backend.tune_table))
     plan.run()

@@ -148,6 +148,8 @@ struct PairArgs {
     const int8_t* fam;        // N
     const int32_t* nodes8;    // 8 x N, 1-based
     int duffy;                // skip Quad8 pairs sharing a corner (done by k_duffy)
+    int self_vf;              // 1: the pairs (i, i) too, raw value to diag (not to part / partT)
+    double* diag;             // [NS] raw_ii per slot (self_vf)
     int mc_nsamp, mc_s;       // Monte Carlo samples per pair, strata per side
     uint64_t mc_seed;
     // [0] FP64 re-tests (mixed), [1] rays cast, [2] AABB tests, [3] triangle

@@ -27,7 +27,8 @@ extern "C" int rvf_launch_duffy(const PairArgs* a, const DuffyPair* pairs, int n
 extern "C" int rvf_launch_zero(void* p, int64_t bytes, hipStream_t st);
 extern "C" int rvf_launch_copy(const void* src, void* dst, int64_t bytes, hipStream_t st);
 extern "C" int rvf_launch_fill(double* F, const double* area, int64_t N, const int2* tiles, int ntiles,
-                               hipStream_t st);
+                               int self_vf, hipStream_t st);
+extern "C" int rvf_launch_diag(const double* diag, const double* area, int64_t N, double* rowsum, hipStream_t st);
 extern "C" int rvf_launch_reduce(const double* part, const double* partT, const Chunk* chunks, int nchunks,
                                  const int32_t* chunk_of, int64_t N, int NS, int G, double* rowsum, double* W,
                                  double* Tseg, double* Wseg, hipStream_t st);
@@ -206,7 +207,7 @@ struct rvf_mesh {
     std::vector<double> hcoords;
     std::vector<int2> fill_tiles;
     double Sm = 1.0;               // max |coordinate|
-    mutable Layout lay[2];         // 0 identity, 1 spatial
+    mutable Layout lay[4];         // bit 0: spatial order; bit 1: tiles for the pairs (i, i) too
     DBuf coords, d_fam, d_nodes8, d_grp, d_fill_tiles;
 };
 
@@ -219,8 +220,8 @@ static inline uint32_t spread10(uint32_t v) {
     return v;
 }
 
-static int get_layout(const rvf_mesh* m, bool spatial, const Layout** out) {
-    Layout& L = m->lay[spatial ? 1 : 0];
+static int get_layout(const rvf_mesh* m, bool spatial, bool self, const Layout** out) {
+    Layout& L = m->lay[(spatial ? 1 : 0) | (self ? 2 : 0)];
     if (!L.built) {
         int64_t N = m->N, NS = m->NS;
         std::vector<int32_t> order(N);
@@ -264,10 +265,11 @@ static int get_layout(const rvf_mesh* m, bool spatial, const Layout** out) {
             for (int64_t k = b; k < e; k++) L.chunk_of[k] = (int32_t)L.chunks.size();
             L.chunks.push_back(c);
         }
-        // tiles (strip s, chunk c) exist iff 64 s < last(c)
+        // tiles (strip s, chunk c) exist iff 64 s < last(c); with self pairs
+        // iff 64 s <= last(c)
         L.tiles.clear();
         for (size_t c = 0; c < L.chunks.size(); c++) {
-            int64_t last = L.chunks[c].end - 1;
+            int64_t last = L.chunks[c].end - 1 + (self ? 1 : 0);
             for (int64_t s = 0; s * RVF_STRIP < last; s++) L.tiles.push_back(make_int2((int)s, (int)c));
         }
         int rc;
@@ -849,7 +851,7 @@ struct rvf_plan {
     rvf_opts opts{};
     int nq_quad = 0, nq_tri = 0, maxpts = 0;
     DBuf rule, P64, Q64, P32, Q32, anchor, area, elem_np, bad, F, part, partT, rowsum, W, audit_c, audit_h, counters;
-    DBuf Tseg, Wseg, area_orig;
+    DBuf Tseg, Wseg, area_orig, diag;
     hipGraphExec_t gexec = nullptr;
     const Layout* L = nullptr;
     // Duffy singular pairs
@@ -889,7 +891,8 @@ int rvf_plan_create(rvf_ctx* ctx, const rvf_mesh* mesh, const rvf_obs* obs, cons
     int G = (int)mesh->tags.size();
     // spatially compact strips for occluded quadrature solves (coherent ray bundles)
     bool spatial = obs && !obs->trees.empty() && opts->method == RVF_METHOD_QUAD;
-    if ((rc0 = get_layout(mesh, spatial, &p->L))) { delete p; return rc0; }
+    const bool self = (opts->flags & RVF_SELF_VF) != 0;
+    if ((rc0 = get_layout(mesh, spatial, self, &p->L))) { delete p; return rc0; }
     size_t nch = p->L->chunks.size(), nstrip = NS / RVF_STRIP;
     int rc;
     if ((rc = upload(p->rule, rule.data(), sizeof(double) * rule.size())) ||
@@ -903,6 +906,7 @@ int rvf_plan_create(rvf_ctx* ctx, const rvf_mesh* mesh, const rvf_obs* obs, cons
         (rc = p->part.alloc(sizeof(double) * std::max<size_t>(nch, 1) * NS)) ||
         (rc = p->partT.alloc(sizeof(double) * std::max<size_t>(nstrip, 1) * NS)) ||
         (rc = p->rowsum.alloc(sizeof(double) * NS)) || (rc = p->W.alloc(sizeof(double) * std::max(G, 1) * NS)) ||
+        (self && (rc = p->diag.alloc(sizeof(double) * NS))) ||
         (rc = p->counters.alloc(sizeof(unsigned long long) * 16)) ||
         (rc = p->Tseg.alloc(sizeof(double) * RVF_RSEG * NS)) ||
         (rc = p->Wseg.alloc(sizeof(double) * RVF_RSEG * std::max(G, 1) * NS))) {
@@ -925,7 +929,7 @@ int rvf_plan_create(rvf_ctx* ctx, const rvf_mesh* mesh, const rvf_obs* obs, cons
             cand.clear();
             for (int q = 0; q < 4; q++)
                 for (int32_t j : by_corner[mesh->nodes8[8 * i + q]])
-                    if (j > i) cand.push_back(j);
+                    if (j > i || (self && j == i)) cand.push_back(j);
             std::sort(cand.begin(), cand.end());
             cand.erase(std::unique(cand.begin(), cand.end()), cand.end());
             for (int32_t j : cand) {
@@ -946,11 +950,14 @@ int rvf_plan_create(rvf_ctx* ctx, const rvf_mesh* mesh, const rvf_obs* obs, cons
         }
         p->npairs = (int)pairs.size();
         std::vector<int32_t> rptr(N + 1, 0), rlist;
-        for (const DuffyPair& d : pairs) { rptr[d.i + 1]++; rptr[d.j + 1]++; }
+        for (const DuffyPair& d : pairs) { rptr[d.i + 1]++; if (d.j != d.i) rptr[d.j + 1]++; }
         for (int64_t r = 0; r < N; r++) rptr[r + 1] += rptr[r];
         rlist.assign(rptr[N], 0);
         std::vector<int32_t> fillp(rptr.begin(), rptr.end() - 1);
-        for (int k = 0; k < (int)pairs.size(); k++) { rlist[fillp[pairs[k].i]++] = k; rlist[fillp[pairs[k].j]++] = k; }
+        for (int k = 0; k < (int)pairs.size(); k++) {
+            rlist[fillp[pairs[k].i]++] = k;
+            if (pairs[k].j != pairs[k].i) rlist[fillp[pairs[k].j]++] = k;
+        }
         double p1[8], w1[8], g01[16] = {0};
         int nq = gl1d(opts->nquad, p1, w1);
         for (int k = 0; k < nq; k++) { g01[k] = (p1[k] + 1.0) / 2.0; g01[8 + k] = w1[k] / 2.0; }
@@ -979,7 +986,7 @@ int rvf_plan_create(rvf_ctx* ctx, const rvf_mesh* mesh, const rvf_obs* obs, cons
     HIPCHK(hipMemsetAsync(p->elem_np.p, 0, p->elem_np.n, ctx->stream));
     for (int k = 0; k < 8; k++) HIPCHK(hipEventCreate(&p->ev[k]));
     p->st.n_elems = N;
-    p->st.unique_pairs = N * (N - 1) / 2;
+    p->st.unique_pairs = self ? N * (N + 1) / 2 : N * (N - 1) / 2;
     p->st.tiles = (int64_t)p->L->tiles.size();
     *out = p;
     return 0;
@@ -997,6 +1004,7 @@ static PairArgs pair_args(const rvf_plan* p) {
     a.P64 = p->P64.as<double>(); a.Q64 = p->Q64.as<double>(); a.P32 = p->P32.as<float>(); a.Q32 = p->Q32.as<float>();
     a.anchor = p->anchor.as<double>();
     a.F = p->F.as<double>(); a.part = p->part.as<double>(); a.partT = p->partT.as<double>();
+    a.self_vf = p->diag.p ? 1 : 0; a.diag = p->diag.as<double>();
     a.G = (int)m->tags.size();
     if (p->obs && !p->obs->trees.empty()) {
         a.tree_root = p->obs->d_root.as<int32_t>();
@@ -1075,7 +1083,8 @@ static int enqueue_prologue(rvf_plan* p, hipStream_t st, bool tim) {
     const rvf_mesh* m = p->mesh;
     if (tim) HIPCHK(hipEventRecord(p->ev[0], st));
     int rc;
-    if ((rc = rvf_launch_zero(p->bad.p, sizeof(int), st)) || (rc = rvf_launch_zero(p->counters.p, p->counters.n, st)))
+    if ((rc = rvf_launch_zero(p->bad.p, sizeof(int), st)) || (rc = rvf_launch_zero(p->counters.p, p->counters.n, st)) ||
+        (rc = rvf_launch_zero(p->diag.p, p->diag.n, st)))
         return fail(RVF_E_HIP, std::string("zero launch: ") + hipGetErrorString((hipError_t)rc));
     PrecompArgs pa{};
     pa.coords = m->coords.as<double>(); pa.fam = m->d_fam.as<int8_t>(); pa.nodes8 = m->d_nodes8.as<int32_t>();
@@ -1127,10 +1136,14 @@ static int enqueue_epilogue(rvf_plan* p, hipStream_t st, bool tim) {
                               p->W.as<double>(), 1, st);
         if (rc) return fail(RVF_E_HIP, std::string("duffy apply launch: ") + hipGetErrorString((hipError_t)rc));
     }
+    if (p->diag.p) {
+        rc = rvf_launch_diag(p->diag.as<double>(), p->area.as<double>(), m->N, p->rowsum.as<double>(), st);
+        if (rc) return fail(RVF_E_HIP, std::string("diagonal launch: ") + hipGetErrorString((hipError_t)rc));
+    }
     if (tim) HIPCHK(hipEventRecord(p->ev[4], st));
     if (p->F.p) {
         rc = rvf_launch_fill(p->F.as<double>(), p->area_orig.as<double>(), m->N, m->d_fill_tiles.as<int2>(),
-                             (int)m->fill_tiles.size(), st);
+                             (int)m->fill_tiles.size(), p->diag.p ? 1 : 0, st);
         if (rc) return fail(RVF_E_HIP, std::string("fill launch: ") + hipGetErrorString((hipError_t)rc));
     }
     if (tim) HIPCHK(hipEventRecord(p->ev[5], st));
@@ -1400,6 +1413,11 @@ int rvf_plan_fetch(rvf_plan* p, double* F, double* A, double* rowsum, double* F_
                 R[(size_t)g * G + h] += v;
                 R[(size_t)h * G + g] += v;
             }
+        }
+        if (p->diag.p) {  // self pairs: raw_ii = A_i F_ii, once
+            std::vector<double> hd(N);
+            HIPCHK(hipMemcpy(hd.data(), p->diag.p, sizeof(double) * N, hipMemcpyDeviceToHost));
+            for (int64_t k = 0; k < N; k++) R[(size_t)L->slot_grp[k] * (G + 1)] += hd[k];
         }
         if (A_group) for (int g = 0; g < G; g++) A_group[g] = Ag[g];
         if (F_group)

@@ -288,6 +288,7 @@ struct TileCtx {
     int s, chunk, lane, jb, je;
     int64_t i;   // slot of this lane's element
     int64_t oi;  // its original index
+    int64_t jlo; // columns below jlo have no pair with any lane (j > i; j >= i with self_vf)
     bool vi;
 };
 
@@ -299,6 +300,7 @@ __device__ __forceinline__ TileCtx tile_ctx(const PairArgs& a, int tile) {
     t.i = (int64_t)t.s * RVF_STRIP + t.lane;
     t.vi = t.i < a.N;
     t.oi = t.vi ? a.slot_orig[t.i] : 0;
+    t.jlo = (int64_t)t.s * RVF_STRIP + (a.self_vf ? 0 : 1);
     Chunk c = a.chunks[t.chunk];
     t.jb = c.begin; t.je = c.end;
     return t;
@@ -329,12 +331,14 @@ __device__ __forceinline__ bool shares_corner(const Corners& x, const Corners& y
 // transposed partial for row j (sum over lanes of raw/A_j).
 __device__ __forceinline__ void pair_out(const PairArgs& a, const TileCtx& t, int64_t j, int64_t oj, bool act,
                                          double raw, double Ai, double& ls, double& lc) {
+    const bool self = t.i == j;  // only active with self_vf: kept out of both partials
     if (act) {
         if (a.F) a.F[min(t.oi, oj) + a.N * max(t.oi, oj)] = raw;
-        neumaier(ls, lc, raw / Ai);
+        if (self) a.diag[j] = raw;
+        else neumaier(ls, lc, raw / Ai);
     }
     double Aj = a.area[j];
-    double v = act ? raw / Aj : 0.0;
+    double v = act && !self ? raw / Aj : 0.0;
     v = wave_sum_d(v);
     if (t.lane == 0) a.partT[(int64_t)t.s * a.NS + j] = v;
 }
@@ -356,11 +360,11 @@ __device__ __forceinline__ void tile_fp64(const PairArgs& a, int tile) {
     const double* __restrict__ P = a.P64;
     double ls = 0.0, lc = 0.0;
     for (int64_t j = t.jb; j < t.je; j++) {
-        if (j <= (int64_t)t.s * RVF_STRIP) {  // no lane has i < j
+        if (j < t.jlo) {  // no lane pairs with column j
             if (t.lane == 0) a.partT[(int64_t)t.s * a.NS + j] = 0.0;
             continue;
         }
-        bool act = t.vi && t.i < j;
+        bool act = t.vi && t.i < j + a.self_vf;
         const int64_t oj = a.slot_orig[j];
         if (a.duffy && shares_corner(cwi, corners_of(a, oj, true))) act = false;
         const int npj = a.elem_np[j];
@@ -434,11 +438,11 @@ __device__ __forceinline__ void tile_fp64_occ(const PairArgs& a, int tile, doubl
     double ls = 0.0, lc = 0.0;
     unsigned long long nrays = 0, naabb = 0, ntri = 0;
     for (int64_t j = t.jb; j < t.je; j++) {
-        if (j <= (int64_t)t.s * RVF_STRIP) {
+        if (j < t.jlo) {
             if (t.lane == 0) a.partT[(int64_t)t.s * a.NS + j] = 0.0;
             continue;
         }
-        bool act = t.vi && t.i < j;
+        bool act = t.vi && t.i < j + a.self_vf;
         const int64_t oj = a.slot_orig[j];
         if (a.duffy && shares_corner(cwi, corners_of(a, oj, true))) act = false;
         const bool rev = t.oi > oj;
@@ -557,11 +561,11 @@ __device__ __forceinline__ void tile_mixed(const PairArgs& a, int tile) {
     if (SINGLE) load_chunk(0);
     double ls = 0.0, lc = 0.0;
     for (int64_t j = t.jb; j < t.je; j++) {
-        if (j <= (int64_t)t.s * RVF_STRIP) {
+        if (j < t.jlo) {
             if (t.lane == 0) a.partT[(int64_t)t.s * a.NS + j] = 0.0;
             continue;
         }
-        bool act = t.vi && t.i < j;
+        bool act = t.vi && t.i < j + a.self_vf;
         const int64_t oj = a.slot_orig[j];
         if (a.duffy && shares_corner(cwi, corners_of(a, oj, true))) act = false;
         const int npj = a.elem_np[j];
@@ -735,11 +739,11 @@ __device__ __forceinline__ void tile_mixed_occ(const PairArgs& a, int tile) {
     double ls = 0.0, lc = 0.0;
     unsigned long long nret = 0, nrays = 0;
     for (int64_t j = t.jb; j < t.je; j++) {
-        if (j <= (int64_t)t.s * RVF_STRIP) {
+        if (j < t.jlo) {
             if (t.lane == 0) a.partT[(int64_t)t.s * a.NS + j] = 0.0;
             continue;
         }
-        bool act = t.vi && t.i < j;
+        bool act = t.vi && t.i < j + a.self_vf;
         const int64_t oj = a.slot_orig[j];
         if (a.duffy && shares_corner(cwi, corners_of(a, oj, true))) act = false;
         const int npj = a.elem_np[j];
@@ -1518,11 +1522,11 @@ __device__ __forceinline__ void tile_occ_queue(const PairArgs& a, int tile, doub
     X.lroot = lroot;
     X.stage = stage;
     for (int64_t j = t.jb; j < t.je; j++) {
-        if (j <= (int64_t)t.s * RVF_STRIP) {
+        if (j < t.jlo) {
             if (lane == 0) a.partT[(int64_t)t.s * a.NS + j] = 0.0;
             continue;
         }
-        bool act = t.vi && t.i < j;
+        bool act = t.vi && t.i < j + a.self_vf;
         const int64_t oj = a.slot_orig[j];
         if (a.duffy && shares_corner(cwi, corners_of(a, oj, true))) act = false;
         const bool rev = t.oi > oj;
@@ -1763,11 +1767,11 @@ __device__ __forceinline__ void tile_mc(const PairArgs& a, int tile) {
     double ls = 0.0, lc = 0.0;
     unsigned long long nrays = 0, naabb = 0, ntri = 0;
     for (int64_t j = t.jb; j < t.je; j++) {
-        if (j <= (int64_t)t.s * RVF_STRIP) {
+        if (j < t.jlo) {
             if (t.lane == 0) a.partT[(int64_t)t.s * NS + j] = 0.0;
             continue;
         }
-        const bool act = t.vi && t.i < j;
+        const bool act = t.vi && t.i < j + a.self_vf;
         const int64_t oj = a.slot_orig[j];
         const int fj = a.fam[oj];
         const int root = a.tree_root ? a.tree_root[gi * a.G + a.elem_grp[j]] : -1;
@@ -1941,6 +1945,11 @@ __global__ __launch_bounds__(256) void k_duffy_apply(PairArgs a, const DuffyPair
         int pidx = rlist[k];
         const DuffyPair P = pairs[pidx];
         double raw = val[pidx];
+        if (P.i == P.j) {  // self pair (self_vf): the diagonal, added by k_diag_apply
+            a.diag[r] = raw;
+            if (a.F) a.F[(int64_t)r + a.N * (int64_t)r] = raw;
+            continue;
+        }
         double f = raw / Ar;
         neumaier(ts, tc, f);
         if (P.i == r) {  // partner j > r: upper part, group of j
@@ -1951,12 +1960,22 @@ __global__ __launch_bounds__(256) void k_duffy_apply(PairArgs a, const DuffyPair
     rowsum[r] += ts + tc;
 }
 
+// Self view factors: F_ii = raw_ii / A_i joins the row sum (the group sums
+// take the diagonal on the host).
+__global__ __launch_bounds__(256) void k_diag_apply(const double* __restrict__ diag, const double* __restrict__ area,
+                                                    int64_t N, double* __restrict__ rowsum) {
+    int64_t r = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
+    if (r >= N) return;
+    rowsum[r] += diag[r] / area[r];
+}
+
 // ---------------------------------------------------------------------------
 // Fill: F[i,j] = raw/A_i and F[j,i] = raw/A_j for i < j (Assembly.jl:211),
-// F[i,i] = 0.  One 64x64 tile per workgroup, transposed half through LDS.
+// F[i,i] = raw_ii/A_i with self view factors, else 0.  One 64x64 tile per
+// workgroup, transposed half through LDS.
 // ---------------------------------------------------------------------------
 __global__ __launch_bounds__(256) void k_fill(double* __restrict__ F, const double* __restrict__ area, int64_t N,
-                                              const int2* __restrict__ tiles) {
+                                              const int2* __restrict__ tiles, int self_vf) {
     __shared__ double T[64][65];
     int2 tl = tiles[blockIdx.x];
     int64_t i0 = (int64_t)tl.x * 64, j0 = (int64_t)tl.y * 64;
@@ -1972,7 +1991,7 @@ __global__ __launch_bounds__(256) void k_fill(double* __restrict__ F, const doub
                 F[i + N * j] = raw / Ai;
                 out = raw / area[j];
             } else if (i == j) {
-                F[i + N * j] = 0.0;
+                F[i + N * j] = self_vf ? F[i + N * j] / Ai : 0.0;
             }
         }
         T[tx][r] = out;  // value for F[j, i]
@@ -2233,9 +2252,16 @@ int rvf_launch_copy(const void* src, void* dst, int64_t bytes, hipStream_t st) {
     return (int)hipGetLastError();
 }
 
-int rvf_launch_fill(double* F, const double* area, int64_t N, const int2* tiles, int ntiles, hipStream_t st) {
+int rvf_launch_fill(double* F, const double* area, int64_t N, const int2* tiles, int ntiles, int self_vf,
+                    hipStream_t st) {
     if (ntiles == 0) return 0;
-    hipLaunchKernelGGL(k_fill, dim3((unsigned)ntiles), dim3(256), 0, st, F, area, N, tiles);
+    hipLaunchKernelGGL(k_fill, dim3((unsigned)ntiles), dim3(256), 0, st, F, area, N, tiles, self_vf);
+    return (int)hipGetLastError();
+}
+
+int rvf_launch_diag(const double* diag, const double* area, int64_t N, double* rowsum, hipStream_t st) {
+    if (N == 0) return 0;
+    hipLaunchKernelGGL(k_diag_apply, dim3((unsigned)((N + 255) / 256)), dim3(256), 0, st, diag, area, N, rowsum);
     return (int)hipGetLastError();
 }
 

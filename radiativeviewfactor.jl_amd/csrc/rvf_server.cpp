@@ -159,7 +159,7 @@ size_t estimate_bytes(const Req& r) {
     b += (size_t)r.Nn * 24 + (size_t)r.N * 64;                          // mesh
     b += (size_t)r.T * 512;                                              // trees, triangles, items
     b += (size_t)NS * maxpts * 8 * (8 + 8 + 4 + 4);                      // P64 Q64 P32 Q32
-    b += (size_t)NS * 64;                                                // anchors, areas, sums
+    b += (size_t)NS * 72;                                                // anchors, areas, sums, diagonal
     b += (size_t)nchunks * NS * 8 + (size_t)(NS / RVF_STRIP) * NS * 8;   // part, partT
     if (r.opts.flags & RVF_WANT_F) b += (size_t)r.N * r.N * 8;
     if (r.opts.flags & RVF_AUDIT) b += (size_t)r.N * r.N * 8;

@@ -423,6 +423,56 @@ def cylinder(n_around: int = 128, n_along: int = 64, radius: float = 0.5, height
                     {1: "side", 2: "bottom", 3: "top"})
 
 
+def sphere_interior(n: int = 4, radius: float = 1.0, order: int = 2) -> MeshData:
+    """Interior of a sphere as a cubed sphere: 6 faces of ``n x n`` Quad8
+    (``order=2``) or Quad4 (``order=1``) elements whose nodes (mid-side nodes
+    too) lie on the sphere, shared between elements, inward normals; groups
+    1..6 by cube face.  Between two points of a sphere's interior the kernel
+    cos(t_i) cos(t_j) / (pi r^2) is 1 / (4 pi R^2), so F_ij -> A_j / (4 pi R^2)
+    for every pair, the element with itself included."""
+    node_id = {}
+    coords = []
+
+    def nid(p):  # p: integer point of the doubled grid [0, 2n]^3
+        k = node_id.get(p)
+        if k is None:
+            k = len(coords) + 1
+            node_id[p] = k
+            x = np.array(p, np.float64) / n - 1.0
+            coords.append(radius * x / np.sqrt(x @ x))
+        return k
+
+    elems, groups, names = [], [], {}
+    g = 0
+    for a in range(3):
+        b, c = (a + 1) % 3, (a + 2) % 3
+        for v in (0, 2 * n):
+            g += 1
+            names[g] = f"{'xyz'[a]}{'-' if v == 0 else '+'}"
+            for w in range(0, 2 * n, 2):
+                for u in range(0, 2 * n, 2):
+                    def P(uu, ww):
+                        q = [0, 0, 0]
+                        q[a] = v; q[b] = uu; q[c] = ww
+                        return nid(tuple(q))
+                    c1, c2, c3, c4 = P(u, w), P(u + 2, w), P(u + 2, w + 2), P(u, w + 2)
+                    m12, m23, m34, m41 = P(u + 1, w), P(u + 2, w + 1), P(u + 1, w + 2), P(u, w + 1)
+                    if v == 0:  # face at -1 along a: inward normal +e_a
+                        e = [c1, c2, c3, c4, m12, m23, m34, m41]
+                    else:
+                        e = [c2, c1, c4, c3, m12, m41, m34, m23]
+                    elems.append(e if order == 2 else e[:4] + [0, 0, 0, 0])
+                    groups.append(g)
+    if order == 1:  # drop the mid-side nodes
+        used = sorted({k for e in elems for k in e[:4]})
+        remap = {k: r + 1 for r, k in enumerate(used)}
+        coords = [coords[k - 1] for k in used]
+        elems = [[remap[k] for k in e[:4]] + [0, 0, 0, 0] for e in elems]
+    N = len(elems)
+    return MeshData(np.array(coords, np.float64), np.full(N, 0 if order == 2 else 2, np.int8),
+                    np.array(elems, np.int32), np.array(groups, np.int32), names)
+
+
 def random_box_enclosure(rng: np.random.Generator, n_elems: int, max_occluders: int = 3,
                          max_occluder_tris: int = 8000):
     """One serving request: a closed box (sides U[0.5, 2]) meshed with about
