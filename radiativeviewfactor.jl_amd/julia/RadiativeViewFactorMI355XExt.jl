@@ -26,6 +26,7 @@ Base.@kwdef struct MI355XBackend
     seed::UInt64      = 0x0
     device::Int       = 0
     parity::Symbol    = :reference   # :reference (Q1 soup reading) | :geometric
+    self_vf::Bool     = false        # the pairs (i, i) too (src/Assembly.jl:173,193); the hook drops the keyword
 end
 
 _gpu_array_type(::MI355XBackend) = Array
@@ -97,7 +98,7 @@ function rocm_solve(mesh::MeshData, nquad::Int, b::MI355XBackend;
     end
     method = monte_carlo ? Int32(1) : (b.use_duffy ? Int32(2) : Int32(0))
     opts = RvfOpts(Int32(nquad), method, b.precision === :mixed ? Int32(1) : Int32(0), Int32(n_samples),
-                   b.seed, Int32(1), Int32(0))
+                   b.seed, Int32(1) | (b.self_vf ? Int32(16) : Int32(0)), Int32(0))  # RVF_WANT_F | RVF_SELF_VF
     G = Ref{Int32}(0)
     _check(ccall((:rvf_mesh_info, LIB), Cint, (Ptr{Cvoid}, Ptr{Int64}, Ref{Int32}, Ptr{Int32}),
                  hmesh[], C_NULL, G, C_NULL))
