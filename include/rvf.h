@@ -43,6 +43,7 @@ extern "C" {
 #define RVF_SOUP_REFERENCE 0    /* AABB reads [xyz,vertex], Moller-Trumbore reads [vertex,xyz] */
 #define RVF_SOUP_XYZ_VERTEX 1   /* geometric reading of a [xyz,vertex,tri] soup (load_mesh) */
 #define RVF_SOUP_VERTEX_XYZ 2   /* geometric reading of a [vertex,xyz,tri] soup */
+#define RVF_SOUP_SEGMENTS 3     /* curve meshes: segment soup (3,2,T) [xyz,endpoint,seg] (src/MeshIO.jl:641-645) */
 
 #define RVF_WANT_F 1     /* materialise the N x N element matrix F on the device */
 #define RVF_AUDIT 2      /* per-pair blocked-ray count + hash (occluded solves) */
@@ -105,6 +106,15 @@ int rvf_version(void);
 int rvf_mesh_upload(rvf_ctx* ctx, const double* coords, int64_t n_nodes, int64_t n_elems,
                     const int8_t* family, const int32_t* elem_node_idx, const int32_t* elem_group,
                     const int32_t* const nodes[4], const int64_t n_per_family[4], rvf_mesh** out);
+/* curve meshes (mesh_dim 1): family 4 = Line3 [left, right, midpoint], 5 =
+ * Line2; line3_nodes 3 x n_line3, line2_nodes 2 x n_line2 column-major.  The
+ * reference computes these on the CPU only (src/Assembly.jl:83-86; kernel
+ * src/ViewFactorKernel.jl:169-180, segment occlusion src/BVH.jl:254-322);
+ * obstruction soups of a curve mesh are segment soups (RVF_SOUP_SEGMENTS). */
+int rvf_curve_mesh_upload(rvf_ctx* ctx, const double* coords, int64_t n_nodes, int64_t n_elems,
+                          const int8_t* family, const int32_t* elem_node_idx, const int32_t* elem_group,
+                          const int32_t* line3_nodes, int64_t n_line3, const int32_t* line2_nodes, int64_t n_line2,
+                          rvf_mesh** out);
 void rvf_mesh_free(rvf_mesh* mesh);
 int rvf_mesh_info(const rvf_mesh* mesh, int64_t* n_elems, int32_t* n_groups, int32_t* group_tags);
 
@@ -112,7 +122,8 @@ int rvf_mesh_info(const rvf_mesh* mesh, int64_t* n_elems, int32_t* n_groups, int
  * with the CPU path's semantics (src/Assembly.jl:119-143): one BVH per
  * distinct active set obstruction_groups \ {g_i, g_j}, built bit-identically
  * to build_bvh (src/BVH.jl:71-148).  soup: (3,3,T) column-major, triangles of
- * each group contiguous in ascending group order, tri_group: tag per triangle. */
+ * each group contiguous in ascending group order, tri_group: tag per triangle.
+ * Curve meshes: soup_read RVF_SOUP_SEGMENTS and a (3,2,T) segment soup. */
 int rvf_obs_build(rvf_ctx* ctx, const rvf_mesh* mesh, const double* soup, const int32_t* tri_group, int64_t T,
                   const int32_t* obstruction_groups, int32_t n_obs, int32_t soup_read, rvf_obs** out);
 void rvf_obs_free(rvf_obs* obs);

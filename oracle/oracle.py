@@ -38,6 +38,9 @@ def lib():
         L.orc_eval_point.argtypes = [C.c_int, P, P, C.c_double, C.c_double, P, P, P]
         L.orc_vf_kernel.argtypes = [P, P, P, P]
         L.orc_vf_kernel.restype = C.c_double
+        L.orc_vf_kernel_2d.argtypes = [P, P, P, P]
+        L.orc_vf_kernel_2d.restype = C.c_double
+        L.orc_seg_hits_brute.argtypes = [P, C.c_int64, P, P]
         L.orc_pair.argtypes = [P, P, P, C.c_int64, C.c_int64, C.c_int, P]
         L.orc_pair.restype = C.c_double
         L.orc_pair_duffy.argtypes = [P, P, P, C.c_int64, C.c_int64, C.c_int]
@@ -117,8 +120,15 @@ def pair_duffy(mesh, i, j, nquad):
     return lib().orc_pair_duffy(_p(mesh.coords), _p(mesh.family), _p(mesh.nodes), i, j, nquad)
 
 
+SEGMENTS = 3  # soup mode of a curve mesh's segment soup (T, 2, 3)
+
+
+def _soup(soup, mode):
+    return np.ascontiguousarray(soup, np.float64).reshape(-1, 2 if mode == SEGMENTS else 3, 3)
+
+
 def bvh_export(soup, mode):
-    soup = np.ascontiguousarray(soup, np.float64).reshape(-1, 3, 3)
+    soup = _soup(soup, mode)
     T = soup.shape[0]
     cap = max(4, 2 * T)
     box = np.zeros(6 * cap); meta = np.zeros(5 * cap, np.int32); tri = np.zeros(max(T, 1), np.int32)
@@ -135,12 +145,24 @@ def ray_hits(soup, mode, o, d, tmax, brute=False):
 
 
 def visible_batch(soup, mode, xi, xj):
-    soup = np.ascontiguousarray(soup, np.float64).reshape(-1, 3, 3)
+    soup = _soup(soup, mode)
     xi = np.ascontiguousarray(xi, np.float64).reshape(-1, 3)
     xj = np.ascontiguousarray(xj, np.float64).reshape(-1, 3)
     out = np.zeros(xi.shape[0], np.uint8)
     lib().orc_visible_batch(_p(soup), soup.shape[0], mode, _p(xi), _p(xj), xi.shape[0], _p(out))
     return out.astype(bool)
+
+
+def seg_hits_brute(soup, xi, xj):
+    """Any segment of a (T, 2, 3) soup blocks the 2-D path xi -> xj (no tree)."""
+    soup = _soup(soup, SEGMENTS)
+    xi = np.ascontiguousarray(xi, np.float64); xj = np.ascontiguousarray(xj, np.float64)
+    return bool(lib().orc_seg_hits_brute(_p(soup), soup.shape[0], _p(xi), _p(xj)))
+
+
+def vf_kernel_2d(xi, ni, xj, nj):
+    a = [np.ascontiguousarray(v, np.float64) for v in (xi, ni, xj, nj)]
+    return lib().orc_vf_kernel_2d(*[_p(v) for v in a])
 
 
 def xorshift_stream(seed, tid, n):

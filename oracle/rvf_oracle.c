@@ -28,6 +28,13 @@
  *                           non-Quad8 == standard quadrature.
  *   Golub-Welsch (nquad>5)  ulp-level parity unpinned (reference uses LAPACK
  *                           eigen); pinned to 1e-12 by runtests.jl:34-48.
+ *   curve meshes (Line2/3)  runtests.jl:241-247 (line2 shape) + Hottel's crossed
+ *                           strings (exact 2-D view factors between straight
+ *                           strips, with and without a blocking strip; arcs of a
+ *                           circle, self view factor 1 - chord/arc) + brute-force
+ *                           any-hit over all segments.
+ *   self_vf                 pairs unchanged, diagonal = pair(i, i), flat = 0,
+ *                           sphere closed form with O(h^2) convergence.
  */
 #include <math.h>
 #include <stdint.h>
@@ -182,7 +189,8 @@ int orc_tri_rule(int nquad, double* pts, double* wts) {
 
 /* ------------------------------------------------------------------------- */
 /* Shape functions.  Family codes as GPUKernels.jl:344-375:                  */
-/* 0 = Quad8 (:quad), 1 = Tri6 (:tri), 2 = Quad4, 3 = Tri3.                  */
+/* 0 = Quad8 (:quad), 1 = Tri6 (:tri), 2 = Quad4, 3 = Tri3; curve meshes     */
+/* (mesh_dim 1, CPU only in the reference) 4 = Line3, 5 = Line2.             */
 /* ------------------------------------------------------------------------- */
 
 /* quad8_shape — Geometry.jl:36-71 */
@@ -237,12 +245,26 @@ static void tri3_shape(double x, double e, double* N, double* dx, double* de) {
     dx[0] = -1.0; dx[1] = 1.0; dx[2] = 0.0;
     de[0] = -1.0; de[1] = 0.0; de[2] = 1.0;
 }
+/* line3_shape — Geometry.jl:226-235 (nodes: left, right, midpoint) */
+static void line3_shape(double x, double* N, double* dx) {
+    N[0] = ((-0.5) * x) * (1.0 - x);
+    N[1] = (0.5 * x) * (1.0 + x);
+    N[2] = 1.0 - x * x;
+    dx[0] = -0.5 + x; dx[1] = 0.5 + x; dx[2] = (-2.0) * x;
+}
+/* line2_shape — Geometry.jl:292-294 */
+static void line2_shape(double x, double* N, double* dx) {
+    N[0] = 0.5 * (1.0 - x); N[1] = 0.5 * (1.0 + x);
+    dx[0] = -0.5; dx[1] = 0.5;
+}
 int orc_shape(int fam, double x, double e, double* N, double* dx, double* de) {
     switch (fam) {
     case 0: quad8_shape(x, e, N, dx, de); return 8;
     case 1: tri6_shape(x, e, N, dx, de); return 6;
     case 2: quad4_shape(x, e, N, dx, de); return 4;
     case 3: tri3_shape(x, e, N, dx, de); return 3;
+    case 4: line3_shape(x, N, dx); return 3;
+    case 5: line2_shape(x, N, dx); return 2;
     }
     return -1;
 }
@@ -254,6 +276,24 @@ void orc_eval_point(int fam, const double* coords, const int32_t* nodes,
                     double xi, double eta, double* x, double* n, double* dA) {
     double N[8], dx[8], de[8];
     int nn = orc_shape(fam, xi, eta, N, dx, de);
+    if (fam >= 4) {
+        /* line*_physical_point + line*_normal_and_length_element (Geometry.jl:242-282,
+         * 302-332): x = sum N_a x_a, dL = norm(dx/dxi), t = (dx/dxi)/dL, n = (-t_y, t_x, 0) */
+        double p0 = 0.0, p1 = 0.0, p2 = 0.0, a0 = 0.0, a1 = 0.0, a2 = 0.0;
+        for (int a = 0; a < nn; a++) {
+            const double* xa = coords + 3 * (int64_t)(nodes[a] - 1);
+            p0 = p0 + N[a] * xa[0]; p1 = p1 + N[a] * xa[1]; p2 = p2 + N[a] * xa[2];
+        }
+        for (int a = 0; a < nn; a++) {
+            const double* xa = coords + 3 * (int64_t)(nodes[a] - 1);
+            a0 = a0 + dx[a] * xa[0]; a1 = a1 + dx[a] * xa[1]; a2 = a2 + dx[a] * xa[2];
+        }
+        double d = sqrt((a0 * a0 + a1 * a1) + a2 * a2);
+        x[0] = p0; x[1] = p1; x[2] = p2;
+        n[0] = -(a1 / d); n[1] = a0 / d; n[2] = 0.0;
+        *dA = d;
+        return;
+    }
     double p0 = 0.0, p1 = 0.0, p2 = 0.0;
     double a0 = 0.0, a1 = 0.0, a2 = 0.0, b0 = 0.0, b1 = 0.0, b2 = 0.0;
     for (int a = 0; a < nn; a++) {
@@ -285,6 +325,19 @@ double orc_vf_kernel(const double* xi, const double* ni, const double* xj, const
     return (ci * cj) / (3.141592653589793 * rr);
 }
 
+/* vf_kernel_2d — ViewFactorKernel.jl:169-180 (per unit depth) */
+double orc_vf_kernel_2d(const double* xi, const double* ni, const double* xj, const double* nj) {
+    double r0 = xj[0] - xi[0], r1 = xj[1] - xi[1], r2 = xj[2] - xi[2];
+    double rr = (r0 * r0 + r1 * r1) + r2 * r2;
+    if (rr < 1e-30) return 0.0;
+    double r = sqrt(rr);
+    double h0 = r0 / r, h1 = r1 / r, h2 = r2 / r;
+    double ci = (ni[0] * h0 + ni[1] * h1) + ni[2] * h2;
+    double cj = (nj[0] * (-h0) + nj[1] * (-h1)) + nj[2] * (-h2);
+    if (ci <= 0.0 || cj <= 0.0) return 0.0;
+    return (ci * cj) / (2.0 * r);
+}
+
 /* ------------------------------------------------------------------------- */
 /* BVH — src/BVH.jl:71-250 (build + stack traversal), GPUBVH.jl:69-82        */
 /* (miss links, used only for the canonical stackless test counts).          */
@@ -297,8 +350,9 @@ typedef struct {
 typedef struct {
     OrcNode* nodes; int nn, cap;
     int* tri_idx;          /* 1-based permutation, length T */
-    const double* soup;    /* 9*T, Julia (3,3,T) column-major */
+    const double* soup;    /* 3*nv*T, Julia (3,3,T) or segment (3,2,T) column-major */
     int64_t T;
+    int nv;                /* vertices per item: 3 (triangles) or 2 (segments, [xyz,endpoint]) */
     int aabb_v, aabb_c;    /* strides for AABB/centroid read: coord c of vertex v at c*aabb_c + v*aabb_v */
     int mt_v, mt_c;        /* strides for Moller-Trumbore read */
     int* miss;             /* 1-based miss links (0 = end) */
@@ -312,26 +366,26 @@ static void set_read(OrcBVH* b, int mode) {
     else { b->aabb_v = 1; b->aabb_c = 3; b->mt_v = 1; b->mt_c = 3; }
 }
 static inline double sa(const OrcBVH* b, int64_t t, int v, int c) {
-    return b->soup[9 * t + v * b->aabb_v + c * b->aabb_c];
+    return b->soup[3 * b->nv * t + v * b->aabb_v + c * b->aabb_c];
 }
 static inline double sm(const OrcBVH* b, int64_t t, int v, int c) {
     return b->soup[9 * t + v * b->mt_v + c * b->mt_c];
 }
 
-/* _triangle_aabb — BVH.jl:81-94 */
+/* _triangle_aabb — BVH.jl:81-94 (also segment soups: nv endpoints) */
 static void tri_aabb(const OrcBVH* b, int64_t t, double* lo, double* hi) {
     for (int c = 0; c < 3; c++) { lo[c] = sa(b, t, 0, c); hi[c] = lo[c]; }
-    for (int v = 1; v < 3; v++)
+    for (int v = 1; v < b->nv; v++)
         for (int c = 0; c < 3; c++) {
             double x = sa(b, t, v, c);
             lo[c] = jl_min(lo[c], x); hi[c] = jl_max(hi[c], x);
         }
 }
-/* _centroid — BVH.jl:100-109: ((0 + x1) + x2) + x3, then / 3 */
+/* _centroid — BVH.jl:100-109: ((0 + x1) + x2) + x3, then / nv */
 static double centroid(const OrcBVH* b, int64_t t, int axis) {
     double s = 0.0;
-    for (int v = 0; v < 3; v++) s += sa(b, t, v, axis);
-    return s / 3.0;
+    for (int v = 0; v < b->nv; v++) s += sa(b, t, v, axis);
+    return s / (double)b->nv;
 }
 
 static int push_node(OrcBVH* b, OrcNode nd) {
@@ -402,9 +456,10 @@ static void assign_miss(OrcBVH* b, int node, int next) {
     }
 }
 
-static OrcBVH* bvh_build(const double* soup, int64_t T, int mode) {
+/* nv = 2: segment soup (3,2,T), read [xyz, endpoint] everywhere (MeshIO.jl:641-645, BVH.jl:307) */
+static OrcBVH* bvh_build(const double* soup, int64_t T, int mode, int nv) {
     OrcBVH* b = (OrcBVH*)calloc(1, sizeof(OrcBVH));
-    b->soup = soup; b->T = T; set_read(b, mode);
+    b->soup = soup; b->T = T; b->nv = nv; set_read(b, nv == 2 ? 1 : mode);
     b->tri_idx = (int*)malloc(sizeof(int) * (T ? T : 1));
     for (int64_t t = 0; t < T; t++) b->tri_idx[t] = (int)(t + 1);
     int* tmpi = (int*)malloc(sizeof(int) * (T ? T : 1));
@@ -511,8 +566,14 @@ static int intersect_stackless_count(const OrcBVH* b, const double* o, const dou
     return 0;
 }
 
-/* is_visible — RayCast.jl:20-32 (mesh_dim = 2) */
+static int intersect_seg_bvh(const OrcBVH* b, const double* xi, const double* xj, OrcCounters* cnt);
+
+/* is_visible — RayCast.jl:20-32 */
 static int is_visible(const OrcBVH* b, const double* xi, const double* xj, OrcCounters* cnt) {
+    if (b->nv == 2) { /* mesh_dim = 1 */
+        if (cnt) cnt->rays++;
+        return !intersect_seg_bvh(b, xi, xj, cnt);
+    }
     double d[3] = {xj[0] - xi[0], xj[1] - xi[1], xj[2] - xi[2]};
     double L = sqrt((d[0] * d[0] + d[1] * d[1]) + d[2] * d[2]);
     if (L < DBL_EPSILON) return 1;
@@ -522,6 +583,47 @@ static int is_visible(const OrcBVH* b, const double* xi, const double* xj, OrcCo
         return !intersect_stackless_count(b, xi, dir, L, cnt);
     }
     return !intersect_ray_bvh(b, xi, dir, L);
+}
+
+/* _seg_blocks_ray — BVH.jl:264-277 (tol = 1e-8; xy only) */
+static inline int seg_blocks_ray(const double* xi, const double* xj, const double* v0, const double* v1) {
+    const double tol = 1e-8;
+    double dx = xj[0] - xi[0], dy = xj[1] - xi[1];
+    double ex = v1[0] - v0[0], ey = v1[1] - v0[1];
+    double denom = dx * ey - dy * ex;
+    if (fabs(denom) < tol) return 0;
+    double fx = v0[0] - xi[0], fy = v0[1] - xi[1];
+    double s = (fx * ey - fy * ex) / denom;
+    double t = (fx * dy - fy * dx) / denom;
+    return (tol < s && s < 1.0 - tol) && (tol < t && t < 1.0 - tol);
+}
+
+/* intersect_seg_bvh — BVH.jl:286-320 (stack: push left then right; 2-D box
+ * overlap with the ray's bounding box) */
+static int intersect_seg_bvh(const OrcBVH* b, const double* xi, const double* xj, OrcCounters* cnt) {
+    if (b->nn == 0) return 0;
+    double xlo = jl_min(xi[0], xj[0]), xhi = jl_max(xi[0], xj[0]);
+    double ylo = jl_min(xi[1], xj[1]), yhi = jl_max(xi[1], xj[1]);
+    int stack[128]; int sp = 0;
+    stack[sp++] = 1;
+    while (sp > 0) {
+        const OrcNode* nd = &b->nodes[stack[--sp] - 1];
+        if (cnt) cnt->aabb++;
+        if (nd->lo[0] > xhi || nd->hi[0] < xlo || nd->lo[1] > yhi || nd->hi[1] < ylo) continue;
+        if (nd->left == 0) {
+            for (int k = nd->tri_start; k <= nd->tri_start + nd->tri_count - 1; k++) {
+                int64_t t = b->tri_idx[k - 1] - 1;
+                const double* s = b->soup + 6 * t;
+                double v0[3] = {s[0], s[1], 0.0}, v1[3] = {s[3], s[4], 0.0};
+                if (cnt) cnt->tri++;
+                if (seg_blocks_ray(xi, xj, v0, v1)) return 1;
+            }
+        } else {
+            stack[sp++] = nd->left;
+            stack[sp++] = nd->right;
+        }
+    }
+    return 0;
 }
 
 /* ------------------------------------------------------------------------- */
@@ -534,13 +636,29 @@ typedef struct {
     const int32_t* nodes;   /* 8*N, 1-based node ids, padded */
     const int32_t* group;   /* N */
     int nquad;
-    int nq_quad, nq_tri;
+    int nq_quad, nq_tri, nq_line;
     double qp[2 * ORC_MAXPTS], qw[ORC_MAXPTS];
     double tp[2 * 13], tw[13];
+    double lp[8], lw[8];
+    int dim2;               /* curve mesh (mesh_dim 1): Line3/Line2 elements, segment occluders */
 } OrcMesh;
+
+static void mesh_rules(OrcMesh* m, int nquad) {
+    m->nquad = nquad;
+    m->nq_quad = orc_quad_rule(nquad, m->qp, m->qw);
+    m->nq_tri = orc_tri_rule(nquad, m->tp, m->tw);
+    m->nq_line = orc_gl1d(nquad, m->lp, m->lw);
+}
 
 static int quad_points(const OrcMesh* m, int64_t e, double* xs, double* ns, double* dA, double* w) {
     int f = m->fam[e];
+    if (f >= 4) { /* _quad_points, curve branch (ViewFactorKernel.jl:284-301): 1-D Gauss-Legendre */
+        for (int k = 0; k < m->nq_line; k++) {
+            orc_eval_point(f, m->coords, m->nodes + 8 * e, m->lp[k], 0.0, xs + 3 * k, ns + 3 * k, dA + k);
+            w[k] = m->lw[k];
+        }
+        return m->nq_line;
+    }
     int isq = (f == 0 || f == 2);
     int np = isq ? m->nq_quad : m->nq_tri;
     const double* pts = isq ? m->qp : m->tp;
@@ -557,7 +675,7 @@ static inline uint32_t mix32(uint32_t x) {
     return x;
 }
 
-/* element_pair_view_factor — ViewFactorKernel.jl:199-241 (mesh_dim = 2) */
+/* element_pair_view_factor — ViewFactorKernel.jl:199-241 */
 static double pair_vf(const OrcMesh* m, int64_t i, int64_t j, const OrcBVH* bvh, double* Li,
                       uint32_t* bl_count, uint32_t* bl_hash, OrcCounters* cnt) {
     double xi[3 * ORC_MAXPTS], ni[3 * ORC_MAXPTS], ai[ORC_MAXPTS], wi[ORC_MAXPTS];
@@ -570,7 +688,8 @@ static double pair_vf(const OrcMesh* m, int64_t i, int64_t j, const OrcBVH* bvh,
         L += wi[p] * ai[p];
         double inner = 0.0;
         for (int q = 0; q < npj; q++) {
-            double K = orc_vf_kernel(xi + 3 * p, ni + 3 * p, xj + 3 * q, nj + 3 * q);
+            double K = m->dim2 ? orc_vf_kernel_2d(xi + 3 * p, ni + 3 * p, xj + 3 * q, nj + 3 * q)
+                               : orc_vf_kernel(xi + 3 * p, ni + 3 * p, xj + 3 * q, nj + 3 * q);
             if (K == 0.0) continue;
             if (bvh && !is_visible(bvh, xi + 3 * p, xj + 3 * q, cnt)) {
                 bc++; bh += mix32((uint32_t)(p * npj + q) + 0x9e3779b9U);
@@ -760,13 +879,40 @@ uint64_t orc_init_rng(uint64_t seed, int64_t tid) {
 static void mc_sample(const OrcMesh* m, int64_t e, double u1, double u2, double* x, double* n, double* dA) {
     int f = m->fam[e];
     double xi, eta;
-    if (f == 0 || f == 2) { xi = 2.0 * u1 - 1.0; eta = 2.0 * u2 - 1.0; }
+    if (f >= 4) { xi = u1; eta = 0.0; } /* curve elements: u1 is the point on [-1, 1] */
+    else if (f == 0 || f == 2) { xi = 2.0 * u1 - 1.0; eta = 2.0 * u2 - 1.0; }
     else { xi = u1; eta = u2; if (xi + eta > 1.0) { xi = 1.0 - xi; eta = 1.0 - eta; } }
     orc_eval_point(f, m->coords, m->nodes + 8 * e, xi, eta, x, n, dA);
 }
 /* per pair raw (and the element-i area estimate), pair (i,j) 0-based, tid = i*N + j + 1 */
+/* Curve meshes (no reference GPU stream exists for them): the same per-pair
+ * stream; sample k of both elements lies in stratum k of n equal strata of
+ * [-1, 1], as _stratified_line_points (MCKernel.jl:107-115) places them:
+ * xi = -1 + ((k + u) * 2) / n, with u1 for element i and u3 for element j. */
+static double mc_pair_line(const OrcMesh* m, int64_t i, int64_t j, int nsamp, uint64_t seed, const OrcBVH* bvh,
+                           double* Ai_out) {
+    uint64_t st = orc_init_rng(seed, i * m->N + j + 1);
+    double Ai = 0.0, Ks = 0.0;
+    for (int k = 0; k < nsamp; k++) {
+        double u1 = xorshift64(&st), u2 = xorshift64(&st);
+        (void)u2;
+        double xi[3], ni[3], ai, xj[3], nj[3], aj;
+        mc_sample(m, i, -1.0 + ((k + u1) * 2.0) / nsamp, 0.0, xi, ni, &ai);
+        Ai += ai;
+        double u3 = xorshift64(&st), u4 = xorshift64(&st);
+        (void)u4;
+        mc_sample(m, j, -1.0 + ((k + u3) * 2.0) / nsamp, 0.0, xj, nj, &aj);
+        double K = orc_vf_kernel_2d(xi, ni, xj, nj);
+        if (K > 0.0 && bvh && !is_visible(bvh, xi, xj, 0)) K = 0.0;
+        Ks += (K * ai) * aj;
+    }
+    if (Ai_out) *Ai_out = (Ai * 2.0) / nsamp;
+    return ((Ks * 2.0) * 2.0) / nsamp;
+}
+
 static double mc_pair(const OrcMesh* m, int64_t i, int64_t j, int nsamp, uint64_t seed, const OrcBVH* bvh,
                       double* Ai_out) {
+    if (m->dim2) return mc_pair_line(m, i, j, nsamp, seed, bvh, Ai_out);
     uint64_t st = orc_init_rng(seed, i * m->N + j + 1);
     int s = (int)floor(sqrt((double)nsamp));
     double Ai = 0.0, Aj = 0.0, Ks = 0.0;
@@ -828,10 +974,13 @@ int orc_compute(const double* coords, int64_t Nn, int64_t N, const int8_t* fam, 
     (void)Nn;
     OrcMesh m;
     memset(&m, 0, sizeof m);
-    m.coords = coords; m.N = N; m.fam = fam; m.nodes = nodes8; m.group = group; m.nquad = nquad;
-    m.nq_quad = orc_quad_rule(nquad, m.qp, m.qw);
-    m.nq_tri = orc_tri_rule(nquad, m.tp, m.tw);
+    m.coords = coords; m.N = N; m.fam = fam; m.nodes = nodes8; m.group = group;
+    mesh_rules(&m, nquad);
     if (m.nq_quad < 0) return -1;
+    /* mesh_dim 1: every element a curve element; occluders are segment soups (3,2,T) */
+    m.dim2 = N > 0 && fam[0] >= 4;
+    for (int64_t e = 0; e < N; e++) if ((fam[e] >= 4) != m.dim2) return -2;
+    const int nv = m.dim2 ? 2 : 3;
 #ifdef _OPENMP
     if (nthreads > 0) omp_set_num_threads(nthreads);
 #endif
@@ -887,12 +1036,15 @@ int orc_compute(const double* coords, int64_t Nn, int64_t N, const int8_t* fam, 
                         for (int64_t t = 0; t < T; t++) if (soup_group[t] == act[g]) tot++;
                     as->merged = NULL; as->bvh = NULL;
                     if (tot > 0) {
-                        as->merged = (double*)malloc(sizeof(double) * 9 * tot);
+                        as->merged = (double*)malloc(sizeof(double) * 3 * nv * tot);
                         int64_t w = 0;
                         for (int g = 0; g < na; g++)
                             for (int64_t t = 0; t < T; t++)
-                                if (soup_group[t] == act[g]) { memcpy(as->merged + 9 * w, soup + 9 * t, 9 * sizeof(double)); w++; }
-                        as->bvh = bvh_build(as->merged, tot, soup_mode);
+                                if (soup_group[t] == act[g]) {
+                                    memcpy(as->merged + 3 * nv * w, soup + 3 * nv * t, 3 * nv * sizeof(double));
+                                    w++;
+                                }
+                        as->bvh = bvh_build(as->merged, tot, soup_mode, nv);
                     }
                     found = nsets++;
                 }
@@ -966,16 +1118,19 @@ double orc_pair(const double* coords, const int8_t* fam, const int32_t* nodes8, 
                 double* Li) {
     OrcMesh m;
     memset(&m, 0, sizeof m);
-    m.coords = coords; m.fam = fam; m.nodes = nodes8; m.nquad = nquad; m.N = (i > j ? i : j) + 1;
-    m.nq_quad = orc_quad_rule(nquad, m.qp, m.qw);
-    m.nq_tri = orc_tri_rule(nquad, m.tp, m.tw);
+    m.coords = coords; m.fam = fam; m.nodes = nodes8; m.N = (i > j ? i : j) + 1;
+    mesh_rules(&m, nquad);
+    m.dim2 = fam[i] >= 4;
     return pair_vf(&m, i, j, 0, Li, 0, 0, 0);
 }
+
+/* Test entries take mode 0..2 (triangle soup readings) or 3 (segment soup (3,2,T)). */
+static int mode_nv(int mode) { return mode == 3 ? 2 : 3; }
 
 /* Build a BVH and export it: nodes (lo3, hi3 as 6 doubles; left,right,start,count,miss as 5 ints),
  * tri_idx (1-based).  Returns node count, or -(needed) if cap is too small. */
 int orc_bvh_export(const double* soup, int64_t T, int mode, int cap, double* box, int32_t* meta, int32_t* tri_idx) {
-    OrcBVH* b = bvh_build(soup, T, mode);
+    OrcBVH* b = bvh_build(soup, T, mode, mode_nv(mode));
     int nn = b->nn;
     if (nn > cap) { bvh_free(b); return -nn; }
     for (int k = 0; k < nn; k++) {
@@ -991,7 +1146,7 @@ int orc_bvh_export(const double* soup, int64_t T, int mode, int cap, double* box
 
 /* intersect_ray_bvh on a freshly built tree (test entry).  mode as above. */
 int orc_ray_hits(const double* soup, int64_t T, int mode, const double* o, const double* d, double tmax) {
-    OrcBVH* b = bvh_build(soup, T, mode);
+    OrcBVH* b = bvh_build(soup, T, mode, 3);
     int r = intersect_ray_bvh(b, o, d, tmax);
     bvh_free(b);
     return r;
@@ -1013,9 +1168,19 @@ int orc_ray_hits_brute(const double* soup, int64_t T, int mode, const double* o,
 /* is_visible on a freshly built tree, batch of rays (test entry) */
 int orc_visible_batch(const double* soup, int64_t T, int mode, const double* xi, const double* xj, int64_t n,
                       uint8_t* out) {
-    OrcBVH* b = bvh_build(soup, T, mode);
+    OrcBVH* b = bvh_build(soup, T, mode, mode_nv(mode));
     for (int64_t k = 0; k < n; k++) out[k] = (uint8_t)is_visible(b, xi + 3 * k, xj + 3 * k, 0);
     bvh_free(b);
+    return 0;
+}
+
+/* brute force: does any segment of a (3,2,T) soup block the 2-D path xi -> xj (test entry) */
+int orc_seg_hits_brute(const double* soup, int64_t T, const double* xi, const double* xj) {
+    for (int64_t t = 0; t < T; t++) {
+        const double* s = soup + 6 * t;
+        double v0[3] = {s[0], s[1], 0.0}, v1[3] = {s[3], s[4], 0.0};
+        if (seg_blocks_ray(xi, xj, v0, v1)) return 1;
+    }
     return 0;
 }
 
@@ -1024,9 +1189,9 @@ double orc_pair_duffy(const double* coords, const int8_t* fam, const int32_t* no
                       int nquad) {
     OrcMesh m;
     memset(&m, 0, sizeof m);
-    m.coords = coords; m.fam = fam; m.nodes = nodes8; m.nquad = nquad; m.N = (i > j ? i : j) + 1;
-    m.nq_quad = orc_quad_rule(nquad, m.qp, m.qw);
-    m.nq_tri = orc_tri_rule(nquad, m.tp, m.tw);
+    m.coords = coords; m.fam = fam; m.nodes = nodes8; m.N = (i > j ? i : j) + 1;
+    mesh_rules(&m, nquad);
+    m.dim2 = fam[i] >= 4;
     return pair_duffy(&m, i, j, 0);
 }
 

@@ -85,6 +85,7 @@ def lib():
         "rvf_last_error": ([], C.c_char_p),
         "rvf_version": ([], C.c_int),
         "rvf_mesh_upload": ([P, P, I64, I64, P, P, P, P, P, PP], C.c_int),
+        "rvf_curve_mesh_upload": ([P, P, I64, I64, P, P, P, P, I64, P, I64, PP], C.c_int),
         "rvf_mesh_free": ([P], None),
         "rvf_mesh_info": ([P, P, P, P], C.c_int),
         "rvf_obs_build": ([P, P, P, P, I64, P, I32, I32, PP], C.c_int),

@@ -45,6 +45,7 @@ MIXED_VARIANTS = (VARIANT_LANE, VARIANT_QUEUE(2), VARIANT_QUEUE(3), VARIANT_QUEU
 EXACT_COUNT_VARIANTS = (VARIANT_LANE, VARIANT_QUEUE(1), VARIANT_QUEUE(2), VARIANT_QUEUE(3))
 
 SOUP_READ = {"reference": 0, ("geometric", "xyz_vertex"): 1, ("geometric", "vertex_xyz"): 2}
+SOUP_SEGMENTS = 3
 
 
 @dataclass
@@ -125,31 +126,37 @@ class Context:
 
 def family_matrices(mesh: MeshData):
     """Per-family node matrices and 1-based elem_node_idx, as build_gpu_arrays
-    (src/GPUKernels.jl:344-385) lays them out."""
-    nn = (8, 6, 4, 3)
-    mats, counts = [], np.zeros(4, np.int64)
+    (src/GPUKernels.jl:344-385) lays them out; curve meshes: Line3, Line2."""
+    nn = (8, 6, 4, 3, 3, 2)
+    fams = (4, 5) if mesh.mesh_dim == 1 else (0, 1, 2, 3)
+    mats, counts = [], np.zeros(len(fams), np.int64)
     idx = np.zeros(mesh.n_elems, np.int32)
-    for f in range(4):
+    for k, f in enumerate(fams):
         sel = np.nonzero(mesh.family == f)[0]
-        counts[f] = sel.size
+        counts[k] = sel.size
         idx[sel] = np.arange(1, sel.size + 1, dtype=np.int32)
         mats.append(np.ascontiguousarray(mesh.nodes[sel, :nn[f]], dtype=np.int32))
     return mats, counts, idx
 
 
 class DeviceMesh:
+    """A mesh in device memory: surface elements, or a curve mesh (mesh_dim
+    1, Line3/Line2, which the reference computes on the CPU only)."""
+
     def __init__(self, ctx: Context, mesh: MeshData):
-        if mesh.mesh_dim == 1:
-            raise ValueError("GPU backend does not support curve meshes (mesh_dim=1). "
-                             "Use the CPU backend for 2D per-unit-depth view factors.")
         self.ctx = ctx
         self.mesh = mesh
         mats, counts, idx = family_matrices(mesh)
         self._keep = (mats, counts, idx)
-        arr = (C.c_void_p * 4)(*[m.ctypes.data if m.size else None for m in mats])
         h = C.c_void_p()
-        check(_lib.lib().rvf_mesh_upload(ctx.h, ptr(mesh.coords), mesh.n_nodes, mesh.n_elems, ptr(mesh.family),
-                                         ptr(idx), ptr(mesh.group), arr, ptr(counts), C.byref(h)))
+        if mesh.mesh_dim == 1:
+            check(_lib.lib().rvf_curve_mesh_upload(ctx.h, ptr(mesh.coords), mesh.n_nodes, mesh.n_elems,
+                                                   ptr(mesh.family), ptr(idx), ptr(mesh.group), ptr(mats[0]),
+                                                   int(counts[0]), ptr(mats[1]), int(counts[1]), C.byref(h)))
+        else:
+            arr = (C.c_void_p * 4)(*[m.ctypes.data if m.size else None for m in mats])
+            check(_lib.lib().rvf_mesh_upload(ctx.h, ptr(mesh.coords), mesh.n_nodes, mesh.n_elems, ptr(mesh.family),
+                                             ptr(idx), ptr(mesh.group), arr, ptr(counts), C.byref(h)))
         self.h = h
         G = C.c_int32()
         _lib.lib().rvf_mesh_info(self.h, None, C.byref(G), None)
@@ -177,9 +184,12 @@ class DeviceObstruction:
             soup = np.ascontiguousarray(np.concatenate([mesh.group_tri_soup[t] for t in tags]), np.float64)
             tg = np.concatenate([np.full(mesh.group_tri_soup[t].shape[0], t, np.int32) for t in tags])
         else:
-            soup = np.zeros((0, 3, 3)); tg = np.zeros(0, np.int32)
+            soup = np.zeros((0, mesh.soup_vertices, 3)); tg = np.zeros(0, np.int32)
         obs = np.ascontiguousarray(list(obstruction_groups), np.int32)
-        mode = SOUP_READ["reference"] if parity == "reference" else SOUP_READ[("geometric", mesh.soup_layout)]
+        if mesh.mesh_dim == 1:  # segment soups read one way only (src/BVH.jl:81-109, 307)
+            mode = SOUP_SEGMENTS
+        else:
+            mode = SOUP_READ["reference"] if parity == "reference" else SOUP_READ[("geometric", mesh.soup_layout)]
         self._keep = (soup, tg, obs)
         h = C.c_void_p()
         check(_lib.lib().rvf_obs_build(ctx.h, dmesh.h, ptr(soup), ptr(tg), soup.shape[0], ptr(obs), len(obs), mode,
@@ -270,6 +280,8 @@ class Server:
 
     def submit(self, mesh: MeshData, nquad: int, obstruction_groups: Sequence[int] = (), parity: str = "reference",
                precision: str = "fp64", want_F: bool = False, variant: int = 0, self_vf: bool = False) -> int:
+        if mesh.mesh_dim == 1:
+            raise ValueError("the server takes surface meshes; solve curve meshes with Plan or compute_view_factors")
         mats, counts, idx = family_matrices(mesh)
         arr = (C.c_void_p * 4)(*[m.ctypes.data if m.size else None for m in mats])
         tags = sorted(set(int(g) for g in obstruction_groups if int(g) in mesh.group_tri_soup))

@@ -6,6 +6,8 @@
 #define RVF_STRIP 64          // i-elements per pair tile (one per lane of a wavefront)
 #define RVF_CHUNK 16          // max j-elements per pair tile (group-pure column chunk)
 #define RVF_MAXPTS 64         // quadrature points per element (nquad <= 8 quads)
+#define RVF_RULE_LINE (3 * 64 + 39)  // offset of the 1-D rule in the precompute rule table
+#define RVF_RULE_SIZE (RVF_RULE_LINE + 24)
 
 // Flattened BVH node, FP64 (64 B).  Preorder layout: an interior node's left
 // child is the next node; `miss` is the DFS successor (-1 = end of tree).
@@ -73,7 +75,8 @@ struct __align__(16) Item64 {
     X(MIX_QUEUE2, mixed_occ_queue2) X(MIX_QUEUE3, mixed_occ_queue3) X(MIX_QUEUE4, mixed_occ_queue4)        \
     X(MIX_QUEUE6, mixed_occ_queue6) X(FP64_SPLIT1, fp64_occ_split1) X(FP64_SPLIT2, fp64_occ_split2)         \
     X(FP64_SPLIT3, fp64_occ_split3) X(MIX_SPLIT1, mixed_occ_split1) X(MIX_SPLIT2, mixed_occ_split2)         \
-    X(MIX_SPLIT3, mixed_occ_split3)
+    X(MIX_SPLIT3, mixed_occ_split3) X(FP64_2D, fp64_curve) X(MIX_2D, mixed_curve)                           \
+    X(FP64_OCC2D, fp64_curve_occ) X(MIX_OCC2D, mixed_curve_occ)
 enum RvfPairKind {
 #define RVF_KIND_ENUM(e, t) RVF_K_##e,
     RVF_PAIR_KINDS(RVF_KIND_ENUM)
@@ -148,6 +151,7 @@ struct PairArgs {
     const int8_t* fam;        // N
     const int32_t* nodes8;    // 8 x N, 1-based
     int duffy;                // skip Quad8 pairs sharing a corner (done by k_duffy)
+    int dim2;                 // curve mesh (mesh_dim 1): 2-D kernel, segment occluders in nodes64 / tris64
     int self_vf;              // 1: the pairs (i, i) too, raw value to diag (not to part / partT)
     double* diag;             // [NS] raw_ii per slot (self_vf)
     int mc_nsamp, mc_s;       // Monte Carlo samples per pair, strata per side
@@ -164,8 +168,8 @@ struct PrecompArgs {
     const int32_t* nodes8;     // N*8, 1-based
     int64_t N;
     int NS, maxpts;
-    int nq_quad, nq_tri;
-    const double* rule;        // [quad pts 2*64][quad w 64][tri pts 2*13][tri w 13]
+    int nq_quad, nq_tri, nq_line;
+    const double* rule;        // [quad pts 2*64][quad w 64][tri pts 2*13][tri w 13][line pts (xi,0) 2*8][line w 8]
     double* P64; double* Q64; float* P32; float* Q32; double* anchor; double* area;
     int32_t* elem_np;
     int* bad;                  // set to 1 if any dA is 0 or not finite

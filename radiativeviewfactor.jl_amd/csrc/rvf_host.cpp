@@ -208,6 +208,7 @@ struct rvf_mesh {
     std::vector<int2> fill_tiles;
     double Sm = 1.0;               // max |coordinate|
     mutable Layout lay[4];         // bit 0: spatial order; bit 1: tiles for the pairs (i, i) too
+    bool dim2 = false;             // curve mesh (Line3 / Line2, mesh_dim 1)
     DBuf coords, d_fam, d_nodes8, d_grp, d_fill_tiles;
 };
 
@@ -232,7 +233,7 @@ static int get_layout(const rvf_mesh* m, bool spatial, bool self, const Layout**
                 for (int c = 0; c < 3; c++) { lo[c] = std::min(lo[c], m->hcoords[3 * k + c]); hi[c] = std::max(hi[c], m->hcoords[3 * k + c]); }
             std::vector<uint64_t> key(N);
             for (int64_t e = 0; e < N; e++) {
-                int nc = (m->fam[e] == 0 || m->fam[e] == 2) ? 4 : 3;
+                int nc = m->fam[e] >= 4 ? 2 : (m->fam[e] == 0 || m->fam[e] == 2) ? 4 : 3;
                 uint32_t q[3];
                 for (int c = 0; c < 3; c++) {
                     double x = 0;
@@ -285,15 +286,17 @@ static int get_layout(const rvf_mesh* m, bool spatial, bool self, const Layout**
     return 0;
 }
 
-int rvf_mesh_upload(rvf_ctx* ctx, const double* coords, int64_t n_nodes, int64_t n_elems, const int8_t* family,
-                    const int32_t* elem_node_idx, const int32_t* elem_group, const int32_t* const nodes[4],
-                    const int64_t n_per_family[4], rvf_mesh** out) {
+// Families 0..3 (surface meshes) or 4..5 (curve meshes), never both: nodes[f]
+// holds NNF[f] x n_per_family[f] node ids for family f.
+static int mesh_upload(rvf_ctx* ctx, const double* coords, int64_t n_nodes, int64_t n_elems, const int8_t* family,
+                       const int32_t* elem_node_idx, const int32_t* elem_group, const int32_t* const nodes[6],
+                       const int64_t n_per_family[6], bool curve, rvf_mesh** out) {
     if (!ctx || !out || n_elems < 0 || n_nodes < 0) return fail(RVF_E_ARG, "rvf_mesh_upload: bad arguments");
     if (n_elems > ((int64_t)1 << 30)) return fail(RVF_E_ARG, "rvf_mesh_upload: too many elements");
     HIPCHK(hipSetDevice(ctx->device));
-    const int NNF[4] = {8, 6, 4, 3};
+    const int NNF[6] = {8, 6, 4, 3, 3, 2};
     rvf_mesh* m = new rvf_mesh();
-    m->ctx = ctx; m->N = n_elems; m->Nn = n_nodes;
+    m->ctx = ctx; m->N = n_elems; m->Nn = n_nodes; m->dim2 = curve;
     m->fam.assign(family, family + n_elems);
     m->nodes8.assign(8 * n_elems, 0);
     double Sm = 0.0;
@@ -304,7 +307,12 @@ int rvf_mesh_upload(rvf_ctx* ctx, const double* coords, int64_t n_nodes, int64_t
     m->Sm = Sm > 0 ? Sm : 1.0;
     for (int64_t e = 0; e < n_elems; e++) {
         int f = family[e];
-        if (f < 0 || f > 3) { delete m; return fail(RVF_E_ARG, "GPU backend does not support element family code " + std::to_string(f)); }
+        if (curve ? (f < 4 || f > 5) : (f < 0 || f > 3)) {
+            delete m;
+            return fail(RVF_E_ARG, std::string(curve ? "curve mesh: element family must be Line3 (4) or Line2 (5), got "
+                                                     : "GPU backend does not support element family code ") +
+                                       std::to_string(f));
+        }
         int64_t col = (int64_t)elem_node_idx[e] - 1;
         if (col < 0 || col >= n_per_family[f] || !nodes[f]) { delete m; return fail(RVF_E_ARG, "rvf_mesh_upload: elem_node_idx out of range"); }
         for (int a = 0; a < NNF[f]; a++) {
@@ -337,6 +345,23 @@ int rvf_mesh_upload(rvf_ctx* ctx, const double* coords, int64_t n_nodes, int64_t
     }
     *out = m;
     return 0;
+}
+
+int rvf_mesh_upload(rvf_ctx* ctx, const double* coords, int64_t n_nodes, int64_t n_elems, const int8_t* family,
+                    const int32_t* elem_node_idx, const int32_t* elem_group, const int32_t* const nodes[4],
+                    const int64_t n_per_family[4], rvf_mesh** out) {
+    if (!nodes || !n_per_family) return fail(RVF_E_ARG, "rvf_mesh_upload: bad arguments");
+    const int32_t* nd[6] = {nodes[0], nodes[1], nodes[2], nodes[3], nullptr, nullptr};
+    const int64_t cnt[6] = {n_per_family[0], n_per_family[1], n_per_family[2], n_per_family[3], 0, 0};
+    return mesh_upload(ctx, coords, n_nodes, n_elems, family, elem_node_idx, elem_group, nd, cnt, false, out);
+}
+
+int rvf_curve_mesh_upload(rvf_ctx* ctx, const double* coords, int64_t n_nodes, int64_t n_elems, const int8_t* family,
+                          const int32_t* elem_node_idx, const int32_t* elem_group, const int32_t* line3_nodes,
+                          int64_t n_line3, const int32_t* line2_nodes, int64_t n_line2, rvf_mesh** out) {
+    const int32_t* nd[6] = {nullptr, nullptr, nullptr, nullptr, line3_nodes, line2_nodes};
+    const int64_t cnt[6] = {0, 0, 0, 0, n_line3, n_line2};
+    return mesh_upload(ctx, coords, n_nodes, n_elems, family, elem_node_idx, elem_group, nd, cnt, true, out);
 }
 
 void rvf_mesh_free(rvf_mesh* m) { delete m; }
@@ -373,15 +398,16 @@ static inline bool jisless(double a, double b) {
 struct HostNode { double lo[3], hi[3]; int left, right, start, count; };  // 1-based like the reference
 
 struct Builder {
-    const double* soup;   // merged (3,3,T)
+    const double* soup;   // merged (3,3,T), or a segment soup (3,2,T) with nv = 2
     int av, ac;           // AABB read strides (vertex, coord)
+    int nv = 3;           // vertices per item
     std::vector<int> idx; // 1-based permutation
     std::vector<HostNode> nodes;
     std::vector<double> key;
-    double rd(int64_t t, int v, int c) const { return soup[9 * t + v * av + c * ac]; }
+    double rd(int64_t t, int v, int c) const { return soup[3 * nv * t + v * av + c * ac]; }
     void tri_box(int64_t t, double* lo, double* hi) const {
         for (int c = 0; c < 3; c++) lo[c] = hi[c] = rd(t, 0, c);
-        for (int v = 1; v < 3; v++)
+        for (int v = 1; v < nv; v++)
             for (int c = 0; c < 3; c++) { double x = rd(t, v, c); lo[c] = jmin(lo[c], x); hi[c] = jmax(hi[c], x); }
     }
     int build(int start, int stop) {  // 1-based inclusive
@@ -404,13 +430,13 @@ struct Builder {
         int axis = 0;
         for (int c = 1; c < 3; c++) if (jisless(ext[axis], ext[c])) axis = c;
         int mid = (start + stop) / 2;
-        // stable sort by centroid: ((0 + x1) + x2) + x3, / 3 (BVH.jl:100-109)
+        // stable sort by centroid: ((0 + x1) + x2) + x3, / nv (BVH.jl:100-109)
         std::vector<std::pair<double, int>> kv(count);
         for (int i = start; i <= stop; i++) {
             int64_t t = idx[i - 1] - 1;
             double s = 0.0;
-            for (int v = 0; v < 3; v++) s += rd(t, v, axis);
-            kv[i - start] = std::make_pair(s / 3.0, idx[i - 1]);
+            for (int v = 0; v < nv; v++) s += rd(t, v, axis);
+            kv[i - start] = std::make_pair(s / (double)nv, idx[i - 1]);
         }
         std::stable_sort(kv.begin(), kv.end(),
                          [](const std::pair<double, int>& a, const std::pair<double, int>& b) {
@@ -557,6 +583,7 @@ struct rvf_obs {
     rvf_ctx* ctx = nullptr;
     int G = 0;
     int soup_read = 0;
+    bool seg = false;            // segment soup of a curve mesh
     std::vector<Tree> trees;
     std::vector<int32_t> root;   // G*G absolute node index or -1
     std::vector<int32_t> troot;  // G*G traversal-tree root or -1
@@ -579,23 +606,29 @@ static float f32_up(double x) {
 
 int rvf_obs_build(rvf_ctx* ctx, const rvf_mesh* mesh, const double* soup, const int32_t* tri_group, int64_t T,
                   const int32_t* obs_groups, int32_t n_obs, int32_t soup_read, rvf_obs** out) {
-    if (!ctx || !mesh || !out || T < 0 || n_obs < 0 || soup_read < 0 || soup_read > 2)
+    if (!ctx || !mesh || !out || T < 0 || n_obs < 0 || soup_read < 0 || soup_read > 3)
         return fail(RVF_E_ARG, "rvf_obs_build: bad arguments");
+    // curve meshes are obstructed by segment soups (3,2,T) (MeshIO.jl:641-645), surfaces by triangles
+    const bool seg = soup_read == RVF_SOUP_SEGMENTS;
+    if (seg != mesh->dim2)
+        return fail(RVF_E_ARG, seg ? "rvf_obs_build: segment soups obstruct curve meshes only"
+                                   : "rvf_obs_build: a curve mesh needs a segment soup (RVF_SOUP_SEGMENTS)");
+    const int nv = seg ? 2 : 3, SW = 3 * nv;  // vertices and doubles per item
     HIPCHK(hipSetDevice(ctx->device));
-    for (int64_t k = 0; k < 9 * T; k++)
+    for (int64_t k = 0; k < SW * T; k++)
         if (!std::isfinite(soup[k])) return fail(RVF_E_ARG, "rvf_obs_build: non-finite soup coordinate");
     rvf_obs* o = new rvf_obs();
-    o->ctx = ctx; o->soup_read = soup_read;
+    o->ctx = ctx; o->soup_read = soup_read; o->seg = seg;
     int G = (int)mesh->tags.size();
     o->G = G;
     o->root.assign((size_t)G * G, -1);
     // strides: coord c of vertex v at v*sv + c*sc
     int aav, aac, mtv, mtc;
     if (soup_read == RVF_SOUP_REFERENCE) { aac = 1; aav = 3; mtv = 1; mtc = 3; }
-    else if (soup_read == RVF_SOUP_XYZ_VERTEX) { aac = 1; aav = 3; mtc = 1; mtv = 3; }
+    else if (soup_read == RVF_SOUP_XYZ_VERTEX || seg) { aac = 1; aav = 3; mtc = 1; mtv = 3; }
     else { aav = 1; aac = 3; mtv = 1; mtc = 3; }
     double Sm = mesh->Sm;
-    for (int64_t k = 0; k < 9 * T; k++) Sm = std::max(Sm, std::fabs(soup[k]));
+    for (int64_t k = 0; k < SW * T; k++) Sm = std::max(Sm, std::fabs(soup[k]));
     o->Sm = (float)(Sm * (1 + 1e-6));
     std::map<std::vector<int>, int> seen;
     std::vector<int> act;
@@ -612,12 +645,12 @@ int rvf_obs_build(rvf_ctx* ctx, const rvf_mesh* mesh, const double* soup, const 
             tr.groups = act;
             for (int g : act)
                 for (int64_t t = 0; t < T; t++)
-                    if (tri_group[t] == g) tr.soup.insert(tr.soup.end(), soup + 9 * t, soup + 9 * t + 9);
-            int64_t nt = (int64_t)tr.soup.size() / 9;
+                    if (tri_group[t] == g) tr.soup.insert(tr.soup.end(), soup + SW * t, soup + SW * t + SW);
+            int64_t nt = (int64_t)tr.soup.size() / SW;
             if (nt == 0) { seen[act] = -1; continue; }
             if (nt >= ((int64_t)1 << 27)) { delete o; return fail(RVF_E_ARG, "rvf_obs_build: too many triangles"); }
             Builder bd;
-            bd.soup = tr.soup.data(); bd.av = aav; bd.ac = aac;
+            bd.soup = tr.soup.data(); bd.av = aav; bd.ac = aac; bd.nv = nv;
             bd.idx.resize(nt);
             for (int64_t t = 0; t < nt; t++) bd.idx[t] = (int)(t + 1);
             bd.build(1, (int)nt);
@@ -674,10 +707,10 @@ int rvf_obs_build(rvf_ctx* ctx, const rvf_mesh* mesh, const double* soup, const 
         int64_t nt = (int64_t)tr.idx.size();
         for (int64_t k = 0; k < nt; k++) {
             int64_t t = tr.idx[k] - 1;
-            const double* s = tr.soup.data() + 9 * t;
+            const double* s = tr.soup.data() + SW * t;
             double v[3][3];
-            for (int vv = 0; vv < 3; vv++)
-                for (int c = 0; c < 3; c++) v[vv][c] = s[vv * mtv + c * mtc];
+            for (int vv = 0; vv < 3; vv++)  // a segment (v0, v1) is stored as v0, e1 = v1 - v0, e2 = 0
+                for (int c = 0; c < 3; c++) v[vv][c] = vv < nv ? s[vv * mtv + c * mtc] : s[c * mtc];
             Tri64& d = t64[tr.tri_base + k];
             for (int c = 0; c < 3; c++) {
                 d.v0[c] = v[0][c]; d.e1[c] = v[1][c] - v[0][c]; d.e2[c] = v[2][c] - v[0][c];
@@ -819,10 +852,11 @@ int rvf_obs_export(const rvf_obs* o, int32_t tree, int64_t cap, double* box, int
 
 int rvf_bvh_build_host(const double* soup, int64_t T, int32_t soup_read, int64_t cap, double* box, int32_t* meta,
                        int32_t* tri_idx, int64_t* n_nodes) {
-    if (!soup || T <= 0 || soup_read < 0 || soup_read > 2) return fail(RVF_E_ARG, "rvf_bvh_build_host: bad arguments");
+    if (!soup || T <= 0 || soup_read < 0 || soup_read > 3) return fail(RVF_E_ARG, "rvf_bvh_build_host: bad arguments");
     Builder bd;
     bd.soup = soup;
     if (soup_read == RVF_SOUP_VERTEX_XYZ) { bd.av = 1; bd.ac = 3; } else { bd.ac = 1; bd.av = 3; }
+    if (soup_read == RVF_SOUP_SEGMENTS) bd.nv = 2;
     bd.idx.resize(T);
     for (int64_t t = 0; t < T; t++) bd.idx[t] = (int)(t + 1);
     bd.build(1, (int)T);
@@ -849,7 +883,7 @@ struct rvf_plan {
     const rvf_mesh* mesh = nullptr;
     const rvf_obs* obs = nullptr;
     rvf_opts opts{};
-    int nq_quad = 0, nq_tri = 0, maxpts = 0;
+    int nq_quad = 0, nq_tri = 0, nq_line = 0, maxpts = 0;
     DBuf rule, P64, Q64, P32, Q32, anchor, area, elem_np, bad, F, part, partT, rowsum, W, audit_c, audit_h, counters;
     DBuf Tseg, Wseg, area_orig, diag;
     hipGraphExec_t gexec = nullptr;
@@ -872,7 +906,8 @@ int rvf_plan_create(rvf_ctx* ctx, const rvf_mesh* mesh, const rvf_obs* obs, cons
     if (opts->method < 0 || opts->method > 2) return fail(RVF_E_ARG, "unknown method");
     if (opts->precision < 0 || opts->precision > 1) return fail(RVF_E_ARG, "unknown precision");
     if (opts->method == RVF_METHOD_MC && opts->n_samples < 1) return fail(RVF_E_ARG, "n_samples must be >= 1");
-    if (obs && obs->G != (int)mesh->tags.size()) return fail(RVF_E_ARG, "obstruction set built for another mesh");
+    if (obs && (obs->G != (int)mesh->tags.size() || obs->seg != mesh->dim2))
+        return fail(RVF_E_ARG, "obstruction set built for another mesh");
     if (obs && !obs->trees.empty() && opts->method != RVF_METHOD_MC &&
         occ_kind(opts->variant, opts->precision == RVF_MIXED) < 0)
         return fail(RVF_E_ARG, "unknown traversal variant for this precision");
@@ -880,12 +915,20 @@ int rvf_plan_create(rvf_ctx* ctx, const rvf_mesh* mesh, const rvf_obs* obs, cons
     rvf_plan* p = new rvf_plan();
     p->ctx = ctx; p->mesh = mesh; p->obs = obs; p->opts = *opts;
     int rc0;
-    std::vector<double> rule(3 * 64 + 39, 0.0);
+    std::vector<double> rule(RVF_RULE_SIZE, 0.0);
     p->nq_quad = quad_rule(opts->nquad, rule.data(), rule.data() + 2 * 64);
     p->nq_tri = tri_rule(opts->nquad, rule.data() + 3 * 64, rule.data() + 3 * 64 + 26);
+    {   // curve elements: 1-D Gauss-Legendre points stored as (xi, 0) (ViewFactorKernel.jl:284-301)
+        double lp[8], lw[8];
+        p->nq_line = gl1d(opts->nquad, lp, lw);
+        for (int k = 0; k < p->nq_line; k++) {
+            rule[RVF_RULE_LINE + 2 * k] = lp[k];
+            rule[RVF_RULE_LINE + 16 + k] = lw[k];
+        }
+    }
     bool anyq = false, anyt = false;
     for (int8_t f : mesh->fam) { if (f == 0 || f == 2) anyq = true; else anyt = true; }
-    p->maxpts = std::max(anyq ? p->nq_quad : 0, anyt ? p->nq_tri : 0);
+    p->maxpts = mesh->dim2 ? p->nq_line : std::max(anyq ? p->nq_quad : 0, anyt ? p->nq_tri : 0);
     if (p->maxpts == 0) p->maxpts = 1;
     int64_t N = mesh->N, NS = mesh->NS;
     int G = (int)mesh->tags.size();
@@ -1006,6 +1049,7 @@ static PairArgs pair_args(const rvf_plan* p) {
     a.F = p->F.as<double>(); a.part = p->part.as<double>(); a.partT = p->partT.as<double>();
     a.self_vf = p->diag.p ? 1 : 0; a.diag = p->diag.as<double>();
     a.G = (int)m->tags.size();
+    a.dim2 = m->dim2 ? 1 : 0;
     if (p->obs && !p->obs->trees.empty()) {
         a.tree_root = p->obs->d_root.as<int32_t>();
         a.nodes64 = p->obs->d_n64.as<BvhNode64>(); a.tris64 = p->obs->d_t64.as<Tri64>();
@@ -1053,6 +1097,10 @@ static int occ_kind(int variant, bool mixed) {
 // Which pair-kernel variant a plan runs, and the LDS it needs.
 static int pair_kind(const rvf_plan* p) {
     const bool mixed = p->opts.precision == RVF_MIXED;
+    if (p->mesh->dim2 && p->opts.method != RVF_METHOD_MC) {  // curve meshes: one traversal form
+        if (plan_occluded_quad(p)) return mixed ? RVF_K_MIX_OCC2D : RVF_K_FP64_OCC2D;
+        return mixed ? RVF_K_MIX_2D : RVF_K_FP64_2D;
+    }
     if (plan_occluded_quad(p)) return occ_kind(p->opts.variant, mixed);
     int pc = p->maxpts;
     if (!mixed && pc > 9) pc = 9;  // register chunks of 9 points
@@ -1063,7 +1111,8 @@ static size_t pair_lds(const rvf_plan* p) {
     const int kind = pair_kind(p);
     const bool mixed = p->opts.precision == RVF_MIXED;
     const size_t accq = mixed ? 0 : sizeof(double) * 64 * (size_t)p->maxpts;
-    if (kind == RVF_K_FP64_OCC_LANE) return accq;
+    if (kind == RVF_K_FP64_OCC_LANE || kind == RVF_K_FP64_OCC2D) return accq;
+    if (kind == RVF_K_MIX_OCC2D) return 0;
     if (kind == RVF_K_MIX_OCC_LANE) return 0;
     const size_t ring =
         kind >= RVF_K_FP64_SPLIT1
@@ -1089,6 +1138,7 @@ static int enqueue_prologue(rvf_plan* p, hipStream_t st, bool tim) {
     PrecompArgs pa{};
     pa.coords = m->coords.as<double>(); pa.fam = m->d_fam.as<int8_t>(); pa.nodes8 = m->d_nodes8.as<int32_t>();
     pa.N = m->N; pa.NS = m->NS; pa.maxpts = p->maxpts; pa.nq_quad = p->nq_quad; pa.nq_tri = p->nq_tri;
+    pa.nq_line = p->nq_line;
     pa.rule = p->rule.as<double>();
     pa.P64 = p->P64.as<double>(); pa.Q64 = p->Q64.as<double>(); pa.P32 = p->P32.as<float>(); pa.Q32 = p->Q32.as<float>();
     pa.anchor = p->anchor.as<double>(); pa.area = p->area.as<double>(); pa.elem_np = p->elem_np.as<int32_t>();
@@ -1222,7 +1272,7 @@ int rvf_plan_autotune(rvf_plan* p, int64_t sample_tiles, rvf_tune_entry* ent, in
     *n = 0;
     if (variant) *variant = p->opts.variant;
     if (refill_min) *refill_min = p->refill_min;
-    if (!plan_occluded_quad(p)) return 0;
+    if (!plan_occluded_quad(p) || p->mesh->dim2) return 0;  // curve meshes have one traversal form
     rvf_ctx* c = p->ctx;
     hipStream_t st = c->stream;
     HIPCHK(hipSetDevice(c->device));

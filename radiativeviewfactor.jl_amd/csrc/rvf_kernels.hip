@@ -94,11 +94,19 @@ __device__ __forceinline__ int shape_fn(int fam, double x, double e, double* N, 
         dx[0] = (-0.25) * (1 - e); dx[1] = 0.25 * (1 - e); dx[2] = 0.25 * (1 + e); dx[3] = (-0.25) * (1 + e);
         de[0] = (-0.25) * (1 - x); de[1] = (-0.25) * (1 + x); de[2] = 0.25 * (1 + x); de[3] = 0.25 * (1 - x);
         return 4;
-    } else {
+    } else if (fam == 3) {
         N[0] = (1.0 - x) - e; N[1] = x; N[2] = e;
         dx[0] = -1.0; dx[1] = 1.0; dx[2] = 0.0;
         de[0] = -1.0; de[1] = 0.0; de[2] = 1.0;
         return 3;
+    } else if (fam == 4) {  // Line3 [left, right, midpoint] (Geometry.jl:226-235)
+        N[0] = ((-0.5) * x) * (1.0 - x); N[1] = (0.5 * x) * (1.0 + x); N[2] = 1.0 - x * x;
+        dx[0] = -0.5 + x; dx[1] = 0.5 + x; dx[2] = (-2.0) * x;
+        return 3;
+    } else {  // Line2 (Geometry.jl:292-294)
+        N[0] = 0.5 * (1.0 - x); N[1] = 0.5 * (1.0 + x);
+        dx[0] = -0.5; dx[1] = 0.5;
+        return 2;
     }
 }
 
@@ -107,6 +115,22 @@ __device__ __forceinline__ void eval_point(int fam, const double* __restrict__ c
                                            double xi, double eta, double* x, double* n, double* dA) {
     double N[8], dx[8], de[8];
     int nn = shape_fn(fam, xi, eta, N, dx, de);
+    if (fam >= 4) {
+        // curve element: dL = |dx/dxi|, n = tangent turned 90 degrees counter-
+        // clockwise in xy (Geometry.jl:265-282, 321-332)
+        double p0 = 0.0, p1 = 0.0, p2 = 0.0, a0 = 0.0, a1 = 0.0, a2 = 0.0;
+        for (int k = 0; k < nn; k++) {
+            const double* xa = coords + 3 * (int64_t)(nodes[k] - 1);
+            double X = xa[0], Y = xa[1], Z = xa[2];
+            p0 = p0 + N[k] * X; p1 = p1 + N[k] * Y; p2 = p2 + N[k] * Z;
+            a0 = a0 + dx[k] * X; a1 = a1 + dx[k] * Y; a2 = a2 + dx[k] * Z;
+        }
+        double d = sqrt((a0 * a0 + a1 * a1) + a2 * a2);
+        x[0] = p0; x[1] = p1; x[2] = p2;
+        n[0] = -(a1 / d); n[1] = a0 / d; n[2] = 0.0;
+        *dA = d;
+        return;
+    }
     double p0 = 0.0, p1 = 0.0, p2 = 0.0, a0 = 0.0, a1 = 0.0, a2 = 0.0, b0 = 0.0, b1 = 0.0, b2 = 0.0;
     for (int k = 0; k < nn; k++) {
         const double* xa = coords + 3 * (int64_t)(nodes[k] - 1);
@@ -135,9 +159,10 @@ __global__ __launch_bounds__(64) void k_precompute(PrecompArgs a) {
     const int64_t o = a.slot_orig[e];                             // original element
     int f = a.fam[o];
     int isq = (f == 0 || f == 2);
-    int np = isq ? a.nq_quad : a.nq_tri;
-    const double* pts = isq ? a.rule : a.rule + 3 * 64;
-    const double* wts = isq ? a.rule + 2 * 64 : a.rule + 3 * 64 + 26;
+    const bool line = f >= 4;  // 1-D Gauss-Legendre (ViewFactorKernel.jl:284-301), points (xi, 0)
+    int np = line ? a.nq_line : isq ? a.nq_quad : a.nq_tri;
+    const double* pts = line ? a.rule + RVF_RULE_LINE : isq ? a.rule : a.rule + 3 * 64;
+    const double* wts = line ? a.rule + RVF_RULE_LINE + 16 : isq ? a.rule + 2 * 64 : a.rule + 3 * 64 + 26;
     const int32_t* nd = a.nodes8 + 8 * o;
     double A = 0.0, cx = 0.0, cy = 0.0, cz = 0.0;
     for (int p = 0; p < np; p++) {
@@ -206,6 +231,61 @@ __device__ __forceinline__ double vf_exact(double xi, double yi, double zi, doub
     ray.ox = xi; ray.oy = yi; ray.oz = zi; ray.dx = h0; ray.dy = h1; ray.dz = h2; ray.L = r;
     double K = (ci * cj) / (PI_D * rr);
     return (rr < 1e-30 || ci <= 0.0 || cj <= 0.0) ? 0.0 : K;
+}
+
+// vf_kernel_2d (ViewFactorKernel.jl:169-180), per unit depth: cos cos / (2 r)
+__device__ __forceinline__ double vf_exact2d(double xi, double yi, double zi, double nxi, double nyi, double nzi,
+                                             double xj, double yj, double zj, double nxj, double nyj, double nzj) {
+    double r0 = xj - xi, r1 = yj - yi, r2 = zj - zi;
+    double rr = (r0 * r0 + r1 * r1) + r2 * r2;
+    double r = sqrt(rr);
+    double h0 = r0 / r, h1 = r1 / r, h2 = r2 / r;
+    double ci = (nxi * h0 + nyi * h1) + nzi * h2;
+    double cj = (nxj * (-h0) + nyj * (-h1)) + nzj * (-h2);
+    double K = (ci * cj) / (2.0 * r);
+    return (rr < 1e-30 || ci <= 0.0 || cj <= 0.0) ? 0.0 : K;
+}
+
+// _seg_blocks_ray (BVH.jl:264-277): the 2-D path xi -> xj against the segment
+// v0 -> v0 + e (e = v1 - v0, the same double the reference forms), tol 1e-8.
+__device__ __forceinline__ bool seg_blocks64(double xi, double yi, double xj, double yj, const Tri64& S) {
+    const double tol = 1e-8;
+    double dx = xj - xi, dy = yj - yi;
+    double ex = S.e1[0], ey = S.e1[1];
+    double denom = dx * ey - dy * ex;
+    if (fabs(denom) < tol) return false;
+    double fx = S.v0[0] - xi, fy = S.v0[1] - yi;
+    double sp = (fx * ey - fy * ex) / denom;
+    double tp = (fx * dy - fy * dx) / denom;
+    return (tol < sp && sp < 1.0 - tol) && (tol < tp && tp < 1.0 - tol);
+}
+
+// intersect_seg_bvh (BVH.jl:286-320) over the reference's tree in stackless
+// preorder: a node is entered iff its box overlaps the path's xy bounding box,
+// as in the reference, so every segment the reference tests is tested and the
+// any-hit decision is the reference's whatever the visiting order.
+__device__ __forceinline__ bool blocked2d(const BvhNode64* __restrict__ nodes, const Tri64* __restrict__ segs, int root,
+                                          double xi, double yi, double xj, double yj, unsigned long long& naabb,
+                                          unsigned long long& ntri) {
+    // finite points: the order of equal values and the sign of zero do not change the comparisons below
+    const double xlo = xi < xj ? xi : xj, xhi = xi < xj ? xj : xi;
+    const double ylo = yi < yj ? yi : yj, yhi = yi < yj ? yj : yi;
+    int n = root;
+    while (n >= 0) {
+        const BvhNode64& nd = nodes[n];
+        naabb++;
+        if (nd.lo[0] > xhi || nd.hi[0] < xlo || nd.lo[1] > yhi || nd.hi[1] < ylo) { n = nd.miss; continue; }
+        if (nd.count > 0) {
+            for (int k = nd.start; k < nd.start + nd.count; k++) {
+                ntri++;
+                if (seg_blocks64(xi, yi, xj, yj, segs[k])) return true;
+            }
+            n = nd.miss;
+        } else {
+            n = n + 1;
+        }
+    }
+    return false;
 }
 
 // _ray_triangle with t_min = 0 and the caller's t < t_max (BVH.jl:181-201,241-242)
@@ -344,9 +424,10 @@ __device__ __forceinline__ void pair_out(const PairArgs& a, const TileCtx& t, in
 }
 
 // ---------------------------------------------------------------------------
-// FP64, unoccluded: i-points in registers, PC per chunk of the p loop.
+// FP64, unoccluded: i-points in registers, PC per chunk of the p loop.  D2:
+// curve meshes (2-D kernel per unit depth).
 // ---------------------------------------------------------------------------
-template <int PC>
+template <int PC, bool D2 = false>
 __device__ __forceinline__ void tile_fp64(const PairArgs& a, int tile) {
     const TileCtx t = tile_ctx(a, tile);
     const Corners cwi = corners_of(a, t.oi, t.vi && a.duffy);
@@ -395,7 +476,8 @@ __device__ __forceinline__ void tile_fp64(const PairArgs& a, int tile) {
                 for (int p = 0; p < PC; p++) {
                     if (c0 + p < npmax) {
                         Ray64 R;
-                        double K = vf_exact(X[p], Y[p], Z[p], NX[p], NY[p], NZ[p], xj, yj, zj, nxj, nyj, nzj, R);
+                        double K = D2 ? vf_exact2d(X[p], Y[p], Z[p], NX[p], NY[p], NZ[p], xj, yj, zj, nxj, nyj, nzj)
+                                      : vf_exact(X[p], Y[p], Z[p], NX[p], NY[p], NZ[p], xj, yj, zj, nxj, nyj, nzj, R);
                         inner[p] += (wj * K) * dAj;
                     }
                 }
@@ -423,6 +505,7 @@ __device__ __forceinline__ void tile_fp64(const PairArgs& a, int tile) {
 // are accumulated in LDS (accq[q][lane]) so the summation order is still the
 // reference's.  K is symmetric bit for bit, so one evaluation serves both.
 // ---------------------------------------------------------------------------
+template <bool D2 = false>
 __device__ __forceinline__ void tile_fp64_occ(const PairArgs& a, int tile, double* accq /* [maxpts][64] LDS */) {
     const TileCtx t = tile_ctx(a, tile);
     const Corners cwi = corners_of(a, t.oi, t.vi && a.duffy);
@@ -466,14 +549,21 @@ __device__ __forceinline__ void tile_fp64_occ(const PairArgs& a, int tile, doubl
                 const double nxj = qj[q * 8 + 3], nyj = qj[q * 8 + 4], nzj = qj[q * 8 + 5];
                 const double wj = qj[q * 8 + 6], dAj = qj[q * 8 + 7];
                 Ray64 R;
-                double K = vf_exact(xi, yi, zi, nxi, nyi, nzi, xj, yj, zj, nxj, nyj, nzj, R);
-                if (rev) {  // is_visible(x_outer = x_j, x_inner = x_i): d = x_i - x_j = -(x_j - x_i) exactly
+                double K = D2 ? vf_exact2d(xi, yi, zi, nxi, nyi, nzi, xj, yj, zj, nxj, nyj, nzj)
+                              : vf_exact(xi, yi, zi, nxi, nyi, nzi, xj, yj, zj, nxj, nyj, nzj, R);
+                if (!D2 && rev) {  // is_visible(x_outer = x_j, x_inner = x_i): d = x_i - x_j = -(x_j - x_i) exactly
                     R.ox = xj; R.oy = yj; R.oz = zj;
                     R.dx = -R.dx; R.dy = -R.dy; R.dz = -R.dz;
                 }
                 const bool need = pv && K != 0.0 && root >= 0;
                 nrays += need;
-                if ((occlusion64(a, need, root, R, naabb, ntri) >> t.lane) & 1) {
+                bool blk;
+                if (D2)  // is_visible(bvh, x_outer, x_inner; mesh_dim = 1)
+                    blk = need && (rev ? blocked2d(a.nodes64, a.tris64, root, xj, yj, xi, yi, naabb, ntri)
+                                       : blocked2d(a.nodes64, a.tris64, root, xi, yi, xj, yj, naabb, ntri));
+                else
+                    blk = (occlusion64(a, need, root, R, naabb, ntri) >> t.lane) & 1;
+                if (blk) {
                     K = 0.0;
                     bc++;
                     bh += mix32((uint32_t)(rev ? q * npi + p : p * npj + q) + 0x9e3779b9U);
@@ -521,7 +611,21 @@ __device__ __forceinline__ float vf_mixed(float bx, float by, float bz, float nx
     return (r2 >= 1e-30f) ? k : 0.0f;
 }
 
-template <int PC, bool SINGLE>
+// Curve meshes: K'' = max(n_i.r, 0) max(-n_j.r, 0) / |r|^3 (1/2 applied once
+// per pair: cos cos / (2 r), ViewFactorKernel.jl:169-180).
+__device__ __forceinline__ float vf_mixed2d(float bx, float by, float bz, float nxi, float nyi, float nzi,
+                                            float qx, float qy, float qz, float nxj, float nyj, float nzj) {
+#pragma clang fp contract(fast)
+    const float rx = bx + qx, ry = by + qy, rz = bz + qz;
+    const float r2 = rx * rx + ry * ry + rz * rz;
+    const float di = nxi * rx + nyi * ry + nzi * rz;
+    const float dj = nxj * rx + nyj * ry + nzj * rz;
+    const float rs = __builtin_amdgcn_rsqf(r2);
+    const float k = ((fmaxf(di, 0.0f) * rs) * (fmaxf(-dj, 0.0f) * rs)) * rs;
+    return (r2 >= 1e-30f) ? k : 0.0f;
+}
+
+template <int PC, bool SINGLE, bool D2 = false>
 __device__ __forceinline__ void tile_mixed(const PairArgs& a, int tile) {
     // SINGLE: every element has <= PC points, so the strip's points stay in
     // registers for the whole tile; otherwise chunks of PC points are reloaded
@@ -590,8 +694,9 @@ __device__ __forceinline__ void tile_mixed(const PairArgs& a, int tile) {
                 for (int p = 0; p < PC; p++) {
                     if (c0 + p < npmax) {
                         float rx, ry, rz, r2;
-                        float k = vf_mixed(BX[p], BY[p], BZ[p], NX[p], NY[p], NZ[p], qx, qy, qz, nxj, nyj, nzj,
-                                           rx, ry, rz, r2);
+                        float k = D2 ? vf_mixed2d(BX[p], BY[p], BZ[p], NX[p], NY[p], NZ[p], qx, qy, qz, nxj, nyj, nzj)
+                                     : vf_mixed(BX[p], BY[p], BZ[p], NX[p], NY[p], NZ[p], qx, qy, qz, nxj, nyj, nzj,
+                                                rx, ry, rz, r2);
                         inner[p] = fmaf(wdj, k, inner[p]);
                     }
                 }
@@ -600,7 +705,7 @@ __device__ __forceinline__ void tile_mixed(const PairArgs& a, int tile) {
             for (int p = 0; p < PC; p++)
                 if (c0 + p < npmax) neumaier(Fs, Fc, WD[p] * (double)inner[p]);
         }
-        pair_out(a, t, j, oj, act, (Fs + Fc) * (1.0 / PI_D), Ai, ls, lc);
+        pair_out(a, t, j, oj, act, (Fs + Fc) * (D2 ? 0.5 : 1.0 / PI_D), Ai, ls, lc);
     }
     a.part[(int64_t)t.chunk * NS + t.i] = ls + lc;
 }
@@ -1492,6 +1597,89 @@ __device__ __forceinline__ bool capsule_meets_box(const double* ai, const double
     return tmin <= tmax;
 }
 
+// ---------------------------------------------------------------------------
+// Mixed, occluded curve meshes: FP32 kernel, each cast path decided by the
+// exact FP64 segment test from the FP64 points (so no decision can differ
+// from the FP64 solve), from the outer element's point as the reference does.
+// ---------------------------------------------------------------------------
+__device__ __forceinline__ void tile_mixed_occ2d(const PairArgs& a, int tile) {
+    const TileCtx t = tile_ctx(a, tile);
+    const int64_t NS = a.NS;
+    const int npi = t.vi ? a.elem_np[t.i] : 0;
+    int npmax = npi;
+#pragma unroll
+    for (int o = 32; o > 0; o >>= 1) npmax = max(npmax, __shfl_xor(npmax, o, 64));
+    npmax = __builtin_amdgcn_readfirstlane(npmax);
+    const double Ai = t.vi ? a.area[t.i] : 1.0;
+    const int gi = t.vi ? a.elem_grp[t.i] : 0;
+    const float* __restrict__ P = a.P32;
+    const double* __restrict__ P64 = a.P64;
+    const double aix = a.anchor[t.i * 4 + 0], aiy = a.anchor[t.i * 4 + 1], aiz = a.anchor[t.i * 4 + 2];
+    double ls = 0.0, lc = 0.0;
+    unsigned long long nrays = 0, naabb = 0, ntri = 0;
+    for (int64_t j = t.jb; j < t.je; j++) {
+        if (j < t.jlo) {
+            if (t.lane == 0) a.partT[(int64_t)t.s * a.NS + j] = 0.0;
+            continue;
+        }
+        const bool act = t.vi && t.i < j + a.self_vf;
+        const int64_t oj = a.slot_orig[j];
+        const int npj = a.elem_np[j];
+        const int root = a.tree_root[gi * a.G + a.elem_grp[j]];
+        const float* __restrict__ qj = a.Q32 + j * a.maxpts * 8;
+        const double* __restrict__ qj64 = a.Q64 + j * a.maxpts * 8;
+        const bool rev = t.oi > oj;
+        const float dax = (float)(a.anchor[j * 4 + 0] - aix);
+        const float day = (float)(a.anchor[j * 4 + 1] - aiy);
+        const float daz = (float)(a.anchor[j * 4 + 2] - aiz);
+        double Fs = 0.0, Fc = 0.0;
+        uint32_t bc = 0, bh = 0;
+        for (int p = 0; p < npmax; p++) {
+            const bool pv = act && p < npi;
+            const float bx = dax - P[(int64_t)(p * 8 + 0) * NS + t.i], by = day - P[(int64_t)(p * 8 + 1) * NS + t.i];
+            const float bz = daz - P[(int64_t)(p * 8 + 2) * NS + t.i];
+            const float nxi = P[(int64_t)(p * 8 + 3) * NS + t.i], nyi = P[(int64_t)(p * 8 + 4) * NS + t.i];
+            const float nzi = P[(int64_t)(p * 8 + 5) * NS + t.i];
+            const double xi = P64[(int64_t)(p * 8 + 0) * NS + t.i], yi = P64[(int64_t)(p * 8 + 1) * NS + t.i];
+            float inner = 0.0f;
+            for (int q = 0; q < npj; q++) {
+                float k = vf_mixed2d(bx, by, bz, nxi, nyi, nzi, qj[q * 8 + 0], qj[q * 8 + 1], qj[q * 8 + 2],
+                                     qj[q * 8 + 3], qj[q * 8 + 4], qj[q * 8 + 5]);
+                const bool need = pv && k != 0.0f && root >= 0;
+                nrays += need;
+                if (need) {
+                    const double xj = qj64[q * 8 + 0], yj = qj64[q * 8 + 1];
+                    const bool blk = rev ? blocked2d(a.nodes64, a.tris64, root, xj, yj, xi, yi, naabb, ntri)
+                                         : blocked2d(a.nodes64, a.tris64, root, xi, yi, xj, yj, naabb, ntri);
+                    if (blk) {
+                        k = 0.0f;
+                        bc++;
+                        bh += mix32((uint32_t)(rev ? q * npi + p : p * npj + q) + 0x9e3779b9U);
+                    }
+                }
+                inner = fmaf(qj[q * 8 + 6], k, inner);
+            }
+            if (p < npi) {
+                double wd = P64[(int64_t)(p * 8 + 6) * NS + t.i] * P64[(int64_t)(p * 8 + 7) * NS + t.i];
+                neumaier(Fs, Fc, wd * (double)inner);
+            }
+        }
+        if (act && a.audit_count) {
+            a.audit_count[min(t.oi, oj) + a.N * max(t.oi, oj)] = bc;
+            a.audit_hash[min(t.oi, oj) + a.N * max(t.oi, oj)] = bh;
+        }
+        pair_out(a, t, j, oj, act, (Fs + Fc) * 0.5, Ai, ls, lc);
+    }
+    a.part[(int64_t)t.chunk * NS + t.i] = ls + lc;
+    if (a.counters) {
+        unsigned long long c1 = nrays, c2 = naabb, c3 = ntri;
+        for (int o = 32; o > 0; o >>= 1) {
+            c1 += __shfl_xor(c1, o, 64); c2 += __shfl_xor(c2, o, 64); c3 += __shfl_xor(c3, o, 64);
+        }
+        if (t.lane == 0) { atomicAdd(&a.counters[1], c1); atomicAdd(&a.counters[2], c2); atomicAdd(&a.counters[3], c3); }
+    }
+}
+
 template <bool MIX, int S, bool SPLIT>
 __device__ __forceinline__ void tile_occ_queue(const PairArgs& a, int tile, double* lds) {
     const TileCtx t = tile_ctx(a, tile);
@@ -1717,10 +1905,17 @@ __device__ __forceinline__ double xorshift64(uint64_t& s) {
 __device__ __forceinline__ void mc_sample(const PairArgs& a, int f, int64_t e, double u1, double u2, double* x,
                                           double* n, double* dA) {
     double xi, eta;
-    if (f == 0 || f == 2) { xi = 2.0 * u1 - 1.0; eta = 2.0 * u2 - 1.0; }
+    if (f >= 4) { xi = u1; eta = 0.0; }  // curve element: u1 is the point on [-1, 1]
+    else if (f == 0 || f == 2) { xi = 2.0 * u1 - 1.0; eta = 2.0 * u2 - 1.0; }
     else { xi = u1; eta = u2; if (xi + eta > 1.0) { xi = 1.0 - xi; eta = 1.0 - eta; } }
     eval_point(f, a.coords, a.nodes8 + 8 * e, xi, eta, x, n, dA);
 }
+
+// Curve meshes (the reference's GPU Monte Carlo has no curve form): the same
+// per-pair stream, sample k of both elements in stratum k of n equal strata of
+// [-1, 1] as _stratified_line_points places them (MCKernel.jl:107-115):
+// xi = -1 + ((k + u) * 2) / n, u1 for element i, u3 for element j.
+__device__ __forceinline__ double mc_line_point(int k, double u, int n) { return -1.0 + ((k + u) * 2.0) / n; }
 
 // MC area of element e from the stream of the pair (e, e), mirroring the CPU
 // path's area pass element_pair_view_factor_mc(i, i) (Assembly.jl:164-169).
@@ -1733,6 +1928,18 @@ __global__ __launch_bounds__(64) void k_mc_area(PairArgs a, double* area, double
     double Ai = 0.0;
     int k = 0;
     double x[3], n[3], dA;
+    if (a.dim2) {
+        for (int kk = 0; kk < a.mc_nsamp; kk++) {
+            double u1 = xorshift64(st);
+            xorshift64(st);
+            mc_sample(a, f, e, mc_line_point(kk, u1, a.mc_nsamp), 0.0, x, n, &dA);
+            Ai += dA;
+            xorshift64(st); xorshift64(st);
+        }
+        area[e] = (Ai * 2.0) / a.mc_nsamp;
+        area_orig[a.slot_orig[e]] = area[e];
+        return;
+    }
     for (int si = 0; si < s; si++)
         for (int sj = 0; sj < s; sj++) {
             k++;
@@ -1759,7 +1966,9 @@ __device__ __forceinline__ void tile_mc(const PairArgs& a, int tile) {
     const int64_t NS = a.NS;
     // Monte Carlo solves use the identity slot layout (slot == element), so
     // i < j is the reference's thread ordering (GPUMCKernels.jl:167).
-    const int fi = t.vi ? a.fam[t.oi] : 2;
+    // lanes past the last element sample element 0 with its own family (its
+    // node list; another family's shape would read node slots it leaves 0)
+    const int fi = a.fam[t.vi ? t.oi : 0];
     const double Ai = t.vi ? a.area[t.i] : 1.0;
     const int gi = t.vi ? a.elem_grp[t.i] : 0;
     const int64_t ii = t.vi ? t.oi : 0;
@@ -1783,20 +1992,28 @@ __device__ __forceinline__ void tile_mc(const PairArgs& a, int tile) {
             double u1 = xorshift64(st), u2 = xorshift64(st);
             double si = (double)(kk / s), sj = (double)(kk % s);
             bool strat = kk < s * s;
-            mc_sample(a, fi, ii, strat ? (si + u1) / s : u1, strat ? (sj + u2) / s : u2, xi, ni, &ai);
+            if (a.dim2) mc_sample(a, fi, ii, mc_line_point(kk, u1, a.mc_nsamp), 0.0, xi, ni, &ai);
+            else mc_sample(a, fi, ii, strat ? (si + u1) / s : u1, strat ? (sj + u2) / s : u2, xi, ni, &ai);
             double u3 = xorshift64(st), u4 = xorshift64(st);
-            mc_sample(a, fj, oj, strat ? (si + u3) / s : u3, strat ? (sj + u4) / s : u4, xj, nj, &aj);
+            if (a.dim2) mc_sample(a, fj, oj, mc_line_point(kk, u3, a.mc_nsamp), 0.0, xj, nj, &aj);
+            else mc_sample(a, fj, oj, strat ? (si + u3) / s : u3, strat ? (sj + u4) / s : u4, xj, nj, &aj);
             Ray64 R;
-            double K = vf_exact(xi[0], xi[1], xi[2], ni[0], ni[1], ni[2], xj[0], xj[1], xj[2], nj[0], nj[1], nj[2], R);
+            double K = a.dim2 ? vf_exact2d(xi[0], xi[1], xi[2], ni[0], ni[1], ni[2], xj[0], xj[1], xj[2], nj[0], nj[1],
+                                           nj[2])
+                              : vf_exact(xi[0], xi[1], xi[2], ni[0], ni[1], ni[2], xj[0], xj[1], xj[2], nj[0], nj[1],
+                                         nj[2], R);
             const bool need = act && K > 0.0 && root >= 0;
             nrays += need;
-            if ((occlusion64(a, need, root, R, naabb, ntri) >> t.lane) & 1) K = 0.0;
+            bool blk;
+            if (a.dim2) blk = need && blocked2d(a.nodes64, a.tris64, root, xi[0], xi[1], xj[0], xj[1], naabb, ntri);
+            else blk = (occlusion64(a, need, root, R, naabb, ntri) >> t.lane) & 1;
+            if (blk) K = 0.0;
             Ks += (K * ai) * aj;
             k++;
         }
         (void)k;
-        double ri = (fi == 0 || fi == 2) ? 4.0 : 0.5;
-        double rj = (fj == 0 || fj == 2) ? 4.0 : 0.5;
+        double ri = fi >= 4 ? 2.0 : (fi == 0 || fi == 2) ? 4.0 : 0.5;
+        double rj = fj >= 4 ? 2.0 : (fj == 0 || fj == 2) ? 4.0 : 0.5;
         double raw = ((Ks * ri) * rj) / a.mc_nsamp;
         pair_out(a, t, j, oj, act, raw, Ai, ls, lc);
     }
@@ -2094,6 +2311,10 @@ __device__ __forceinline__ void run_tile(const PairArgs& a, int tile, double* ld
     else if (KIND == RVF_K_MIX_SPLIT1) tile_occ_queue<true, 1, true>(a, tile, lds);
     else if (KIND == RVF_K_MIX_SPLIT2) tile_occ_queue<true, 2, true>(a, tile, lds);
     else if (KIND == RVF_K_MIX_SPLIT3) tile_occ_queue<true, 3, true>(a, tile, lds);
+    else if (KIND == RVF_K_FP64_2D) tile_fp64<8, true>(a, tile);
+    else if (KIND == RVF_K_MIX_2D) tile_mixed<8, true, true>(a, tile);
+    else if (KIND == RVF_K_FP64_OCC2D) tile_fp64_occ<true>(a, tile, lds);
+    else if (KIND == RVF_K_MIX_OCC2D) tile_mixed_occ2d(a, tile);
     else tile_mc(a, tile);
 }
 

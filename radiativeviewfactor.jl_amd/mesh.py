@@ -28,9 +28,9 @@ from typing import Dict, List, Optional, Sequence
 
 import numpy as np
 
-FAMILY_CODE = {"quad": 0, "tri": 1, "quad4": 2, "tri3": 3}
+FAMILY_CODE = {"quad": 0, "tri": 1, "quad4": 2, "tri3": 3, "line3": 4, "line2": 5}
 FAMILY_NAME = {v: k for k, v in FAMILY_CODE.items()}
-FAMILY_NNODES = {0: 8, 1: 6, 2: 4, 3: 3}
+FAMILY_NNODES = {0: 8, 1: 6, 2: 4, 3: 3, 4: 3, 5: 2}
 
 
 @dataclass
@@ -57,8 +57,20 @@ class MeshData:
         self.family = np.ascontiguousarray(self.family, dtype=np.int8)
         self.nodes = np.ascontiguousarray(self.nodes, dtype=np.int32).reshape(-1, 8)
         self.group = np.ascontiguousarray(self.group, dtype=np.int32)
-        self.group_tri_soup = {int(k): np.ascontiguousarray(v, dtype=np.float64).reshape(-1, 3, 3)
+        curve = self.family >= 4
+        if curve.size and curve.all():
+            self.mesh_dim = 1
+        elif curve.any():
+            raise ValueError("a mesh holds either surface or curve (Line2/Line3) elements, not both")
+        nv = self.soup_vertices
+        self.group_tri_soup = {int(k): np.ascontiguousarray(v, dtype=np.float64).reshape(-1, nv, 3)
                                for k, v in self.group_tri_soup.items()}
+
+    @property
+    def soup_vertices(self) -> int:
+        """Vertices per obstruction item: 3 (triangles), 2 for curve meshes
+        (segment soups, src/MeshIO.jl:641-645)."""
+        return 2 if self.mesh_dim == 1 else 3
 
     # -- reference-shaped views -------------------------------------------------
     @property
@@ -104,7 +116,7 @@ class MeshData:
         reference (src/Assembly.jl:123-141) is derived from this on the host."""
         tags = sorted(self.group_tri_soup)
         if not tags:
-            return np.zeros((0, 3, 3)), np.zeros(0, np.int32)
+            return np.zeros((0, self.soup_vertices, 3)), np.zeros(0, np.int32)
         soup = np.concatenate([self.group_tri_soup[t] for t in tags], axis=0)
         grp = np.concatenate([np.full(self.group_tri_soup[t].shape[0], t, np.int32) for t in tags])
         return np.ascontiguousarray(soup), grp
@@ -117,6 +129,11 @@ def build_group_obs_soups(mesh: MeshData, groups: Optional[Sequence[int]] = None
     soups = {}
     tags = groups if groups is not None else sorted(mesh.group_tags)
     X = mesh.coords
+    if mesh.mesh_dim == 1:  # segment soups from the first two corners (src/MeshIO.jl:672-680)
+        for g in tags:
+            c = mesh.nodes[np.nonzero(mesh.group == g)[0], :2].astype(np.int64) - 1
+            soups[int(g)] = np.ascontiguousarray(np.stack([X[c[:, 0]], X[c[:, 1]]], axis=1))
+        return soups
     for g in tags:
         idx = np.nonzero(mesh.group == g)[0]
         c = mesh.nodes[idx, :4].astype(np.int64) - 1
@@ -471,6 +488,88 @@ def sphere_interior(n: int = 4, radius: float = 1.0, order: int = 2) -> MeshData
     N = len(elems)
     return MeshData(np.array(coords, np.float64), np.full(N, 0 if order == 2 else 2, np.int8),
                     np.array(elems, np.int32), np.array(groups, np.int32), names)
+
+
+# ----------------------------------------------------------------------------
+# Curve meshes (mesh_dim 1, per unit depth): Line3 = [left, right, midpoint],
+# Line2 = [left, right]; the normal is the tangent turned 90 degrees counter-
+# clockwise in xy (src/Geometry.jl:219-221), so walking a closed curve
+# counter-clockwise gives inward normals.
+# ----------------------------------------------------------------------------
+def _polyline(points, order: int, closed: bool, node_base: int, mid=None):
+    """Elements along ``points`` (K, 3); ``mid(k)`` gives element k's midpoint
+    node for Line3 (default: the chord midpoint).  Returns coords, nodes."""
+    P = np.asarray(points, np.float64)
+    K = P.shape[0]
+    nel = K if closed else K - 1
+    coords = [P]
+    nodes = np.zeros((nel, 8), np.int32)
+    for k in range(nel):
+        a, b = k, (k + 1) % K
+        nodes[k, 0], nodes[k, 1] = node_base + a + 1, node_base + b + 1
+    if order == 2:
+        M = np.array([mid(k) if mid is not None else 0.5 * (P[k] + P[(k + 1) % K]) for k in range(nel)])
+        coords.append(M)
+        nodes[:, 2] = node_base + K + 1 + np.arange(nel)
+    return np.concatenate(coords), nodes
+
+
+def parallel_lines(n: int = 16, gap: float = 1.0, width: float = 1.0, order: int = 1,
+                   offset: float = 0.0) -> MeshData:
+    """Two parallel strips of ``width`` (per unit depth), ``gap`` apart, each
+    ``n`` Line2 (order 1) or Line3 (order 2) elements, normals facing each
+    other; the top strip is shifted by ``offset`` along x.  Groups 1 (bottom,
+    normal +y), 2 (top, normal -y)."""
+    xs = np.linspace(0.0, width, n + 1)
+    bot = np.stack([xs, np.zeros_like(xs), np.zeros_like(xs)], axis=1)           # walks +x: normal +y
+    top = np.stack([offset + xs[::-1], np.full_like(xs, gap), np.zeros_like(xs)], axis=1)  # walks -x: normal -y
+    c1, n1 = _polyline(bot, order, False, 0)
+    c2, n2 = _polyline(top, order, False, c1.shape[0])
+    fam = 5 if order == 1 else 4
+    N = n1.shape[0] + n2.shape[0]
+    return MeshData(np.concatenate([c1, c2]), np.full(N, fam, np.int8), np.concatenate([n1, n2]),
+                    np.array([1] * n1.shape[0] + [2] * n2.shape[0], np.int32), {1: "bottom", 2: "top"})
+
+
+def circle_interior(n: int = 32, radius: float = 1.0, order: int = 2, groups: int = 1) -> MeshData:
+    """Interior of a circle as ``n`` arcs: Line3 with every node on the circle
+    (order 2) or its inscribed polygon of Line2 chords (order 1); inward
+    normals.  Element k spans angles [2 pi k / n, 2 pi (k+1) / n]; groups
+    1..``groups`` split the circle into equal runs.  Between two arcs of a
+    circle Hottel's crossed strings give F_ij exactly, and an arc sees itself
+    with F_ii = 1 - chord / arc."""
+    th = 2 * np.pi * np.arange(n) / n
+    P = radius * np.stack([np.cos(th), np.sin(th), np.zeros(n)], axis=1)
+
+    def mid(k):
+        t = 2 * np.pi * (k + 0.5) / n
+        return (radius * np.cos(t), radius * np.sin(t), 0.0)
+    coords, nodes = _polyline(P, order, True, 0, mid)
+    grp = (np.arange(n) * groups // n + 1).astype(np.int32)
+    return MeshData(coords, np.full(n, 5 if order == 1 else 4, np.int8), nodes, grp,
+                    {g: f"arc{g}" for g in range(1, groups + 1)})
+
+
+def square_enclosure_2d(n: int = 16, order: int = 1, baffle: bool = True) -> MeshData:
+    """Interior of the unit square, ``n`` elements per side (groups 1..4:
+    bottom, right, top, left; inward normals), and optionally an
+    obstruction-only baffle (group 5 segment soup): the vertical strip
+    x = 0.5, y in [0.2, 0.8], 8 segments."""
+    s = np.linspace(0.0, 1.0, n + 1)[:-1]
+    z = np.zeros(n)
+    sides = [np.stack([s, z, z], 1), np.stack([1 + z, s, z], 1), np.stack([1 - s, 1 + z, z], 1),
+             np.stack([z, 1 - s, z], 1)]
+    P = np.concatenate(sides)  # counter-clockwise: inward normals
+    coords, nodes = _polyline(P, order, True, 0)
+    grp = np.repeat(np.arange(1, 5), n).astype(np.int32)
+    m = MeshData(coords, np.full(4 * n, 5 if order == 1 else 4, np.int8), nodes, grp,
+                 {1: "bottom", 2: "right", 3: "top", 4: "left"})
+    if baffle:
+        y = np.linspace(0.2, 0.8, 9)
+        seg = np.stack([np.stack([0.5 + 0 * y[:-1], y[:-1], 0 * y[:-1]], 1),
+                        np.stack([0.5 + 0 * y[1:], y[1:], 0 * y[1:]], 1)], axis=1)
+        m.group_tri_soup = {5: seg}
+    return m
 
 
 def random_box_enclosure(rng: np.random.Generator, n_elems: int, max_occluders: int = 3,

@@ -1,0 +1,177 @@
+"""GPU parity for curve meshes (mesh_dim 1: Line3 / Line2, the 2-D kernel per
+unit depth and segment occluders), which the reference computes on the CPU
+only (src/Assembly.jl:83-86), against the oracle's restatement of that CPU
+path (checked against closed forms in test_oracle_curves.py).
+
+Tolerances, as for surface meshes:
+  FP64       F and A bit-identical (diagonal included with self_vf); row sums
+             and F_group <= 1e-12 relative; per-pair blocked-path counts and
+             hashes identical.
+  mixed      max |dF| <= 1e-6 and closure <= 1e-6 against FP64, no flip (the
+             mixed kernels decide visibility with the exact FP64 segment test).
+  Monte Carlo  bit-identical (same per-pair streams).
+"""
+import ctypes
+import math
+
+import numpy as np
+import pytest
+
+from rvf_amd import _lib, api
+from rvf_amd import mesh as M
+from rvf_amd._lib import ptr
+
+pytestmark = pytest.mark.gpu
+
+
+def gpu(mesh, nquad, obs=(), precision="fp64", method="quad", self_vf=False, n_samples=64, seed=0, audit=True):
+    ctx = api.context(0)
+    dm = api.DeviceMesh(ctx, mesh)
+    do = api.DeviceObstruction(ctx, dm, obs, "geometric") if len(obs) else None
+    plan = api.Plan(ctx, dm, do, nquad, method, precision, n_samples=n_samples, seed=seed, want_F=True,
+                    audit=audit and do is not None, self_vf=self_vf)
+    plan.run(); plan.sync()
+    F, A, rs, Fg, Ag, tags, ac, ah = plan.fetch()
+    out = dict(F=F, A=A, rs=rs, Fg=Fg, Ag=Ag, tags=tags, ac=ac, ah=ah, stats=plan.stats(), kernel=plan.kernel_name())
+    plan.close(); dm.close()
+    if do is not None:
+        do.close()
+    return out
+
+
+def check(orc, mesh, nquad, obs=(), method="quad", self_vf=False, **kw):
+    g = gpu(mesh, nquad, obs, "fp64", method, self_vf, **kw)
+    r = orc.compute(mesh, nquad=nquad, obstruction_groups=obs, parity_mode="geometric", method=method,
+                    audit=len(obs) > 0, self_vf=self_vf, n_samples=kw.get("n_samples", 64), seed=kw.get("seed", 0))
+    F = r["raw"] / r["A"][:, None]
+    assert np.array_equal(g["A"], r["A"]), "lengths differ"
+    assert np.array_equal(g["F"], F), f"F differs: max {np.abs(g['F'] - F).max()}"
+    tags, names, Fg, Ag = orc.aggregate(mesh, F, r["A"])
+    assert g["tags"] == tags
+    assert np.allclose(g["Fg"], Fg, rtol=1e-12, atol=1e-15)
+    assert np.allclose(g["rs"], F.sum(1), rtol=1e-12, atol=1e-15)
+    if len(obs):
+        iu = np.triu_indices(mesh.n_elems, 0 if self_vf else 1)
+        assert np.array_equal(g["ac"][iu], r["audit_count"][iu]), "blocked-path counts differ"
+        assert np.array_equal(g["ah"][iu], r["audit_hash"][iu]), "blocked-path masks differ"
+    return g, F
+
+
+def mixed_families_circle(n=40, groups=3):
+    """Circle of Line3 arcs with every third element a Line2 chord."""
+    m = M.circle_interior(n, 1.0, 2, groups)
+    fam = m.family.copy()
+    nodes = m.nodes.copy()
+    fam[::3] = 5
+    nodes[::3, 2] = 0
+    return M.MeshData(m.coords, fam, nodes, m.group, m.group_tags)
+
+
+def random_curves(seed, n_elems=150):
+    """Ragged random polylines (Line2 and Line3) in the unit square, facing
+    random ways, in 4 groups, plus an obstruction-only group 9 of random
+    segments."""
+    rng = np.random.default_rng(seed)
+    coords, fam, nodes, grp = [], [], [], []
+    for e in range(n_elems):
+        c = rng.uniform(0, 1, 2)
+        t = rng.uniform(0, 2 * np.pi)
+        L = rng.uniform(0.02, 0.08)
+        d = np.array([np.cos(t), np.sin(t)]) * L / 2
+        p0, p1 = c - d, c + d
+        k = len(coords)
+        coords += [[*p0, 0.0], [*p1, 0.0]]
+        if e % 2:
+            bow = rng.uniform(-0.2, 0.2) * L * np.array([-d[1], d[0]]) / (L / 2)
+            coords.append([*(c + bow), 0.0])
+            fam.append(4); nodes.append([k + 1, k + 2, k + 3, 0, 0, 0, 0, 0])
+        else:
+            fam.append(5); nodes.append([k + 1, k + 2, 0, 0, 0, 0, 0, 0])
+        grp.append(1 + e % 4)
+    m = M.MeshData(np.array(coords), np.array(fam, np.int8), np.array(nodes, np.int32), np.array(grp, np.int32),
+                   {g: f"g{g}" for g in range(1, 5)})
+    a = rng.uniform(0, 1, (40, 2)); b = a + rng.normal(0, 0.1, (40, 2))
+    seg = np.zeros((40, 2, 3)); seg[:, 0, :2] = a; seg[:, 1, :2] = b
+    m.group_tri_soup = {9: seg, **M.build_group_obs_soups(m, [2])}  # group 2 radiates and obstructs
+    return m
+
+
+@pytest.mark.parametrize("order", [1, 2])
+def test_parallel_lines_fp64(orc, order):
+    g, F = check(orc, M.parallel_lines(16, 1.0, 1.0, order), 4)
+    assert g["kernel"] == "k_pair<fp64_curve>"
+    assert abs(g["Fg"][0, 1] - (math.sqrt(2) - 1)) < 1e-10
+
+
+@pytest.mark.parametrize("nquad", [1, 2, 3, 4, 5, 6, 7, 8])
+def test_circle_self_vf_nquad(orc, nquad):
+    g, F = check(orc, mixed_families_circle(), nquad, self_vf=True)
+    if nquad > 1:  # one point per element has only the coincident pair (kernel 0)
+        assert np.all(np.diag(F)[mixed_families_circle().family == 4] > 0)
+
+
+@pytest.mark.parametrize("self_vf", [False, True])
+def test_square_with_baffle_occluded(orc, self_vf):
+    for order in (1, 2):
+        g, F = check(orc, M.square_enclosure_2d(12, order, baffle=True), 3, obs=[5], self_vf=self_vf)
+        assert g["kernel"] == "k_pair<fp64_curve_occ>" and g["stats"]["rays"] > 0
+        assert int(g["ac"].sum()) > 0
+
+
+@pytest.mark.parametrize("seed", [1, 2])
+def test_random_curves_occluded(orc, seed):
+    m = random_curves(seed)
+    check(orc, m, 4, obs=[9, 2])
+    check(orc, m, 2, obs=[9], self_vf=True)
+
+
+def test_monte_carlo_bitexact(orc):
+    check(orc, mixed_families_circle(24), 3, method="mc", n_samples=49, seed=3)
+    check(orc, M.square_enclosure_2d(6, 1, baffle=True), 3, obs=[5], method="mc", n_samples=16, seed=9)
+
+
+def test_duffy_is_quadrature_on_curves(orc):
+    m = mixed_families_circle(24)
+    a = gpu(m, 3, method="duffy")
+    b = gpu(m, 3, method="quad")
+    assert np.array_equal(a["F"], b["F"]) and a["stats"]["singular_pairs"] == 0
+
+
+@pytest.mark.parametrize("case", ["circle", "baffle", "random"])
+def test_mixed_tolerance(case):
+    if case == "circle":
+        m, obs = mixed_families_circle(), ()
+    elif case == "baffle":
+        m, obs = M.square_enclosure_2d(16, 2, baffle=True), (5,)
+    else:
+        m, obs = random_curves(3), (9,)
+    a = gpu(m, 4, obs, "fp64", self_vf=True)
+    b = gpu(m, 4, obs, "mixed", self_vf=True)
+    assert b["kernel"] == ("k_pair<mixed_curve_occ>" if obs else "k_pair<mixed_curve>")
+    assert float(np.abs(a["F"] - b["F"]).max()) <= 1e-6
+    assert float(np.abs(a["rs"] - b["rs"]).max()) <= 1e-6
+    if obs:
+        assert np.array_equal(a["ac"], b["ac"]) and np.array_equal(a["ah"], b["ah"])
+
+
+def test_curve_mesh_errors():
+    ctx = api.context(0)
+    m = M.square_enclosure_2d(4, 1, baffle=True)
+    srv = api.Server(ctx)
+    with pytest.raises(ValueError):
+        srv.submit(m, 3)
+    srv.close()
+    # a curve mesh with a triangle soup reading, and a surface mesh with a segment soup, are refused
+    h = ctypes.c_void_p()
+    obs = np.array([5], np.int32); tg = np.array([5], np.int32)
+    dm = api.DeviceMesh(ctx, m)
+    tri = np.zeros((1, 3, 3)); tri[0, 1, 0] = 1; tri[0, 2, 1] = 1
+    rc = _lib.lib().rvf_obs_build(ctx.h, dm.h, ptr(tri), ptr(tg), 1, ptr(obs), 1, 1, ctypes.byref(h))
+    assert rc != 0 and "segment soup" in _lib.lib().rvf_last_error().decode()
+    dm.close()
+    cube = M.unit_cube(2)
+    dm = api.DeviceMesh(ctx, cube)
+    seg = np.zeros((1, 2, 3)); seg[0, 1, 0] = 1
+    rc = _lib.lib().rvf_obs_build(ctx.h, dm.h, ptr(seg), ptr(tg), 1, ptr(obs), 1, 3, ctypes.byref(h))
+    assert rc != 0 and "curve meshes only" in _lib.lib().rvf_last_error().decode()
+    dm.close()

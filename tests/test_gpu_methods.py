@@ -43,6 +43,22 @@ def test_mc_bitexact_curved_mixed_families(orc):
     assert np.array_equal(F, r["raw"] / r["A"][:, None])
 
 
+def test_mc_bitexact_tri3_first_ragged(orc):
+    # element 0 has 3 nodes and the last strip is partly empty: lanes past the
+    # last element must sample element 0 with its own (Tri3) shape
+    q = M.parallel_plates(3)
+    tris = []
+    for e in range(q.n_elems):
+        a, b, c, d = q.nodes[e, :4]
+        tris += [[a, b, c, 0, 0, 0, 0, 0], [a, c, d, 0, 0, 0, 0, 0]]
+    m = M.MeshData(q.coords, np.full(len(tris), 3, np.int8), np.array(tris, np.int32), np.repeat(q.group, 2),
+                   q.group_tags)
+    F, A, *_ = gpu(m, 3, "mc", n_samples=16, seed=21)
+    r = orc.compute(m, method="mc", n_samples=16, seed=21)
+    assert np.array_equal(A, r["A"])
+    assert np.array_equal(F, r["raw"] / r["A"][:, None])
+
+
 def test_mc_occluded_bitexact(orc):
     m = M.occluded_plates(n=6, subdiv=2)
     F, A, *_ = gpu(m, 3, "mc", obs=[3], n_samples=25, seed=5)
