@@ -213,20 +213,24 @@ int rvf_ctx_sync(rvf_ctx* ctx);
  * load_vtu (ext/RadiativeViewFactorReadVTKExt.jl:71-157) do: Gmsh .msh in
  * format 2.2 or 4.1 (ASCII or binary), or an XML VTK unstructured grid (.vtu,
  * or .vtk in XML form; ascii, base64 or appended data, raw or zlib).
- * Elements of the supported surface families in every physical group of
- * dimension 2 (Quad9 read as Quad8); without physical groups a single
- * "default" group over every surface entity; VTK groups from a per-cell
- * integer array (group_field, or NULL to try CellEntityIds, gmsh:physical,
- * RegionId, MaterialIds, region, group).  reverse_normals swaps nodes as
- * _reverse_all_normals! (src/MeshIO.jl:545-562).  surface_dim must be 2. */
+ * Elements of the supported families in every physical group of dimension
+ * surface_dim: 2 = surfaces (Quad9 read as Quad8), 1 = curves (Line2, Line3;
+ * a .msh file's line elements are turned to face the surface next to their
+ * curve, as _orient_line3_normals!, src/MeshIO.jl:469-599); without physical
+ * groups a single "default" group over every entity of that dimension; VTK
+ * groups from a per-cell integer array (group_field, or NULL to try
+ * CellEntityIds, gmsh:physical, RegionId, MaterialIds, region, group).
+ * reverse_normals swaps nodes as _reverse_all_normals! (src/MeshIO.jl:613-632). */
 typedef struct rvf_meshfile rvf_meshfile;
 int rvf_meshfile_read(const char* path, int32_t surface_dim, int32_t reverse_normals, const char* group_field,
                       rvf_meshfile** out);
 /* flags: 1 = default group synthesised, 2 = Quad9 centre nodes dropped,
- * 4 = the requested VTK group array was absent (single default group) */
+ * 4 = the requested VTK group array was absent (single default group),
+ * 8 = some line elements had no curve entity or adjacent surface to orient by,
+ * 16 = some line elements were reoriented */
 int rvf_meshfile_info(const rvf_meshfile* mf, int64_t* n_nodes, int64_t* n_elems, int32_t* n_groups,
                       int32_t* flags);
-/* coords 3 x Nn column-major; family N (0 Quad8, 1 Tri6, 2 Quad4, 3 Tri3);
+/* coords 3 x Nn column-major; family N (0 Quad8, 1 Tri6, 2 Quad4, 3 Tri3, 4 Line3, 5 Line2);
  * nodes 8 x N column-major, 1-based, unused slots 0; group N (tags) */
 int rvf_meshfile_arrays(const rvf_meshfile* mf, double* coords, int8_t* family, int32_t* nodes, int32_t* group);
 /* group k (ascending tag order): tag and name; returns the name's length */

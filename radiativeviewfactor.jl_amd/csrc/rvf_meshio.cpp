@@ -18,6 +18,7 @@
 //     triangle blocks before quadrangle blocks (Gmsh's storage order), each
 //     in file order.
 #include <algorithm>
+#include <array>
 #include <cctype>
 #include <cmath>
 #include <cstdint>
@@ -51,9 +52,17 @@ struct MeshErr {
 };
 [[noreturn]] void bad(const std::string& m) { throw MeshErr{m}; }
 
-// element families of surface_dim 2: Gmsh type -> (family, nodes read, nodes kept)
+// element families read at surface_dim (ELEM_INFO, src/MeshIO.jl:20-34):
+// Gmsh type -> (family, nodes read, nodes kept)
 struct Fam { int code, nread, nkeep; };
-bool msh_family(int type, Fam& f) {
+bool msh_family(int type, int dim, Fam& f) {
+    if (dim == 1) {
+        switch (type) {
+            case 1: f = {5, 2, 2}; return true;    // Line2
+            case 8: f = {4, 3, 3}; return true;    // Line3 [left, right, midpoint]
+            default: return false;
+        }
+    }
     switch (type) {
         case 2: f = {3, 3, 3}; return true;    // Tri3
         case 3: f = {2, 4, 4}; return true;    // Quad4
@@ -61,6 +70,18 @@ bool msh_family(int type, Fam& f) {
         case 16: f = {0, 8, 8}; return true;   // Quad8
         case 10: f = {0, 9, 8}; return true;   // Quad9: centre node dropped
         default: return false;
+    }
+}
+// dimension of a Gmsh element type (-1: unknown)
+int msh_type_dim(int type) {
+    switch (type) {
+        case 15: return 0;
+        case 1: case 8: case 26: case 27: case 28: return 1;
+        case 2: case 3: case 9: case 10: case 16: case 20: case 21: case 22: case 23: case 24: case 25: case 36:
+        case 37: return 2;
+        case 4: case 5: case 6: case 7: case 11: case 12: case 13: case 14: case 17: case 18: case 19: case 29:
+        case 30: case 31: case 92: case 93: return 3;
+        default: return -1;
     }
 }
 bool msh_is_tri(int type) { return type == 2 || type == 9; }
@@ -146,15 +167,23 @@ struct Cur {
 struct RawElem { int type; int entity; int phys; size_t off; };  // node tags at MshModel::ntags[off]
 
 struct MshModel {
+    int dim = 2;                                                // surface_dim: 2 surfaces, 1 curves
     std::vector<long long> node_tags;
     std::vector<double> xyz;
     std::map<std::pair<int, int>, std::string> phys_names;      // (dim, tag) -> name
-    std::map<int, std::set<int>> surf_phys;                     // surface entity -> physical tags (4.1)
-    std::set<int> phys2;                                        // physical tags of dim 2
-    std::vector<RawElem> elems;                                 // dim-2 elements, file order
+    std::map<int, std::set<int>> surf_phys;                     // entity of dim -> physical tags (4.1)
+    std::set<int> phys2;                                        // physical tags of dim
+    std::vector<RawElem> elems;                                 // elements of dim, file order
     std::vector<long long> ntags;                               // their node tags, flat
-    std::set<int> surf_entities;                                // surface entities seen
+    std::set<int> surf_entities;                                // entities of dim seen
     bool v2 = false;
+    // curve meshes: what _orient_line3_normals! asks Gmsh for (src/MeshIO.jl:469-599)
+    std::map<int, std::set<int>> face_phys;                     // dim-2 entity -> physical tags
+    std::set<int> face_entities;                                // dim-2 entities
+    std::map<int, std::vector<long long>> face_nodes;           // dim-2 entity -> node tags of its elements
+    std::map<int, std::vector<long long>> curve_nodes;          // dim-1 entity -> nodes classified on it
+    std::map<int, std::vector<long long>> curve_elem_nodes;     // dim-1 entity -> node tags of its elements (2.2)
+    std::set<long long> point_nodes;                            // nodes of point elements (2.2)
 };
 
 void read_physical_names(Cur& c, MshModel& M) {
@@ -191,8 +220,20 @@ void v2_nodes(Cur& c, MshModel& M, bool binary) {
 }
 
 void v2_add(MshModel& M, int type, int phys, int ent, const long long* nt, int nn) {
+    const int td = msh_type_dim(type);
+    if (M.dim == 1) {
+        if (td == 2) {
+            M.face_entities.insert(ent);
+            if (phys != 0) M.face_phys[ent].insert(phys);
+            M.face_nodes[ent].insert(M.face_nodes[ent].end(), nt, nt + nn);
+        } else if (td == 1) {
+            M.curve_elem_nodes[ent].insert(M.curve_elem_nodes[ent].end(), nt, nt + nn);
+        } else if (td == 0) {
+            M.point_nodes.insert(nt, nt + nn);
+        }
+    }
     Fam f;
-    if (!msh_family(type, f)) return;
+    if (!msh_family(type, M.dim, f)) return;
     if (nn < f.nread) bad("element of type " + std::to_string(type) + " with too few nodes");
     M.elems.push_back({type, ent, phys, M.ntags.size()});
     M.ntags.insert(M.ntags.end(), nt, nt + f.nread);
@@ -261,13 +302,15 @@ void v4_entities(Cur& c, MshModel& M, bool binary) {
             const long long np = binary ? (long long)c.bin<uint64_t>() : c.integer();
             for (long long q = 0; q < np; q++) {
                 const int pt = binary ? c.bin<int32_t>() : (int)c.integer();
-                if (d == 2) { M.surf_phys[tag].insert(std::abs(pt)); M.phys2.insert(std::abs(pt)); }
+                if (d == M.dim) { M.surf_phys[tag].insert(std::abs(pt)); M.phys2.insert(std::abs(pt)); }
+                if (d == 2 && M.dim == 1) M.face_phys[tag].insert(std::abs(pt));
             }
             if (d > 0) {
                 const long long nb = binary ? (long long)c.bin<uint64_t>() : c.integer();
                 for (long long q = 0; q < nb; q++) binary ? (void)c.bin<int32_t>() : (void)c.integer();
             }
-            if (d == 2) M.surf_entities.insert(tag);
+            if (d == M.dim) M.surf_entities.insert(tag);
+            if (d == 2 && M.dim == 1) M.face_entities.insert(tag);
         }
     c.expect_end("Entities");
 }
@@ -281,10 +324,13 @@ void v4_nodes(Cur& c, MshModel& M, bool binary) {
     M.xyz.reserve(3 * nnodes);
     for (long long b = 0; b < nblocks; b++) {
         const int edim = i32();
-        (void)i32();
+        const int etag = i32();
         const int param = i32();
         const long long n = sz();
+        const size_t first = M.node_tags.size();
         for (long long k = 0; k < n; k++) M.node_tags.push_back(sz());
+        if (M.dim == 1 && edim == 1)  // nodes classified on a curve entity (gmsh getNodes(1, ent))
+            M.curve_nodes[etag].insert(M.curve_nodes[etag].end(), M.node_tags.begin() + first, M.node_tags.end());
         const int extra = param ? edim : 0;
         for (long long k = 0; k < n; k++) {
             for (int d = 0; d < 3; d++) M.xyz.push_back(binary ? c.bin<double>() : c.real());
@@ -305,7 +351,8 @@ void v4_elements(Cur& c, MshModel& M, bool binary) {
         const int nn = msh_nodes_of(type);
         if (nn < 0) bad("unknown element type " + std::to_string(type));
         Fam f;
-        const bool keep = edim == 2 && msh_family(type, f);
+        const bool keep = edim == M.dim && msh_family(type, M.dim, f);
+        const bool face = M.dim == 1 && edim == 2;
         std::vector<long long> nt(nn);
         for (long long k = 0; k < n; k++) {
             (void)sz();
@@ -314,8 +361,10 @@ void v4_elements(Cur& c, MshModel& M, bool binary) {
                 M.elems.push_back({type, etag, 0, M.ntags.size()});
                 M.ntags.insert(M.ntags.end(), nt.begin(), nt.begin() + f.nread);
             }
+            if (face) M.face_nodes[etag].insert(M.face_nodes[etag].end(), nt.begin(), nt.end());
         }
-        if (edim == 2) M.surf_entities.insert(etag);
+        if (edim == M.dim) M.surf_entities.insert(etag);
+        if (face) M.face_entities.insert(etag);
     }
     c.expect_end("Elements");
 }
@@ -352,6 +401,103 @@ void read_msh(const std::string& s, MshModel& M) {
     if (!have_nodes || !have_elems) bad("mesh file without $Nodes or $Elements");
 }
 
+// _orient_line3_normals! (src/MeshIO.jl:469-599) with _build_curve_to_surface_centroid
+// (:330-445), from the file itself instead of a Gmsh session: a curve entity's
+// nodes are those classified on it (its $Nodes block in format 4.1; format
+// 2.2 classifies none, so they are taken as the nodes of its line elements
+// that no point element and no other curve entity uses), a surface entity's
+// nodes those of its 2-D elements (entities of physical surface groups, else
+// every surface entity).  Each line element whose normal at xi = 0 points away
+// from the centroid of the surface next to its curve has its endpoints swapped.
+template <class Lookup>
+void orient_lines(const MshModel& M, const std::set<int>& curve_ents, const Lookup& lookup, rvf_meshfile& out) {
+    const double* X = out.coords.data();
+    auto pos_of = [&](const std::vector<long long>& tags) {
+        std::set<int32_t> p;
+        for (long long t : tags) { const int32_t ix = lookup(t); if (ix) p.insert(ix); }
+        return p;
+    };
+    std::map<int, std::set<int32_t>> cnodes;
+    if (!M.v2) {
+        for (int ent : curve_ents) {
+            auto it = M.curve_nodes.find(ent);
+            cnodes[ent] = it == M.curve_nodes.end() ? std::set<int32_t>() : pos_of(it->second);
+        }
+    } else {
+        std::map<long long, int> uses;  // node tag -> curve entities using it
+        for (const auto& kv : M.curve_elem_nodes) {
+            std::set<long long> u(kv.second.begin(), kv.second.end());
+            for (long long t : u) uses[t]++;
+        }
+        for (int ent : curve_ents) {
+            std::vector<long long> own;
+            auto it = M.curve_elem_nodes.find(ent);
+            if (it != M.curve_elem_nodes.end())
+                for (long long t : it->second)
+                    if (uses[t] == 1 && !M.point_nodes.count(t)) own.push_back(t);
+            cnodes[ent] = pos_of(own);
+        }
+    }
+    std::map<int32_t, int> pos_to_curve;  // first entity wins
+    for (const auto& kv : cnodes)
+        for (int32_t p : kv.second) pos_to_curve.emplace(p, kv.first);
+    std::set<int> faces;
+    for (const auto& kv : M.face_phys)
+        if (!kv.second.empty()) faces.insert(kv.first);
+    if (faces.empty()) faces = M.face_entities;
+    std::map<int, std::set<int32_t>> fnodes;
+    std::map<int, std::array<double, 3>> fcen;
+    auto centroid = [&](const std::set<int32_t>& ps) {
+        std::array<double, 3> c = {0.0, 0.0, 0.0};
+        for (int32_t p : ps)
+            for (int d = 0; d < 3; d++) c[d] += X[3 * (int64_t)(p - 1) + d];
+        for (int d = 0; d < 3; d++) c[d] /= (double)ps.size();
+        return c;
+    };
+    for (int f : faces) {
+        auto it = M.face_nodes.find(f);
+        if (it == M.face_nodes.end()) continue;
+        std::set<int32_t> ps = pos_of(it->second);
+        if (ps.empty()) continue;
+        fcen[f] = centroid(ps);
+        fnodes[f] = std::move(ps);
+    }
+    std::map<int, std::array<double, 3>> target;  // curve entity -> nearest adjacent surface centroid
+    for (const auto& kv : cnodes) {
+        if (kv.second.empty()) continue;
+        const std::array<double, 3> cc = centroid(kv.second);
+        bool found = false;
+        double best = 0.0;
+        for (const auto& fv : fnodes) {
+            bool adj = false;
+            for (int32_t p : kv.second) if (fv.second.count(p)) { adj = true; break; }
+            if (!adj) continue;
+            const std::array<double, 3>& sc = fcen[fv.first];
+            const double d = (cc[0] - sc[0]) * (cc[0] - sc[0]) + (cc[1] - sc[1]) * (cc[1] - sc[1]) +
+                             (cc[2] - sc[2]) * (cc[2] - sc[2]);
+            if (!found || d < best) { best = d; target[kv.first] = sc; found = true; }
+        }
+    }
+    for (size_t e = 0; e < out.family.size(); e++) {
+        int32_t* n = &out.nodes[8 * e];
+        auto it = pos_to_curve.find(n[0]);
+        if (it == pos_to_curve.end()) it = pos_to_curve.find(n[1]);
+        if (it == pos_to_curve.end()) { out.flags |= 8; continue; }
+        auto jt = target.find(it->second);
+        if (jt == target.end()) { out.flags |= 8; continue; }
+        const double* x1 = X + 3 * (int64_t)(n[0] - 1);
+        const double* x2 = X + 3 * (int64_t)(n[1] - 1);
+        double mid[3];
+        for (int d = 0; d < 3; d++) mid[d] = out.family[e] == 4 ? X[3 * (int64_t)(n[2] - 1) + d] : 0.5 * (x1[d] + x2[d]);
+        const double dx = x2[0] - x1[0], dy = x2[1] - x1[1];
+        const double tl = std::sqrt(dx * dx + dy * dy);
+        if (tl < 2.220446049250313e-16) continue;
+        const double nx = -dy / tl, ny = dx / tl;
+        const double dot = (nx * (jt->second[0] - mid[0]) + ny * (jt->second[1] - mid[1])) + 0.0 * (jt->second[2] - mid[2]);
+        if (dot < 0.0) { std::swap(n[0], n[1]); out.flags |= 16; }
+    }
+}
+
 void build_from_msh(const MshModel& M, rvf_meshfile& out) {
     const int64_t Nn = (int64_t)M.node_tags.size();
     // node tag -> 1-based index: a dense table when tags are compact, else a map
@@ -374,18 +520,19 @@ void build_from_msh(const MshModel& M, rvf_meshfile& out) {
         return it == sparse_idx.end() ? 0 : it->second;
     };
     out.coords = M.xyz;
-    // physical groups of dimension 2 (_read_physical_groups, src/MeshIO.jl:217-226)
-    std::map<int, std::vector<int>> ents_of;  // physical tag -> surface entities (ascending)
+    const std::string D = std::to_string(M.dim);
+    // physical groups of dimension surface_dim (_read_physical_groups, src/MeshIO.jl:217-226)
+    std::map<int, std::vector<int>> ents_of;  // physical tag -> entities of that dimension (ascending)
     std::set<int> phys = M.phys2;
     for (const auto& kv : M.phys_names)
-        if (kv.first.first == 2) phys.insert(kv.first.second);
+        if (kv.first.first == M.dim) phys.insert(kv.first.second);
     if (!M.v2)
         for (const auto& kv : M.surf_phys)
             for (int p : kv.second) ents_of[p].push_back(kv.first);
     if (phys.empty()) {
         // _synthesize_default_group (src/MeshIO.jl:236-246): one group over every
         // surface entity, tagged one above the largest physical tag of any dimension
-        if (M.surf_entities.empty()) bad("No entities of dimension 2 found in mesh; nothing to load.");
+        if (M.surf_entities.empty()) bad("No entities of dimension " + D + " found in mesh; nothing to load.");
         int top = 0;
         for (const auto& kv : M.phys_names) top = std::max(top, kv.first.second);
         const int g = top + 1;
@@ -395,7 +542,7 @@ void build_from_msh(const MshModel& M, rvf_meshfile& out) {
         phys.insert(g);
     } else {
         for (int g : phys) {
-            auto it = M.phys_names.find({2, g});
+            auto it = M.phys_names.find({M.dim, g});
             out.groups[g] = (it == M.phys_names.end() || it->second.empty()) ? "group_" + std::to_string(g)
                                                                              : it->second;
         }
@@ -407,9 +554,11 @@ void build_from_msh(const MshModel& M, rvf_meshfile& out) {
         std::stable_sort(kv.second.begin(), kv.second.end(), [](const RawElem* a, const RawElem* b) {
             return msh_is_tri(a->type) && !msh_is_tri(b->type);
         });
+    std::set<int> used_ents;  // entities of the groups read (curve orientation)
     auto push = [&](const RawElem& e, int g) {
         Fam f;
-        msh_family(e.type, f);
+        msh_family(e.type, M.dim, f);
+        used_ents.insert(e.entity);
         out.family.push_back((int8_t)f.code);
         for (int a = 0; a < 8; a++) {
             if (a < f.nkeep) {
@@ -452,8 +601,10 @@ void build_from_msh(const MshModel& M, rvf_meshfile& out) {
         }
     }
     if (out.family.empty())
-        bad("No supported elements found in physical groups (dimension 2). Supported types (1st or 2nd "
-            "order): Tri3 (2), Quad4 (3), Tri6 (9), Quad8 (16), Quad9 (10)");
+        bad("No supported elements found in physical groups (dimension " + D + "). Supported types (1st or 2nd "
+            "order): " + (M.dim == 1 ? std::string("Line2 (1), Line3 (8)")
+                                     : std::string("Tri3 (2), Quad4 (3), Tri6 (9), Quad8 (16), Quad9 (10)")));
+    if (M.dim == 1) orient_lines(M, used_ents, lookup, out);
 }
 
 // ---------------------------------------------------------------------------
@@ -732,7 +883,8 @@ void read_vtu(const std::string& s, int32_t surface_dim, const char* group_field
         else if (t == 22) { code = 1; nn = 6; dim = 2; }
         else if (t == 9) { code = 2; nn = 4; dim = 2; }
         else if (t == 23) { code = 0; nn = 8; dim = 2; }
-        else if (t == 3 || t == 21) dim = 1;
+        else if (t == 3) { code = 5; nn = 2; dim = 1; }
+        else if (t == 21) { code = 4; nn = 3; dim = 1; }  // VTK_QUADRATIC_EDGE: ends, then midpoint
         if (code < 0 || dim != surface_dim) continue;
         long long lo, hi;
         if ((long long)offs.size() == ncells + 1) { lo = (long long)offs[k]; hi = (long long)offs[k + 1]; }
@@ -754,7 +906,7 @@ void read_vtu(const std::string& s, int32_t surface_dim, const char* group_field
         out.group.push_back(g);
         out.groups[g] = have_g ? "group_" + std::to_string(g) : "default";
     }
-    if (out.family.empty()) bad("No VTK cells of dimension 2 found");
+    if (out.family.empty()) bad("No VTK cells of dimension " + std::to_string(surface_dim) + " found");
 }
 
 bool is_xml_vtk(const std::string& path, const std::string& head) {
@@ -767,10 +919,11 @@ bool is_xml_vtk(const std::string& path, const std::string& head) {
     return head.find("<?xml") != std::string::npos || head.find("<VTKFile") != std::string::npos;
 }
 
-// _reverse_all_normals! (src/MeshIO.jl:545-562)
+// _reverse_all_normals! (src/MeshIO.jl:613-632)
 void reverse_normals(rvf_meshfile& m) {
     for (size_t e = 0; e < m.family.size(); e++) {
         int32_t* n = &m.nodes[8 * e];
+        if (m.family[e] >= 4) { std::swap(n[0], n[1]); continue; }  // Line2 / Line3: endpoints
         std::swap(n[0], n[2]);
         if (m.family[e] == 0) std::swap(n[4], n[6]);
         if (m.family[e] == 1) std::swap(n[3], n[5]);
@@ -785,8 +938,8 @@ int rvf_meshfile_read(const char* path, int32_t surface_dim, int32_t reverse, co
                       rvf_meshfile** out) {
     if (!path || !out) return rvfi_fail(RVF_E_ARG, "rvf_meshfile_read: bad arguments");
     *out = nullptr;
-    if (surface_dim != 2)
-        return rvfi_fail(RVF_E_ARG, "rvf_meshfile_read: surface_dim must be 2 (curve meshes are not supported)");
+    if (surface_dim != 1 && surface_dim != 2)
+        return rvfi_fail(RVF_E_ARG, "rvf_meshfile_read: surface_dim must be 1 or 2, got " + std::to_string(surface_dim));
     FILE* f = std::fopen(path, "rb");
     if (!f) return rvfi_fail(RVF_E_ARG, std::string("Mesh file not found: ") + path);
     std::string s;
@@ -802,6 +955,7 @@ int rvf_meshfile_read(const char* path, int32_t surface_dim, int32_t reverse, co
             read_vtu(s, surface_dim, group_field, *m);
         } else {
             MshModel M;
+            M.dim = surface_dim;
             read_msh(s, M);
             build_from_msh(M, *m);
         }

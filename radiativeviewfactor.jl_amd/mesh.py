@@ -155,8 +155,9 @@ def load_mesh(filename: str, surface_dim: int = 2, reverse_normals: bool = False
     """Mirror of load_mesh (src/MeshIO.jl:94-145): a Gmsh .msh (format 2.2 or
     4.1, ASCII or binary) or an XML VTK unstructured grid, read by the
     library's native reader (rvf_meshfile_read), with the obstruction soups
-    of every group built as _build_group_obs_soups does.  Only surface meshes
-    (surface_dim 2) are read; curve meshes are CPU-only in the reference."""
+    of every group built as _build_group_obs_soups does.  surface_dim 2 reads
+    surfaces; 1 reads curves (Line2, Line3), whose elements in a .msh file are
+    turned to face the surface next to their curve (_orient_line3_normals!)."""
     import ctypes as C
     from . import _lib
     L = _lib.lib()
@@ -182,7 +183,17 @@ def load_mesh(filename: str, surface_dim: int = 2, reverse_normals: bool = False
         L.rvf_meshfile_free(h)
     m = MeshData(coords, fam, nodes, grp, tags)
     m.group_tri_soup = build_group_obs_soups(m)
-    if verbose:
+    if verbose and surface_dim == 1:
+        if flags & 1:
+            print('No physical groups of dimension 1 found; created a synthetic "default" group.')
+        if flags & 16:
+            print("  Reoriented line element(s) to point toward the adjacent surface interior.")
+        if flags & 8:
+            print("  Warning: some line elements have no curve entity or adjacent surface; their normals "
+                  "follow the node order.")
+        print(f"Loaded {m.n_elems} curve elements (line3: {int(np.sum(fam == 4))}, line2: "
+              f"{int(np.sum(fam == 5))}) in {len(tags)} physical group(s).")
+    elif verbose:
         counts = {n: int(np.sum(fam == c)) for n, c in (("quad8", 0), ("quad4", 2), ("tri6", 1), ("tri3", 3))}
         if flags & 1:
             print('No physical groups of dimension 2 found; created a synthetic "default" group.')

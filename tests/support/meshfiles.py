@@ -169,7 +169,7 @@ def write_msh(m: GmshModel, path: str, version: str = "4.1", binary: bool = Fals
 # ---------------------------------------------------------------------------
 # XML VTK
 # ---------------------------------------------------------------------------
-VTK_TYPE = {3: 5, 1: 22, 2: 9, 0: 23}  # family code -> VTK cell type
+VTK_TYPE = {3: 5, 1: 22, 2: 9, 0: 23, 5: 3, 4: 21}  # family code -> VTK cell type
 
 
 def write_vtu(path: str, xyz: np.ndarray, cells: List[Tuple[int, List[int]]], cell_data: Dict[str, np.ndarray],
@@ -393,6 +393,74 @@ def two_plates_model(order: int, h: float = 0.2, seed: int = 0x5EED0102) -> Gmsh
     order_tags = np.concatenate([e.node_tags for e in model_ents])
     names = {(2, 1): "bottom", (2, 2): "top"}
     return GmshModel(order_tags, xyz[order_tags - 1], model_ents, names)
+
+
+def square_curves_model(order: int, n: int = 4, seed: int = 0x5EED0103, physical_surface: bool = True,
+                        tag_offset: int = 50) -> GmshModel:
+    """The unit square as a 2-D Gmsh model: a structured n x n Quad4 / Quad8
+    surface (entity 1, physical group 20 "domain" unless physical_surface is
+    false) bounded by four curves (entities 1..4: bottom, right, top, left;
+    physical groups 11..14) of n Line2 / Line3 elements, corners on point
+    entities with point elements.  A seeded half of the line elements is wound
+    backwards, so their counter-clockwise normals face out of the square until
+    the reader reorients them."""
+    rng = np.random.default_rng(seed)
+    m2 = 2 * n  # doubled grid: (even, even) corners, one odd index: edge midpoints
+    keys, xyz = {}, []
+
+    def node(i, j):
+        if (i, j) not in keys:
+            keys[(i, j)] = len(xyz)
+            xyz.append((i / m2, j / m2, 0.0))
+        return keys[(i, j)]
+    quads = []
+    for a in range(0, m2, 2):
+        for b in range(0, m2, 2):
+            q = [node(a, b), node(a + 2, b), node(a + 2, b + 2), node(a, b + 2)]
+            if order == 2:
+                q += [node(a + 1, b), node(a + 2, b + 1), node(a + 1, b + 2), node(a, b + 1)]
+            quads.append(q)
+    sides = {1: lambda t: (t, 0), 2: lambda t: (m2, t), 3: lambda t: (m2 - t, m2), 4: lambda t: (0, m2 - t)}
+    lines = {}
+    for c, f in sides.items():  # walking counter-clockwise: the inward-facing order
+        rows = []
+        for t in range(0, m2, 2):
+            a, b = node(*f(t)), node(*f(t + 2))
+            r = [a, b] + ([node(*f(t + 1))] if order == 2 else [])
+            if rng.random() < 0.5:
+                r[0], r[1] = r[1], r[0]
+            rows.append(r)
+        lines[c] = np.array(rows, np.int64)
+    xyz = np.array(xyz, np.float64)
+    ntag = np.arange(len(xyz), dtype=np.int64) + tag_offset
+    rng.shuffle(ntag)
+    corner = {(0, 0): 1, (m2, 0): 2, (m2, m2): 3, (0, m2): 4}
+    pt, cv, sf = {k: [] for k in range(1, 5)}, {k: [] for k in range(1, 5)}, []
+    for (i, j), k in keys.items():
+        if (i, j) in corner:
+            pt[corner[(i, j)]].append(ntag[k])
+        elif j == 0:
+            cv[1].append(ntag[k])
+        elif i == m2:
+            cv[2].append(ntag[k])
+        elif j == m2:
+            cv[3].append(ntag[k])
+        elif i == 0:
+            cv[4].append(ntag[k])
+        else:
+            sf.append(ntag[k])
+    lt = 8 if order == 2 else 1
+    ents = [Entity(0, k, [], [(15, np.array([[v] for v in pt[k]], np.int64))], np.array(pt[k], np.int64))
+            for k in range(1, 5)]
+    ents += [Entity(1, k, [10 + k], [(lt, ntag[lines[k]])], np.array(cv[k], np.int64)) for k in range(1, 5)]
+    ents.append(Entity(2, 1, [20] if physical_surface else [], [(16 if order == 2 else 3, ntag[np.array(quads)])],
+                       np.array(sf, np.int64)))
+    order_tags = np.concatenate([e.node_tags for e in ents if len(e.node_tags)])
+    pos = {int(t): k for k, t in enumerate(ntag)}
+    names = {(1, 11): "bottom", (1, 12): "right", (1, 13): "top", (1, 14): "left"}
+    if physical_surface:
+        names[(2, 20)] = "domain"
+    return GmshModel(order_tags, xyz[[pos[int(t)] for t in order_tags]], ents, names)
 
 
 def model_elements(m: GmshModel):

@@ -12,7 +12,12 @@ Beyond those: the elements read (family, node coordinates, group) equal the
 written ones in every format; Quad9 keeps its first 8 nodes; reverse_normals
 swaps nodes as _reverse_all_normals! (src/MeshIO.jl:545-562); unnamed groups
 are named group_<tag>; an entity in two groups contributes to both; bad
-files fail with a message.  Areas and view factors come from the oracle.
+files fail with a message.  Curve meshes (surface_dim 1, src/MeshIO.jl:20-34,
+469-599): a square's four boundary curves of Line2 / Line3, half wound
+backwards, are read and turned to face the square (the surface next to their
+curves) in every .msh encoding, reverse_normals then turns them all out, and
+the square's opposite sides see each other with Hottel's sqrt(2) - 1; VTK line
+cells keep their winding.  Areas and view factors come from the oracle.
 """
 import os
 
@@ -185,4 +190,90 @@ def test_errors(tmp_path):
     with pytest.raises(RVFError):
         M.load_mesh(str(p), verbose=False)
     with pytest.raises(RVFError, match="surface_dim"):
+        M.load_mesh(_write_msh(tmp_path, MF.plate_model(1)), surface_dim=3, verbose=False)
+    with pytest.raises(RVFError, match=r"dimension 1\). Supported types .*Line2"):
         M.load_mesh(_write_msh(tmp_path, MF.plate_model(1)), surface_dim=1, verbose=False)
+
+
+# ------------------------------------------------------------------------------
+# curve meshes (surface_dim 1): Line2 / Line3 in physical groups of dimension 1,
+# turned to face the adjacent surface (_orient_line3_normals!, src/MeshIO.jl:469-599)
+# ------------------------------------------------------------------------------
+def _inward(orc, m):
+    """Cosine between each element's normal at xi = 0 and the direction from
+    its midpoint to the square's centre."""
+    out = []
+    for e in range(m.n_elems):
+        x, n, _ = orc.eval_point(int(m.family[e]), m.coords, m.nodes[e], 0.0, 0.0)
+        d = np.array([0.5, 0.5, 0.0]) - x
+        out.append(float(n @ d) / np.linalg.norm(d))
+    return np.array(out)
+
+
+@pytest.mark.parametrize("version,binary", MSH_FORMATS)
+@pytest.mark.parametrize("order", [1, 2])
+def test_curves_read_and_oriented(orc, tmp_path, version, binary, order):
+    model = MF.square_curves_model(order)
+    p = str(tmp_path / "sq.msh")
+    MF.write_msh(model, p, version, binary)
+    m = M.load_mesh(p, surface_dim=1, verbose=False)
+    assert m.mesh_dim == 1 and set(m.family) == {5 if order == 1 else 4}
+    assert m.group_tags == {11: "bottom", 12: "right", 13: "top", 14: "left"}
+    assert m.n_elems == 16 and list(np.bincount(m.group)[11:15]) == [4, 4, 4, 4]
+    assert np.all(_inward(orc, m) > 0.5)        # every element faces into the square
+    # the written elements, up to the endpoint swap the reader makes
+    pos = {int(t): k for k, t in enumerate(model.node_tags)}
+    want = []
+    for e in sorted((e for e in model.entities if e.dim == 1), key=lambda e: e.tag):
+        for row in e.blocks[0][1]:
+            want.append({tuple(model.xyz[pos[int(x)]]) for x in row})
+    got = [{tuple(m.coords[k - 1]) for k in m.nodes[e, :M.FAMILY_NNODES[int(m.family[e])]]} for e in range(m.n_elems)]
+    assert got == want
+    if order == 2:  # the midpoint stays the third node
+        for e in range(m.n_elems):
+            a, b, c = (m.coords[k - 1] for k in m.nodes[e, :3])
+            assert np.allclose(c, 0.5 * (a + b))
+    # segment soups from the two endpoints (src/MeshIO.jl:672-680)
+    assert m.group_tri_soup[11].shape == (4, 2, 3)
+    r = orc.compute(m, nquad=4)
+    F = r["raw"] / r["A"][:, None]
+    tags, names, Fg, Ag = orc.aggregate(m, F, r["A"])
+    assert abs(Fg[names.index("bottom"), names.index("top")] - (2 ** 0.5 - 1)) < 1e-9
+    # reverse_normals: applied after the orientation, every element faces out
+    mr = M.load_mesh(p, surface_dim=1, reverse_normals=True, verbose=False)
+    assert np.all(_inward(orc, mr) < -0.5)
+
+
+def test_curves_without_adjacent_surface(tmp_path):
+    # no physical surface and (format 2.2) no surface elements written: the
+    # winding is kept and the reader says so
+    from rvf_amd import _lib
+    import ctypes as C
+    model = MF.square_curves_model(1)
+    model.entities = [e for e in model.entities if e.dim != 2]
+    model.node_tags = np.concatenate([e.node_tags for e in model.entities if len(e.node_tags)])
+    pos = {int(t): k for k, t in enumerate(MF.square_curves_model(1).node_tags)}
+    full = MF.square_curves_model(1)
+    model.xyz = full.xyz[[pos[int(t)] for t in model.node_tags]]
+    p = str(tmp_path / "c.msh")
+    MF.write_msh(model, p, "4.1", False)
+    h = C.c_void_p()
+    _lib.check(_lib.lib().rvf_meshfile_read(p.encode(), 1, 0, None, C.byref(h)))
+    fl = C.c_int32()
+    _lib.lib().rvf_meshfile_info(h, None, None, None, C.byref(fl))
+    _lib.lib().rvf_meshfile_free(h)
+    assert fl.value & 8 and not fl.value & 16
+
+
+def test_curves_vtu(orc, tmp_path):
+    m0 = M.circle_interior(12, 1.0, 2)
+    cells = [(21, list(m0.nodes[e, :3] - 1)) for e in range(m0.n_elems)] + [(5, [0, 1, 2])]
+    p = str(tmp_path / "circle.vtu")
+    MF.write_vtu(p, m0.coords, cells, {"CellEntityIds": np.concatenate([m0.group, [7]])}, encoding="binary")
+    v = M.load_mesh(p, surface_dim=1, verbose=False)
+    assert np.array_equal(v.nodes, m0.nodes) and np.array_equal(v.family, m0.family)
+    s = M.load_mesh(p, surface_dim=2, verbose=False)   # the triangle only
+    assert s.n_elems == 1 and s.family[0] == 3
+    r0 = orc.compute(m0, nquad=3)
+    r1 = orc.compute(v, nquad=3)
+    assert np.array_equal(r0["raw"], r1["raw"])
