@@ -241,6 +241,7 @@ def main():
         out["unoccluded_cube"] = secondary_cube(api, ctx, args.steps)
     if args.secondary:
         out["self_vf_sphere"] = secondary_self_vf(api, ctx, args.steps)
+        out["curves"] = secondary_curves(api, ctx, args.steps, 0.0 if args.no_cpu else 5.0)
     if args.serving and args.serving_requests > 0:
         out["serving"] = serving_stream(api, ctx, args.serving_requests, args.serving_rate)
     print(json.dumps(out))
@@ -372,6 +373,56 @@ def secondary_self_vf(api, ctx, steps, n=48):
     res["rowsum_minus_1_max_fp64"] = {"self_vf": float(np.abs(rsd - 1).max()),
                                       "without": float(np.abs(out["fp64", False][1] - 1).max())}
     dm.close()
+    return res
+
+
+def secondary_curves(api, ctx, steps, cpu_seconds):
+    """Curve meshes (per unit depth; CPU only in the reference): the interior
+    of a circle as 8,192 Line3 arcs with self view factors, and the unit
+    square's interior as 4 x 2,048 Line2 elements around an 8-segment baffle
+    (obstruction-only), nquad 4; mixed and FP64 solve times, the mixed-vs-FP64
+    differences, and the oracle's rate on a sample of the circle's rows."""
+    from rvf_amd import mesh as M
+    res = {}
+    for name, mesh, obs, self_vf in (
+            ("circle", M.circle_interior(8192, 1.0, 2), [], True),
+            ("square_baffle", M.square_enclosure_2d(2048, 1, baffle=True), [5], False)):
+        N = mesh.n_elems
+        dm = api.DeviceMesh(ctx, mesh)
+        do = api.DeviceObstruction(ctx, dm, obs, "geometric") if obs else None
+        r = {"workload": f"{name}: {N:,} {'Line3' if mesh.family[0] == 4 else 'Line2'} elements, nquad=4"
+                         + (", self_vf" if self_vf else "") + (", 8 occluding segments" if obs else "")}
+        Fs = {}
+        for prec in ("mixed", "fp64"):
+            plan, ms, lat, pair = run_mode(api, ctx, dm, do, 4, prec, steps, 1, self_vf=self_vf)
+            st = plan.stats()
+            F, A, rs, *_ = plan.fetch()
+            Fs[prec] = (F, rs)
+            pairs = N * (N + 1) / 2 if self_vf else N * (N - 1) / 2
+            r[prec] = {"ms_per_step": ms, "pair_kernel_ms": float(pair.mean()), "unique_pairs_per_s": pairs / (ms * 1e-3),
+                       "kernel": plan.kernel_name(), "rays": st.get("rays")}
+            plan.close()
+        r["max_abs_dF"] = float(np.abs(Fs["mixed"][0] - Fs["fp64"][0]).max())
+        r["closure_err"] = float(np.abs(Fs["mixed"][1] - Fs["fp64"][1]).max())
+        r["rowsum_minus_1_max_fp64"] = float(np.abs(Fs["fp64"][1] - 1.0).max())
+        if do is not None:
+            do.close()
+        dm.close()
+        if cpu_seconds > 0 and name == "circle":
+            sys.path.insert(0, os.path.join(ROOT, "oracle"))
+            import oracle as orc
+            rng = np.random.default_rng(2)
+            t0, pairs, rows = time.perf_counter(), 0, 0
+            while time.perf_counter() - t0 < cpu_seconds and rows < N:
+                r0 = int(rng.integers(0, N - 64))
+                orc.compute(mesh, nquad=4, rows=(r0, r0 + 64), want_raw=False, nthreads=os.cpu_count() or 1,
+                            self_vf=True)
+                pairs += sum(N - i for i in range(r0, r0 + 64))
+                rows += 64
+            cpu = pairs / (time.perf_counter() - t0)
+            r["cpu_oracle_unique_pairs_per_s"] = cpu
+            r["gpu_fp64_vs_cpu"] = r["fp64"]["unique_pairs_per_s"] / cpu
+        res[name] = r
     return res
 
 

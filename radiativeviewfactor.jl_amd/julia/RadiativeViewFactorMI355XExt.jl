@@ -10,6 +10,8 @@
 # chained, not replaced: other backends still reach the previous target.
 # Knobs the reference hook drops (precision, Duffy, MC seed, device, soup
 # reading) travel on the backend struct (src/Assembly.jl:77-78,243-252).
+# Curve meshes (mesh_dim 1), which compute_view_factors refuses on any GPU
+# backend (src/Assembly.jl:83-86), go through compute_view_factors_mi355x.
 # ---------------------------------------------------------------------------
 module RadiativeViewFactorMI355XExt
 
@@ -53,7 +55,8 @@ function _ctx(device::Int)
     return _CTX[]
 end
 
-const _FAM = Dict(:quad => Int8(0), :tri => Int8(1), :quad4 => Int8(2), :tri3 => Int8(3))
+const _FAM = Dict(:quad => Int8(0), :tri => Int8(1), :quad4 => Int8(2), :tri3 => Int8(3),
+                  :line3 => Int8(4), :line2 => Int8(5))
 
 function rocm_solve(mesh::MeshData, nquad::Int, b::MI355XBackend;
                     obstruction_groups::Vector{Int} = Int[], verbose::Bool = true,
@@ -61,32 +64,44 @@ function rocm_solve(mesh::MeshData, nquad::Int, b::MI355XBackend;
     ctx = _ctx(b.device)
     elems = mesh.surface_elems
     N = length(elems)
-    # build_gpu_arrays layout (src/GPUKernels.jl:344-385)
-    lists = [Vector{Int32}[] for _ in 1:4]
+    curve = mesh.mesh_dim == 1
+    # build_gpu_arrays layout (src/GPUKernels.jl:344-385); curve meshes: Line3, Line2
+    nfam = (8, 6, 4, 3, 3, 2)
+    lists = [Vector{Int32}[] for _ in 1:6]
     fam = Vector{Int8}(undef, N); idx = Vector{Int32}(undef, N); grp = Vector{Int32}(undef, N)
     for (i, el) in enumerate(elems)
         f = get(_FAM, el.family) do
-            error("GPU backend does not support element family $(el.family) (curve/Line elements are CPU-only).")
+            error("GPU backend does not support element family $(el.family).")
         end
+        (f >= 4) == curve || error("element family $(el.family) in a mesh of mesh_dim $(mesh.mesh_dim)")
         push!(lists[f + 1], Int32.(el.nodes))
         fam[i] = f; idx[i] = Int32(length(lists[f + 1])); grp[i] = Int32(el.group)
     end
-    mats = [isempty(l) ? zeros(Int32, (8, 6, 4, 3)[k], 0) : reduce(hcat, l) for (k, l) in enumerate(lists)]
-    ptrs = [pointer(m) for m in mats]
+    mats = [isempty(l) ? zeros(Int32, nfam[k], 0) : reduce(hcat, l) for (k, l) in enumerate(lists)]
     counts = Int64[size(m, 2) for m in mats]
     hmesh = Ref{Ptr{Cvoid}}(C_NULL)
     GC.@preserve mats begin
-        _check(ccall((:rvf_mesh_upload, LIB), Cint,
-                     (Ptr{Cvoid}, Ptr{Float64}, Int64, Int64, Ptr{Int8}, Ptr{Int32}, Ptr{Int32},
-                      Ptr{Ptr{Int32}}, Ptr{Int64}, Ref{Ptr{Cvoid}}),
-                     ctx, mesh.coords, size(mesh.coords, 2), N, fam, idx, grp, ptrs, counts, hmesh))
+        if curve
+            _check(ccall((:rvf_curve_mesh_upload, LIB), Cint,
+                         (Ptr{Cvoid}, Ptr{Float64}, Int64, Int64, Ptr{Int8}, Ptr{Int32}, Ptr{Int32},
+                          Ptr{Int32}, Int64, Ptr{Int32}, Int64, Ref{Ptr{Cvoid}}),
+                         ctx, mesh.coords, size(mesh.coords, 2), N, fam, idx, grp, mats[5], counts[5],
+                         mats[6], counts[6], hmesh))
+        else
+            ptrs = [pointer(m) for m in mats[1:4]]
+            _check(ccall((:rvf_mesh_upload, LIB), Cint,
+                         (Ptr{Cvoid}, Ptr{Float64}, Int64, Int64, Ptr{Int8}, Ptr{Int32}, Ptr{Int32},
+                          Ptr{Ptr{Int32}}, Ptr{Int64}, Ref{Ptr{Cvoid}}),
+                         ctx, mesh.coords, size(mesh.coords, 2), N, fam, idx, grp, ptrs, counts[1:4], hmesh))
+        end
     end
     hobs = Ref{Ptr{Cvoid}}(C_NULL)
     tags = sort([g for g in unique(obstruction_groups) if haskey(mesh.group_tri_soup, g)])
     if !isempty(tags)
         soup = cat([mesh.group_tri_soup[g] for g in tags]...; dims = 3)
         tg = reduce(vcat, [fill(Int32(g), size(mesh.group_tri_soup[g], 3)) for g in tags])
-        soup_read = b.parity === :reference ? Int32(0) : Int32(1)
+        # segment soups (3,2,T) of curve meshes have one reading (RVF_SOUP_SEGMENTS)
+        soup_read = curve ? Int32(3) : b.parity === :reference ? Int32(0) : Int32(1)
         _check(ccall((:rvf_obs_build, LIB), Cint,
                      (Ptr{Cvoid}, Ptr{Cvoid}, Ptr{Float64}, Ptr{Int32}, Int64, Ptr{Int32}, Int32, Int32,
                       Ref{Ptr{Cvoid}}),
@@ -119,12 +134,25 @@ function rocm_solve(mesh::MeshData, nquad::Int, b::MI355XBackend;
     return ViewFactorResult(F, A, Fg, Ag, group_tags, group_names)
 end
 
+"""
+    compute_view_factors_mi355x(mesh; nquad=4, obstruction_groups=Int[], backend=MI355XBackend(),
+                                monte_carlo=false, n_samples=10000, verbose=true)
+
+The MI355X solve for any mesh, curve meshes (mesh_dim 1) included, with the
+keywords of compute_view_factors (src/Assembly.jl:64-73).
+"""
+function compute_view_factors_mi355x(mesh::MeshData; nquad::Int = 4, obstruction_groups::Vector{Int} = Int[],
+                                     backend::MI355XBackend = MI355XBackend(), monte_carlo::Bool = false,
+                                     n_samples::Int = 10000, verbose::Bool = true)
+    return rocm_solve(mesh, nquad, backend; obstruction_groups, verbose, monte_carlo, n_samples)
+end
+
 function __init__()
     prev = _GPU_HOOK_REF[]
     register_gpu_hook!((m, nq, b, FT, AT; kw...) ->
         b isa MI355XBackend ? rocm_solve(m, nq, b; kw...) : prev(m, nq, b, FT, AT; kw...))
 end
 
-export MI355XBackend
+export MI355XBackend, compute_view_factors_mi355x
 
 end # module
