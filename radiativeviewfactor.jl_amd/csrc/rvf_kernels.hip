@@ -1680,6 +1680,72 @@ __device__ __forceinline__ void tile_mixed_occ2d(const PairArgs& a, int tile) {
     }
 }
 
+// The same capsule walked down a tree of padded FP32 boxes (preorder, miss
+// links, interior start_count = right child << 4).  Returns -1 when it reaches
+// no leaf's box: then no ray of the pair can reach a triangle, and none is
+// blocked.  Otherwise returns the node the pair's rays may start from: the
+// descent from the root follows the only child the capsule meets while there
+// is one, so every node a ray of the pair could pass is in that node's
+// subtree or is a sibling the capsule misses (which a ray leaving the subtree
+// then tests and fails).  Box tests use the capsule's inverse direction; their
+// rounding is far inside the boxes' padding.  After `limit` box tests the walk
+// stops without culling.
+__device__ __forceinline__ int capsule_entry(const double* ai, const double* aj, double R,
+                                             const BvhNode32* __restrict__ nodes, int root, int limit) {
+    double inv[3];
+    bool flat[3];
+#pragma unroll
+    for (int c = 0; c < 3; c++) {
+        const double d = aj[c] - ai[c];
+        flat[c] = d == 0.0;
+        inv[c] = flat[c] ? 0.0 : 1.0 / d;
+    }
+    auto meets = [&](const BvhNode32& nd) {
+        double tmin = 0.0, tmax = 1.0;
+#pragma unroll
+        for (int c = 0; c < 3; c++) {
+            const double lo = (double)nd.lo[c] - R, hi = (double)nd.hi[c] + R;
+            if (flat[c]) {
+                if (ai[c] < lo || ai[c] > hi) tmax = -1.0;
+            } else {
+                double t1 = (lo - ai[c]) * inv[c], t2 = (hi - ai[c]) * inv[c];
+                if (t1 > t2) { const double t = t1; t1 = t2; t2 = t; }
+                tmin = fmax(tmin, t1);
+                tmax = fmin(tmax, t2);
+            }
+        }
+        return tmin <= tmax;
+    };
+    if (limit < 1) return root;
+    int n = root, k = 1;
+    BvhNode32 nd = nodes[n];
+    if (!meets(nd)) return -1;
+    while ((nd.start_count & 15) == 0) {  // descend while one child alone is met
+        if (k + 2 > limit) return n;
+        const int l = n + 1, r = nd.start_count >> 4;
+        const BvhNode32 L = nodes[l], Rt = nodes[r];
+        const bool ml = meets(L), mr = meets(Rt);
+        k += 2;
+        if (!ml && !mr) return -1;
+        if (ml && mr) break;
+        n = ml ? l : r;
+        nd = ml ? L : Rt;
+    }
+    if (nd.start_count & 15) return n;  // a leaf the capsule meets
+    // both children met: is any leaf below n met?
+    for (int m = n + 1; m != nd.miss;) {
+        if (k++ >= limit) return n;
+        const BvhNode32 x = nodes[m];
+        if (meets(x)) {
+            if (x.start_count & 15) return n;
+            m = m + 1;
+        } else {
+            m = x.miss;
+        }
+    }
+    return -1;
+}
+
 template <bool MIX, int S, bool SPLIT>
 __device__ __forceinline__ void tile_occ_queue(const PairArgs& a, int tile, double* lds) {
     const TileCtx t = tile_ctx(a, tile);
@@ -1721,15 +1787,24 @@ __device__ __forceinline__ void tile_occ_queue(const PairArgs& a, int tile, doub
         const int npj = a.elem_np[j];
         const int gj = a.elem_grp[j];
         // mixed split traversal walks its own tree over the same triangles
-        const int root = (MIX && SPLIT ? a.ttree_root : a.tree_root)[gi * a.G + gj];
-        // pairs whose rays cannot reach the occluders' root box trace nothing
-        // (their rays count as cast with one box test, as the reference's)
+        int root = (MIX && SPLIT ? a.ttree_root : a.tree_root)[gi * a.G + gj];
+        // pairs whose rays cannot reach the occluders trace nothing (their
+        // rays count as cast; FP64: with one box test, as the reference's)
         bool cap = false;
         if (act && root >= 0) {
             const double ai[3] = {a.anchor[t.i * 4 + 0], a.anchor[t.i * 4 + 1], a.anchor[t.i * 4 + 2]};
             const double aj[3] = {a.anchor[j * 4 + 0], a.anchor[j * 4 + 1], a.anchor[j * 4 + 2]};
             const double R = fmax(a.anchor[t.i * 4 + 3], a.anchor[j * 4 + 3]);
-            cap = capsule_meets_box(ai, aj, R, a.nodes32[a.tree_root[gi * a.G + gj]]);
+            // mixed: down the tree the rays walk, which also gives their start
+            // node; FP64 keeps the root box alone, so its box-test counts stay
+            // the reference's
+            if (MIX) {
+                const int e = capsule_entry(ai, aj, R, SPLIT ? a.tnodes32 : a.nodes32, root, a.cull_limit);
+                cap = e >= 0;
+                if (cap) root = e;
+            } else {
+                cap = capsule_meets_box(ai, aj, R, a.nodes32[a.tree_root[gi * a.G + gj]]);
+            }
         }
         X.j = j; X.oj = oj; X.npj = npj; X.gj = gj;
         X.revmask = __ballot(rev);
