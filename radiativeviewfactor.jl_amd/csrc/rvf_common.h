@@ -143,6 +143,7 @@ struct PairArgs {
     int refill_min;           // wave-queue kernels: refill a slot once this many lanes have it empty
     int verify;               // mixed: confirm FP64 hits by the leaf's exact slab chain
     int cull_limit;           // mixed occluded: box tests a pair's capsule walk may spend (1: root box only)
+    int hint;                 // mixed split traversal: try the pair's last certain blocker before walking
     float Sm;                 // mixed: scene coordinate magnitude (FP32 error bounds)
     // audit (nullable): per pair blocked-ray count / order-free hash at [i + N*j]
     uint32_t* audit_count;

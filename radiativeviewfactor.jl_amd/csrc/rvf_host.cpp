@@ -1062,6 +1062,7 @@ static PairArgs pair_args(const rvf_plan* p) {
         a.refill_min = p->refill_min;
         a.verify = p->obs->soup_read == RVF_SOUP_REFERENCE ? 1 : 0;
         a.cull_limit = 256;  // box tests per pair; measured: 64 leaves 2 % of the node steps, more gains nothing
+        a.hint = 1;
         a.Sm = p->obs->Sm;
     } else {
         a.Sm = 1.0f;
@@ -1120,7 +1121,7 @@ static size_t pair_lds(const rvf_plan* p) {
             ? (mixed ? (RVF_TQCAP_MIX + RVF_CQCAP) * sizeof(uint32_t) + 6 * 64 * sizeof(float)
                      : RVF_TQCAP * sizeof(uint32_t))
             : 0;
-    return accq + RVF_QCOMBO * (sizeof(uint64_t) + 2 * sizeof(uint32_t)) + 64 * sizeof(int32_t) + ring;
+    return accq + RVF_QCOMBO * (sizeof(uint64_t) + 2 * sizeof(uint32_t)) + 2 * 64 * sizeof(int32_t) + ring;
 }
 
 // A solve is three stages on one stream: the prologue (zero fills, point

@@ -1029,6 +1029,7 @@ struct QCtx {
     uint32_t magic;      // floor(c / npj) = (c * magic) >> 16 for c < RVF_QCOMBO
     uint64_t revmask;    // lanes whose element is the pair's inner element
     const int* lroot;    // LDS: per lane, tree root of (lane's group, column group)
+    int* lhint;          // LDS (mixed, split form): per lane, a triangle that certainly blocked a ray of its pair, or -1
     const float* stage;  // LDS (mixed, split form): per lane [dax day daz fax fay faz] x 64
     float fjx, fjy, fjz; // column element's anchor as FP32
 };
@@ -1466,8 +1467,18 @@ __device__ __forceinline__ void split_drain(const PairArgs& a, const QCtx& X, in
                 if (Q.c < Q.nc && req != 0 && (idle || __popcll(req) >= a.refill_min)) {
                     int c = 0, b = 0;
                     if (queue_take(req, Q, need, lane, c, b)) {
-                        tag[s] = (c << 6) | b;
-                        cur[s] = nray_setup(a, X, c, b, R[s]);
+                        // mixed: a triangle that certainly blocked another ray
+                        // of the pair often blocks this one too; a certain
+                        // hit on it settles the ray without a traversal
+                        int h = -1;
+                        if (MIX && a.hint) h = X.lhint[b];
+                        if (h >= 0 && ttest_mixed(a, X, c, b, h) == 2) {
+                            K.c1++;
+                            atomicOr(&blk[c * 2 + (b >> 5)], 1u << (b & 31));
+                        } else {
+                            tag[s] = (c << 6) | b;
+                            cur[s] = nray_setup(a, X, c, b, R[s]);
+                        }
                     }
                 }
             }
@@ -1535,6 +1546,7 @@ __device__ __forceinline__ void split_drain(const PairArgs& a, const QCtx& X, in
                     if (cls == 2 && !a.verify) {
                         K.c1++;
                         atomicOr(&blk[c * 2 + (b >> 5)], 1u << (b & 31));
+                        X.lhint[b] = tri;  // any lane's certain hit will do
                     }
                 } else {
                     K.c1++;
@@ -1764,7 +1776,8 @@ __device__ __forceinline__ void tile_occ_queue(const PairArgs& a, int tile, doub
     uint64_t* need = (uint64_t*)(lds + (MIX ? 0 : a.maxpts * 64));
     uint32_t* blk = (uint32_t*)(need + RVF_QCOMBO);
     int* lroot = (int*)(blk + 2 * RVF_QCOMBO);
-    uint32_t* tq = (uint32_t*)(lroot + 64);
+    int* lhint = lroot + 64;
+    uint32_t* tq = (uint32_t*)(lhint + 64);
     float* stage = (float*)(tq + (SPLIT ? (MIX ? RVF_TQCAP_MIX + RVF_CQCAP : RVF_TQCAP) : 0));
     const double* __restrict__ P64 = a.P64;
     const float* __restrict__ P = a.P32;
@@ -1774,6 +1787,7 @@ __device__ __forceinline__ void tile_occ_queue(const PairArgs& a, int tile, doub
     QCtx X;
     X.sbase = (int64_t)t.s * RVF_STRIP;
     X.lroot = lroot;
+    X.lhint = lhint;
     X.stage = stage;
     for (int64_t j = t.jb; j < t.je; j++) {
         if (j < t.jlo) {
@@ -1809,6 +1823,7 @@ __device__ __forceinline__ void tile_occ_queue(const PairArgs& a, int tile, doub
         X.j = j; X.oj = oj; X.npj = npj; X.gj = gj;
         X.revmask = __ballot(rev);
         lroot[lane] = root;
+        lhint[lane] = -1;
         X.pch = max(1, RVF_QCOMBO / npj);
         X.magic = (65536u + (uint32_t)npj - 1u) / (uint32_t)npj;
         const double* __restrict__ qj64 = a.Q64 + j * a.maxpts * 8;
