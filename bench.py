@@ -229,7 +229,8 @@ def main():
         "traversal": {"mixed_kernel": plan_m.kernel_name(), "fp64_kernel": plan_d.kernel_name(),
                       "fp64_box_tests": st_d.get("aabb_tests"), "fp64_triangle_tests": st_d.get("tri_tests"),
                       "mixed_slot_steps": st_m.get("slot_steps"), "mixed_task_batches": st_m.get("task_batches"),
-                      "mixed_sure_hits": st_m.get("sure_hits"), "mixed_exact_batches": st_m.get("exact_batches")},
+                      "mixed_sure_hits": st_m.get("sure_hits"), "mixed_exact_batches": st_m.get("exact_batches"),
+                      "mixed_culled_rays": st_m.get("culled_rays"), "fp64_culled_rays": st_d.get("culled_rays")},
         "verification": ver, "roofline": roof, "cpu_baseline": cpu,
     }
     if args.autotune:

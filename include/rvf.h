@@ -90,6 +90,9 @@ typedef struct rvf_stats {
     /* mixed mode, split traversal: triangles the FP32 classifier proved hit
      * (no FP64 re-test needed) */
     int64_t sure_hits;
+    /* occluded quadrature, wave-queue kernels: rays of pairs culled before any
+     * traversal (their rays cannot reach an occluder); counted in `rays` */
+    int64_t culled_rays;
 } rvf_stats;
 
 /* --- lifecycle (no reference counterpart: the reference is one Julia process) */

@@ -1470,9 +1470,12 @@ __device__ __forceinline__ void split_drain(const PairArgs& a, const QCtx& X, in
                         // mixed: a triangle that certainly blocked another ray
                         // of the pair often blocks this one too; a certain
                         // hit on it settles the ray without a traversal
-                        int h = -1;
-                        if (MIX && a.hint) h = X.lhint[b];
-                        if (h >= 0 && ttest_mixed(a, X, c, b, h) == 2) {
+                        bool settled = false;
+                        if (MIX && a.hint) {
+                            const int h = X.lhint[b];
+                            settled = h >= 0 && ttest_mixed(a, X, c, b, h) == 2;
+                        }
+                        if (settled) {
                             K.c1++;
                             atomicOr(&blk[c * 2 + (b >> 5)], 1u << (b & 31));
                         } else {
@@ -1782,7 +1785,7 @@ __device__ __forceinline__ void tile_occ_queue(const PairArgs& a, int tile, doub
     const double* __restrict__ P64 = a.P64;
     const float* __restrict__ P = a.P32;
     double ls = 0.0, lc = 0.0;
-    uint32_t nrays = 0, culled = 0;  // rays cast; of those, rays of pairs culled at the root box
+    uint32_t nrays = 0, culled = 0;  // rays cast; of those, rays of pairs culled before any traversal
     QCount K;
     QCtx X;
     X.sbase = (int64_t)t.s * RVF_STRIP;
@@ -1862,6 +1865,7 @@ __device__ __forceinline__ void tile_occ_queue(const PairArgs& a, int tile, doub
                         const bool nd = pv && k != 0.0f;
                         const uint64_t m = __ballot(nd && cap);
                         nrays += nd;
+                        culled += nd && !cap;
                         const int c = (p - p0) * npj + q;
                         if (lane == 0) { need[c] = m; blk[2 * c] = 0u; blk[2 * c + 1] = 0u; }
                     }
@@ -1959,16 +1963,17 @@ __device__ __forceinline__ void tile_occ_queue(const PairArgs& a, int tile, doub
     a.part[(int64_t)t.chunk * NS + t.i] = ls + lc;
     if (a.counters) {
         if (!MIX) K.c0 += culled;  // FP64 box tests: a culled ray's root test
-        unsigned long long v[7] = {nrays, K.c0, K.c1, K.iters, K.steps, K.tbatch, K.ebatch};
+        unsigned long long v[8] = {nrays, K.c0, K.c1, K.iters, K.steps, K.tbatch, K.ebatch, culled};
 #pragma unroll
         for (int o = 32; o > 0; o >>= 1)
 #pragma unroll
-            for (int e = 0; e < 7; e++) v[e] += __shfl_xor(v[e], o, 64);
+            for (int e = 0; e < 8; e++) v[e] += __shfl_xor(v[e], o, 64);
         if (lane == 0) {
             atomicAdd(&a.counters[1], v[0]);
             if (MIX) { atomicAdd(&a.counters[0], v[1]); atomicAdd(&a.counters[8], v[2]); }
             else { atomicAdd(&a.counters[2], v[1]); atomicAdd(&a.counters[3], v[2]); }
             for (int e = 3; e < 7; e++) atomicAdd(&a.counters[e + 1], v[e]);
+            atomicAdd(&a.counters[9], v[7]);
         }
     }
 }
