@@ -143,8 +143,8 @@ def main():
     ap.add_argument("--probe-precision", default="mixed", choices=["mixed", "fp64"])
     args = ap.parse_args()
     if args.gpus != 1:
-        print("bench.py: this build runs on one GPU; multi-GPU sharding of pair tiles is designed "
-              "(see the sharding notes) but not built", file=sys.stderr)
+        print("bench.py: measures one GPU; sharding pair tiles over several GPUs is designed but not built",
+              file=sys.stderr)
         sys.exit(2)
 
     import __graft_entry__ as G

@@ -1,6 +1,6 @@
 """ctypes wrapper of the C oracle — TEST INFRASTRUCTURE ONLY.
 
-Loaded by tests/, ``__graft_entry__.smoke()`` (as the checker) and
+Loaded by tests/, ``__graft_entry__.smoke()`` (to check the GPU result) and
 ``bench.py`` (``cpu_baseline`` leg).  Never imported by the product package.
 
 Also holds numpy restatements of the reference's host-side reductions

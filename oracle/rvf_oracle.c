@@ -2,7 +2,7 @@
  * rvf_oracle.c — TEST INFRASTRUCTURE ONLY.
  *
  * A plain-C, Float64 restatement of RadiativeViewFactor.jl's CPU view-factor
- * path, used as the checker for the HIP library and as the CPU baseline that
+ * path, used as the parity reference for the HIP library and as the CPU baseline that
  * bench.py times beside the GPU.  Only tests/, __graft_entry__.smoke() and
  * bench.py's cpu_baseline leg may load it.  The product path never links,
  * loads or calls this file.

@@ -1,5 +1,6 @@
 """One small occluded solve on cuda:0 through the C ABI, checked against the
-oracle (the checker, imported only here and in tests/bench)."""
+oracle (test infrastructure, imported only here, in tests and in bench.py's
+CPU baseline)."""
 import os
 import sys
 

@@ -18,7 +18,7 @@ is pinned by the mathematics:
     visibility or a shared corner cuts the integrand).
   * segment occlusion: the reference's tree traversal decides as a brute-force
     any-hit over all segments (the path's box test only prunes).
-  * Monte Carlo on curves (a stream layout of this build, stratum pairing of
+  * Monte Carlo on curves (the library's own stream layout, stratum pairing of
     the reference's CPU sampler): converges to the exact value as h^2.
 """
 import math
