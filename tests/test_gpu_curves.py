@@ -175,3 +175,30 @@ def test_curve_mesh_errors():
     rc = _lib.lib().rvf_obs_build(ctx.h, dm.h, ptr(seg), ptr(tg), 1, ptr(obs), 1, 3, ctypes.byref(h))
     assert rc != 0 and "curve meshes only" in _lib.lib().rvf_last_error().decode()
     dm.close()
+
+
+def test_single_arc_graph_and_tuning(orc):
+    # one Line3 arc alone (N = 1): only its self pair; a captured graph replays
+    # it and the tuner has no traversal choice to make on curve meshes
+    m = M.circle_interior(2, 1.0, 2)  # element 0: the upper half circle as one parabolic arc
+    m = M.MeshData(m.coords, m.family[:1], m.nodes[:1], m.group[:1], m.group_tags)
+    g, F = check(orc, m, 5, self_vf=True)
+    assert F.shape == (1, 1) and F[0, 0] > 0 and np.allclose(g["rs"], F[0, 0], rtol=1e-12)
+    ctx = api.context(0)
+    sq = M.square_enclosure_2d(8, 2, baffle=True)
+    dm = api.DeviceMesh(ctx, sq)
+    do = api.DeviceObstruction(ctx, dm, [5], "geometric")
+    eager = api.Plan(ctx, dm, do, 3, "quad", "mixed")
+    eager.run(); eager.sync()
+    ref = eager.fetch()
+    pg = api.Plan(ctx, dm, do, 3, "quad", "mixed", graph=True)
+    v, r, meas = pg.autotune()
+    assert meas == []
+    for _ in range(3):
+        pg.run(); pg.sync()
+        got = pg.fetch()
+        assert np.array_equal(got[0], ref[0]) and np.array_equal(got[2], ref[2])
+        eager.run(); eager.sync()
+    for p in (eager, pg):
+        p.close()
+    do.close(); dm.close()
