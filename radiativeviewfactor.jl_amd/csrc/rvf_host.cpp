@@ -1327,7 +1327,11 @@ int rvf_plan_autotune(rvf_plan* p, int64_t sample_tiles, rvf_tune_entry* ent, in
         rank.push_back({ms, v});
     }
     // 2. re-time the three fastest on the whole problem (or a four times
-    //    larger sample), then the refill thresholds of the winner
+    //    larger sample), then the refill thresholds of the winner.  The
+    //    library default (variant, refill 16) counts 1 % faster than measured:
+    //    a sample can rank near-equal configurations either way.
+    const int vdef = mixed ? RVF_VARIANT_SPLIT(3) : RVF_VARIANT_QUEUE(1);
+    auto score = [&](int v, int refill, float ms) { return v == vdef && refill == 16 ? 0.99f * ms : ms; };
     int best_v = v0, best_r = r0;
     float best = 3.4e38f;
     const int64_t big = std::min<int64_t>(nt, 4 * base);
@@ -1337,7 +1341,8 @@ int rvf_plan_autotune(rvf_plan* p, int64_t sample_tiles, rvf_tune_entry* ent, in
             float ms = 0;
             if ((rc = measure(rank[k].second, 16, big, &ms))) break;
             add(rank[k].second, 16, big, ms);
-            if (ms < best) { best = ms; best_v = rank[k].second; best_r = 16; }
+            const float sc = score(rank[k].second, 16, ms);
+            if (sc < best) { best = sc; best_v = rank[k].second; best_r = 16; }
         }
     }
     if (!rc && best_v != RVF_VARIANT_LANE) {
@@ -1346,7 +1351,7 @@ int rvf_plan_autotune(rvf_plan* p, int64_t sample_tiles, rvf_tune_entry* ent, in
             float ms = 0;
             if ((rc = measure(bv, refill, big, &ms))) break;
             add(bv, refill, big, ms);
-            if (ms < best) { best = ms; best_r = refill; }
+            if (score(bv, refill, ms) < best) { best = score(bv, refill, ms); best_r = refill; }
         }
     }
     if (rc) { p->opts.variant = v0; p->refill_min = r0; return rc; }

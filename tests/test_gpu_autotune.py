@@ -45,11 +45,13 @@ def test_autotune_measures_all_variants_and_keeps_results(prec, tmp_path):
     assert set(variants) <= measured, (sorted(measured), variants)
     assert all(e["ms"] > 0 for e in meas)
     # the first len(variants) entries rank the sample; the choice is the
-    # fastest of the re-timed ones that follow
+    # fastest of the re-timed ones that follow, the library default (split 3
+    # mixed, queue 1 FP64, refill 16) counted 1 % faster
     ranked = sorted(meas[:len(variants)], key=lambda e: e["ms"])
     again = meas[len(variants):]
     assert [e["variant"] for e in again[:3]] == [e["variant"] for e in ranked[:3]]
-    best = min(again, key=lambda e: e["ms"])
+    vdef = api.VARIANT_SPLIT(3) if prec == "mixed" else api.VARIANT_QUEUE(1)
+    best = min(again, key=lambda e: e["ms"] * (0.99 if (e["variant"], e["refill_min"]) == (vdef, 16) else 1.0))
     assert (v, r) == (best["variant"], best["refill_min"]) and v in variants
     for a, b in zip(got, ref):
         if isinstance(a, np.ndarray):
