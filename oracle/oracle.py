@@ -53,6 +53,9 @@ def lib():
         L.orc_init_rng.argtypes = [C.c_uint64, C.c_int64]
         L.orc_init_rng.restype = C.c_uint64
         L.orc_singularity.argtypes = [P, C.c_int, P, C.c_int, P, P]
+        L.orc_duffy_rule.argtypes = [C.c_int, C.c_int, P, P, P, P, P]
+        L.orc_duffy_rule.restype = C.c_int64
+        L.orc_duffy_map.argtypes = [C.c_int, C.c_int, C.c_double, C.c_double, C.c_double, C.c_double, P]
         L.orc_compute.argtypes = [P, C.c_int64, C.c_int64, P, P, P, C.c_int, P, P, C.c_int64, P, C.c_int,
                                   C.c_int, C.c_int, C.c_int, C.c_uint64, C.c_int, C.c_int, C.c_int64,
                                   C.c_int64, P, P, P, P, C.c_int, P, C.c_int]
@@ -118,6 +121,27 @@ def pair(mesh, i, j, nquad):
 
 def pair_duffy(mesh, i, j, nquad):
     return lib().orc_pair_duffy(_p(mesh.coords), _p(mesh.family), _p(mesh.nodes), i, j, nquad)
+
+
+def duffy_rule(kind, nquad, ci, cj):
+    """The Duffy point rule for one pair of shared corners (kind 1: common
+    vertex, corners ci[0], cj[0]; kind 2: common edge, ci[0:2], cj[0:2]):
+    points (u, v, s, t) in [0,1]^4, weights without the kernel and area
+    elements, and the region of each point."""
+    a = np.ascontiguousarray(list(ci) + [0] * (2 - len(ci)), np.int32)
+    b = np.ascontiguousarray(list(cj) + [0] * (2 - len(cj)), np.int32)
+    n = lib().orc_duffy_rule(kind, nquad, _p(a), _p(b), None, None, None)
+    uvst = np.zeros((n, 4)); w = np.zeros(n); reg = np.zeros(n, np.int32)
+    lib().orc_duffy_rule(kind, nquad, _p(a), _p(b), _p(uvst), _p(w), _p(reg))
+    return uvst, w, reg
+
+
+def duffy_map(kind, region, r, e1, e2, e3):
+    """One Duffy region map: kind 1 common-vertex offsets (du, dv, ds, dt),
+    kind 2 common-edge (u, v, s, t) in the bottom-edge frame."""
+    out = np.zeros(4)
+    lib().orc_duffy_map(kind, region, r, e1, e2, e3, _p(out))
+    return out
 
 
 SEGMENTS = 3  # soup mode of a curve mesh's segment soup (T, 2, 3)

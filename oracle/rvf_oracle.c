@@ -23,9 +23,14 @@
  *                           triangles (tree-independent) on random soups
  *   MC stream               closed-form integer recurrences (splitmix64,
  *                           xorshift64) + 3-sigma agreement with quadrature
- *   Duffy                   parity unpinned: no reference test covers it; pinned
- *                           only by the invariants non-adjacent == standard and
- *                           non-Quad8 == standard quadrature.
+ *   Duffy                   no reference test covers it; pinned by the
+ *                           invariants non-adjacent == standard and non-Quad8 ==
+ *                           standard quadrature, by closed-form moments of the
+ *                           common-vertex rule (a constant integrates to 2, the
+ *                           Q2 Jacobian), by the common-edge rule's vanishing-
+ *                           Jacobian hyperplanes, in-square weight sums and
+ *                           square rotations, and bit for bit by vectors of the
+ *                           reference's own region-map expressions.
  *   Golub-Welsch (nquad>5)  ulp-level parity unpinned (reference uses LAPACK
  *                           eigen); pinned to 1e-12 by runtests.jl:34-48.
  *   curve meshes (Line2/3)  runtests.jl:241-247 (line2 shape) + Hottel's crossed
@@ -806,11 +811,23 @@ static void rotate_to_edge(double* u, double* v, int e) {
     else if (e == 4) { *u = b; *v = 1.0 - a; }
 }
 
-/* _vertex_integral / _edge_integral — DuffyKernel.jl:222-283, 326-371 */
-static double duffy_integral(const OrcMesh* m, int64_t i, int64_t j, int kind, const int* ci, const int* cj,
-                             const OrcBVH* bvh) {
+/* One region map at one point (test entry): kind 1 the common-vertex map's
+ * corner-relative offsets (du, dv, ds, dt), kind 2 the common-edge map in the
+ * bottom-edge frame (u, v, s, t). */
+void orc_duffy_map(int kind, int region, double r, double e1, double e2, double e3, double* out) {
+    if (kind == 1) vertex_map(region, r, e1, e2, e3, 0.0, 0.0, 0.0, 0.0, out, out + 1, out + 2, out + 3);
+    else edge_map(region, r, e1, e2, e3, out, out + 1, out + 2, out + 3);
+}
+
+/* The point rule of _vertex_integral / _edge_integral (DuffyKernel.jl:222-283,
+ * 326-371) for one pair of shared corners: every (region, rho, eta1, eta2, eta3)
+ * in the reference's loop order whose (u, v, s, t) falls inside [0,1]^4, with its
+ * weight ((((w_rho w_eta1) w_eta2) w_eta3) rho^3).  kind 1: common vertex
+ * (ci[0], cj[0]); kind 2: common edge (ci[0..1], cj[0..1]).  uvst / w / region
+ * may be NULL (count only); returns the number of points. */
+int64_t orc_duffy_rule(int kind, int nquad, const int* ci, const int* cj, double* uvst, double* w, int* region_out) {
     double p1[8], w1[8], p01[8], w01[8];
-    int nq = orc_gl1d(m->nquad, p1, w1);
+    int nq = orc_gl1d(nquad, p1, w1);
     for (int k = 0; k < nq; k++) { p01[k] = (p1[k] + 1.0) / 2.0; w01[k] = w1[k] / 2.0; }
     double u0i = 0, v0i = 0, u0j = 0, v0j = 0;
     int edge_i = 1, edge_j = 1;
@@ -822,7 +839,7 @@ static double duffy_integral(const OrcMesh* m, int64_t i, int64_t j, int kind, c
         edge_j = corners_to_edge(cj[0], cj[1]);
     }
     int nreg = kind == 1 ? 8 : 5;
-    double F = 0.0;
+    int64_t n = 0;
     for (int region = 1; region <= nreg; region++)
         for (int a = 0; a < nq; a++)
             for (int b = 0; b < nq; b++)
@@ -838,15 +855,34 @@ static double duffy_integral(const OrcMesh* m, int64_t i, int64_t j, int kind, c
                             rotate_to_edge(&s, &t, edge_j);
                         }
                         if (u < 0 || u > 1 || v < 0 || v > 1 || s < 0 || s > 1 || t < 0 || t > 1) continue;
-                        double xi[3], ni[3], ai, xj[3], nj[3], aj;
-                        eval_quad01(m, i, u, v, xi, ni, &ai);
-                        eval_quad01(m, j, s, t, xj, nj, &aj);
-                        double K = orc_vf_kernel(xi, ni, xj, nj);
-                        if (K == 0.0) continue;
-                        if (bvh && !is_visible(bvh, xi, xj, 0)) continue;
-                        double jac = (r * r) * r;
-                        F += ((((((wr * we1) * we2) * we3) * jac) * K) * ai) * aj;
+                        if (uvst) { uvst[4 * n] = u; uvst[4 * n + 1] = v; uvst[4 * n + 2] = s; uvst[4 * n + 3] = t; }
+                        if (w) w[n] = (((wr * we1) * we2) * we3) * ((r * r) * r);
+                        if (region_out) region_out[n] = region;
+                        n++;
                     }
+    return n;
+}
+
+/* _vertex_integral / _edge_integral — DuffyKernel.jl:222-283, 326-371: the
+ * kernel summed over the point rule above, term by term in the same order */
+static double duffy_integral(const OrcMesh* m, int64_t i, int64_t j, int kind, const int* ci, const int* cj,
+                             const OrcBVH* bvh) {
+    int64_t n = orc_duffy_rule(kind, m->nquad, ci, cj, NULL, NULL, NULL);
+    double* uvst = (double*)malloc(sizeof(double) * 4 * (size_t)(n ? n : 1));
+    double* w = (double*)malloc(sizeof(double) * (size_t)(n ? n : 1));
+    orc_duffy_rule(kind, m->nquad, ci, cj, uvst, w, NULL);
+    double F = 0.0;
+    for (int64_t k = 0; k < n; k++) {
+        double xi[3], ni[3], ai, xj[3], nj[3], aj;
+        eval_quad01(m, i, uvst[4 * k], uvst[4 * k + 1], xi, ni, &ai);
+        eval_quad01(m, j, uvst[4 * k + 2], uvst[4 * k + 3], xj, nj, &aj);
+        double K = orc_vf_kernel(xi, ni, xj, nj);
+        if (K == 0.0) continue;
+        if (bvh && !is_visible(bvh, xi, xj, 0)) continue;
+        F += ((w[k] * K) * ai) * aj;
+    }
+    free(uvst);
+    free(w);
     return F;
 }
 
