@@ -31,8 +31,11 @@
  *                           Jacobian hyperplanes, in-square weight sums and
  *                           square rotations, and bit for bit by vectors of the
  *                           reference's own region-map expressions.
- *   Golub-Welsch (nquad>5)  ulp-level parity unpinned (reference uses LAPACK
- *                           eigen); pinned to 1e-12 by runtests.jl:34-48.
+ *   Golub-Welsch (nquad>5)  runtests.jl:34-48, numpy's leggauss to 1e-14, and
+ *                           the reference's eigenvalue route through LAPACK
+ *                           dsyevr to 8e-15 (plate view factor to 1e-13); the
+ *                           reference's own last bits depend on the machine's
+ *                           BLAS kernels, so no bit-level target exists.
  *   curve meshes (Line2/3)  runtests.jl:241-247 (line2 shape) + Hottel's crossed
  *                           strings (exact 2-D view factors between straight
  *                           strips, with and without a blocking strip; arcs of a

@@ -50,6 +50,35 @@ def test_gl_large_n(orc, n):
         assert abs(np.dot(w, p ** 10) - 2 / 11) <= 1e-12
 
 
+def golub_welsch_lapack(n):
+    """The reference's _golub_welsch (src/Quadrature.jl:75-84) through the
+    LAPACK routine Julia's eigen uses for a dense symmetric matrix (dsyevr,
+    upper triangle); its last bits depend on the BLAS kernels of the machine."""
+    import scipy.linalg as sl
+    b = np.array([i / math.sqrt(4 * i * i - 1.0) for i in range(1, n)])
+    vals, vecs = sl.eigh(np.diag(b, 1) + np.diag(b, -1), driver="evr", lower=False)
+    idx = np.argsort(vals, kind="stable")
+    return vals[idx], 2.0 * vecs[0, idx] ** 2
+
+
+@pytest.mark.parametrize("n", [6, 7, 8])
+def test_gl_large_n_vs_reference_algorithm(orc, n):
+    # the oracle's nodes (Newton in long double, correctly rounded) and the
+    # reference's eigenvalue route agree to LAPACK's accuracy, and so do the
+    # view factors built on them
+    p, w = orc.gl1d(n)
+    rp, rw = golub_welsch_lapack(n)
+    assert np.abs(p - rp).max() <= 8e-15 and np.abs(w - rw).max() <= 8e-15
+
+    def plates(x, wt):  # coaxial unit squares 1 apart: kernel 1 / (pi (|dx|^2 + 1)^2)
+        x, W = (1 + x) / 2, wt / 2
+        dx = x[:, None] - x[None, :]
+        k = 1 / (math.pi * (dx[:, :, None, None] ** 2 + dx[None, None, :, :] ** 2 + 1) ** 2)
+        return np.einsum("a,b,c,d,acbd->", W, W, W, W, k)
+
+    assert abs(plates(p, w) / plates(rp, rw) - 1) <= 1e-13
+
+
 # --- test/runtests.jl:258-275 (Dunavant; Q3 constant kept) ------------------------
 @pytest.mark.parametrize("n", range(1, 9))
 def test_dunavant_sum(orc, n):
