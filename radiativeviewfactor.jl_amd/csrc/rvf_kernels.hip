@@ -1304,12 +1304,14 @@ __device__ __forceinline__ int nray_setup(const PairArgs& a, const QCtx& X, int 
 }
 struct SRay { float ox, oy, oz, dx, dy, dz, L, dd; };  // FP32 ray and its direction error bound
 __device__ __forceinline__ SRay sray(const PairArgs& a, const QCtx& X, int c, int b);
+__device__ __forceinline__ int nray_from(const SRay& S, int root, NRay<true>& N);
 __device__ __forceinline__ int nray_setup(const PairArgs& a, const QCtx& X, int c, int b, NRay<true>& N) {
-    const SRay S = sray(a, X, c, b);  // the same ray as the triangle tasks'
+    return nray_from(sray(a, X, c, b), X.lroot[b], N);  // the same ray as the triangle tasks'
+}
+__device__ __forceinline__ int nray_from(const SRay& S, int root, NRay<true>& N) {
     struct { float ox, oy, oz, ix, iy, iz, L; } R;
     R.ox = S.ox; R.oy = S.oy; R.oz = S.oz; R.L = S.L;
     R.ix = __builtin_amdgcn_rcpf(S.dx); R.iy = __builtin_amdgcn_rcpf(S.dy); R.iz = __builtin_amdgcn_rcpf(S.dz);
-    const int root = X.lroot[b];
     // A zero direction component gives an infinite inverse; clamped to 1e30 the
     // slab of that axis is still (-huge, +huge) inside the box's extent and
     // empty outside, and fma(bound, i, -o*i) stays finite (|bound|, |o| <= Sm).
@@ -1416,8 +1418,11 @@ __device__ __forceinline__ SRay sray(const PairArgs& a, const QCtx& X, int c, in
 
 // Mixed mode triangle task: the conservative classifier on that ray:
 // 0 certain miss, 1 candidate for the exact FP64 test, 2 certain hit.
+__device__ __forceinline__ int tclass(const PairArgs& a, const SRay& S, int tri);
 __device__ __forceinline__ int ttest_mixed(const PairArgs& a, const QCtx& X, int c, int b, int tri) {
-    const SRay S = sray(a, X, c, b);
+    return tclass(a, sray(a, X, c, b), tri);
+}
+__device__ __forceinline__ int tclass(const PairArgs& a, const SRay& S, int tri) {
     const Tri32 T = a.ttris32[tri];  // traversal-tree order
     RvfFRay R;
     R.ox = S.ox; R.oy = S.oy; R.oz = S.oz; R.dx = S.dx; R.dy = S.dy; R.dz = S.dz; R.L = S.L;
@@ -1467,17 +1472,24 @@ __device__ __forceinline__ void split_drain(const PairArgs& a, const QCtx& X, in
                 if (Q.c < Q.nc && req != 0 && (idle || __popcll(req) >= a.refill_min)) {
                     int c = 0, b = 0;
                     if (queue_take(req, Q, need, lane, c, b)) {
-                        // mixed: a triangle that certainly blocked another ray
-                        // of the pair often blocks this one too; a certain
-                        // hit on it settles the ray without a traversal
-                        bool settled = false;
-                        if (MIX && a.hint) {
-                            const int h = X.lhint[b];
-                            settled = h >= 0 && ttest_mixed(a, X, c, b, h) == 2;
-                        }
-                        if (settled) {
-                            K.c1++;
-                            atomicOr(&blk[c * 2 + (b >> 5)], 1u << (b & 31));
+                        if constexpr (MIX) {
+                            // a triangle that certainly blocked another ray of
+                            // the pair often blocks this one too; a certain hit
+                            // on it settles the ray without a traversal (the
+                            // ray is formed once for the test and the walk)
+                            const SRay SR = sray(a, X, c, b);
+                            bool settled = false;
+                            if (a.hint) {
+                                const int h = X.lhint[b];
+                                settled = h >= 0 && tclass(a, SR, h) == 2;
+                            }
+                            if (settled) {
+                                K.c1++;
+                                atomicOr(&blk[c * 2 + (b >> 5)], 1u << (b & 31));
+                            } else {
+                                tag[s] = (c << 6) | b;
+                                cur[s] = nray_from(SR, X.lroot[b], R[s]);
+                            }
                         } else {
                             tag[s] = (c << 6) | b;
                             cur[s] = nray_setup(a, X, c, b, R[s]);
