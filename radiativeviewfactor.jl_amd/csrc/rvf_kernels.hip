@@ -1466,31 +1466,48 @@ __device__ __forceinline__ void split_drain(const PairArgs& a, const QCtx& X, in
 #pragma unroll
             for (int s = 0; s < S; s++) any |= cur[s] >= 0;
             const bool idle = __ballot(any) == 0;
+            if constexpr (MIX) {
+                // mixed: a pass gives each lane that has an empty slot one ray
+                // for its first empty slot, once enough lanes wait (however
+                // their empty slots are spread); a further pass only while
+                // enough lanes still wait.  A refilled ray is formed once for
+                // the blocker-hint test and the walk: a triangle that
+                // certainly blocked another ray of the pair often blocks this
+                // one too, and a certain hit on it settles the ray at once.
 #pragma unroll
-            for (int s = 0; s < S; s++) {
-                const uint64_t req = __ballot(cur[s] < 0);
-                if (Q.c < Q.nc && req != 0 && (idle || __popcll(req) >= a.refill_min)) {
+                for (int pass = 0; pass < S; pass++) {
+                    int es = -1;
+#pragma unroll
+                    for (int s = S - 1; s >= 0; s--) es = cur[s] < 0 ? s : es;
+                    const uint64_t req = __ballot(es >= 0);
+                    if (!(Q.c < Q.nc && req != 0 && (idle || __popcll(req) >= a.refill_min))) break;
                     int c = 0, b = 0;
                     if (queue_take(req, Q, need, lane, c, b)) {
-                        if constexpr (MIX) {
-                            // a triangle that certainly blocked another ray of
-                            // the pair often blocks this one too; a certain hit
-                            // on it settles the ray without a traversal (the
-                            // ray is formed once for the test and the walk)
-                            const SRay SR = sray(a, X, c, b);
-                            bool settled = false;
-                            if (a.hint) {
-                                const int h = X.lhint[b];
-                                settled = h >= 0 && tclass(a, SR, h) == 2;
-                            }
-                            if (settled) {
-                                K.c1++;
-                                atomicOr(&blk[c * 2 + (b >> 5)], 1u << (b & 31));
-                            } else {
-                                tag[s] = (c << 6) | b;
-                                cur[s] = nray_from(SR, X.lroot[b], R[s]);
-                            }
+                        const SRay SR = sray(a, X, c, b);
+                        bool settled = false;
+                        if (a.hint) {
+                            const int h = X.lhint[b];
+                            settled = h >= 0 && tclass(a, SR, h) == 2;
+                        }
+                        if (settled) {
+                            K.c1++;
+                            atomicOr(&blk[c * 2 + (b >> 5)], 1u << (b & 31));
                         } else {
+                            NRay<MIX> nr;
+                            const int root = nray_from(SR, X.lroot[b], nr);
+#pragma unroll
+                            for (int s = 0; s < S; s++)
+                                if (s == es) { tag[s] = (c << 6) | b; cur[s] = root; R[s] = nr; }
+                        }
+                    }
+                }
+            } else {
+#pragma unroll
+                for (int s = 0; s < S; s++) {
+                    const uint64_t req = __ballot(cur[s] < 0);
+                    if (Q.c < Q.nc && req != 0 && (idle || __popcll(req) >= a.refill_min)) {
+                        int c = 0, b = 0;
+                        if (queue_take(req, Q, need, lane, c, b)) {
                             tag[s] = (c << 6) | b;
                             cur[s] = nray_setup(a, X, c, b, R[s]);
                         }
