@@ -1893,12 +1893,18 @@ __device__ __forceinline__ void tile_occ_queue(const PairArgs& a, int tile, doub
         }
         double Fij = 0.0, Fs = 0.0, Fc = 0.0;
         uint32_t bc = 0, bh = 0;
+        // mixed: when no pair of the wave can reach an occluder (all culled),
+        // the need masks and the trace are skipped; the sums below count the
+        // rays instead, from the same kernel values
+        const bool trace = !MIX || __ballot(cap) != 0;
         for (int p0 = 0; p0 < npmax; p0 += X.pch) {
             const int p1 = min(npmax, p0 + X.pch);
             const int nc = (p1 - p0) * npj;
             X.p0 = p0;
+            if (!trace)
+                for (int w = lane; w < 2 * nc; w += 64) blk[w] = 0u;
             // Phase A: need masks
-            for (int p = p0; p < p1; p++) {
+            for (int p = p0; p < p1 && trace; p++) {
                 const bool pv = act && p < npi && root >= 0;
                 if (MIX) {
                     const float dxi = P[(int64_t)(p * 8 + 0) * NS + t.i], dyi = P[(int64_t)(p * 8 + 1) * NS + t.i];
@@ -1938,8 +1944,10 @@ __device__ __forceinline__ void tile_occ_queue(const PairArgs& a, int tile, doub
             }
             __syncthreads();
             // Phase B: trace
-            if constexpr (SPLIT) split_drain<MIX, S>(a, X, nc, need, blk, lane, tq, K);
-            else queue_drain<MIX, S>(a, X, nc, need, blk, lane, K);
+            if (trace) {
+                if constexpr (SPLIT) split_drain<MIX, S>(a, X, nc, need, blk, lane, tq, K);
+                else queue_drain<MIX, S>(a, X, nc, need, blk, lane, K);
+            }
             __syncthreads();
             // Phase C: sums in each lane's own order
             for (int p = p0; p < p1; p++) {
@@ -1955,6 +1963,11 @@ __device__ __forceinline__ void tile_occ_queue(const PairArgs& a, int tile, doub
                         float k = vf_mixed(bx, by, bz, nxi, nyi, nzi, qj32[q * 8 + 0], qj32[q * 8 + 1], qj32[q * 8 + 2],
                                            qj32[q * 8 + 3], qj32[q * 8 + 4], qj32[q * 8 + 5], rx, ry, rz, r2);
                         const int c = (p - p0) * npj + q;
+                        if (!trace) {  // as the need masks count them: every such ray is culled
+                            const bool nd = act && p < npi && root >= 0 && k != 0.0f;
+                            nrays += nd;
+                            culled += nd;
+                        }
                         if ((blk[2 * c + (lane >> 5)] >> (lane & 31)) & 1u) {
                             k = 0.0f;
                             bc++;
