@@ -1893,10 +1893,10 @@ __device__ __forceinline__ void tile_occ_queue(const PairArgs& a, int tile, doub
         }
         double Fij = 0.0, Fs = 0.0, Fc = 0.0;
         uint32_t bc = 0, bh = 0;
-        // mixed: when no pair of the wave can reach an occluder (all culled),
-        // the need masks and the trace are skipped; the sums below count the
-        // rays instead, from the same kernel values
-        const bool trace = !MIX || __ballot(cap) != 0;
+        // when no pair of the wave can reach an occluder (all culled), the need
+        // masks and the trace are skipped; the sums below count the rays
+        // instead, from the same kernel values
+        const bool trace = __ballot(cap) != 0;
         for (int p0 = 0; p0 < npmax; p0 += X.pch) {
             const int p1 = min(npmax, p0 + X.pch);
             const int nc = (p1 - p0) * npj;
@@ -1992,6 +1992,11 @@ __device__ __forceinline__ void tile_occ_queue(const PairArgs& a, int tile, doub
                         double K = vf_exact(xi, yi, zi, nxi, nyi, nzi, qj64[q * 8 + 0], qj64[q * 8 + 1], qj64[q * 8 + 2],
                                             qj64[q * 8 + 3], qj64[q * 8 + 4], qj64[q * 8 + 5], Rr);
                         const int c = (p - p0) * npj + q;
+                        if (!trace) {  // as the need masks count them: every such ray is culled
+                            const bool nd = act && p < npi && root >= 0 && K != 0.0;
+                            nrays += nd;
+                            culled += nd;
+                        }
                         if ((blk[2 * c + (lane >> 5)] >> (lane & 31)) & 1u) {
                             K = 0.0;
                             bc++;
