@@ -289,3 +289,26 @@ def test_graph_replay_matches_eager():
     for pl in (eager, other, g):
         pl.close()
     do.close(); dm.close()
+
+
+@pytest.mark.parametrize("variant", api.EXACT_COUNT_VARIANTS)
+def test_culled_waves_counts_and_values(orc, variant):
+    # a small occluder near one corner of the cube: most waves' pairs are all
+    # culled and skip the trace; their rays are still counted as the oracle
+    # counts them, and F, audits and counts stay exact
+    m = M.unit_cube(6)
+    V, T = M.icosphere(1, 0.05, (0.12, 0.1, 0.15), seed=3)
+    m.group_tri_soup = {7: V[T]}
+    g = gpu(m, 2, [7], "fp64", "geometric", audit=True, variant=variant)
+    r = orc.compute(m, nquad=2, obstruction_groups=[7], parity_mode="geometric", audit=True, count=True)
+    assert (g["stats"]["rays"], g["stats"]["aabb_tests"], g["stats"]["tri_tests"]) == \
+        (r["rays"], r["aabb_tests"], r["tri_tests"])
+    if variant != api.VARIANT_LANE:  # (the per-lane kernel traces every ray; only the wave kernels cull)
+        assert g["stats"]["culled_rays"] > g["stats"]["rays"] // 2
+    assert np.array_equal(g["F"], r["raw"] / r["A"][:, None])
+    iu = np.triu_indices(m.n_elems, 1)
+    assert np.array_equal(g["ac"][iu], r["audit_count"][iu]) and int(g["ac"][iu].sum()) > 0
+    mx = gpu(m, 2, [7], "mixed", "geometric", audit=True)
+    assert mx["stats"]["culled_rays"] > mx["stats"]["rays"] // 2
+    assert np.array_equal(mx["ac"][iu], g["ac"][iu]) and np.array_equal(mx["ah"][iu], g["ah"][iu])
+    assert float(np.abs(mx["F"] - g["F"]).max()) <= 1e-6
