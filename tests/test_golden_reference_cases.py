@@ -244,3 +244,13 @@ def test_device_visibility(orc, c, parity):
         # the oracle's occluded solve agrees
         o = orc.compute(m, nquad=1, obstruction_groups=[3], parity_mode=parity)
         assert (o["raw"][0, 1] > 0) == visible
+
+
+def test_fixture_matches_its_generator():
+    # the committed fixture is what tests/golden/make_reference_cases.py writes
+    import importlib.util
+    spec = importlib.util.spec_from_file_location("make_reference_cases",
+                                                  os.path.join(HERE, "golden", "make_reference_cases.py"))
+    gen = importlib.util.module_from_spec(spec)
+    spec.loader.exec_module(gen)
+    assert json.loads(json.dumps(gen.cases())) == CASES
