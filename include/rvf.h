@@ -93,6 +93,9 @@ typedef struct rvf_stats {
     /* occluded quadrature, wave-queue kernels: rays of pairs culled before any
      * traversal (their rays cannot reach an occluder); counted in `rays` */
     int64_t culled_rays;
+    /* occluded quadrature, wave-queue kernels: (wavefront, column element)
+     * passes that skipped the trace because every pair of the wave was culled */
+    int64_t culled_waves;
 } rvf_stats;
 
 /* --- lifecycle (no reference counterpart: the reference is one Julia process) */

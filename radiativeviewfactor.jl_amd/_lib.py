@@ -62,7 +62,7 @@ class Stats(C.Structure):
                 ("retests", C.c_int64), ("singular_pairs", C.c_int64), ("aabb_tests", C.c_int64),
                 ("tri_tests", C.c_int64), ("trav_iters", C.c_int64), ("slot_steps", C.c_int64),
                 ("task_batches", C.c_int64), ("exact_batches", C.c_int64), ("sure_hits", C.c_int64),
-                ("culled_rays", C.c_int64)]
+                ("culled_rays", C.c_int64), ("culled_waves", C.c_int64)]
 
     def as_dict(self):
         return {k: getattr(self, k) for k, _ in self._fields_}

@@ -88,6 +88,10 @@ enum RvfPairKind {
 #define RVF_TQCAP_MIX 1024  // the same for the mixed split variants (small leaves)
 #define RVF_TLEAF_MAX 4  // triangles per leaf of the mixed traversal trees
 #define RVF_CQCAP 128    // mixed split variants: candidates for the exact FP64 test
+// A split-variant triangle task packs (tag << RVF_TQ_TRI_BITS) | triangle into
+// 32 bits (tag = combination << 6 | lane < 2^13), so the split variants serve
+// obstruction sets of fewer than 2^19 triangles over all their trees.
+#define RVF_TQ_TRI_BITS 19
 
 // Column chunk: a run of <= RVF_CHUNK consecutive elements of one group.
 struct Chunk {

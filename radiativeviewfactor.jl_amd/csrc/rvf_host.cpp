@@ -897,7 +897,8 @@ struct rvf_plan {
     int refill_min = 16;
 };
 
-static int occ_kind(int variant, bool mixed);
+static int occ_kind(int variant, bool mixed, int64_t n_tris);
+static const char* occ_kind_error(int variant, int64_t n_tris);
 static int pair_kind(const rvf_plan* p);
 
 int rvf_plan_create(rvf_ctx* ctx, const rvf_mesh* mesh, const rvf_obs* obs, const rvf_opts* opts, rvf_plan** out) {
@@ -909,8 +910,8 @@ int rvf_plan_create(rvf_ctx* ctx, const rvf_mesh* mesh, const rvf_obs* obs, cons
     if (obs && (obs->G != (int)mesh->tags.size() || obs->seg != mesh->dim2))
         return fail(RVF_E_ARG, "obstruction set built for another mesh");
     if (obs && !obs->trees.empty() && opts->method != RVF_METHOD_MC &&
-        occ_kind(opts->variant, opts->precision == RVF_MIXED) < 0)
-        return fail(RVF_E_ARG, "unknown traversal variant for this precision");
+        occ_kind(opts->variant, opts->precision == RVF_MIXED, obs->n_tris) < 0)
+        return fail(RVF_E_ARG, std::string("rvf_plan_create: ") + occ_kind_error(opts->variant, obs->n_tris));
     HIPCHK(hipSetDevice(ctx->device));
     rvf_plan* p = new rvf_plan();
     p->ctx = ctx; p->mesh = mesh; p->obs = obs; p->opts = *opts;
@@ -1081,11 +1082,19 @@ static bool plan_occluded_quad(const rvf_plan* p) {
     return p->obs && !p->obs->trees.empty() && p->opts.method != RVF_METHOD_MC;
 }
 
-// Kernel kind for an occluded quadrature plan's variant, -1 if invalid.
-static int occ_kind(int variant, bool mixed) {
-    if (variant == RVF_VARIANT_DEFAULT) variant = mixed ? RVF_VARIANT_SPLIT(3) : RVF_VARIANT_QUEUE(1);
+// Kernel kind for an occluded quadrature plan's variant, -1 if invalid.  The
+// split variants address triangles with RVF_TQ_TRI_BITS bits of a task word:
+// with 2^19 or more triangles over all trees they are invalid, and the mixed
+// default falls back to the queue form.
+static int default_variant(bool mixed, int64_t n_tris) {
+    if (!mixed) return RVF_VARIANT_QUEUE(1);
+    return n_tris < ((int64_t)1 << RVF_TQ_TRI_BITS) ? RVF_VARIANT_SPLIT(3) : RVF_VARIANT_QUEUE(3);
+}
+static int occ_kind(int variant, bool mixed, int64_t n_tris) {
+    if (variant == RVF_VARIANT_DEFAULT) variant = default_variant(mixed, n_tris);
     if (variant == RVF_VARIANT_LANE) return mixed ? RVF_K_MIX_OCC_LANE : RVF_K_FP64_OCC_LANE;
     if (variant > 20) {  // split node / triangle-task form
+        if (n_tris >= ((int64_t)1 << RVF_TQ_TRI_BITS)) return -1;
         const int S = variant - 20;
         if (!mixed) return S == 1 ? RVF_K_FP64_SPLIT1 : S == 2 ? RVF_K_FP64_SPLIT2 : S == 3 ? RVF_K_FP64_SPLIT3 : -1;
         return S == 1 ? RVF_K_MIX_SPLIT1 : S == 2 ? RVF_K_MIX_SPLIT2 : S == 3 ? RVF_K_MIX_SPLIT3 : -1;
@@ -1095,6 +1104,11 @@ static int occ_kind(int variant, bool mixed) {
     return S == 2 ? RVF_K_MIX_QUEUE2 : S == 3 ? RVF_K_MIX_QUEUE3 : S == 4 ? RVF_K_MIX_QUEUE4
          : S == 6 ? RVF_K_MIX_QUEUE6 : -1;
 }
+static const char* occ_kind_error(int variant, int64_t n_tris) {
+    if (variant > 20 && variant <= 23 && n_tris >= ((int64_t)1 << RVF_TQ_TRI_BITS))
+        return "split traversal variants need fewer than 2^19 obstruction triangles over all trees";
+    return "unknown traversal variant for this precision";
+}
 
 // Which pair-kernel variant a plan runs, and the LDS it needs.
 static int pair_kind(const rvf_plan* p) {
@@ -1103,7 +1117,7 @@ static int pair_kind(const rvf_plan* p) {
         if (plan_occluded_quad(p)) return mixed ? RVF_K_MIX_OCC2D : RVF_K_FP64_OCC2D;
         return mixed ? RVF_K_MIX_2D : RVF_K_FP64_2D;
     }
-    if (plan_occluded_quad(p)) return occ_kind(p->opts.variant, mixed);
+    if (plan_occluded_quad(p)) return occ_kind(p->opts.variant, mixed, p->obs->n_tris);
     int pc = p->maxpts;
     if (!mixed && pc > 9) pc = 9;  // register chunks of 9 points
     return rvf_pair_kind(mixed, 0, pc, p->opts.method == RVF_METHOD_MC);
@@ -1257,8 +1271,8 @@ int rvf_plan_tune(rvf_plan* p, int32_t variant, int32_t refill_min) {
     if (!p) return fail(RVF_E_ARG, "rvf_plan_tune: null plan");
     if (refill_min < 0 || refill_min > 64) return fail(RVF_E_ARG, "rvf_plan_tune: refill_min must be in 0..64");
     if (p->obs && !p->obs->trees.empty() && p->opts.method != RVF_METHOD_MC &&
-        occ_kind(variant, p->opts.precision == RVF_MIXED) < 0)
-        return fail(RVF_E_ARG, "rvf_plan_tune: unknown traversal variant for this precision");
+        occ_kind(variant, p->opts.precision == RVF_MIXED, p->obs->n_tris) < 0)
+        return fail(RVF_E_ARG, std::string("rvf_plan_tune: ") + occ_kind_error(variant, p->obs->n_tris));
     HIPCHK(hipSetDevice(p->ctx->device));
     HIPCHK(hipStreamSynchronize(p->ctx->stream));
     p->opts.variant = variant;
@@ -1310,10 +1324,12 @@ int rvf_plan_autotune(rvf_plan* p, int64_t sample_tiles, rvf_tune_entry* ent, in
         HIPCHK(hipEventElapsedTime(ms, p->ev[6], p->ev[7]));
         return 0;
     };
+    const int64_t ntri = p->obs->n_tris;
     std::vector<int> cands = {RVF_VARIANT_LANE};
     for (int S = 1; S <= 6; S++)
-        if (occ_kind(RVF_VARIANT_QUEUE(S), mixed) >= 0) cands.push_back(RVF_VARIANT_QUEUE(S));
-    for (int S = 1; S <= 3; S++) cands.push_back(RVF_VARIANT_SPLIT(S));
+        if (occ_kind(RVF_VARIANT_QUEUE(S), mixed, ntri) >= 0) cands.push_back(RVF_VARIANT_QUEUE(S));
+    for (int S = 1; S <= 3; S++)
+        if (occ_kind(RVF_VARIANT_SPLIT(S), mixed, ntri) >= 0) cands.push_back(RVF_VARIANT_SPLIT(S));
     auto add = [&](int v, int refill, int64_t want, float ms) {
         if (*n < cap) ent[*n] = rvf_tune_entry{v, v == RVF_VARIANT_LANE ? 0 : refill, std::min(want, nt), (double)ms};
         ++*n;
@@ -1330,7 +1346,7 @@ int rvf_plan_autotune(rvf_plan* p, int64_t sample_tiles, rvf_tune_entry* ent, in
     //    larger sample), then the refill thresholds of the winner.  The
     //    library default (variant, refill 16) counts 1 % faster than measured:
     //    a sample can rank near-equal configurations either way.
-    const int vdef = mixed ? RVF_VARIANT_SPLIT(3) : RVF_VARIANT_QUEUE(1);
+    const int vdef = default_variant(mixed, ntri);
     auto score = [&](int v, int refill, float ms) { return v == vdef && refill == 16 ? 0.99f * ms : ms; };
     int best_v = v0, best_r = r0;
     float best = 3.4e38f;
@@ -1415,6 +1431,7 @@ static int plan_collect(rvf_plan* p) {
     p->st.exact_batches = (int64_t)cnt[7];
     p->st.sure_hits = (int64_t)cnt[8];
     p->st.culled_rays = (int64_t)cnt[9];
+    p->st.culled_waves = (int64_t)cnt[10];
     return 0;
 }
 

@@ -1285,7 +1285,8 @@ __device__ __forceinline__ void queue_drain(const PairArgs& a, const QCtx& X, in
 // RVF_TQCAP, RVF_TQCAP_MIX (powers of two): one iteration of S <= 3 slots
 // pushes at most S x 64 lanes x the leaf size (8 in the reference trees,
 // RVF_TLEAF_MAX in the traversal trees); RVF_CQCAP >= 64 + 63.
-#define RVF_TQ_TRI_BITS 19  // a task packs (tag << 19) | triangle
+// (RVF_TQ_TRI_BITS in rvf_common.h: a task packs (tag << 19) | triangle)
+static_assert(RVF_QCOMBO * 64 <= (1 << (32 - RVF_TQ_TRI_BITS)), "task tag does not fit");
 
 template <bool MIX>
 struct NRay;
@@ -1834,6 +1835,7 @@ __device__ __forceinline__ void tile_occ_queue(const PairArgs& a, int tile, doub
     const float* __restrict__ P = a.P32;
     double ls = 0.0, lc = 0.0;
     uint32_t nrays = 0, culled = 0;  // rays cast; of those, rays of pairs culled before any traversal
+    uint32_t skipped = 0;  // columns whose trace the wave skipped (all its pairs culled; lane 0)
     QCount K;
     QCtx X;
     X.sbase = (int64_t)t.s * RVF_STRIP;
@@ -1897,6 +1899,7 @@ __device__ __forceinline__ void tile_occ_queue(const PairArgs& a, int tile, doub
         // masks and the trace are skipped; the sums below count the rays
         // instead, from the same kernel values
         const bool trace = __ballot(cap) != 0;
+        skipped += !trace && lane == 0 && __ballot(act && root >= 0) != 0;
         for (int p0 = 0; p0 < npmax; p0 += X.pch) {
             const int p1 = min(npmax, p0 + X.pch);
             const int nc = (p1 - p0) * npj;
@@ -2029,17 +2032,18 @@ __device__ __forceinline__ void tile_occ_queue(const PairArgs& a, int tile, doub
     a.part[(int64_t)t.chunk * NS + t.i] = ls + lc;
     if (a.counters) {
         if (!MIX) K.c0 += culled;  // FP64 box tests: a culled ray's root test
-        unsigned long long v[8] = {nrays, K.c0, K.c1, K.iters, K.steps, K.tbatch, K.ebatch, culled};
+        unsigned long long v[9] = {nrays, K.c0, K.c1, K.iters, K.steps, K.tbatch, K.ebatch, culled, skipped};
 #pragma unroll
         for (int o = 32; o > 0; o >>= 1)
 #pragma unroll
-            for (int e = 0; e < 8; e++) v[e] += __shfl_xor(v[e], o, 64);
+            for (int e = 0; e < 9; e++) v[e] += __shfl_xor(v[e], o, 64);
         if (lane == 0) {
             atomicAdd(&a.counters[1], v[0]);
             if (MIX) { atomicAdd(&a.counters[0], v[1]); atomicAdd(&a.counters[8], v[2]); }
             else { atomicAdd(&a.counters[2], v[1]); atomicAdd(&a.counters[3], v[2]); }
             for (int e = 3; e < 7; e++) atomicAdd(&a.counters[e + 1], v[e]);
             atomicAdd(&a.counters[9], v[7]);
+            atomicAdd(&a.counters[10], v[8]);
         }
     }
 }

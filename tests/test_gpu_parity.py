@@ -305,10 +305,47 @@ def test_culled_waves_counts_and_values(orc, variant):
         (r["rays"], r["aabb_tests"], r["tri_tests"])
     if variant != api.VARIANT_LANE:  # (the per-lane kernel traces every ray; only the wave kernels cull)
         assert g["stats"]["culled_rays"] > g["stats"]["rays"] // 2
+        assert g["stats"]["culled_waves"] > 0  # some waves took the branch that skips the trace
     assert np.array_equal(g["F"], r["raw"] / r["A"][:, None])
     iu = np.triu_indices(m.n_elems, 1)
     assert np.array_equal(g["ac"][iu], r["audit_count"][iu]) and int(g["ac"][iu].sum()) > 0
     mx = gpu(m, 2, [7], "mixed", "geometric", audit=True)
-    assert mx["stats"]["culled_rays"] > mx["stats"]["rays"] // 2
+    assert mx["stats"]["culled_rays"] > mx["stats"]["rays"] // 2 and mx["stats"]["culled_waves"] > 0
+    assert np.array_equal(mx["ac"][iu], g["ac"][iu]) and np.array_equal(mx["ah"][iu], g["ah"][iu])
+    assert float(np.abs(mx["F"] - g["F"]).max()) <= 1e-6
+
+
+def test_split_variants_refused_past_task_index_width(orc):
+    # a split-variant task word holds a triangle index in 19 bits: with 2^19 or
+    # more obstruction triangles over all trees the split variants are refused
+    # (explicitly or by the autotuner) and the mixed default is the queue form;
+    # results stay exact there
+    rng = np.random.default_rng(0x5EED0019)
+    T = (1 << 19) + 1500
+    cen = rng.uniform((0.15, 0.15, 0.35), (0.85, 0.85, 0.65), (T, 1, 3))
+    soup = np.ascontiguousarray(cen + rng.uniform(-0.004, 0.004, (T, 3, 3)))
+    m = M.parallel_plates(6)
+    m.group_tri_soup = {3: soup}
+    m.soup_layout = "xyz_vertex"
+    ctx = api.context(0)
+    dm = api.DeviceMesh(ctx, m)
+    do = api.DeviceObstruction(ctx, dm, [3], "geometric")
+    assert do.n_tris == T
+    for prec, v in (("mixed", api.VARIANT_SPLIT(3)), ("mixed", api.VARIANT_SPLIT(1)), ("fp64", api.VARIANT_SPLIT(2))):
+        with pytest.raises(api.RVFError, match="2\\^19"):
+            api.Plan(ctx, dm, do, 2, "quad", prec, variant=v)
+    pl = api.Plan(ctx, dm, do, 2, "quad", "mixed")
+    assert pl.kernel_name() == "k_pair<mixed_occ_queue3>"
+    with pytest.raises(api.RVFError, match="2\\^19"):
+        pl.tune(api.VARIANT_SPLIT(3), 16)
+    v, r, meas = pl.autotune(sample_tiles=64)
+    assert meas and all(e["variant"] < 20 for e in meas) and v < 20
+    pl.close(); do.close(); dm.close()
+    g = gpu(m, 2, [3], "fp64", "geometric", audit=True)
+    ro = orc.compute(m, nquad=2, obstruction_groups=[3], parity_mode="geometric", audit=True)
+    iu = np.triu_indices(m.n_elems, 1)
+    assert np.array_equal(g["F"], ro["raw"] / ro["A"][:, None])
+    assert np.array_equal(g["ac"][iu], ro["audit_count"][iu]) and int(g["ac"][iu].sum()) > 0
+    mx = gpu(m, 2, [3], "mixed", "geometric", audit=True)
     assert np.array_equal(mx["ac"][iu], g["ac"][iu]) and np.array_equal(mx["ah"][iu], g["ah"][iu])
     assert float(np.abs(mx["F"] - g["F"]).max()) <= 1e-6
