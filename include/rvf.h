@@ -198,6 +198,24 @@ int rvf_compute_view_factors(rvf_ctx* ctx, const rvf_mesh* mesh, const rvf_obs* 
                              double* F, double* A, double* F_group, double* A_group, int32_t* group_tags,
                              rvf_stats* st);
 
+/* No counterpart in the reference: for callers that cannot allocate device
+ * memory (code under HIP graph capture, tight serving loops, harnesses that
+ * forbid hipMalloc).  rvf_compute_view_factors without any device allocation:
+ * the solve's intermediates (point tables, slot layout, tiles, partial and
+ * row sums, counters, Duffy pair lists) live in one static device array of
+ * RVF_NOALLOC_SCRATCH_BYTES per GPU, held until the process ends; F, when
+ * F_dev is non-null, is written to that caller-owned device buffer (N x N
+ * doubles, column-major), the other results to host memory as
+ * rvf_compute_view_factors writes them.  A solve whose intermediates exceed
+ * the array (about 8,000 elements at nquad 3) returns RVF_E_NOMEM with the
+ * size it needs; RVF_AUDIT, RVF_GRAPH and RVF_TIMING are refused (RVF_E_ARG).
+ * Every call on a GPU shares that GPU's array: calls that overlap on one GPU
+ * race and must be serialised by the caller.  Returns when the solve is done. */
+#define RVF_NOALLOC_SCRATCH_BYTES (64ll << 20)
+int rvf_compute_view_factors_noalloc(rvf_ctx* ctx, const rvf_mesh* mesh, const rvf_obs* obs, const rvf_opts* opts,
+                                     double* F_dev, double* A, double* F_group, double* A_group,
+                                     int32_t* group_tags, rvf_stats* st);
+
 /* --- quadrature tables as the device sees them (gauss_legendre_2d,
  * src/Quadrature.jl:96-109; tri_quad_rule, src/ViewFactorKernel.jl:42-71) */
 int rvf_quad_rule(int32_t nquad, int32_t tri, double* pts, double* wts);

@@ -120,6 +120,7 @@ def lib():
         "rvf_server_stats": ([P, P], C.c_int),
         "rvf_plan_free": ([P], None),
         "rvf_compute_view_factors": ([P, P, P, P, P, P, P, P, P, P], C.c_int),
+        "rvf_compute_view_factors_noalloc": ([P, P, P, P, P, P, P, P, P, P], C.c_int),
         "rvf_quad_rule": ([I32, I32, P, P], C.c_int),
         "rvf_ctx_event_record": ([P, I32], C.c_int),
         "rvf_ctx_event_elapsed": ([P, I32, I32, C.POINTER(C.c_float)], C.c_int),

@@ -8,6 +8,7 @@
 #include <cstdio>
 #include <cstring>
 #include <map>
+#include <mutex>
 #include <string>
 #include <vector>
 
@@ -56,14 +57,23 @@ static int fail(int code, const std::string& msg) {
 struct DBuf {
     void* p = nullptr;
     size_t n = 0;
-    ~DBuf() { if (p) (void)hipFree(p); }
+    bool own = true;  // false: a view of memory someone else owns (never freed here)
+    ~DBuf() { release(); }
+    void release() {
+        if (p && own) (void)hipFree(p);
+        p = nullptr; n = 0; own = true;
+    }
     int alloc(size_t bytes) {
-        if (p) { (void)hipFree(p); p = nullptr; n = 0; }
+        release();
         if (bytes == 0) bytes = 16;
         hipError_t e = hipMalloc(&p, bytes);
         if (e != hipSuccess) { p = nullptr; return fail(RVF_E_NOMEM, std::string("hipMalloc: ") + hipGetErrorString(e)); }
         n = bytes;
         return 0;
+    }
+    void view(void* q, size_t bytes) {
+        release();
+        p = q; n = q ? bytes : 0; own = false;
     }
     template <class T> T* as() const { return (T*)p; }
 };
@@ -221,58 +231,66 @@ static inline uint32_t spread10(uint32_t v) {
     return v;
 }
 
+// Host part of a layout: slot order, group-pure column chunks and the tile list.
+static void layout_host(const rvf_mesh* m, bool spatial, bool self, Layout& L) {
+    int64_t N = m->N, NS = m->NS;
+    std::vector<int32_t> order(N);
+    for (int64_t e = 0; e < N; e++) order[e] = (int32_t)e;
+    if (spatial) {
+        double lo[3] = {1e300, 1e300, 1e300}, hi[3] = {-1e300, -1e300, -1e300};
+        for (int64_t k = 0; k < m->Nn; k++)
+            for (int c = 0; c < 3; c++) { lo[c] = std::min(lo[c], m->hcoords[3 * k + c]); hi[c] = std::max(hi[c], m->hcoords[3 * k + c]); }
+        std::vector<uint64_t> key(N);
+        for (int64_t e = 0; e < N; e++) {
+            int nc = m->fam[e] >= 4 ? 2 : (m->fam[e] == 0 || m->fam[e] == 2) ? 4 : 3;
+            uint32_t q[3];
+            for (int c = 0; c < 3; c++) {
+                double x = 0;
+                for (int a = 0; a < nc; a++) x += m->hcoords[3 * (int64_t)(m->nodes8[8 * e + a] - 1) + c];
+                x /= nc;
+                double u = hi[c] > lo[c] ? (x - lo[c]) / (hi[c] - lo[c]) : 0.0;
+                q[c] = (uint32_t)std::min(1023.0, std::max(0.0, u * 1024.0));
+            }
+            uint64_t morton = spread10(q[0]) | (spread10(q[1]) << 1) | (spread10(q[2]) << 2);
+            key[e] = ((uint64_t)m->grp_rank[e] << 32) | morton;
+        }
+        std::stable_sort(order.begin(), order.end(), [&](int32_t a, int32_t b) { return key[a] < key[b]; });
+    }
+    L.slot_orig.assign(NS, -1);
+    L.orig_slot.assign(N, 0);
+    L.slot_grp.assign(NS, 0);
+    for (int64_t k = 0; k < N; k++) {
+        L.slot_orig[k] = order[k];
+        L.orig_slot[order[k]] = (int32_t)k;
+        L.slot_grp[k] = m->grp_rank[order[k]];
+    }
+    // group-pure column chunks of <= RVF_CHUNK slots
+    L.chunk_of.assign(NS, 0);
+    L.chunks.clear();
+    for (int64_t e = 0; e < N;) {
+        int64_t b = e;
+        int g = L.slot_grp[e];
+        while (e < N && e - b < RVF_CHUNK && L.slot_grp[e] == g) e++;
+        Chunk c; c.begin = (int32_t)b; c.end = (int32_t)e; c.grp = g; c.pad = 0;
+        for (int64_t k = b; k < e; k++) L.chunk_of[k] = (int32_t)L.chunks.size();
+        L.chunks.push_back(c);
+    }
+    // tiles (strip s, chunk c) exist iff 64 s < last(c); with self pairs
+    // iff 64 s <= last(c)
+    L.tiles.clear();
+    for (size_t c = 0; c < L.chunks.size(); c++) {
+        int64_t last = L.chunks[c].end - 1 + (self ? 1 : 0);
+        for (int64_t s = 0; s * RVF_STRIP < last; s++) L.tiles.push_back(make_int2((int)s, (int)c));
+    }
+}
+
+// Layouts are cached per mesh: bit 0 of the index is the spatial order, bit 1
+// the self-pair tiles.
 static int get_layout(const rvf_mesh* m, bool spatial, bool self, const Layout** out) {
     Layout& L = m->lay[(spatial ? 1 : 0) | (self ? 2 : 0)];
     if (!L.built) {
-        int64_t N = m->N, NS = m->NS;
-        std::vector<int32_t> order(N);
-        for (int64_t e = 0; e < N; e++) order[e] = (int32_t)e;
-        if (spatial) {
-            double lo[3] = {1e300, 1e300, 1e300}, hi[3] = {-1e300, -1e300, -1e300};
-            for (int64_t k = 0; k < m->Nn; k++)
-                for (int c = 0; c < 3; c++) { lo[c] = std::min(lo[c], m->hcoords[3 * k + c]); hi[c] = std::max(hi[c], m->hcoords[3 * k + c]); }
-            std::vector<uint64_t> key(N);
-            for (int64_t e = 0; e < N; e++) {
-                int nc = m->fam[e] >= 4 ? 2 : (m->fam[e] == 0 || m->fam[e] == 2) ? 4 : 3;
-                uint32_t q[3];
-                for (int c = 0; c < 3; c++) {
-                    double x = 0;
-                    for (int a = 0; a < nc; a++) x += m->hcoords[3 * (int64_t)(m->nodes8[8 * e + a] - 1) + c];
-                    x /= nc;
-                    double u = hi[c] > lo[c] ? (x - lo[c]) / (hi[c] - lo[c]) : 0.0;
-                    q[c] = (uint32_t)std::min(1023.0, std::max(0.0, u * 1024.0));
-                }
-                uint64_t morton = spread10(q[0]) | (spread10(q[1]) << 1) | (spread10(q[2]) << 2);
-                key[e] = ((uint64_t)m->grp_rank[e] << 32) | morton;
-            }
-            std::stable_sort(order.begin(), order.end(), [&](int32_t a, int32_t b) { return key[a] < key[b]; });
-        }
-        L.slot_orig.assign(NS, -1);
-        L.orig_slot.assign(N, 0);
-        L.slot_grp.assign(NS, 0);
-        for (int64_t k = 0; k < N; k++) {
-            L.slot_orig[k] = order[k];
-            L.orig_slot[order[k]] = (int32_t)k;
-            L.slot_grp[k] = m->grp_rank[order[k]];
-        }
-        // group-pure column chunks of <= RVF_CHUNK slots
-        L.chunk_of.assign(NS, 0);
-        L.chunks.clear();
-        for (int64_t e = 0; e < N;) {
-            int64_t b = e;
-            int g = L.slot_grp[e];
-            while (e < N && e - b < RVF_CHUNK && L.slot_grp[e] == g) e++;
-            Chunk c; c.begin = (int32_t)b; c.end = (int32_t)e; c.grp = g; c.pad = 0;
-            for (int64_t k = b; k < e; k++) L.chunk_of[k] = (int32_t)L.chunks.size();
-            L.chunks.push_back(c);
-        }
-        // tiles (strip s, chunk c) exist iff 64 s < last(c); with self pairs
-        // iff 64 s <= last(c)
-        L.tiles.clear();
-        for (size_t c = 0; c < L.chunks.size(); c++) {
-            int64_t last = L.chunks[c].end - 1 + (self ? 1 : 0);
-            for (int64_t s = 0; s * RVF_STRIP < last; s++) L.tiles.push_back(make_int2((int)s, (int)c));
-        }
+        const int64_t NS = m->NS;
+        layout_host(m, spatial, self, L);
         int rc;
         if ((rc = upload(L.d_slot_orig, L.slot_orig.data(), sizeof(int32_t) * NS)) ||
             (rc = upload(L.d_slot_grp, L.slot_grp.data(), sizeof(int32_t) * NS)) ||
@@ -901,8 +919,31 @@ static int occ_kind(int variant, bool mixed, int64_t n_tris);
 static const char* occ_kind_error(int variant, int64_t n_tris);
 static int pair_kind(const rvf_plan* p);
 
-int rvf_plan_create(rvf_ctx* ctx, const rvf_mesh* mesh, const rvf_obs* obs, const rvf_opts* opts, rvf_plan** out) {
-    if (!ctx || !mesh || !opts || !out) return fail(RVF_E_ARG, "rvf_plan_create: bad arguments");
+// Device buffers of a plan come either from hipMalloc (rvf_plan_create) or
+// from consecutive 256-byte-aligned pieces of the library's static scratch
+// array (rvf_compute_view_factors_noalloc).  In the second form `used` counts
+// the bytes the plan needs even past `cap`, so a refusal can say how many.
+struct BufSource {
+    char* base = nullptr;  // null: hipMalloc
+    size_t cap = 0, used = 0;
+    bool over = false;
+    int get(DBuf& b, size_t bytes) {
+        if (!base) return b.alloc(bytes);
+        if (bytes == 0) bytes = 16;
+        const size_t off = (used + 255) & ~(size_t)255;
+        used = off + bytes;
+        if (used > cap) { over = true; b.view(nullptr, 0); return 0; }
+        b.view(base + off, bytes);
+        return 0;
+    }
+};
+
+// Sets up a plan: quadrature tables, slot layout, Duffy pair list and every
+// device buffer of the solve.  With src.base set (the no-allocation form) the
+// layout goes into *own_layout and the plan's device buffers, layout and
+// uploads included, into the scratch pieces; F is then the caller's F_dev.
+static int plan_init(rvf_ctx* ctx, const rvf_mesh* mesh, const rvf_obs* obs, const rvf_opts* opts, rvf_plan* p,
+                     BufSource& src, Layout* own_layout, double* F_dev) {
     if (opts->nquad < 1 || opts->nquad > 8) return fail(RVF_E_ARG, "nquad must be in 1..8");
     if (opts->method < 0 || opts->method > 2) return fail(RVF_E_ARG, "unknown method");
     if (opts->precision < 0 || opts->precision > 1) return fail(RVF_E_ARG, "unknown precision");
@@ -912,10 +953,9 @@ int rvf_plan_create(rvf_ctx* ctx, const rvf_mesh* mesh, const rvf_obs* obs, cons
     if (obs && !obs->trees.empty() && opts->method != RVF_METHOD_MC &&
         occ_kind(opts->variant, opts->precision == RVF_MIXED, obs->n_tris) < 0)
         return fail(RVF_E_ARG, std::string("rvf_plan_create: ") + occ_kind_error(opts->variant, obs->n_tris));
+    const bool scratch = src.base != nullptr;
     HIPCHK(hipSetDevice(ctx->device));
-    rvf_plan* p = new rvf_plan();
     p->ctx = ctx; p->mesh = mesh; p->obs = obs; p->opts = *opts;
-    int rc0;
     std::vector<double> rule(RVF_RULE_SIZE, 0.0);
     p->nq_quad = quad_rule(opts->nquad, rule.data(), rule.data() + 2 * 64);
     p->nq_tri = tri_rule(opts->nquad, rule.data() + 3 * 64, rule.data() + 3 * 64 + 26);
@@ -936,37 +976,23 @@ int rvf_plan_create(rvf_ctx* ctx, const rvf_mesh* mesh, const rvf_obs* obs, cons
     // spatially compact strips for occluded quadrature solves (coherent ray bundles)
     bool spatial = obs && !obs->trees.empty() && opts->method == RVF_METHOD_QUAD;
     const bool self = (opts->flags & RVF_SELF_VF) != 0;
-    if ((rc0 = get_layout(mesh, spatial, self, &p->L))) { delete p; return rc0; }
-    size_t nch = p->L->chunks.size(), nstrip = NS / RVF_STRIP;
     int rc;
-    if ((rc = upload(p->rule, rule.data(), sizeof(double) * rule.size())) ||
-        (rc = p->P64.alloc(sizeof(double) * 8 * p->maxpts * NS)) ||
-        (rc = p->Q64.alloc(sizeof(double) * 8 * p->maxpts * NS)) ||
-        (rc = p->P32.alloc(sizeof(float) * 8 * p->maxpts * NS)) ||
-        (rc = p->Q32.alloc(sizeof(float) * 8 * p->maxpts * NS)) ||
-        (rc = p->anchor.alloc(sizeof(double) * 4 * NS)) || (rc = p->area.alloc(sizeof(double) * NS)) ||
-        (rc = p->area_orig.alloc(sizeof(double) * std::max<int64_t>(N, 1))) ||
-        (rc = p->elem_np.alloc(sizeof(int32_t) * NS)) || (rc = p->bad.alloc(sizeof(int))) ||
-        (rc = p->part.alloc(sizeof(double) * std::max<size_t>(nch, 1) * NS)) ||
-        (rc = p->partT.alloc(sizeof(double) * std::max<size_t>(nstrip, 1) * NS)) ||
-        (rc = p->rowsum.alloc(sizeof(double) * NS)) || (rc = p->W.alloc(sizeof(double) * std::max(G, 1) * NS)) ||
-        (self && (rc = p->diag.alloc(sizeof(double) * NS))) ||
-        (rc = p->counters.alloc(sizeof(unsigned long long) * 16)) ||
-        (rc = p->Tseg.alloc(sizeof(double) * RVF_RSEG * NS)) ||
-        (rc = p->Wseg.alloc(sizeof(double) * RVF_RSEG * std::max(G, 1) * NS))) {
-        delete p;
+    if (scratch) {
+        layout_host(mesh, spatial, self, *own_layout);
+        p->L = own_layout;
+    } else if ((rc = get_layout(mesh, spatial, self, &p->L))) {
         return rc;
     }
-    if (opts->flags & RVF_WANT_F) {
-        if ((rc = p->F.alloc(sizeof(double) * N * N))) { delete p; return rc; }
-    }
+    size_t nch = p->L->chunks.size(), nstrip = NS / RVF_STRIP;
+    // Duffy: singular Quad8 pairs (singularity_type, DuffyKernel.jl:95-115)
+    std::vector<DuffyPair> pairs;
+    std::vector<int32_t> rptr, rlist;
+    double g01[16] = {0};
     if (opts->method == RVF_METHOD_DUFFY) {
-        // singular Quad8 pairs (singularity_type, DuffyKernel.jl:95-115)
         std::map<int32_t, std::vector<int32_t>> by_corner;
         for (int64_t e = 0; e < N; e++)
             if (mesh->fam[e] == 0)
                 for (int q = 0; q < 4; q++) by_corner[mesh->nodes8[8 * e + q]].push_back((int32_t)e);
-        std::vector<DuffyPair> pairs;
         std::vector<int32_t> cand;
         for (int64_t i = 0; i < N; i++) {
             if (mesh->fam[i] != 0) continue;
@@ -993,7 +1019,7 @@ int rvf_plan_create(rvf_ctx* ctx, const rvf_mesh* mesh, const rvf_obs* obs, cons
             }
         }
         p->npairs = (int)pairs.size();
-        std::vector<int32_t> rptr(N + 1, 0), rlist;
+        rptr.assign(N + 1, 0);
         for (const DuffyPair& d : pairs) { rptr[d.i + 1]++; if (d.j != d.i) rptr[d.j + 1]++; }
         for (int64_t r = 0; r < N; r++) rptr[r + 1] += rptr[r];
         rlist.assign(rptr[N], 0);
@@ -1002,36 +1028,104 @@ int rvf_plan_create(rvf_ctx* ctx, const rvf_mesh* mesh, const rvf_obs* obs, cons
             rlist[fillp[pairs[k].i]++] = k;
             if (pairs[k].j != pairs[k].i) rlist[fillp[pairs[k].j]++] = k;
         }
-        double p1[8], w1[8], g01[16] = {0};
+        double p1[8], w1[8];
         int nq = gl1d(opts->nquad, p1, w1);
         for (int k = 0; k < nq; k++) { g01[k] = (p1[k] + 1.0) / 2.0; g01[8 + k] = w1[k] / 2.0; }
-        if ((rc = upload(p->d_pairs, pairs.data(), sizeof(DuffyPair) * pairs.size())) ||
-            (rc = upload(p->d_gl01, g01, sizeof g01)) || (rc = p->d_dval.alloc(sizeof(double) * pairs.size())) ||
-            (rc = upload(p->d_rptr, rptr.data(), sizeof(int32_t) * rptr.size())) ||
-            (rc = upload(p->d_rlist, rlist.data(), sizeof(int32_t) * rlist.size()))) {
-            delete p;
-            return rc;
-        }
         p->st.singular_pairs = p->npairs;
     }
+    // device buffers
+    if ((rc = src.get(p->rule, sizeof(double) * rule.size())) ||
+        (rc = src.get(p->P64, sizeof(double) * 8 * p->maxpts * NS)) ||
+        (rc = src.get(p->Q64, sizeof(double) * 8 * p->maxpts * NS)) ||
+        (rc = src.get(p->P32, sizeof(float) * 8 * p->maxpts * NS)) ||
+        (rc = src.get(p->Q32, sizeof(float) * 8 * p->maxpts * NS)) ||
+        (rc = src.get(p->anchor, sizeof(double) * 4 * NS)) || (rc = src.get(p->area, sizeof(double) * NS)) ||
+        (rc = src.get(p->area_orig, sizeof(double) * std::max<int64_t>(N, 1))) ||
+        (rc = src.get(p->elem_np, sizeof(int32_t) * NS)) || (rc = src.get(p->bad, sizeof(int))) ||
+        (rc = src.get(p->part, sizeof(double) * std::max<size_t>(nch, 1) * NS)) ||
+        (rc = src.get(p->partT, sizeof(double) * std::max<size_t>(nstrip, 1) * NS)) ||
+        (rc = src.get(p->rowsum, sizeof(double) * NS)) || (rc = src.get(p->W, sizeof(double) * std::max(G, 1) * NS)) ||
+        (self && (rc = src.get(p->diag, sizeof(double) * NS))) ||
+        (rc = src.get(p->counters, sizeof(unsigned long long) * 16)) ||
+        (rc = src.get(p->Tseg, sizeof(double) * RVF_RSEG * NS)) ||
+        (rc = src.get(p->Wseg, sizeof(double) * RVF_RSEG * std::max(G, 1) * NS)))
+        return rc;
+    if (opts->flags & RVF_WANT_F) {
+        if (scratch) p->F.view(F_dev, sizeof(double) * N * N);
+        else if ((rc = p->F.alloc(sizeof(double) * N * N))) return rc;
+    }
+    if (opts->method == RVF_METHOD_DUFFY &&
+        ((rc = src.get(p->d_pairs, sizeof(DuffyPair) * pairs.size())) || (rc = src.get(p->d_gl01, sizeof g01)) ||
+         (rc = src.get(p->d_dval, sizeof(double) * pairs.size())) ||
+         (rc = src.get(p->d_rptr, sizeof(int32_t) * rptr.size())) ||
+         (rc = src.get(p->d_rlist, sizeof(int32_t) * rlist.size()))))
+        return rc;
+    Layout* own = scratch ? own_layout : nullptr;
+    if (own &&
+        ((rc = src.get(own->d_slot_orig, sizeof(int32_t) * NS)) || (rc = src.get(own->d_slot_grp, sizeof(int32_t) * NS)) ||
+         (rc = src.get(own->d_chunks, sizeof(Chunk) * own->chunks.size())) ||
+         (rc = src.get(own->d_chunk_of, sizeof(int32_t) * NS)) ||
+         (rc = src.get(own->d_tiles, sizeof(int2) * own->tiles.size()))))
+        return rc;
     if ((opts->flags & RVF_AUDIT) && obs) {
-        if ((rc = p->audit_c.alloc(sizeof(uint32_t) * N * N)) || (rc = p->audit_h.alloc(sizeof(uint32_t) * N * N))) {
-            delete p;
+        if (scratch) return fail(RVF_E_ARG, "RVF_AUDIT needs device allocations");
+        if ((rc = p->audit_c.alloc(sizeof(uint32_t) * N * N)) || (rc = p->audit_h.alloc(sizeof(uint32_t) * N * N)))
             return rc;
-        }
         HIPCHK(hipMemsetAsync(p->audit_c.p, 0, p->audit_c.n, ctx->stream));
         HIPCHK(hipMemsetAsync(p->audit_h.p, 0, p->audit_h.n, ctx->stream));
     }
-    // zero point arrays once (padding lanes / points read but never accumulated)
-    HIPCHK(hipMemsetAsync(p->P64.p, 0, p->P64.n, ctx->stream));
-    HIPCHK(hipMemsetAsync(p->P32.p, 0, p->P32.n, ctx->stream));
-    HIPCHK(hipMemsetAsync(p->anchor.p, 0, p->anchor.n, ctx->stream));
-    HIPCHK(hipMemsetAsync(p->area.p, 0, p->area.n, ctx->stream));
-    HIPCHK(hipMemsetAsync(p->elem_np.p, 0, p->elem_np.n, ctx->stream));
-    for (int k = 0; k < 8; k++) HIPCHK(hipEventCreate(&p->ev[k]));
+    if (src.over)
+        return fail(RVF_E_NOMEM, "this solve needs " + std::to_string(src.used) + " bytes of scratch; the static scratch array holds " +
+                                     std::to_string(src.cap) + " bytes (use rvf_compute_view_factors)");
+    // uploads and zero fills, ordered on the context stream
+    auto h2d = [&](DBuf& b, const void* h, size_t bytes) -> int {
+        if (!bytes) return 0;
+        if (scratch) HIPCHK(hipMemcpyAsync(b.p, h, bytes, hipMemcpyHostToDevice, ctx->stream));
+        else HIPCHK(hipMemcpy(b.p, h, bytes, hipMemcpyHostToDevice));
+        return 0;
+    };
+    if (scratch) {
+        // the array keeps the previous call's contents: clear this solve's part
+        if ((rc = rvf_launch_zero(src.base, (int64_t)src.used, ctx->stream)))
+            return fail(RVF_E_HIP, std::string("zero launch: ") + hipGetErrorString((hipError_t)rc));
+    }
+    if ((rc = h2d(p->rule, rule.data(), sizeof(double) * rule.size()))) return rc;
+    if (opts->method == RVF_METHOD_DUFFY &&
+        ((rc = h2d(p->d_pairs, pairs.data(), sizeof(DuffyPair) * pairs.size())) || (rc = h2d(p->d_gl01, g01, sizeof g01)) ||
+         (rc = h2d(p->d_rptr, rptr.data(), sizeof(int32_t) * rptr.size())) ||
+         (rc = h2d(p->d_rlist, rlist.data(), sizeof(int32_t) * rlist.size()))))
+        return rc;
+    if (own &&
+        ((rc = h2d(own->d_slot_orig, own->slot_orig.data(), sizeof(int32_t) * NS)) ||
+         (rc = h2d(own->d_slot_grp, own->slot_grp.data(), sizeof(int32_t) * NS)) ||
+         (rc = h2d(own->d_chunks, own->chunks.data(), sizeof(Chunk) * own->chunks.size())) ||
+         (rc = h2d(own->d_chunk_of, own->chunk_of.data(), sizeof(int32_t) * NS)) ||
+         (rc = h2d(own->d_tiles, own->tiles.data(), sizeof(int2) * own->tiles.size()))))
+        return rc;
+    if (scratch) {
+        // the host arrays above are released on return: finish the copies first
+        HIPCHK(hipStreamSynchronize(ctx->stream));
+    } else {
+        // zero point arrays once (padding lanes / points read but never accumulated)
+        HIPCHK(hipMemsetAsync(p->P64.p, 0, p->P64.n, ctx->stream));
+        HIPCHK(hipMemsetAsync(p->P32.p, 0, p->P32.n, ctx->stream));
+        HIPCHK(hipMemsetAsync(p->anchor.p, 0, p->anchor.n, ctx->stream));
+        HIPCHK(hipMemsetAsync(p->area.p, 0, p->area.n, ctx->stream));
+        HIPCHK(hipMemsetAsync(p->elem_np.p, 0, p->elem_np.n, ctx->stream));
+        for (int k = 0; k < 8; k++) HIPCHK(hipEventCreate(&p->ev[k]));
+    }
     p->st.n_elems = N;
     p->st.unique_pairs = self ? N * (N + 1) / 2 : N * (N - 1) / 2;
     p->st.tiles = (int64_t)p->L->tiles.size();
+    return 0;
+}
+
+int rvf_plan_create(rvf_ctx* ctx, const rvf_mesh* mesh, const rvf_obs* obs, const rvf_opts* opts, rvf_plan** out) {
+    if (!ctx || !mesh || !opts || !out) return fail(RVF_E_ARG, "rvf_plan_create: bad arguments");
+    rvf_plan* p = new rvf_plan();
+    BufSource src;
+    const int rc = plan_init(ctx, mesh, obs, opts, p, src, nullptr, nullptr);
+    if (rc) { rvf_plan_free(p); return rc; }
     *out = p;
     return 0;
 }
@@ -1546,6 +1640,79 @@ int rvf_compute_view_factors(rvf_ctx* ctx, const rvf_mesh* mesh, const rvf_obs* 
 }
 
 // ---------------------------------------------------------------------------
+// Solve without device allocation (no reference counterpart): for callers that
+// cannot call hipMalloc.  Every intermediate of the solve lives in `scratch`,
+// one array per GPU for the life of the process; its device address is looked
+// up once per device when a context is created (scratch_ptr), and host code
+// uses only that address.  Calls on one GPU share the array: calls that
+// overlap on one GPU race, so callers serialise them.
+// ---------------------------------------------------------------------------
+static __device__ __align__(256) unsigned char scratch[RVF_NOALLOC_SCRATCH_BYTES];
+
+#define RVF_MAX_DEVICES 64
+static std::mutex g_scratch_mu;
+static void* g_scratch_ptr[RVF_MAX_DEVICES] = {};
+static hipError_t g_scratch_err[RVF_MAX_DEVICES] = {};
+
+// With `device` current: record the device address of `scratch` on it.
+static void scratch_lookup(int device) {
+    if (device < 0 || device >= RVF_MAX_DEVICES) return;
+    std::lock_guard<std::mutex> lk(g_scratch_mu);
+    if (g_scratch_ptr[device]) return;
+    void* ptr = nullptr;
+    const hipError_t e = hipGetSymbolAddress(&ptr, HIP_SYMBOL(scratch));
+    if (e != hipSuccess || !ptr) {
+        g_scratch_err[device] = e != hipSuccess ? e : hipErrorInvalidSymbol;
+        (void)hipGetLastError();
+        return;
+    }
+    g_scratch_ptr[device] = ptr;
+    g_scratch_err[device] = hipSuccess;
+}
+
+static int scratch_ptr(int device, char** out) {
+    std::lock_guard<std::mutex> lk(g_scratch_mu);
+    if (device < 0 || device >= RVF_MAX_DEVICES) return fail(RVF_E_ARG, "device index beyond the scratch table");
+    if (!g_scratch_ptr[device])
+        return fail(RVF_E_HIP, std::string("static scratch array unavailable on this device: ") +
+                                   hipGetErrorString(g_scratch_err[device]));
+    *out = (char*)g_scratch_ptr[device];
+    return 0;
+}
+
+int rvf_compute_view_factors_noalloc(rvf_ctx* ctx, const rvf_mesh* mesh, const rvf_obs* obs, const rvf_opts* opts,
+                                     double* F_dev, double* A, double* F_group, double* A_group, int32_t* group_tags,
+                                     rvf_stats* st) {
+    if (!ctx || !mesh || !opts) return fail(RVF_E_ARG, "rvf_compute_view_factors_noalloc: bad arguments");
+    if (opts->flags & (RVF_AUDIT | RVF_GRAPH | RVF_TIMING))
+        return fail(RVF_E_ARG, "rvf_compute_view_factors_noalloc: RVF_AUDIT, RVF_GRAPH and RVF_TIMING need device "
+                               "allocations (use rvf_plan_create)");
+    char* base = nullptr;
+    int rc = scratch_ptr(ctx->device, &base);
+    if (rc) return rc;
+    rvf_opts o = *opts;
+    o.flags &= ~RVF_WANT_F;
+    if (F_dev) o.flags |= RVF_WANT_F;
+    rvf_plan p;    // buffers are views into the scratch array and F_dev
+    Layout lay;    // the slot layout, built for this call
+    BufSource src;
+    src.base = base;
+    src.cap = (size_t)RVF_NOALLOC_SCRATCH_BYTES;
+    rc = plan_init(ctx, mesh, obs, &o, &p, src, &lay, F_dev);
+    if (!rc) rc = enqueue_solve(&p, ctx->stream, false);
+    if (!rc) {
+        p.ran = true;
+        rc = rvf_plan_sync(&p);
+    } else {
+        // leave nothing of this call running on the shared array
+        (void)hipStreamSynchronize(ctx->stream);
+    }
+    if (!rc) rc = rvf_plan_fetch(&p, nullptr, A, nullptr, F_group, A_group, group_tags, nullptr, nullptr);
+    if (!rc && st) *st = p.st;
+    return rc;
+}
+
+// ---------------------------------------------------------------------------
 // context + misc
 // ---------------------------------------------------------------------------
 int rvf_ctx_create(int device, rvf_ctx** out) {
@@ -1554,6 +1721,7 @@ int rvf_ctx_create(int device, rvf_ctx** out) {
     HIPCHK(hipGetDeviceCount(&n));
     if (device < 0 || device >= n) return fail(RVF_E_ARG, "no such device");
     HIPCHK(hipSetDevice(device));
+    scratch_lookup(device);  // per device, with the device current
     rvf_ctx* c = new rvf_ctx();
     c->device = device;
     hipError_t e = hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking);
