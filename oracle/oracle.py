@@ -45,6 +45,8 @@ def lib():
         L.orc_pair.restype = C.c_double
         L.orc_pair_duffy.argtypes = [P, P, P, C.c_int64, C.c_int64, C.c_int]
         L.orc_pair_duffy.restype = C.c_double
+        L.orc_mc_expect.argtypes = [P, P, P, C.c_int64, C.c_int64, C.c_int, C.c_int]
+        L.orc_mc_expect.restype = C.c_double
         L.orc_bvh_export.argtypes = [P, C.c_int64, C.c_int, C.c_int, P, P, P]
         L.orc_ray_hits.argtypes = [P, C.c_int64, C.c_int, P, P, C.c_double]
         L.orc_ray_hits_brute.argtypes = [P, C.c_int64, C.c_int, P, P, C.c_double]
@@ -121,6 +123,13 @@ def pair(mesh, i, j, nquad):
 
 def pair_duffy(mesh, i, j, nquad):
     return lib().orc_pair_duffy(_p(mesh.coords), _p(mesh.family), _p(mesh.nodes), i, j, nquad)
+
+
+def mc_expect(mesh, i, j, n_samples, g=3):
+    """Expectation of the Monte Carlo pair estimate raw_ij of two quadrilaterals
+    (the stream's sample k puts both points in parameter cell k), by Gauss
+    quadrature inside each paired cell."""
+    return lib().orc_mc_expect(_p(mesh.coords), _p(mesh.family), _p(mesh.nodes), i, j, n_samples, g)
 
 
 def duffy_rule(kind, nquad, ci, cj):

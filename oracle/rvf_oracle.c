@@ -23,6 +23,8 @@
  *                           triangles (tree-independent) on random soups
  *   MC stream               closed-form integer recurrences (splitmix64,
  *                           xorshift64) + 3-sigma agreement with quadrature
+ *                           (coaxial plates) and with orc_mc_expect, the
+ *                           paired-cell expectation of the stream's estimate
  *   Duffy                   no reference test covers it; pinned by the
  *                           invariants non-adjacent == standard and non-Quad8 ==
  *                           standard quadrature, by closed-form moments of the
@@ -1232,6 +1234,46 @@ double orc_pair_duffy(const double* coords, const int8_t* fam, const int32_t* no
     mesh_rules(&m, nquad);
     m.dim2 = fam[i] >= 4;
     return pair_duffy(&m, i, j, 0);
+}
+
+/* Expectation of the Monte Carlo pair estimate of two quadrilaterals (mc_pair,
+ * GPUMCKernels.jl:146-262) with n_samples = s^2: sample (si, sj) puts element
+ * i's point and element j's point in the SAME parameter cell (si, sj), so
+ *   E[raw] = (ri rj / n) sum_cells E_{u in cell x cell}[K a_i a_j],
+ * evaluated here by a g x g Gauss-Legendre rule on [0,1]^2 inside each cell of
+ * each element (g^4 points per cell pair).  No visibility (unoccluded pairs).
+ * This pins the stream's stratum mapping statistically; it is not the pair's
+ * view-factor integral, which sums over all cell pairs (si, sj) x (si', sj'). */
+double orc_mc_expect(const double* coords, const int8_t* fam, const int32_t* nodes8, int64_t i, int64_t j,
+                     int nsamp, int g) {
+    OrcMesh m;
+    memset(&m, 0, sizeof m);
+    m.coords = coords; m.fam = fam; m.nodes = nodes8; m.N = (i > j ? i : j) + 1;
+    if (!((fam[i] == 0 || fam[i] == 2) && (fam[j] == 0 || fam[j] == 2))) return NAN;
+    int s = (int)floor(sqrt((double)nsamp));
+    if (s * s != nsamp || g < 1 || g > 8) return NAN;
+    double p[8], w[8];
+    orc_gl1d(g, p, w);
+    for (int k = 0; k < g; k++) { p[k] = (p[k] + 1.0) / 2.0; w[k] = w[k] / 2.0; }  /* [0,1] */
+    double tot = 0.0;
+    for (int si = 0; si < s; si++)
+        for (int sj = 0; sj < s; sj++) {
+            double cell = 0.0;
+            for (int a1 = 0; a1 < g; a1++)
+                for (int a2 = 0; a2 < g; a2++) {
+                    double xi[3], ni[3], ai;
+                    mc_sample(&m, i, (si + p[a1]) / s, (sj + p[a2]) / s, xi, ni, &ai);
+                    for (int b1 = 0; b1 < g; b1++)
+                        for (int b2 = 0; b2 < g; b2++) {
+                            double xj[3], nj[3], aj;
+                            mc_sample(&m, j, (si + p[b1]) / s, (sj + p[b2]) / s, xj, nj, &aj);
+                            double K = orc_vf_kernel(xi, ni, xj, nj);
+                            cell += (((w[a1] * w[a2]) * (w[b1] * w[b2])) * K) * (ai * aj);
+                        }
+                }
+            tot += cell;
+        }
+    return (tot * 16.0) / nsamp;
 }
 
 /* xorshift stream (test entry) */

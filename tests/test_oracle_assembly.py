@@ -204,8 +204,9 @@ def test_mc_stream_closed_form(orc):
 def test_mc_matches_quadrature_3sigma(orc):
     # The reference pairs stratum k of element i with stratum k of element j
     # (GPUMCKernels.jl:176-216, MCKernel.jl:105-121), which biases single
-    # element pairs (~0.3% at 8x8 plates) but cancels at group level; the
-    # 3-sigma agreement (sigma from R=8 seeds) is therefore checked on F_group.
+    # element pairs (~0.3% at 8x8 plates); between these coaxial plates the
+    # biases cancel at group level, so the 3-sigma agreement (sigma from R=8
+    # seeds) holds on F_group here (not in general: see the next test).
     m = M.parallel_plates(8)
     q = orc.compute(m, nquad=5)
     Fg_q = orc.aggregate(m, q["raw"] / q["A"][:, None], q["A"])[2][0, 1]
@@ -217,6 +218,28 @@ def test_mc_matches_quadrature_3sigma(orc):
     sd = ests.std(ddof=1) / math.sqrt(len(ests))
     assert abs(ests.mean() - Fg_q) < 3 * sd
     assert abs(ests.mean() - F_PAR) < 1e-5
+
+
+def test_mc_expectation_pins_the_stream_cells(orc):
+    # Sample (si, sj) of both elements lies in parameter cell (si, sj)
+    # (GPUMCKernels.jl:172-195): the estimate converges to the paired-cell
+    # expectation, which orc.mc_expect evaluates by Gauss quadrature inside
+    # each cell, independently of the stream.  The MC mean over 48 seeds is
+    # within 3 sigma of it for near, far and edge-sharing side pairs of a
+    # curved cylinder, and far (> 50 sigma) from the pairs' integrals.
+    m = M.cylinder(n_around=16, n_along=4, cap_tris=40)
+    q = orc.compute(m, nquad=8)
+    for i, j in ((0, 2), (0, 5), (0, 18), (3, 40), (0, 1)):
+        est = np.array([orc.compute(m, method="mc", n_samples=256, seed=500 + s, rows=(i, i + 1))["raw"][i, j]
+                        for s in range(48)])
+        sd = est.std(ddof=1) / math.sqrt(len(est))
+        E = orc.mc_expect(m, i, j, 256, 3)
+        assert abs(orc.mc_expect(m, i, j, 256, 5) - E) <= 1e-7 * E  # the cell rule has converged
+        assert abs(est.mean() - E) < 3 * sd, (i, j)
+        assert abs(est.mean() - q["raw"][i, j]) > 50 * sd, (i, j)
+    # quadrilaterals only, and n_samples must be a square
+    tri = int(np.nonzero(m.family == 1)[0][0])
+    assert math.isnan(orc.mc_expect(m, 0, tri, 256, 3)) and math.isnan(orc.mc_expect(m, 0, 2, 200, 3))
 
 
 # ------------------------------------------------------------------------------
