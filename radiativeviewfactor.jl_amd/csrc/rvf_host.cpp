@@ -1226,7 +1226,8 @@ static size_t pair_lds(const rvf_plan* p) {
     if (kind == RVF_K_MIX_OCC_LANE) return 0;
     const size_t ring =
         kind >= RVF_K_FP64_SPLIT1
-            ? (mixed ? (RVF_TQCAP_MIX + RVF_CQCAP) * sizeof(uint32_t) + 6 * 64 * sizeof(float)
+            ? (mixed ? (RVF_TQCAP_MIX + RVF_CQCAP) * sizeof(uint32_t) + 6 * 64 * sizeof(float) +
+                           RVF_FRONTIER * 64 * sizeof(int32_t)
                      : RVF_TQCAP * sizeof(uint32_t))
             : 0;
     return accq + RVF_QCOMBO * (sizeof(uint64_t) + 2 * sizeof(uint32_t)) + 2 * 64 * sizeof(int32_t) + ring;

@@ -1030,6 +1030,7 @@ struct QCtx {
     uint64_t revmask;    // lanes whose element is the pair's inner element
     const int* lroot;    // LDS: per lane, tree root of (lane's group, column group)
     int* lhint;          // LDS (mixed, split form): per lane, a triangle that certainly blocked a ray of its pair, or -1
+    const int* lfr;      // LDS (mixed, split form): per lane, the frontier subtrees its pair's rays walk, [k * 64 + lane]
     const float* stage;  // LDS (mixed, split form): per lane [dax day daz fax fay faz] x 64
     float fjx, fjy, fjz; // column element's anchor as FP32
 };
@@ -1432,6 +1433,23 @@ __device__ __forceinline__ int tclass(const PairArgs& a, const SRay& S, int tri)
     return rvf_tri_classify(T.v0, T.e1, T.e2, T.E1, T.E2, R, 80.0f * uu * a.Sm);
 }
 
+// Mixed split traversal, after a node step of a slot: the first step in a
+// subtree reads its root, whose miss link ends the subtree's preorder range;
+// a walk that leaves the range (or the tree) moves on to the pair's next
+// frontier subtree, or ends after the last.  The slot's tag holds the
+// frontier index (bits 16-19) and count (bits 20-23).  Branch-free: the next
+// entry is read from LDS whether or not it is taken.
+__device__ __forceinline__ void frontier_next(const QCtx& X, int miss, int& cur, int& tag, int& end) {
+    end = end == -2 ? (miss >= 0 ? miss : 0x7fffffff) : end;
+    const bool leave = cur < 0 || cur >= end;
+    const int k = ((tag >> 16) & 15) + 1;
+    const int nxt = X.lfr[min(k, RVF_FRONTIER - 1) * 64 + (tag & 63)];
+    const bool more = leave && k < (tag >> 20);
+    cur = more ? nxt : (leave ? -1 : cur);
+    end = more ? -2 : end;
+    tag = more ? (tag & 0xf0ffff) | (k << 16) : tag;
+}
+
 template <bool MIX, int S>
 __device__ __forceinline__ void split_drain(const PairArgs& a, const QCtx& X, int nc, const uint64_t* need,
                                             uint32_t* blk, int lane, uint32_t* tq, QCount& K) {
@@ -1446,10 +1464,13 @@ __device__ __forceinline__ void split_drain(const PairArgs& a, const QCtx& X, in
     uint32_t chead = 0, ctail = 0; // candidate ring (mixed)
     bool bdirty = false;           // a batch may have set blocked bits (uniform)
     uint32_t* cq = tq + TQC;
-    int cur[S], tag[S];
+    // a slot: current node, tag = (combination << 6 | lane) | frontier index << 16
+    // (mixed), and (mixed) the end of the current subtree's preorder range
+    // (-2 until its root has been read)
+    int cur[S], tag[S], end[S];
     NRay<MIX> R[S];
 #pragma unroll
-    for (int s = 0; s < S; s++) { cur[s] = -1; tag[s] = 0; }
+    for (int s = 0; s < S; s++) { cur[s] = -1; tag[s] = 0; end[s] = -2; }
     for (;;) {
         // A ray whose blocked bit a task has set stops here, and its slot is
         // refilled below.  Blocked bits change only in task and exact batches,
@@ -1457,7 +1478,7 @@ __device__ __forceinline__ void split_drain(const PairArgs& a, const QCtx& X, in
         if (bdirty) {
 #pragma unroll
             for (int s = 0; s < S; s++) {
-                const int c = tag[s] >> 6, b = tag[s] & 63;
+                const int c = (tag[s] & 0xffff) >> 6, b = tag[s] & 63;
                 if (cur[s] >= 0 && ((blk[c * 2 + (b >> 5)] >> (b & 31)) & 1u)) cur[s] = -1;
             }
             bdirty = false;
@@ -1494,11 +1515,13 @@ __device__ __forceinline__ void split_drain(const PairArgs& a, const QCtx& X, in
                             K.c1++;
                             atomicOr(&blk[c * 2 + (b >> 5)], 1u << (b & 31));
                         } else {
+                            // the walk starts at the pair's first frontier subtree
                             NRay<MIX> nr;
-                            const int root = nray_from(SR, X.lroot[b], nr);
+                            const int root = nray_from(SR, X.lfr[b], nr);
+                            const int nf = X.lroot[b];
 #pragma unroll
                             for (int s = 0; s < S; s++)
-                                if (s == es) { tag[s] = (c << 6) | b; cur[s] = root; R[s] = nr; }
+                                if (s == es) { tag[s] = (c << 6) | b | (nf << 20); cur[s] = root; end[s] = -2; R[s] = nr; }
                         }
                     }
                 }
@@ -1533,6 +1556,13 @@ __device__ __forceinline__ void split_drain(const PairArgs& a, const QCtx& X, in
                 K.steps += act;
                 cnt[s] = act ? cnt[s] : 0;
                 cur[s] = act ? nx : -1;
+                if (MIX) {
+                    int c2 = cur[s], t2 = tag[s], e2 = end[s];
+                    frontier_next(X, nd[s].miss, c2, t2, e2);
+                    cur[s] = act ? c2 : cur[s];
+                    tag[s] = act ? t2 : tag[s];
+                    end[s] = act ? e2 : end[s];
+                }
             }
             // mixed: a slot whose step queued no leaf takes one more step, so
             // the iteration's fixed cost (refill, ring offsets, batches) is
@@ -1549,16 +1579,25 @@ __device__ __forceinline__ void split_drain(const PairArgs& a, const QCtx& X, in
                     int f2 = 0, c2 = 0;
                     const int nx = nstep(nd[s], max(cur[s], 0), R[s], f2, c2);
                     K.steps += act;
-                    if (act) { first[s] = f2; cnt[s] = c2; cur[s] = nx; }
+                    int cx = nx, tx = tag[s], ex = end[s];
+                    frontier_next(X, nd[s].miss, cx, tx, ex);
+                    first[s] = act ? f2 : first[s];
+                    cnt[s] = act ? c2 : cnt[s];
+                    cur[s] = act ? cx : cur[s];
+                    tag[s] = act ? tx : tag[s];
+                    end[s] = act ? ex : end[s];
                 }
             }
 #pragma unroll
             for (int s = 0; s < S; s++) total += (uint32_t)cnt[s];
         }
-        // ring offsets: exclusive prefix of the lane totals (<= 8 S < 32) over the wave
+        // ring offsets: exclusive prefix of the lane totals over the wave (FP64:
+        // <= 8 S < 32; mixed, single-triangle leaves: <= S <= 3)
+        static_assert(!MIX || S * RVF_TLEAF_MAX < 4, "mixed lane totals need more bits");
+        constexpr int TBITS = MIX ? 2 : 5;
         uint32_t off = 0, tot = 0;
 #pragma unroll
-        for (int bit = 0; bit < 5; bit++) {
+        for (int bit = 0; bit < TBITS; bit++) {
             const uint64_t bm = __ballot((total >> bit) & 1u);
             off += (uint32_t)lane_rank(bm) << bit;
             tot += (uint32_t)__popcll(bm) << bit;
@@ -1568,7 +1607,7 @@ __device__ __forceinline__ void split_drain(const PairArgs& a, const QCtx& X, in
             for (int s = 0; s < S; s++) {
                 for (int k = 0; k < cnt[s]; k++)
                     tq[(tail + off + k) & (TQC - 1)] =
-                        ((uint32_t)tag[s] << RVF_TQ_TRI_BITS) | (uint32_t)(first[s] + k);
+                        ((uint32_t)(tag[s] & 0xffff) << RVF_TQ_TRI_BITS) | (uint32_t)(first[s] + k);
                 off += (uint32_t)cnt[s];
             }
             tail += tot;
@@ -1633,6 +1672,97 @@ __device__ __forceinline__ void split_drain(const PairArgs& a, const QCtx& X, in
             }
         }
         if (!wany && Q.c >= Q.nc) break;
+    }
+}
+
+// The subtrees a pair's rays walk (mixed split traversal): the maximal nodes
+// of the traversal tree whose padded box the pair's capsule meets and whose
+// subtree holds at most `lmax` leaves, in preorder, written to out[k * 64]
+// (LDS, one column per lane); the finest such cut of at most RVF_FRONTIER
+// subtrees over lmax = 2, 8, 32, ...  Every triangle a ray of the pair can
+// reach lies in a leaf box the capsule meets, and every such leaf lies in one
+// of these subtrees, so walking each subtree from its root (stopping at its
+// root's miss link) visits every leaf the ray's walk from the root would pass.
+// Returns their number (0: the capsule meets no leaf box, nothing can block).
+// After `limit` box tests the walk stops and the root alone is returned.
+// Trees are preorder with miss links; an interior node's subtree spans the
+// nodes up to its miss link (2 lmax - 1 nodes hold lmax leaves), a node on the
+// tree's rightmost path (miss -1) is always descended.
+__device__ __forceinline__ int capsule_frontier(const double* ai, const double* aj, double R,
+                                                const BvhNode32* __restrict__ nodes, int root, int limit, int* out,
+                                                float Sm) {
+#ifdef RVF_FRONTIER_FP64
+    double inv[3];
+    bool flat[3];
+#pragma unroll
+    for (int c = 0; c < 3; c++) {
+        const double d = aj[c] - ai[c];
+        flat[c] = d == 0.0;
+        inv[c] = flat[c] ? 0.0 : 1.0 / d;
+    }
+    auto meets = [&](const BvhNode32& nd) {
+        double tmin = 0.0, tmax = 1.0;
+#pragma unroll
+        for (int c = 0; c < 3; c++) {
+            const double lo = (double)nd.lo[c] - R, hi = (double)nd.hi[c] + R;
+            if (flat[c]) {
+                if (ai[c] < lo || ai[c] > hi) tmax = -1.0;
+            } else {
+                double t1 = (lo - ai[c]) * inv[c], t2 = (hi - ai[c]) * inv[c];
+                if (t1 > t2) { const double t = t1; t1 = t2; t2 = t; }
+                tmin = fmax(tmin, t1);
+                tmax = fmin(tmax, t2);
+            }
+        }
+        return tmin <= tmax;
+    };
+#else
+    // FP32 slab test of the segment a_i -> a_j (t in [0, 1]) against the box
+    // grown by R, in the traversal's FMA form.  The boxes are padded outward
+    // by 128 u Sm; the anchors' rounding to FP32 (u Sm), the FMA form's
+    // rounding (a few u |o i| in t, i.e. a few u Sm in space) and the radius
+    // (rounded up, plus 4 u Sm) stay inside that padding, so a capsule that
+    // meets the exact box is never reported missing it.
+    const float uu = 5.9604645e-08f;
+    const float Rf = (float)R * (1.0f + 4.0f * uu) + 4.0f * uu * Sm;
+    float o[3], iv[3], nn[3];
+#pragma unroll
+    for (int c = 0; c < 3; c++) {
+        o[c] = (float)ai[c];
+        const float d = (float)aj[c] - o[c];
+        iv[c] = fminf(fmaxf(__builtin_amdgcn_rcpf(d), -1e30f), 1e30f);
+        nn[c] = -o[c] * iv[c];
+    }
+    auto meets = [&](const BvhNode32& nd) {
+        float tmin = 0.0f, tmax = 1.0001f;
+#pragma unroll
+        for (int c = 0; c < 3; c++) {
+            const float t1 = fmaf(nd.lo[c] - Rf, iv[c], nn[c]), t2 = fmaf(nd.hi[c] + Rf, iv[c], nn[c]);
+            tmin = fmaxf(tmin, fminf(t1, t2));
+            tmax = fminf(tmax, fmaxf(t1, t2));
+        }
+        return tmin <= tmax;
+    };
+#endif
+    int tests = 0;
+    for (int lmax = 2;; lmax *= 4) {
+        int k = 0;
+        bool over = false;
+        for (int n = root; n >= 0;) {
+            if (tests++ >= limit) { out[0] = root; return 1; }
+            const BvhNode32 nd = nodes[n];
+            if (!meets(nd)) { n = nd.miss; continue; }
+            const bool leaf = (nd.start_count & 15) != 0;
+            if (leaf || (nd.miss >= 0 && nd.miss - n <= 2 * lmax - 1)) {
+                if (k == RVF_FRONTIER) { over = true; break; }
+                out[k * 64] = n;
+                k++;
+                n = nd.miss;
+            } else {
+                n = n + 1;
+            }
+        }
+        if (!over) return k;
     }
 }
 
@@ -1831,6 +1961,7 @@ __device__ __forceinline__ void tile_occ_queue(const PairArgs& a, int tile, doub
     int* lhint = lroot + 64;
     uint32_t* tq = (uint32_t*)(lhint + 64);
     float* stage = (float*)(tq + (SPLIT ? (MIX ? RVF_TQCAP_MIX + RVF_CQCAP : RVF_TQCAP) : 0));
+    int* lfr = (int*)(stage + 6 * 64);  // (mixed split) RVF_FRONTIER x 64 frontier subtrees
     const double* __restrict__ P64 = a.P64;
     const float* __restrict__ P = a.P32;
     double ls = 0.0, lc = 0.0;
@@ -1842,6 +1973,7 @@ __device__ __forceinline__ void tile_occ_queue(const PairArgs& a, int tile, doub
     X.lroot = lroot;
     X.lhint = lhint;
     X.stage = stage;
+    X.lfr = lfr;
     for (int64_t j = t.jb; j < t.je; j++) {
         if (j < t.jlo) {
             if (lane == 0) a.partT[(int64_t)t.s * a.NS + j] = 0.0;
@@ -1865,8 +1997,13 @@ __device__ __forceinline__ void tile_occ_queue(const PairArgs& a, int tile, doub
             // mixed: down the tree the rays walk, which also gives their start
             // node; FP64 keeps the root box alone, so its box-test counts stay
             // the reference's
-            if (MIX) {
-                const int e = capsule_entry(ai, aj, R, SPLIT ? a.tnodes32 : a.nodes32, root, a.cull_limit);
+            if (MIX && SPLIT) {
+                // the subtrees the pair's rays walk; lroot keeps their number
+                const int nf = capsule_frontier(ai, aj, R, a.tnodes32, root, 2 * a.cull_limit, lfr + lane, a.Sm);
+                cap = nf > 0;
+                if (cap) root = nf;
+            } else if (MIX) {
+                const int e = capsule_entry(ai, aj, R, a.nodes32, root, a.cull_limit);
                 cap = e >= 0;
                 if (cap) root = e;
             } else {
