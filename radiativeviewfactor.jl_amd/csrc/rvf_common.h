@@ -87,7 +87,7 @@ enum RvfPairKind {
 #define RVF_TQCAP 2048   // triangle tasks per wavefront ring (FP64 split variants)
 #define RVF_TQCAP_MIX 256   // the same for the mixed split variants (single-triangle leaves)
 #define RVF_TLEAF_MAX 1  // triangles per leaf of the mixed traversal trees
-#define RVF_FRONTIER 12  // mixed split variants: subtrees a pair's rays start from (per lane, LDS)
+#define RVF_FRONTIER 16  // mixed split variants: subtrees a pair's rays start from (per lane, LDS)
 #define RVF_CQCAP 128    // mixed split variants: candidates for the exact FP64 test
 // A split-variant triangle task packs (tag << RVF_TQ_TRI_BITS) | triangle into
 // 32 bits (tag = combination << 6 | lane < 2^13), so the split variants serve

@@ -1288,6 +1288,7 @@ __device__ __forceinline__ void queue_drain(const PairArgs& a, const QCtx& X, in
 // RVF_TLEAF_MAX in the traversal trees); RVF_CQCAP >= 64 + 63.
 // (RVF_TQ_TRI_BITS in rvf_common.h: a task packs (tag << 19) | triangle)
 static_assert(RVF_QCOMBO * 64 <= (1 << (32 - RVF_TQ_TRI_BITS)), "task tag does not fit");
+static_assert(RVF_FRONTIER <= 16, "frontier index and count take 4 and 5 bits of a slot tag");
 
 template <bool MIX>
 struct NRay;
@@ -1437,7 +1438,7 @@ __device__ __forceinline__ int tclass(const PairArgs& a, const SRay& S, int tri)
 // subtree reads its root, whose miss link ends the subtree's preorder range;
 // a walk that leaves the range (or the tree) moves on to the pair's next
 // frontier subtree, or ends after the last.  The slot's tag holds the
-// frontier index (bits 16-19) and count (bits 20-23).  Branch-free: the next
+// frontier index (bits 16-19) and count (bits 20-24).  Branch-free: the next
 // entry is read from LDS whether or not it is taken.
 __device__ __forceinline__ void frontier_next(const QCtx& X, int miss, int& cur, int& tag, int& end) {
     end = end == -2 ? (miss >= 0 ? miss : 0x7fffffff) : end;
@@ -1447,7 +1448,7 @@ __device__ __forceinline__ void frontier_next(const QCtx& X, int miss, int& cur,
     const bool more = leave && k < (tag >> 20);
     cur = more ? nxt : (leave ? -1 : cur);
     end = more ? -2 : end;
-    tag = more ? (tag & 0xf0ffff) | (k << 16) : tag;
+    tag = more ? (tag & ~0xf0000) | (k << 16) : tag;
 }
 
 template <bool MIX, int S>
@@ -1679,7 +1680,8 @@ __device__ __forceinline__ void split_drain(const PairArgs& a, const QCtx& X, in
 // of the traversal tree whose padded box the pair's capsule meets and whose
 // subtree holds at most `lmax` leaves, in preorder, written to out[k * 64]
 // (LDS, one column per lane); the finest such cut of at most RVF_FRONTIER
-// subtrees over lmax = 2, 8, 32, ...  Every triangle a ray of the pair can
+// subtrees over lmax = 2, 8, 32, ... (RVF_FRONTIER <= 16: the count takes
+// 5 bits of a slot's tag)  Every triangle a ray of the pair can
 // reach lies in a leaf box the capsule meets, and every such leaf lies in one
 // of these subtrees, so walking each subtree from its root (stopping at its
 // root's miss link) visits every leaf the ray's walk from the root would pass.
