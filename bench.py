@@ -59,8 +59,14 @@ def flop_model(N, npts, rays=0, aabb=0, tri=0):
     return pairs * (npts * npts * 28 + npts * 3) + 15 * rays + 24 * aabb + 52 * tri
 
 
-def run_mode(api, ctx, dm, do, nquad, precision, steps, warmup, audit=False, tune=None, self_vf=False):
-    plan = api.Plan(ctx, dm, do, nquad, "quad", precision, want_F=True, audit=audit, timing=True, self_vf=self_vf)
+def run_mode(api, ctx, dm, do, nquad, precision, steps, warmup, audit=False, tune=None, self_vf=False, lat_min=0,
+             method="quad", n_samples=256, seed=0):
+    """Times `steps` solves (HIP events on the context stream around them, the
+    pair kernel's own events per solve); then, untimed, further solves until
+    `lat_min` latency samples exist.  Returns (plan, ms per solve, latencies,
+    pair-kernel ms of the timed solves)."""
+    plan = api.Plan(ctx, dm, do, nquad, method, precision, want_F=True, audit=audit, timing=True, self_vf=self_vf,
+                    n_samples=n_samples, seed=seed)
     if tune is not None and do is not None:
         # autotuner (setup, outside the timed region): time each traversal
         # configuration over a tile sample and keep the fastest
@@ -80,7 +86,40 @@ def run_mode(api, ctx, dm, do, nquad, precision, steps, warmup, audit=False, tun
         pair_ms.append(plan.stats()["pair_ms"])
     ctx.record(1)
     total_ms = ctx.elapsed(0, 1)
+    while len(lat) < lat_min:  # latency samples past the timed region
+        t0 = time.perf_counter()
+        plan.run(); plan.sync()
+        lat.append((time.perf_counter() - t0) * 1e3)
     return plan, total_ms / steps, np.array(lat), np.array(pair_ms)
+
+
+def end_to_end(plan, reps=3):
+    """Solve plus the D2H copy of F (N x N doubles), areas, row sums and the
+    group sums, as the reference's hook returns them (GPUAssembly.jl:83-104)."""
+    ms = []
+    for _ in range(reps):
+        t0 = time.perf_counter()
+        plan.run(); plan.sync()
+        plan.fetch()
+        ms.append((time.perf_counter() - t0) * 1e3)
+    return float(np.median(ms))
+
+
+def graph_steady_state(api, ctx, dm, do, nquad, precision, tune, reps=10):
+    """Steady-state solves replayed from a captured hipGraph (RVF_GRAPH) with
+    the tuned traversal configuration: median solve time."""
+    plan = api.Plan(ctx, dm, do, nquad, "quad", precision, want_F=True, timing=False, graph=True)
+    if tune is not None and do is not None and "variant" in tune:
+        plan.tune(tune["variant"], tune["refill_min"])
+    plan.run(); plan.sync()  # capture
+    ms = []
+    for _ in range(reps):
+        t0 = time.perf_counter()
+        plan.run(); plan.sync()
+        ms.append((time.perf_counter() - t0) * 1e3)
+    name = plan.kernel_name()
+    plan.close()
+    return {"ms_p50": float(np.median(ms)), "replays": reps, "kernel": name}
 
 
 def measured_traffic(workload, kernel):
@@ -141,6 +180,9 @@ def main():
     ap.add_argument("--traffic-probe", action="store_true",
                     help="one copy of known size, then one solve (for rocprofv3 --pmc runs)")
     ap.add_argument("--probe-precision", default="mixed", choices=["mixed", "fp64"])
+    ap.add_argument("--section", default=None,
+                    choices=["occluded_plates", "curved_cylinder", "unoccluded_cube", "self_vf_sphere", "curves"],
+                    help="run only this secondary section and print its JSON (for profiler runs)")
     args = ap.parse_args()
     if args.gpus != 1:
         print("bench.py: measures one GPU; sharding pair tiles over several GPUs is designed but not built",
@@ -150,6 +192,15 @@ def main():
     import __graft_entry__ as G
     G.load_package()
     from rvf_amd import api
+    if args.section:
+        ctx = api.context(0)
+        fn = {"occluded_plates": lambda: secondary_plates(api, ctx, args.steps),
+              "curved_cylinder": lambda: secondary_cylinder(api, ctx),
+              "unoccluded_cube": lambda: secondary_cube(api, ctx, args.steps),
+              "self_vf_sphere": lambda: secondary_self_vf(api, ctx, args.steps),
+              "curves": lambda: secondary_curves(api, ctx, args.steps, 0.0)}[args.section]
+        print(json.dumps({args.section: fn()}))
+        return
     mesh, obs, nquad, parity, desc = build_workload(args.workload)
     N = mesh.n_elems
     ctx = api.context(0)
@@ -167,11 +218,14 @@ def main():
 
     tune_m = {"sample_tiles": args.tune_tiles} if args.autotune else None
     tune_d = {"sample_tiles": args.tune_tiles} if args.autotune else None
-    plan_m, ms_m, lat_m, pair_m = run_mode(api, ctx, dm, do, nquad, "mixed", args.steps, args.warmup, tune=tune_m)
+    plan_m, ms_m, lat_m, pair_m = run_mode(api, ctx, dm, do, nquad, "mixed", args.steps, args.warmup, tune=tune_m,
+                                           lat_min=30)
     st_m = plan_m.stats()
     plan_d, ms_d, lat_d, pair_d = run_mode(api, ctx, dm, do, nquad, "fp64", max(2, args.steps // 2),
-                                           min(args.warmup, 1), tune=tune_d)
+                                           min(args.warmup, 1), tune=tune_d, lat_min=10)
     st_d = plan_d.stats()
+    e2e_m = end_to_end(plan_m)
+    graph_m = graph_steady_state(api, ctx, dm, do, nquad, "mixed", tune_m)
 
     # verification of mixed against FP64 on the same inputs (outside the timed region)
     ver = {}
@@ -184,13 +238,20 @@ def main():
         ver["rowsum_minus_1_max_fp64"] = float(np.abs(rsd - 1.0).max())
         del Fm, Fd
         if do is not None:
+            # the audited plans run the timed (tuned) traversal configurations
             pa = api.Plan(ctx, dm, do, nquad, "quad", "fp64", want_F=False, audit=True, timing=False)
+            if tune_d and "variant" in tune_d:
+                pa.tune(tune_d["variant"], tune_d["refill_min"])
             pa.run(); pa.sync()
             *_, ca, ha = pa.fetch(F=False)
+            ver["audited_kernels"] = {"fp64": pa.kernel_name()}
             pa.close()
             pb = api.Plan(ctx, dm, do, nquad, "quad", "mixed", want_F=False, audit=True, timing=False)
+            if tune_m and "variant" in tune_m:
+                pb.tune(tune_m["variant"], tune_m["refill_min"])
             pb.run(); pb.sync()
             *_, cb, hb = pb.fetch(F=False)
+            ver["audited_kernels"]["mixed"] = pb.kernel_name()
             pb.close()
             iu = np.triu_indices(N, 1)
             ca, cb, ha, hb = ca[iu].astype(np.int64), cb[iu].astype(np.int64), ha[iu], hb[iu]
@@ -222,7 +283,11 @@ def main():
                  "pair_kernel_ms": float(pair_d.mean())},
         "speedup_mixed_vs_fp64": ms_d / ms_m,
         "latency_ms": {"p50": float(np.percentile(lat_m, 50)), "p99": float(np.percentile(lat_m, 99)),
-                       "fp64_p50": float(np.percentile(lat_d, 50))},
+                       "solves": int(len(lat_m)), "fp64_p50": float(np.percentile(lat_d, 50)),
+                       "fp64_p99": float(np.percentile(lat_d, 99)), "fp64_solves": int(len(lat_d))},
+        "end_to_end": {"ms": e2e_m, "pairs_per_s": ordered / (e2e_m * 1e-3),
+                       "includes": "solve + D2H of F (N^2 x 8 B) + areas, row sums, group sums (PCIe-inclusive)"},
+        "graph_replay": graph_m,
         "kernel_ms": {"mixed": {k: st_m[k] for k in ("precompute_ms", "pair_ms", "fill_ms", "reduce_ms")},
                       "fp64": {k: st_d[k] for k in ("precompute_ms", "pair_ms", "fill_ms", "reduce_ms")}},
         "rays": st_m.get("rays"), "fp64_retests": st_m.get("retests"),
@@ -238,6 +303,10 @@ def main():
     if cpu:
         out["gpu_vs_cpu_unique_pairs"] = (N * (N - 1) / 2) / (ms_m * 1e-3) / cpu["value"]
     plan_m.close(); plan_d.close()
+    if args.secondary and args.workload != "plates_occluded":
+        out["occluded_plates"] = secondary_plates(api, ctx, args.steps)
+    if args.secondary:
+        out["curved_cylinder"] = secondary_cylinder(api, ctx)
     if args.secondary and args.workload != "cube":
         out["unoccluded_cube"] = secondary_cube(api, ctx, args.steps)
     if args.secondary:
@@ -273,8 +342,9 @@ def serving_stream(api, ctx, nreq, rate, seed=0x5EED0005):
     wmesh, wobs, wnq = M.random_box_enclosure(np.random.default_rng(seed + 1), 600, max_occluders=1,
                                               max_occluder_tris=500)
     outs = {}
-    for prec in ("mixed", "fp64"):
-        srv = api.Server(ctx)
+    for prec, direct in (("mixed", 0), ("fp64", 0), ("mixed_batched", -1)):
+        srv = api.Server(ctx, direct_tiles=direct)
+        prec_name, prec = prec, prec.split("_")[0]
         # warm-up request (untimed): first use of the server's streams and of
         # the batched and direct kernels, allocator growth
         w = srv.submit(wmesh, wnq, wobs, "geometric", prec)
@@ -304,16 +374,190 @@ def serving_stream(api, ctx, nreq, rate, seed=0x5EED0005):
         span_s = (max(s["t_done_ms"] for s in st) - min(s["t_submit_ms"] for s in st)) * 1e-3
         pairs = float(sum(float(m.n_elems) ** 2 for m, _, _ in reqs))
         c = srv.stats()
+        for t in tickets:
+            srv.release(t)
         srv.close()
-        outs[prec] = got
-        res[prec] = {"pairs_per_s": pairs / span_s, "makespan_s": span_s,
-                     "latency_ms": {"p50": float(np.percentile(lat, 50)), "p99": float(np.percentile(lat, 99)),
-                                    "max": float(lat.max())},
-                     "rounds": c["rounds"] - c0["rounds"], "peak_device_bytes": c["peak_bytes"],
-                     "worker_ms": {k: c[k] - c0[k] for k in ("launch_ms", "admit_ms", "wait_ms", "retire_ms")}}
+        outs[prec_name] = got
+        tiles = c["tiles"] - c0["tiles"]
+        res[prec_name] = {"pairs_per_s": pairs / span_s, "makespan_s": span_s,
+                          "latency_ms": {"p50": float(np.percentile(lat, 50)), "p99": float(np.percentile(lat, 99)),
+                                         "max": float(lat.max())},
+                          "rounds": c["rounds"] - c0["rounds"], "peak_device_bytes": c["peak_bytes"],
+                          "tiles": tiles, "batched_tiles_share": (c["batched_tiles"] - c0["batched_tiles"]) / max(1, tiles),
+                          "direct_tiles_option": "default (512)" if direct == 0 else "never (every tile batched)",
+                          "worker_ms": {k: c[k] - c0[k] for k in ("launch_ms", "admit_ms", "wait_ms", "retire_ms")}}
     res["speedup_mixed_vs_fp64"] = res["fp64"]["makespan_s"] / res["mixed"]["makespan_s"]
     res["closure_err"] = float(max(np.abs(a[2] - b[2]).max() for a, b in zip(outs["mixed"], outs["fp64"])))
     res["max_abs_dF_group"] = float(max(np.abs(a[3] - b[3]).max() for a, b in zip(outs["mixed"], outs["fp64"])))
+    # per request, untimed: standalone solves of both precisions with audits;
+    # element-level max |dF| and flipped decisions of mixed against FP64, and
+    # the server's results (row sums, F_group) against the standalone ones
+    per = []
+    identical = 0
+    for k, (mesh, obs, nq) in enumerate(reqs):
+        dm = api.DeviceMesh(ctx, mesh)
+        do = api.DeviceObstruction(ctx, dm, obs, "geometric") if obs else None
+        alone = {}
+        for prec in ("mixed", "fp64"):
+            pl = api.Plan(ctx, dm, do, nq, "quad", prec, want_F=True, audit=do is not None, timing=False)
+            pl.run(); pl.sync()
+            alone[prec] = pl.fetch()
+            pl.close()
+        Fm, Fd = alone["mixed"][0], alone["fp64"][0]
+        flips = 0
+        if do is not None:
+            iu = np.triu_indices(mesh.n_elems, 1)
+            ca, cb = alone["fp64"][6][iu].astype(np.int64), alone["mixed"][6][iu].astype(np.int64)
+            ha, hb = alone["fp64"][7][iu], alone["mixed"][7][iu]
+            flips = int(np.abs(ca - cb).sum() + 2 * np.sum((ca == cb) & (ha != hb)))
+            do.close()
+        dm.close()
+        same = all(np.array_equal(outs[p][k][2], alone[p.split("_")[0]][2]) and
+                   np.array_equal(outs[p][k][3], alone[p.split("_")[0]][3]) for p in ("mixed", "fp64", "mixed_batched"))
+        identical += same
+        per.append({"elements": int(mesh.n_elems), "max_abs_dF": float(np.abs(Fm - Fd).max()),
+                    "closure_err": float(np.abs(alone["mixed"][2] - alone["fp64"][2]).max()), "flips": flips})
+        del alone, Fm, Fd
+    res["per_request"] = per
+    res["max_abs_dF"] = max(r["max_abs_dF"] for r in per)
+    res["flipped_visibility"] = sum(r["flips"] for r in per)
+    res["server_equals_standalone"] = f"{identical} of {len(reqs)} requests (row sums and F_group bit-identical)"
+    return res
+
+
+def audit_flips(api, ctx, dm, do, nquad, N):
+    """Blocked-ray counts and order-free hashes of the default FP64 and mixed
+    traversals: flipped visibility decisions of mixed against FP64."""
+    aud = {}
+    for prec in ("fp64", "mixed"):
+        pl = api.Plan(ctx, dm, do, nquad, "quad", prec, want_F=False, audit=True, timing=False)
+        pl.run(); pl.sync()
+        *_, c, h = pl.fetch(F=False)
+        aud[prec] = (c, h, pl.kernel_name())
+        pl.close()
+    iu = np.triu_indices(N, 1)
+    ca, ha = aud["fp64"][0][iu].astype(np.int64), aud["fp64"][1][iu]
+    cb, hb = aud["mixed"][0][iu].astype(np.int64), aud["mixed"][1][iu]
+    flips = int(np.abs(ca - cb).sum() + 2 * np.sum((ca == cb) & (ha != hb)))
+    return flips, int(ca.sum()), {"fp64": aud["fp64"][2], "mixed": aud["mixed"][2]}
+
+
+def secondary_plates(api, ctx, steps):
+    """Occluded parallel plates: 2 x 96 x 96 Quad4 (18,432 elements) with the
+    20,480-triangle perturbed sphere between them, nquad 3, geometric soup
+    reading, default traversal configurations.  Mixed and FP64 solve times,
+    mixed against FP64 on the whole matrix (max |dF|, closure, flipped
+    decisions from audits) and the mixed pair kernel's roofline."""
+    mesh, obs, nquad, parity, desc = build_workload("plates_occluded")
+    N = mesh.n_elems
+    dm = api.DeviceMesh(ctx, mesh)
+    do = api.DeviceObstruction(ctx, dm, obs, parity)
+    res = {"workload": desc}
+    Fs, st = {}, {}
+    for prec, k in (("mixed", max(3, steps // 4)), ("fp64", 2)):
+        plan, ms, lat, pair = run_mode(api, ctx, dm, do, nquad, prec, k, 1)
+        st[prec] = plan.stats()
+        F, A, rs, *_ = plan.fetch()
+        Fs[prec] = (F, rs)
+        res[prec] = {"ms_per_step": ms, "pairs_per_s": float(N) * N / (ms * 1e-3), "pair_kernel_ms": float(pair.mean()),
+                     "kernel": plan.kernel_name(), "steps": k}
+        plan.close()
+    res["speedup_mixed_vs_fp64"] = res["fp64"]["ms_per_step"] / res["mixed"]["ms_per_step"]
+    res["max_abs_dF"] = float(np.abs(Fs["mixed"][0] - Fs["fp64"][0]).max())
+    res["closure_err"] = float(np.abs(Fs["mixed"][1] - Fs["fp64"][1]).max())
+    del Fs
+    res["flipped_visibility"], res["blocked_rays_fp64"], res["audited_kernels"] = audit_flips(api, ctx, dm, do, nquad, N)
+    d = st["fp64"]
+    flops = flop_model(N, nquad * nquad, d.get("rays", 0), d.get("aabb_tests", 0), d.get("tri_tests", 0))
+    ach = flops / (res["mixed"]["pair_kernel_ms"] * 1e-3)
+    res["roofline_mixed"] = {"bound": "valu", "achieved": ach / 1e12, "peak": VALU_PEAK_FP32 / 1e12,
+                             "unit": "TFLOP/s", "frac": ach / VALU_PEAK_FP32, "flops_per_launch": flops,
+                             "rays": d.get("rays"), "box_tests": d.get("aabb_tests"), "triangle_tests": d.get("tri_tests")}
+    do.close(); dm.close()
+    return res
+
+
+# Flops per evaluation of a curved element (point, normal, area element) and
+# per Duffy map, for the Duffy and Monte Carlo rooflines.
+EVAL_FLOPS = 90
+MAP_FLOPS = 20
+
+
+def secondary_cylinder(api, ctx, steps=3, mc_steps=2, seeds=4):
+    """Second-order curved geometry: the interior of the closed cylinder,
+    128 x 64 Quad8 side and 2 x 3,000 Tri6 Delaunay caps (14,192 elements).
+    Duffy (Sauter-Schwab) on the Quad8 pairs sharing a corner with nquad 5
+    elsewhere: solve time, the k_duffy and pair kernels' durations and
+    rooflines; stratified Monte Carlo at 256 samples per pair: solve time and
+    roofline, and F_group over `seeds` seeds against the deterministic solve
+    (the reference's stream pairs cell k of one element with cell k of the
+    other, which biases each pair's estimate)."""
+    from rvf_amd import mesh as M
+    mesh = M.cylinder()
+    N = mesh.n_elems
+    dm = api.DeviceMesh(ctx, mesh)
+    res = {"workload": f"cylinder interior r=0.5 h=2: 128x64 Quad8 side + 2x3,000 Tri6 caps ({N:,} elements)"}
+    # Duffy, nquad 5
+    plan, ms, lat, pair = run_mode(api, ctx, dm, None, 5, "fp64", steps, 1, method="duffy")
+    st = plan.stats()
+    Fd, Ad, rsd, Fgd, Agd, tags, *_ = plan.fetch()
+    nquad_pts = {0: 25, 1: 13}
+    fam = mesh.family
+    nq, nt = int((fam == 0).sum()), int((fam == 1).sum())
+    quad_flops = (nq * (nq - 1) // 2 * (25 * 25 * 28 + 25 * 3) + nq * nt * (25 * 13 * 28 + 25 * 3)
+                  + nt * (nt - 1) // 2 * (13 * 13 * 28 + 13 * 3))
+    # singular pairs by shared corners: 1 corner -> vertex rule (8 regions), 2 -> edge rule (5)
+    by = {}
+    for e in np.nonzero(fam == 0)[0]:
+        for c in mesh.nodes[e, :4]:
+            by.setdefault(int(c), set()).add(int(e))
+    shared = {}
+    for es in by.values():
+        for x in es:
+            for y in es:
+                if x < y:
+                    shared[x, y] = shared.get((x, y), 0) + 1
+    nv = sum(1 for v in shared.values() if v == 1)
+    ne = sum(1 for v in shared.values() if v >= 2)
+    duffy_evals = (8 * nv + 5 * ne) * 5 ** 4
+    duffy_flops = duffy_evals * (28 + 2 * EVAL_FLOPS + MAP_FLOPS)
+    res["duffy_nquad5"] = {
+        "ms_per_step": ms, "pair_kernel_ms": float(pair.mean()), "duffy_kernel_ms": st["duffy_ms"],
+        "singular_pairs": st["singular_pairs"], "vertex_pairs": nv, "edge_pairs": ne, "kernel": plan.kernel_name(),
+        "pairs_per_s": float(N) * N / (ms * 1e-3),
+        "roofline_pair": {"achieved": quad_flops / (float(pair.mean()) * 1e-3) / 1e12, "peak": VALU_PEAK_FP64 / 1e12,
+                          "unit": "TFLOP/s", "flops_per_launch": quad_flops},
+        "roofline_duffy": {"achieved": duffy_flops / (st["duffy_ms"] * 1e-3) / 1e12, "peak": VALU_PEAK_FP64 / 1e12,
+                           "unit": "TFLOP/s", "flops_per_launch": duffy_flops,
+                           "model": f"(8 vertex + 5 edge regions) x nquad^4 evaluations x (28 + 2 x {EVAL_FLOPS} + {MAP_FLOPS})"},
+        "F_group": Fgd.tolist(), "group_tags": tags}
+    for k in ("roofline_pair", "roofline_duffy"):
+        r = res["duffy_nquad5"][k]
+        r["frac"] = r["achieved"] / r["peak"]
+    plan.close()
+    del Fd
+    # Monte Carlo, 256 samples per pair
+    mc_flops = N * (N - 1) // 2 * 256 * (2 * EVAL_FLOPS + 28)
+    fg = []
+    for k in range(seeds):
+        plan, ms, lat, pair = run_mode(api, ctx, dm, None, 3, "fp64", mc_steps if k == 0 else 1, 0 if k else 1,
+                                       method="mc", n_samples=256, seed=0x5EED0004 + k)
+        if k == 0:
+            res["monte_carlo_256"] = {"ms_per_step": ms, "pair_kernel_ms": float(pair.mean()), "kernel": plan.kernel_name(),
+                                      "pairs_per_s": float(N) * N / (ms * 1e-3),
+                                      "roofline": {"achieved": mc_flops / (float(pair.mean()) * 1e-3) / 1e12,
+                                                   "peak": VALU_PEAK_FP64 / 1e12, "unit": "TFLOP/s",
+                                                   "flops_per_launch": mc_flops,
+                                                   "model": f"256 x (2 x {EVAL_FLOPS} + 28) per unique pair"}}
+            r = res["monte_carlo_256"]["roofline"]
+            r["frac"] = r["achieved"] / r["peak"]
+        fg.append(plan.fetch(F=False)[3])
+        plan.close()
+    fg = np.array(fg)
+    res["monte_carlo_256"]["F_group_mean"] = fg.mean(0).tolist()
+    res["monte_carlo_256"]["F_group_sigma_of_mean"] = (fg.std(0, ddof=1) / np.sqrt(seeds)).tolist()
+    res["monte_carlo_256"]["F_group_minus_deterministic"] = (fg.mean(0) - Fgd).tolist()
+    dm.close()
     return res
 
 

@@ -181,9 +181,11 @@ typedef struct {
 /* Autotuner (no reference counterpart): runs the plan's prologue, then times
  * its pair kernel under every traversal variant valid for its precision over
  * sample_tiles tiles (0 = 8192) taken at a fixed stride through the tile
- * list, re-times the three fastest over the whole problem (or four times the
- * sample when the problem is larger), tries the refill thresholds 8 and 32
- * for the winner, and keeps the fastest configuration as rvf_plan_tune would
+ * list, times the library default twice and the three fastest others once
+ * over the whole problem (or four times the sample when the problem is
+ * larger), tries the refill thresholds 8 and 32 for the choice, and keeps the
+ * default unless another configuration is faster by more than twice the
+ * default's run-to-run spread (at least 1 %), as rvf_plan_tune would apply it
  * (*variant, *refill_min).  Writes up to cap measurements to entries and
  * their count to *n.  Plans without obstruction trees have a single pair
  * kernel: *n = 0 and nothing changes.  Results of later runs are the same
@@ -281,6 +283,7 @@ typedef struct rvf_server_counters {
     /* worker time: launching (direct ranges and batched rounds), admitting
      * (upload, BVH, plan), waiting for the device, retiring (result copies) */
     double launch_ms, admit_ms, wait_ms, retire_ms;
+    int64_t batched_tiles;  /* of `tiles`, those run by the persistent batched kernel */
 } rvf_server_counters;
 /* request states returned by rvf_server_poll / rvf_server_wait */
 #define RVF_REQ_PENDING 0

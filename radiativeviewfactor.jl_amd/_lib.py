@@ -45,7 +45,8 @@ class RequestStats(C.Structure):
 class ServerCounters(C.Structure):
     _fields_ = [(k, C.c_int64) for k in ("submitted", "completed", "failed", "cancelled", "rounds", "tiles",
                                           "peak_bytes", "used_bytes", "pending", "active")] + \
-               [(k, C.c_double) for k in ("launch_ms", "admit_ms", "wait_ms", "retire_ms")]
+               [(k, C.c_double) for k in ("launch_ms", "admit_ms", "wait_ms", "retire_ms")] + \
+               [("batched_tiles", C.c_int64)]
 
     def as_dict(self):
         return {k: getattr(self, k) for k, _ in self._fields_}

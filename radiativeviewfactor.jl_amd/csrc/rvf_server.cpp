@@ -426,6 +426,7 @@ void worker_loop(rvf_server* s) {
                 std::lock_guard<std::mutex> g(s->mu);
                 s->st.rounds++;
                 s->st.tiles += nitems;
+                s->st.batched_tiles += nitems;
             }
         }
         const double t2 = s->now_ms();

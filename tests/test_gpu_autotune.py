@@ -1,5 +1,6 @@
 """Autotuner (rvf_plan_autotune, api.TuneTable): it measures every traversal
-variant valid for the plan's precision, keeps the fastest, and the solve that
+variant valid for the plan's precision, keeps the library default unless
+another configuration beats it by more than its run-to-run spread, and the solve that
 follows gives the same result as the default configuration (bit for bit in
 FP64 and in mixed precision, whose blocked rays come from exact FP64 tests);
 a persisted table entry is applied without measuring."""
@@ -44,15 +45,31 @@ def test_autotune_measures_all_variants_and_keeps_results(prec, tmp_path):
     measured = {e["variant"] for e in meas}
     assert set(variants) <= measured, (sorted(measured), variants)
     assert all(e["ms"] > 0 for e in meas)
-    # the first len(variants) entries rank the sample; the choice is the
-    # fastest of the re-timed ones that follow, the library default (split 3
-    # mixed, queue 1 FP64, refill 16) counted 1 % faster
-    ranked = sorted(meas[:len(variants)], key=lambda e: e["ms"])
-    again = meas[len(variants):]
-    assert [e["variant"] for e in again[:3]] == [e["variant"] for e in ranked[:3]]
+    # the first len(variants) entries rank the sample; then the library
+    # default (split 3 mixed, queue 1 FP64, refill 16) is timed twice on the
+    # larger set, then the three fastest others, then refill 8 and 32 of the
+    # choice.  Another configuration replaces the current choice only if it is
+    # faster than the best so far by more than twice the default's spread (at
+    # least 1 %).
     vdef = api.VARIANT_SPLIT(3) if prec == "mixed" else api.VARIANT_QUEUE(1)
-    best = min(again, key=lambda e: e["ms"] * (0.99 if (e["variant"], e["refill_min"]) == (vdef, 16) else 1.0))
-    assert (v, r) == (best["variant"], best["refill_min"]) and v in variants
+    ranked = [e["variant"] for e in sorted(meas[:len(variants)], key=lambda e: e["ms"]) if e["variant"] != vdef]
+    again = meas[len(variants):]
+    d1, d2 = again[0], again[1]
+    assert (d1["variant"], d1["refill_min"]) == (vdef, 16) == (d2["variant"], d2["refill_min"])
+    assert [e["variant"] for e in again[2:5]] == ranked[:3]
+    best_ms = min(d1["ms"], d2["ms"])
+    margin = max(0.01, 2 * abs(d1["ms"] - d2["ms"]) / best_ms)
+    choice = (vdef, 16)
+    for e in again[2:5]:
+        if e["ms"] < best_ms * (1 - margin):
+            best_ms, choice = e["ms"], (e["variant"], 16)
+    for e in again[5:]:
+        assert e["variant"] == choice[0] and e["refill_min"] in (8, 32)
+        if e["ms"] < best_ms * (1 - margin):
+            best_ms, choice = e["ms"], (e["variant"], e["refill_min"])
+    if choice[0] == api.VARIANT_LANE:
+        choice = (choice[0], 0)
+    assert (v, r) == choice and v in variants
     for a, b in zip(got, ref):
         if isinstance(a, np.ndarray):
             assert np.array_equal(a, b)
