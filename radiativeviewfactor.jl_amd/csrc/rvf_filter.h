@@ -17,7 +17,9 @@
 //     |V' - V| <= Sm E1 (6 dd + 32u)     |T' - T| <= 20u Sm E1 E2
 //     |L' - L| <= L (2 dd + 4u).
 // The per-ray margins below are at least twice these bounds; the FP64 test's
-// own rounding (2^-53 relative) is far inside them.
+// own rounding (2^-53 relative) is far inside them.  Second-order terms (the
+// products of two first-order relative errors) stay below 1e-3 of the
+// first-order bounds because rays with dd > 1e-3 are always left undecided.
 #pragma once
 
 #ifndef RVF_HD
@@ -69,7 +71,10 @@ RVF_HD inline int rvf_tri_classify(const float* v0, const float* e1, const float
     const float aa = neg ? -a : a;
     const float ua = neg ? -U : U, va = neg ? -V : V, ta = neg ? -T : T;
     const float top = aa + da;
-    const bool undecided = aa <= da + 1e-10f;
+    // rays whose direction error bound dd exceeds 1e-3 (A > 4e-3) are left to
+    // the exact test: below it the second-order terms of the error expansion
+    // stay far inside the margins' slack over the first-order bounds
+    const bool undecided = aa <= da + 1e-10f || R.A > 4.0e-3f;
     const bool miss = (ua < -dU) | (ua > top + dU) | (va < -dV) | (ua + va > top + dU + dV) | (ta < -dT) |
                       (ta > R.L * top * (1.0f + R.A) + dT);
     if (!undecided && miss) return 0;
