@@ -45,6 +45,7 @@ def lib():
         L.orc_pair.restype = C.c_double
         L.orc_pair_duffy.argtypes = [P, P, P, C.c_int64, C.c_int64, C.c_int]
         L.orc_pair_duffy.restype = C.c_double
+        L.orc_set_norm.argtypes = [C.c_int]
         L.orc_mc_expect.argtypes = [P, P, P, C.c_int64, C.c_int64, C.c_int, C.c_int]
         L.orc_mc_expect.restype = C.c_double
         L.orc_bvh_export.argtypes = [P, C.c_int64, C.c_int, C.c_int, P, P, P]
@@ -206,11 +207,23 @@ def xorshift_stream(seed, tid, n):
 
 def compute(mesh, nquad=4, obstruction_groups=(), parity_mode="reference", method="quad", n_samples=256,
             seed=0, nthreads=0, schedule="static", rows=None, want_raw=True, audit=False, count=False,
-            self_vf=False):
+            self_vf=False, norm="plain"):
     """Restatement of ``_compute_cpu`` (src/Assembly.jl:102-225).  Returns a
     dict with ``raw`` (N x N, Fortran-ordered like Julia; rows outside
     ``rows`` stay zero), ``A``, optional audit arrays and test counters.
-    ``self_vf`` integrates the diagonal pairs (i, i) too (Assembly.jl:173,193)."""
+    ``self_vf`` integrates the diagonal pairs (i, i) too (Assembly.jl:173,193).
+    ``norm``: "plain" or "scaled", the two forms StaticArrays' norm takes
+    depending on its version (quirk Q13)."""
+    lib().orc_set_norm(1 if norm == "scaled" else 0)
+    try:
+        return _compute(mesh, nquad, obstruction_groups, parity_mode, method, n_samples, seed, nthreads, schedule,
+                        rows, want_raw, audit, count, self_vf)
+    finally:
+        lib().orc_set_norm(0)
+
+
+def _compute(mesh, nquad, obstruction_groups, parity_mode, method, n_samples, seed, nthreads, schedule, rows,
+             want_raw, audit, count, self_vf):
     N = mesh.n_elems
     soup, sg = mesh.merged_soup()
     obs = np.ascontiguousarray(list(obstruction_groups), np.int32)

@@ -279,9 +279,27 @@ int orc_shape(int fam, double x, double e, double* N, double* dx, double* de) {
     return -1;
 }
 
+/* norm of a 3-vector (quirk Q13: the reference calls StaticArrays' norm,
+ * Geometry.jl:108,171, ViewFactorKernel.jl:108,146, RayCast.jl:28, and
+ * Project.toml pins no version).  Mode 0 (default): the plain form
+ * sqrt((x^2 + y^2) + z^2), which the current StaticArrays returns whenever
+ * the result is finite and nonzero.  Mode 1: the scaled form of the versions
+ * that always scale, m = max |x_i|, m * sqrt(((x/m)^2 + (y/m)^2) + (z/m)^2),
+ * 0 when m = 0.  Set by orc_set_norm before a computation. */
+static int g_norm_mode = 0;
+void orc_set_norm(int mode) { g_norm_mode = mode; }
+static inline double norm3(double x, double y, double z) {
+    if (g_norm_mode == 0) return sqrt((x * x + y * y) + z * z);
+    const double m = fmax(fmax(fabs(x), fabs(y)), fabs(z));
+    if (m == 0.0) return 0.0;
+    const double a = x / m, b = y / m, c = z / m;
+    return m * sqrt((a * a + b * b) + c * c);
+}
+
 /* *_physical_point + *_normal_and_area_element (Geometry.jl:78-110,142-173;
  * ViewFactorKernel.jl:85-110,123-148): x = sum N_a x_a, c = dxdxi x dxdeta,
- * dA = norm(c) (plain sqrt of the left-folded sum of squares, Q13), n = c/dA. */
+ * dA = norm(c) (norm3: plain sqrt of the left-folded sum of squares, or the
+ * scaled form, Q13), n = c/dA. */
 void orc_eval_point(int fam, const double* coords, const int32_t* nodes,
                     double xi, double eta, double* x, double* n, double* dA) {
     double N[8], dx[8], de[8];
@@ -298,7 +316,7 @@ void orc_eval_point(int fam, const double* coords, const int32_t* nodes,
             const double* xa = coords + 3 * (int64_t)(nodes[a] - 1);
             a0 = a0 + dx[a] * xa[0]; a1 = a1 + dx[a] * xa[1]; a2 = a2 + dx[a] * xa[2];
         }
-        double d = sqrt((a0 * a0 + a1 * a1) + a2 * a2);
+        double d = norm3(a0, a1, a2);
         x[0] = p0; x[1] = p1; x[2] = p2;
         n[0] = -(a1 / d); n[1] = a0 / d; n[2] = 0.0;
         *dA = d;
@@ -316,7 +334,7 @@ void orc_eval_point(int fam, const double* coords, const int32_t* nodes,
         b0 = b0 + de[a] * xa[0]; b1 = b1 + de[a] * xa[1]; b2 = b2 + de[a] * xa[2];
     }
     double c0 = a1 * b2 - a2 * b1, c1 = a2 * b0 - a0 * b2, c2 = a0 * b1 - a1 * b0;
-    double d = sqrt((c0 * c0 + c1 * c1) + c2 * c2);
+    double d = norm3(c0, c1, c2);
     x[0] = p0; x[1] = p1; x[2] = p2;
     n[0] = c0 / d; n[1] = c1 / d; n[2] = c2 / d;
     *dA = d;
@@ -585,7 +603,7 @@ static int is_visible(const OrcBVH* b, const double* xi, const double* xj, OrcCo
         return !intersect_seg_bvh(b, xi, xj, cnt);
     }
     double d[3] = {xj[0] - xi[0], xj[1] - xi[1], xj[2] - xi[2]};
-    double L = sqrt((d[0] * d[0] + d[1] * d[1]) + d[2] * d[2]);
+    double L = norm3(d[0], d[1], d[2]);
     if (L < DBL_EPSILON) return 1;
     double dir[3] = {d[0] / L, d[1] / L, d[2] / L};
     if (cnt) {

@@ -190,3 +190,33 @@ def test_vf_kernel(orc):
         ci, cj = ni @ r / d, -(nj @ r) / d
         ref = ci * cj / (math.pi * d * d) if (ci > 0 and cj > 0) else 0.0
         assert abs(orc.vf_kernel(xi, ni, xj, nj) - ref) <= 1e-14 * max(1.0, abs(ref))
+
+
+def test_scaled_norm_mode_q13(orc):
+    # Q13: StaticArrays' norm is the plain sqrt of the summed squares or, in
+    # versions that always scale, m * sqrt(sum (x/m)^2) with m = max |x_i|.
+    # Flat axis-aligned elements give identical results; on curved Quad8 /
+    # Tri6 elements the area elements differ in the last bit or two and F by a
+    # few ulps; no visibility decision moves on an occluded curved mesh.
+    from rvf_amd import mesh as M
+    plates = M.parallel_plates(4)
+    a = orc.compute(plates, nquad=3, norm="plain")
+    b = orc.compute(plates, nquad=3, norm="scaled")
+    assert np.array_equal(a["A"], b["A"]) and np.array_equal(a["raw"], b["raw"])
+    cyl = M.cylinder(n_around=16, n_along=4, cap_tris=40)
+    a = orc.compute(cyl, nquad=4, norm="plain")
+    b = orc.compute(cyl, nquad=4, norm="scaled")
+    assert not np.array_equal(a["A"], b["A"])  # the two forms do differ here
+    assert float(np.abs(a["A"] - b["A"]).max() / a["A"].max()) <= 1e-15
+    Fa, Fb = a["raw"] / a["A"][:, None], b["raw"] / b["A"][:, None]
+    assert float((np.abs(Fa - Fb) / np.maximum(np.abs(Fa), 1e-300)).max()) <= 1e-14
+    s = M.sphere_interior(6, 1.0, 2)
+    V, T = M.icosphere(2, 0.3, (0.0, 0.0, 0.0), seed=3)
+    s.group_tri_soup = {99: V[T]}
+    s.soup_layout = "xyz_vertex"
+    a = orc.compute(s, nquad=3, obstruction_groups=[99], parity_mode="geometric", audit=True, norm="plain")
+    b = orc.compute(s, nquad=3, obstruction_groups=[99], parity_mode="geometric", audit=True, norm="scaled")
+    iu = np.triu_indices(s.n_elems, 1)
+    assert int(a["audit_count"][iu].sum()) > 10_000
+    assert np.array_equal(a["audit_count"][iu], b["audit_count"][iu])
+    assert np.array_equal(a["audit_hash"][iu], b["audit_hash"][iu])
