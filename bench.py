@@ -72,8 +72,8 @@ def run_mode(api, ctx, dm, do, nquad, precision, steps, warmup, audit=False, tun
         # configuration over a tile sample and keep the fastest
         t0 = time.perf_counter()
         v, r, meas = plan.autotune(sample_tiles=tune["sample_tiles"])
-        tune.update(variant=v, refill_min=r, kernel=plan.kernel_name(), seconds=time.perf_counter() - t0,
-                    measurements=meas)
+        tune.update(variant=v, refill_min=r, columns=plan.columns, kernel=plan.kernel_name(),
+                    seconds=time.perf_counter() - t0, measurements=meas)
     for _ in range(warmup):
         plan.run(); plan.sync()
     lat, pair_ms = [], []
@@ -111,6 +111,7 @@ def graph_steady_state(api, ctx, dm, do, nquad, precision, tune, reps=10):
     plan = api.Plan(ctx, dm, do, nquad, "quad", precision, want_F=True, timing=False, graph=True)
     if tune is not None and do is not None and "variant" in tune:
         plan.tune(tune["variant"], tune["refill_min"])
+        plan.tune_tiles(tune["columns"])
     plan.run(); plan.sync()  # capture
     ms = []
     for _ in range(reps):
@@ -242,6 +243,7 @@ def main():
             pa = api.Plan(ctx, dm, do, nquad, "quad", "fp64", want_F=False, audit=True, timing=False)
             if tune_d and "variant" in tune_d:
                 pa.tune(tune_d["variant"], tune_d["refill_min"])
+                pa.tune_tiles(tune_d["columns"])
             pa.run(); pa.sync()
             *_, ca, ha = pa.fetch(F=False)
             ver["audited_kernels"] = {"fp64": pa.kernel_name()}
@@ -249,6 +251,7 @@ def main():
             pb = api.Plan(ctx, dm, do, nquad, "quad", "mixed", want_F=False, audit=True, timing=False)
             if tune_m and "variant" in tune_m:
                 pb.tune(tune_m["variant"], tune_m["refill_min"])
+                pb.tune_tiles(tune_m["columns"])
             pb.run(); pb.sync()
             *_, cb, hb = pb.fetch(F=False)
             ver["audited_kernels"]["mixed"] = pb.kernel_name()

@@ -96,6 +96,8 @@ typedef struct rvf_stats {
     /* occluded quadrature, wave-queue kernels: (wavefront, column element)
      * passes that skipped the trace because every pair of the wave was culled */
     int64_t culled_waves;
+    /* columns (column elements) per pair tile of the plan's layout */
+    int64_t tile_columns;
 } rvf_stats;
 
 /* --- lifecycle (no reference counterpart: the reference is one Julia process) */
@@ -166,8 +168,15 @@ int rvf_plan_device_ptr(const rvf_plan* plan, int which, void** ptr);
 /* Tuning knobs of an existing plan (no counterpart in the reference; used by
  * the autotuner): traversal variant (RVF_VARIANT_*, 0 = default) and the
  * wave-queue refill threshold (lanes with an empty slot before a refill,
- * 1..64; 0 = default 16).  Results do not depend on either. */
+ * 1..64; 0 = default 16).  F, A and every visibility decision do not depend
+ * on either. */
 int rvf_plan_tune(rvf_plan* plan, int32_t variant, int32_t refill_min);
+/* Columns (column elements) per pair tile, 1..16; 0 = the pair kernel's
+ * default (4 for the wave-queue traversal variants, whose launches claim
+ * tiles dynamically; 16 otherwise).  No reference counterpart.  F, A and
+ * every visibility decision do not depend on it; row sums and group sums are
+ * added in another order (differences of a few ulps). */
+int rvf_plan_tune_tiles(rvf_plan* plan, int32_t columns);
 /* Name of the pair kernel the plan launches (as profilers show it), e.g.
  * "k_pair<mixed_occ_split3>" (no reference counterpart; for measurement). */
 int rvf_plan_kernel(const rvf_plan* plan, char* name, int32_t cap);
@@ -177,19 +186,24 @@ typedef struct {
     int32_t refill_min;  /* wave-queue refill threshold (0 where it does not apply) */
     int64_t tiles;       /* tiles timed */
     double ms;           /* pair-kernel time over those tiles */
+    int32_t columns;     /* columns per tile of the layout timed */
+    int32_t reserved;
 } rvf_tune_entry;
 /* Autotuner (no reference counterpart): runs the plan's prologue, then times
- * its pair kernel under every traversal variant valid for its precision over
- * sample_tiles tiles (0 = 8192) taken at a fixed stride through the tile
- * list, times the library default twice and the three fastest others once
- * over the whole problem (or four times the sample when the problem is
- * larger), tries the refill thresholds 8 and 32 for the choice, and keeps the
- * default unless another configuration is faster by more than twice the
- * default's run-to-run spread (at least 1 %), as rvf_plan_tune would apply it
- * (*variant, *refill_min).  Writes up to cap measurements to entries and
- * their count to *n.  Plans without obstruction trees have a single pair
- * kernel: *n = 0 and nothing changes.  Results of later runs are the same
- * whichever configuration is kept. */
+ * its pair kernel under every traversal variant valid for its precision (each
+ * at its default tile width) over a strided share of the tiles (sample_tiles
+ * of the plan's current layout, 0 = 8192; the same share of the pairs for
+ * every width), times the library default twice and the three fastest others
+ * once over the whole problem (or four times the share when the problem is
+ * larger), tries the refill thresholds 8 and 32 for the choice, then its tile
+ * widths 4, 8 and 16 columns, and keeps the default unless another
+ * configuration is faster by more than twice the default's run-to-run spread
+ * (at least 1 %), applied as rvf_plan_tune and rvf_plan_tune_tiles would
+ * (*variant, *refill_min; the width in rvf_stats.tile_columns).  Writes up to
+ * cap measurements to entries and their count to *n.  Plans without
+ * obstruction trees have a single pair kernel: *n = 0 and nothing changes.
+ * F, A and every visibility decision of later runs are the same whichever
+ * configuration is kept. */
 int rvf_plan_autotune(rvf_plan* plan, int64_t sample_tiles, rvf_tune_entry* entries, int32_t cap, int32_t* n,
                       int32_t* variant, int32_t* refill_min);
 void rvf_plan_free(rvf_plan* plan);

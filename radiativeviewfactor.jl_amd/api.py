@@ -230,7 +230,7 @@ class TuneTable:
         self.entries = {}
         if os.path.exists(path):
             with open(path) as f:
-                self.entries = {k: tuple(v) for k, v in json.load(f).items()}
+                self.entries = {k: tuple(v) + (0,) * (3 - len(v)) for k, v in json.load(f).items()}
 
     @staticmethod
     def key(plan: "Plan") -> Optional[str]:
@@ -251,7 +251,8 @@ class TuneTable:
         return self.entries[key]
 
     def __setitem__(self, key, value):
-        self.entries[key] = (int(value[0]), int(value[1]))
+        # (variant, refill_min, columns per tile; 0 = the kernel's default)
+        self.entries[key] = (int(value[0]), int(value[1]), int(value[2]))
 
     def save(self):
         d = os.path.dirname(os.path.abspath(self.path))
@@ -377,9 +378,23 @@ class Plan:
         self.want_F, self.audit = want_F, audit and dobs is not None
 
     def tune(self, variant: int = 0, refill_min: int = 0):
-        """Switch the traversal variant / wave-queue refill threshold (results unchanged)."""
+        """Switch the traversal variant / wave-queue refill threshold (F, A and
+        visibility decisions unchanged)."""
         check(_lib.lib().rvf_plan_tune(self.h, variant, refill_min))
         self.opts.variant = variant
+
+    def tune_tiles(self, columns: int = 0):
+        """Columns per pair tile, 1..16 (0: the kernel's default).  F, A and
+        visibility decisions are unchanged; row and group sums are added in
+        another order."""
+        check(_lib.lib().rvf_plan_tune_tiles(self.h, columns))
+
+    @property
+    def columns(self) -> int:
+        """Columns per pair tile of the plan's current layout."""
+        st = _lib.Stats()
+        check(_lib.lib().rvf_plan_stats(self.h, C.byref(st)))
+        return int(st.tile_columns)
 
     def autotune(self, sample_tiles: int = 0, table: Optional["TuneTable"] = None):
         """Pick the fastest traversal configuration for this plan.
@@ -388,23 +403,25 @@ class Plan:
         entry is applied without measuring; otherwise the native autotuner
         times every variant over a tile sample (rvf_plan_autotune) and the
         winner is stored in the table.  Returns (variant, refill_min,
-        measurements), measurements being [] for a table hit.  Results of
-        later runs do not depend on the choice."""
+        measurements), measurements being [] for a table hit; the tile width
+        chosen is `plan.columns`.  F, A and visibility decisions of later runs
+        do not depend on the choice."""
         key = table.key(self) if table is not None else None
         if key is not None and key in table:
-            v, r = table[key]
+            v, r, c = table[key]
             self.tune(v, r)
+            self.tune_tiles(c)
             return v, r, []
         ents = (_lib.TuneEntry * 32)()
         n, v, r = C.c_int32(), C.c_int32(), C.c_int32()
         check(_lib.lib().rvf_plan_autotune(self.h, sample_tiles, ents, 32, C.byref(n), C.byref(v), C.byref(r)))
-        meas = [{"variant": e.variant, "refill_min": e.refill_min, "tiles": e.tiles, "ms": e.ms}
-                for e in ents[:n.value]]
+        meas = [{"variant": e.variant, "refill_min": e.refill_min, "tiles": e.tiles, "ms": e.ms,
+                 "columns": e.columns} for e in ents[:n.value]]
         if not meas:
             return v.value, r.value, []
         self.opts.variant = v.value
         if table is not None and key is not None:
-            table[key] = (v.value, r.value)
+            table[key] = (v.value, r.value, self.columns)
             table.save()
         return v.value, r.value, meas
 

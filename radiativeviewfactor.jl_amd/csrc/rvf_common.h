@@ -5,6 +5,7 @@
 
 #define RVF_STRIP 64          // i-elements per pair tile (one per lane of a wavefront)
 #define RVF_CHUNK 16          // max j-elements per pair tile (group-pure column chunk)
+#define RVF_CHUNK_QUEUE 4     // default columns per tile of the wave-queue kernels (dynamic tile claims)
 #define RVF_MAXPTS 64         // quadrature points per element (nquad <= 8 quads)
 #define RVF_RULE_LINE (3 * 64 + 39)  // offset of the 1-D rule in the precompute rule table
 #define RVF_RULE_SIZE (RVF_RULE_LINE + 24)
@@ -89,17 +90,20 @@ enum RvfPairKind {
 #define RVF_TLEAF_MAX 1  // triangles per leaf of the mixed traversal trees
 #define RVF_FRONTIER 16  // mixed split variants: subtrees a pair's rays start from (per lane, LDS)
 #define RVF_CQCAP 128    // mixed split variants: candidates for the exact FP64 test
+#define RVF_QSTATS 10    // per-wave statistics of the wave-queue kernels (LDS, u64)
 // A split-variant triangle task packs (tag << RVF_TQ_TRI_BITS) | triangle into
 // 32 bits (tag = combination << 6 | lane < 2^13), so the split variants serve
 // obstruction sets of fewer than 2^19 triangles over all their trees.
 #define RVF_TQ_TRI_BITS 19
 
-// Column chunk: a run of <= RVF_CHUNK consecutive elements of one group.
+// Column chunk: a run of <= RVF_CHUNK consecutive elements of one group.  The
+// tiles of a chunk are consecutive in the tile list (strips 0, 1, ...), from
+// first_tile on, so a row's partial of chunk c sits at tile first_tile + strip.
 struct Chunk {
     int32_t begin;
     int32_t end;     // exclusive
     int32_t grp;     // group rank
-    int32_t pad;
+    int32_t first_tile;
 };
 
 struct PairArgs {
@@ -108,6 +112,7 @@ struct PairArgs {
     int tile0;                // first tile of a direct launch (tile = tile0 + block * tile_step)
     int tile_step;            // tile stride of a direct launch (1 but for sampled timing)
     int pad_ts;
+    int* tile_ctr;            // wave-queue kernels: tile claim counter (zeroed before each launch)
     int NS;                   // slots = N rounded up to RVF_STRIP
     int maxpts;               // per-element point stride in Q arrays
     int64_t N;                // elements
@@ -125,7 +130,7 @@ struct PairArgs {
     const double* anchor;     // [e*4 + c]
     // outputs
     double* F;                // N*N col-major: raw_ij at [i + N*j], i < j (nullable)
-    double* part;             // [nchunks * NS]: sum_{j in chunk, j > i} F_ij
+    double* part;             // [ntiles * 64]: per tile and lane, sum_{j in chunk, j > i} F_ij
     double* partT;            // [nstrips * NS]: sum_{i in strip, i < j} F_ji
     // obstruction
     int G;                    // radiating groups

@@ -217,7 +217,10 @@ struct rvf_mesh {
     std::vector<double> hcoords;
     std::vector<int2> fill_tiles;
     double Sm = 1.0;               // max |coordinate|
-    mutable Layout lay[4];         // bit 0: spatial order; bit 1: tiles for the pairs (i, i) too
+    // layouts by key: bit 0 spatial order, bit 1 tiles for the pairs (i, i)
+    // too, bits 2.. columns per tile
+    mutable std::map<int, Layout> lay;
+    mutable std::mutex lay_mu;     // plans on one mesh may be created from several threads
     bool dim2 = false;             // curve mesh (Line3 / Line2, mesh_dim 1)
     DBuf coords, d_fam, d_nodes8, d_grp, d_fill_tiles;
 };
@@ -231,8 +234,9 @@ static inline uint32_t spread10(uint32_t v) {
     return v;
 }
 
-// Host part of a layout: slot order, group-pure column chunks and the tile list.
-static void layout_host(const rvf_mesh* m, bool spatial, bool self, Layout& L) {
+// Host part of a layout: slot order, group-pure column chunks of at most `cw`
+// columns and the tile list (chunk-major: each chunk's strips consecutively).
+static void layout_host(const rvf_mesh* m, bool spatial, bool self, int cw, Layout& L) {
     int64_t N = m->N, NS = m->NS;
     std::vector<int32_t> order(N);
     for (int64_t e = 0; e < N; e++) order[e] = (int32_t)e;
@@ -264,14 +268,14 @@ static void layout_host(const rvf_mesh* m, bool spatial, bool self, Layout& L) {
         L.orig_slot[order[k]] = (int32_t)k;
         L.slot_grp[k] = m->grp_rank[order[k]];
     }
-    // group-pure column chunks of <= RVF_CHUNK slots
+    // group-pure column chunks of <= cw slots
     L.chunk_of.assign(NS, 0);
     L.chunks.clear();
     for (int64_t e = 0; e < N;) {
         int64_t b = e;
         int g = L.slot_grp[e];
-        while (e < N && e - b < RVF_CHUNK && L.slot_grp[e] == g) e++;
-        Chunk c; c.begin = (int32_t)b; c.end = (int32_t)e; c.grp = g; c.pad = 0;
+        while (e < N && e - b < cw && L.slot_grp[e] == g) e++;
+        Chunk c; c.begin = (int32_t)b; c.end = (int32_t)e; c.grp = g; c.first_tile = 0;
         for (int64_t k = b; k < e; k++) L.chunk_of[k] = (int32_t)L.chunks.size();
         L.chunks.push_back(c);
     }
@@ -279,18 +283,19 @@ static void layout_host(const rvf_mesh* m, bool spatial, bool self, Layout& L) {
     // iff 64 s <= last(c)
     L.tiles.clear();
     for (size_t c = 0; c < L.chunks.size(); c++) {
+        L.chunks[c].first_tile = (int32_t)L.tiles.size();
         int64_t last = L.chunks[c].end - 1 + (self ? 1 : 0);
         for (int64_t s = 0; s * RVF_STRIP < last; s++) L.tiles.push_back(make_int2((int)s, (int)c));
     }
 }
 
-// Layouts are cached per mesh: bit 0 of the index is the spatial order, bit 1
-// the self-pair tiles.
-static int get_layout(const rvf_mesh* m, bool spatial, bool self, const Layout** out) {
-    Layout& L = m->lay[(spatial ? 1 : 0) | (self ? 2 : 0)];
+// Layouts are cached per mesh and (spatial order, self-pair tiles, columns).
+static int get_layout(const rvf_mesh* m, bool spatial, bool self, int cw, const Layout** out) {
+    std::lock_guard<std::mutex> lk(m->lay_mu);
+    Layout& L = m->lay[(spatial ? 1 : 0) | (self ? 2 : 0) | (cw << 2)];
     if (!L.built) {
         const int64_t NS = m->NS;
-        layout_host(m, spatial, self, L);
+        layout_host(m, spatial, self, cw, L);
         int rc;
         if ((rc = upload(L.d_slot_orig, L.slot_orig.data(), sizeof(int32_t) * NS)) ||
             (rc = upload(L.d_slot_grp, L.slot_grp.data(), sizeof(int32_t) * NS)) ||
@@ -913,11 +918,28 @@ struct rvf_plan {
     rvf_stats st{};
     bool ran = false;
     int refill_min = 16;
+    int cw = 0;      // columns per tile of the current layout
+    int cw_req = 0;  // columns asked for by rvf_plan_tune_tiles (0: the kernel's default)
+    bool scratch = false;  // buffers are pieces of the static scratch array
 };
 
 static int occ_kind(int variant, bool mixed, int64_t n_tris);
 static const char* occ_kind_error(int variant, int64_t n_tris);
 static int pair_kind(const rvf_plan* p);
+
+// Columns per tile.  The wave-queue kernels claim tiles dynamically and run
+// best on narrow ones (the tail of a launch is one tile's time; measured on
+// the occluded 13,824-element cube, mixed 250 / 240 / 234 ms at 16 / 8 / 4
+// columns); the other kernels amortise their per-tile set-up over
+// RVF_CHUNK columns.
+static bool queue_kind(int kind) { return kind >= RVF_K_FP64_QUEUE1 && kind <= RVF_K_MIX_SPLIT3; }
+static int default_columns(int kind) { return queue_kind(kind) ? RVF_CHUNK_QUEUE : RVF_CHUNK; }
+static int plan_columns(const rvf_plan* p) { return p->cw_req ? p->cw_req : default_columns(pair_kind(p)); }
+// spatially compact strips for occluded quadrature solves (coherent ray bundles)
+static bool plan_spatial(const rvf_plan* p) {
+    return p->obs && !p->obs->trees.empty() && p->opts.method == RVF_METHOD_QUAD;
+}
+static size_t part_bytes(const Layout* L) { return sizeof(double) * std::max<size_t>(L->tiles.size(), 1) * RVF_STRIP; }
 
 // Device buffers of a plan come either from hipMalloc (rvf_plan_create) or
 // from consecutive 256-byte-aligned pieces of the library's static scratch
@@ -973,17 +995,18 @@ static int plan_init(rvf_ctx* ctx, const rvf_mesh* mesh, const rvf_obs* obs, con
     if (p->maxpts == 0) p->maxpts = 1;
     int64_t N = mesh->N, NS = mesh->NS;
     int G = (int)mesh->tags.size();
-    // spatially compact strips for occluded quadrature solves (coherent ray bundles)
-    bool spatial = obs && !obs->trees.empty() && opts->method == RVF_METHOD_QUAD;
+    const bool spatial = plan_spatial(p);
     const bool self = (opts->flags & RVF_SELF_VF) != 0;
+    p->scratch = scratch;
+    p->cw = plan_columns(p);
     int rc;
     if (scratch) {
-        layout_host(mesh, spatial, self, *own_layout);
+        layout_host(mesh, spatial, self, p->cw, *own_layout);
         p->L = own_layout;
-    } else if ((rc = get_layout(mesh, spatial, self, &p->L))) {
+    } else if ((rc = get_layout(mesh, spatial, self, p->cw, &p->L))) {
         return rc;
     }
-    size_t nch = p->L->chunks.size(), nstrip = NS / RVF_STRIP;
+    size_t nstrip = NS / RVF_STRIP;
     // Duffy: singular Quad8 pairs (singularity_type, DuffyKernel.jl:95-115)
     std::vector<DuffyPair> pairs;
     std::vector<int32_t> rptr, rlist;
@@ -1042,7 +1065,7 @@ static int plan_init(rvf_ctx* ctx, const rvf_mesh* mesh, const rvf_obs* obs, con
         (rc = src.get(p->anchor, sizeof(double) * 4 * NS)) || (rc = src.get(p->area, sizeof(double) * NS)) ||
         (rc = src.get(p->area_orig, sizeof(double) * std::max<int64_t>(N, 1))) ||
         (rc = src.get(p->elem_np, sizeof(int32_t) * NS)) || (rc = src.get(p->bad, sizeof(int))) ||
-        (rc = src.get(p->part, sizeof(double) * std::max<size_t>(nch, 1) * NS)) ||
+        (rc = src.get(p->part, part_bytes(p->L))) ||
         (rc = src.get(p->partT, sizeof(double) * std::max<size_t>(nstrip, 1) * NS)) ||
         (rc = src.get(p->rowsum, sizeof(double) * NS)) || (rc = src.get(p->W, sizeof(double) * std::max(G, 1) * NS)) ||
         (self && (rc = src.get(p->diag, sizeof(double) * NS))) ||
@@ -1164,6 +1187,7 @@ static PairArgs pair_args(const rvf_plan* p) {
     }
     a.audit_count = p->audit_c.as<uint32_t>(); a.audit_hash = p->audit_h.as<uint32_t>();
     a.counters = p->counters.as<unsigned long long>();
+    a.tile_ctr = (int*)(a.counters + 15);  // slots 0..10 hold the statistics
     a.coords = m->coords.as<double>(); a.fam = m->d_fam.as<int8_t>(); a.nodes8 = m->d_nodes8.as<int32_t>();
     a.duffy = p->opts.method == RVF_METHOD_DUFFY ? 1 : 0;
     a.mc_nsamp = p->opts.n_samples;
@@ -1230,7 +1254,8 @@ static size_t pair_lds(const rvf_plan* p) {
                            RVF_FRONTIER * 64 * sizeof(int32_t)
                      : RVF_TQCAP * sizeof(uint32_t))
             : 0;
-    return accq + RVF_QCOMBO * (sizeof(uint64_t) + 2 * sizeof(uint32_t)) + 2 * 64 * sizeof(int32_t) + ring;
+    return accq + RVF_QCOMBO * (sizeof(uint64_t) + 2 * sizeof(uint32_t)) + 2 * 64 * sizeof(int32_t) + ring +
+           RVF_QSTATS * sizeof(uint64_t);
 }
 
 // A solve is three stages on one stream: the prologue (zero fills, point
@@ -1336,6 +1361,7 @@ int rvfi_plan_batch(const rvf_plan* p, PairArgs* a, int* kind, size_t* lds) {
     *lds = pair_lds(p);
     return 0;
 }
+int rvfi_plan_columns(const rvf_plan* p) { return p->cw; }
 size_t rvfi_plan_bytes(const rvf_plan* p) {
     const DBuf* bufs[] = {&p->rule, &p->P64, &p->Q64, &p->P32, &p->Q32, &p->anchor, &p->area, &p->elem_np, &p->bad,
                           &p->F, &p->part, &p->partT, &p->rowsum, &p->W, &p->audit_c, &p->audit_h, &p->counters,
@@ -1362,6 +1388,24 @@ int rvf_plan_kernel(const rvf_plan* p, char* name, int32_t cap) {
     return 0;
 }
 
+// Moves a plan to the layout of plan_columns(p) (cached on the mesh) and
+// grows its row-partial buffer when the new layout has more tiles.  The
+// caller has drained the context stream.
+static int plan_relayout(rvf_plan* p) {
+    const int cw = plan_columns(p);
+    if (cw == p->cw) return 0;
+    if (p->scratch) return fail(RVF_E_STATE, "a plan in the static scratch array keeps its layout");
+    const Layout* L = nullptr;
+    int rc = get_layout(p->mesh, plan_spatial(p), (p->opts.flags & RVF_SELF_VF) != 0, cw, &L);
+    if (rc) return rc;
+    if (p->part.n < part_bytes(L) && (rc = p->part.alloc(part_bytes(L)))) return rc;
+    p->L = L;
+    p->cw = cw;
+    p->st.tiles = (int64_t)L->tiles.size();
+    if (p->gexec) { (void)hipGraphExecDestroy(p->gexec); p->gexec = nullptr; }  // re-captured on the next run
+    return 0;
+}
+
 int rvf_plan_tune(rvf_plan* p, int32_t variant, int32_t refill_min) {
     if (!p) return fail(RVF_E_ARG, "rvf_plan_tune: null plan");
     if (refill_min < 0 || refill_min > 64) return fail(RVF_E_ARG, "rvf_plan_tune: refill_min must be in 0..64");
@@ -1370,10 +1414,26 @@ int rvf_plan_tune(rvf_plan* p, int32_t variant, int32_t refill_min) {
         return fail(RVF_E_ARG, std::string("rvf_plan_tune: ") + occ_kind_error(variant, p->obs->n_tris));
     HIPCHK(hipSetDevice(p->ctx->device));
     HIPCHK(hipStreamSynchronize(p->ctx->stream));
+    const int v0 = p->opts.variant;
     p->opts.variant = variant;
+    const int rc = plan_relayout(p);
+    if (rc) { p->opts.variant = v0; return rc; }
     p->refill_min = refill_min ? refill_min : 16;
     if (p->gexec) { (void)hipGraphExecDestroy(p->gexec); p->gexec = nullptr; }  // re-captured on the next run
     return 0;
+}
+
+int rvf_plan_tune_tiles(rvf_plan* p, int32_t columns) {
+    if (!p) return fail(RVF_E_ARG, "rvf_plan_tune_tiles: null plan");
+    if (columns < 0 || columns > RVF_CHUNK)
+        return fail(RVF_E_ARG, "rvf_plan_tune_tiles: columns must be in 0.." + std::to_string(RVF_CHUNK));
+    HIPCHK(hipSetDevice(p->ctx->device));
+    HIPCHK(hipStreamSynchronize(p->ctx->stream));
+    const int c0 = p->cw_req;
+    p->cw_req = columns;
+    const int rc = plan_relayout(p);
+    if (rc) p->cw_req = c0;
+    return rc;
 }
 
 int rvf_plan_autotune(rvf_plan* p, int64_t sample_tiles, rvf_tune_entry* ent, int32_t cap, int32_t* n,
@@ -1384,23 +1444,34 @@ int rvf_plan_autotune(rvf_plan* p, int64_t sample_tiles, rvf_tune_entry* ent, in
     if (variant) *variant = p->opts.variant;
     if (refill_min) *refill_min = p->refill_min;
     if (!plan_occluded_quad(p) || p->mesh->dim2) return 0;  // curve meshes have one traversal form
+    if (p->scratch) return fail(RVF_E_STATE, "rvf_plan_autotune: a plan in the static scratch array is not tuned");
     rvf_ctx* c = p->ctx;
     hipStream_t st = c->stream;
     HIPCHK(hipSetDevice(c->device));
     int rc = enqueue_prologue(p, st, false);  // the point tables the pair kernel reads
     if (rc) return rc;
-    const int64_t nt = (int64_t)p->L->tiles.size();
-    if (nt == 0) return 0;
-    const int64_t base = std::min<int64_t>(nt, sample_tiles ? sample_tiles : 8192);
+    const int64_t nt0 = (int64_t)p->L->tiles.size();
+    if (nt0 == 0) return 0;
+    // Samples are fractions of the problem: sample_tiles (0 = 8192) tiles of
+    // the plan's layout as it stands, so that configurations with other tile
+    // widths time the same share of the pairs.
+    const double frac = std::min(1.0, (double)(sample_tiles ? sample_tiles : 8192) / (double)nt0);
+    const double big = std::min(1.0, 4.0 * frac);
     const bool mixed = p->opts.precision == RVF_MIXED;
-    const int v0 = p->opts.variant, r0 = p->refill_min;
-    // Times the pair kernel over `want` tiles taken at a fixed stride through
-    // the tile list (all tiles when want >= nt).  Tile costs vary with where
-    // the pair sits relative to the occluders, so a strided sample follows the
-    // whole problem's mix where contiguous ranges do not.
-    auto measure = [&](int v, int refill, int64_t want, float* ms) -> int {
+    const int v0 = p->opts.variant, r0 = p->refill_min, c0 = p->cw_req;
+    // Times the pair kernel over a share f of the tiles taken at a fixed
+    // stride through the tile list (all tiles when f = 1).  Tile costs vary
+    // with where the pair sits relative to the occluders, so a strided sample
+    // follows the whole problem's mix where contiguous ranges do not.
+    int64_t timed = 0;
+    auto measure = [&](int v, int refill, int cols, double f, float* ms) -> int {
         p->opts.variant = v;
         p->refill_min = refill;
+        p->cw_req = cols;
+        int e = plan_relayout(p);
+        if (e) return e;
+        const int64_t nt = (int64_t)p->L->tiles.size();
+        const int64_t want = f >= 1.0 ? nt : std::max<int64_t>(1, (int64_t)(f * (double)nt));
         PairArgs a = pair_args(p);
         const int kind = pair_kind(p);
         const size_t lds = pair_lds(p);
@@ -1409,7 +1480,7 @@ int rvf_plan_autotune(rvf_plan* p, int64_t sample_tiles, rvf_tune_entry* ent, in
         a.tile0 = 0;
         a.tile_step = (int)step;
         a.ntiles = (int)std::min<int64_t>(cnt, 256);  // first launch of a kernel loads its code
-        int e = rvf_launch_pair(&a, kind, lds, st);
+        e = rvf_launch_pair(&a, kind, lds, st);
         HIPCHK(hipEventRecord(p->ev[6], st));
         a.ntiles = (int)cnt;
         if (!e) e = rvf_launch_pair(&a, kind, lds, st);
@@ -1417,6 +1488,7 @@ int rvf_plan_autotune(rvf_plan* p, int64_t sample_tiles, rvf_tune_entry* ent, in
         HIPCHK(hipEventRecord(p->ev[7], st));
         HIPCHK(hipEventSynchronize(p->ev[7]));
         HIPCHK(hipEventElapsedTime(ms, p->ev[6], p->ev[7]));
+        timed = cnt;
         return 0;
     };
     const int64_t ntri = p->obs->n_tris;
@@ -1425,35 +1497,37 @@ int rvf_plan_autotune(rvf_plan* p, int64_t sample_tiles, rvf_tune_entry* ent, in
         if (occ_kind(RVF_VARIANT_QUEUE(S), mixed, ntri) >= 0) cands.push_back(RVF_VARIANT_QUEUE(S));
     for (int S = 1; S <= 3; S++)
         if (occ_kind(RVF_VARIANT_SPLIT(S), mixed, ntri) >= 0) cands.push_back(RVF_VARIANT_SPLIT(S));
-    auto add = [&](int v, int refill, int64_t want, float ms) {
-        if (*n < cap) ent[*n] = rvf_tune_entry{v, v == RVF_VARIANT_LANE ? 0 : refill, std::min(want, nt), (double)ms};
+    auto add = [&](int v, int refill, float ms) {
+        if (*n < cap)
+            ent[*n] = rvf_tune_entry{v, v == RVF_VARIANT_LANE ? 0 : refill, timed, (double)ms, p->cw, 0};
         ++*n;
     };
-    // 1. rank every variant on the sample
+    // 1. rank every variant (at its default tile width) on the sample
     std::vector<std::pair<float, int>> rank;
     for (int v : cands) {
         float ms = 0;
-        if ((rc = measure(v, 16, base, &ms))) break;
-        add(v, 16, base, ms);
+        if ((rc = measure(v, 16, 0, frac, &ms))) break;
+        add(v, 16, ms);
         rank.push_back({ms, v});
     }
     // 2. re-time the three fastest on the whole problem (or a four times
-    //    larger sample), then the refill thresholds of the winner.  The
-    //    library default (variant, refill 16) is timed twice there: another
-    //    configuration replaces it only if faster than both by more than
-    //    twice their spread (and at least 1 %), so run-to-run noise does not
-    //    swap the kernel that runs.
+    //    larger sample), then the refill thresholds of the winner, then its
+    //    tile widths.  The library default (variant, refill 16, default
+    //    width) is timed twice there: another configuration replaces the
+    //    current choice only if faster by more than twice their spread (and
+    //    at least 1 %), so run-to-run noise does not swap the kernel that runs.
     const int vdef = default_variant(mixed, ntri);
-    const int64_t big = std::min<int64_t>(nt, 4 * base);
-    int best_v = vdef, best_r = 16;
+    int best_v = vdef, best_r = 16, best_c = 0;
     float best = 3.4e38f, margin = 0.01f;
     if (!rc) {
         float d1 = 0, d2 = 0;
-        if (!(rc = measure(vdef, 16, big, &d1)) && !(rc = measure(vdef, 16, big, &d2))) {
-            add(vdef, 16, big, d1);
-            add(vdef, 16, big, d2);
-            best = std::min(d1, d2);
-            margin = std::max(0.01f, 2.0f * std::fabs(d1 - d2) / best);
+        if (!(rc = measure(vdef, 16, 0, big, &d1))) {
+            add(vdef, 16, d1);
+            if (!(rc = measure(vdef, 16, 0, big, &d2))) {
+                add(vdef, 16, d2);
+                best = std::min(d1, d2);
+                margin = std::max(0.01f, 2.0f * std::fabs(d1 - d2) / best);
+            }
         }
     }
     auto better = [&](float ms) { return ms < best * (1.0f - margin); };
@@ -1462,8 +1536,8 @@ int rvf_plan_autotune(rvf_plan* p, int64_t sample_tiles, rvf_tune_entry* ent, in
         for (size_t k = 0, done = 0; k < rank.size() && done < 3; k++) {
             if (rank[k].second == vdef) continue;
             float ms = 0;
-            if ((rc = measure(rank[k].second, 16, big, &ms))) break;
-            add(rank[k].second, 16, big, ms);
+            if ((rc = measure(rank[k].second, 16, 0, big, &ms))) break;
+            add(rank[k].second, 16, ms);
             done++;
             if (better(ms)) { best = ms; best_v = rank[k].second; best_r = 16; }
         }
@@ -1472,15 +1546,32 @@ int rvf_plan_autotune(rvf_plan* p, int64_t sample_tiles, rvf_tune_entry* ent, in
         const int bv = best_v;
         for (int refill : {8, 32}) {
             float ms = 0;
-            if ((rc = measure(bv, refill, big, &ms))) break;
-            add(bv, refill, big, ms);
+            if ((rc = measure(bv, refill, 0, big, &ms))) break;
+            add(bv, refill, ms);
             if (better(ms)) { best = ms; best_r = refill; }
         }
     }
-    if (rc) { p->opts.variant = v0; p->refill_min = r0; return rc; }
+    if (!rc) {
+        p->opts.variant = best_v;
+        const int cdef = plan_columns(p);  // the choice's default width (cw_req is 0 here)
+        for (int cols : {RVF_CHUNK_QUEUE, 2 * RVF_CHUNK_QUEUE, RVF_CHUNK}) {
+            if (cols == cdef) continue;
+            float ms = 0;
+            if ((rc = measure(best_v, best_r, cols, big, &ms))) break;
+            add(best_v, best_r, ms);
+            if (better(ms)) { best = ms; best_c = cols; }
+        }
+    }
     if (*n > cap) *n = cap;
+    if (rc) {
+        p->opts.variant = v0; p->refill_min = r0; p->cw_req = c0;
+        (void)plan_relayout(p);
+        return rc;
+    }
     p->opts.variant = best_v;
     p->refill_min = best_r;
+    p->cw_req = best_c;
+    if ((rc = plan_relayout(p))) return rc;
     if (variant) *variant = best_v;
     if (refill_min) *refill_min = best_v == RVF_VARIANT_LANE ? 0 : best_r;
     if (p->gexec) { (void)hipGraphExecDestroy(p->gexec); p->gexec = nullptr; }  // re-captured on the next run
@@ -1618,6 +1709,7 @@ int rvf_plan_fetch(rvf_plan* p, double* F, double* A, double* rowsum, double* F_
 int rvf_plan_stats(const rvf_plan* p, rvf_stats* st) {
     if (!p || !st) return fail(RVF_E_ARG, "null plan");
     *st = p->st;
+    st->tile_columns = p->cw;
     return 0;
 }
 
@@ -1754,7 +1846,7 @@ void rvf_ctx_destroy(rvf_ctx* c) {
 }
 
 const char* rvf_last_error(void) { return g_err.c_str(); }
-int rvf_version(void) { return 1; }
+int rvf_version(void) { return 2; }  // 2: rvf_stats.tile_columns, rvf_tune_entry.columns, rvf_plan_tune_tiles
 
 int rvf_quad_rule(int32_t nquad, int32_t tri, double* pts, double* wts) {
     if (!pts || !wts) return fail(RVF_E_ARG, "null output");

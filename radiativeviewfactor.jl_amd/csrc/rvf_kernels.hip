@@ -365,7 +365,7 @@ __device__ __forceinline__ uint64_t occlusion64(const PairArgs& a, bool need, in
 // Pair tile prologue shared by all pair kernels
 // ---------------------------------------------------------------------------
 struct TileCtx {
-    int s, chunk, lane, jb, je;
+    int tile, s, chunk, lane, jb, je;
     int64_t i;   // slot of this lane's element
     int64_t oi;  // its original index
     int64_t jlo; // columns below jlo have no pair with any lane (j > i; j >= i with self_vf)
@@ -375,6 +375,7 @@ struct TileCtx {
 __device__ __forceinline__ TileCtx tile_ctx(const PairArgs& a, int tile) {
     TileCtx t;
     int2 tl = a.tiles[tile];
+    t.tile = tile;
     t.s = tl.x; t.chunk = tl.y;
     t.lane = threadIdx.x;
     t.i = (int64_t)t.s * RVF_STRIP + t.lane;
@@ -493,7 +494,7 @@ __device__ __forceinline__ void tile_fp64(const PairArgs& a, int tile) {
         }
         pair_out(a, t, j, oj, act, Fij, Ai, ls, lc);
     }
-    a.part[(int64_t)t.chunk * NS + t.i] = ls + lc;
+    a.part[(int64_t)t.tile * RVF_STRIP + t.lane] = ls + lc;
 }
 
 // ---------------------------------------------------------------------------
@@ -583,7 +584,7 @@ __device__ __forceinline__ void tile_fp64_occ(const PairArgs& a, int tile, doubl
         }
         pair_out(a, t, j, oj, act, Fij, Ai, ls, lc);
     }
-    a.part[(int64_t)t.chunk * NS + t.i] = ls + lc;
+    a.part[(int64_t)t.tile * RVF_STRIP + t.lane] = ls + lc;
     if (a.counters) {
         unsigned long long c1 = nrays, c2 = naabb, c3 = ntri;
         for (int o = 32; o > 0; o >>= 1) {
@@ -707,7 +708,7 @@ __device__ __forceinline__ void tile_mixed(const PairArgs& a, int tile) {
         }
         pair_out(a, t, j, oj, act, (Fs + Fc) * (D2 ? 0.5 : 1.0 / PI_D), Ai, ls, lc);
     }
-    a.part[(int64_t)t.chunk * NS + t.i] = ls + lc;
+    a.part[(int64_t)t.tile * RVF_STRIP + t.lane] = ls + lc;
 }
 
 // ---------------------------------------------------------------------------
@@ -911,7 +912,7 @@ __device__ __forceinline__ void tile_mixed_occ(const PairArgs& a, int tile) {
         }
         pair_out(a, t, j, oj, act, (Fs + Fc) * (1.0 / PI_D), Ai, ls, lc);
     }
-    a.part[(int64_t)t.chunk * NS + t.i] = ls + lc;
+    a.part[(int64_t)t.tile * RVF_STRIP + t.lane] = ls + lc;
     if (a.counters) {
         unsigned long long r0 = nret, r1 = nrays;
         for (int o = 32; o > 0; o >>= 1) { r0 += __shfl_xor(r0, o, 64); r1 += __shfl_xor(r1, o, 64); }
@@ -1035,13 +1036,33 @@ struct QCtx {
     float fjx, fjy, fjz; // column element's anchor as FP32
 };
 
-// Per-lane traversal counters of the wave-queue kernels.
+// Per-wave statistics of the wave-queue kernels, kept in LDS (u64 each) so
+// that no counter stays in a register across a trace: rays cast, c0, c1,
+// loop iterations, slot steps, task batches, exact batches, culled rays,
+// skipped columns.
+enum { QS_RAYS, QS_C0, QS_C1, QS_ITERS, QS_STEPS, QS_TBATCH, QS_EBATCH, QS_CULLED, QS_SKIPPED };
+static_assert(QS_SKIPPED < RVF_QSTATS, "per-wave statistics block too small");
+
+// Traversal counters of one drain of the wave-queue kernels: per lane, or
+// wave-uniform where the whole wave counts one event.
 struct QCount {
     uint32_t c0 = 0, c1 = 0;     // FP64: box tests, triangle tests; mixed: exact re-tests, certain hits
-    uint32_t iters = 0;          // traversal loop iterations (counted by lane 0)
+    uint32_t iters = 0;          // traversal loop iterations (uniform)
     uint32_t steps = 0;          // slot steps that advanced a ray
-    uint32_t tbatch = 0;         // triangle-task batches (lane 0)
-    uint32_t ebatch = 0;         // exact-test batches (lane 0)
+    uint32_t tbatch = 0;         // triangle-task batches (uniform)
+    uint32_t ebatch = 0;         // exact-test batches (uniform)
+    // adds the drain's wave totals to the per-wave block (every lane active)
+    __device__ __forceinline__ void flush(uint64_t* w, int lane) const {
+        uint32_t v[3] = {c0, c1, steps};
+#pragma unroll
+        for (int o = 32; o > 0; o >>= 1)
+#pragma unroll
+            for (int e = 0; e < 3; e++) v[e] += __shfl_xor(v[e], o, 64);
+        if (lane == 0) {
+            w[QS_C0] += v[0]; w[QS_C1] += v[1]; w[QS_STEPS] += v[2];
+            w[QS_ITERS] += iters; w[QS_TBATCH] += tbatch; w[QS_EBATCH] += ebatch;
+        }
+    }
 };
 
 template <bool MIX>
@@ -1237,7 +1258,7 @@ __device__ __forceinline__ void queue_drain(const PairArgs& a, const QCtx& X, in
 #pragma unroll
         for (int s = 0; s < S; s++) any |= cur[s] >= 0;
         if (__ballot(any) == 0) break;
-        K.iters += lane == 0;
+        K.iters++;
         bool hit[S];
         {
             QItem<MIX> it[S];
@@ -1541,7 +1562,7 @@ __device__ __forceinline__ void split_drain(const PairArgs& a, const QCtx& X, in
             }
         }
         // node steps of every slot; passed leaves queue their triangles
-        K.iters += lane == 0;
+        K.iters++;
         int first[S], cnt[S];
         uint32_t total = 0;
         {
@@ -1645,7 +1666,7 @@ __device__ __forceinline__ void split_drain(const PairArgs& a, const QCtx& X, in
                     if (ttest(a, X, c, b, tri, K.c1)) atomicOr(&blk[c * 2 + (b >> 5)], 1u << (b & 31));
                 }
             }
-            K.tbatch += lane == 0 && n > 0;
+            K.tbatch += n > 0;
             head += n;
             bdirty = true;
             if (MIX) {
@@ -1666,7 +1687,7 @@ __device__ __forceinline__ void split_drain(const PairArgs& a, const QCtx& X, in
                         if (qexact_hit(a, X, c, b, rt, a.items32[a.n_nodes + rt].link_y))
                             atomicOr(&blk[c * 2 + (b >> 5)], 1u << (b & 31));
                     }
-                    K.ebatch += lane == 0;
+                    K.ebatch++;
                     chead += m;
                     __builtin_amdgcn_wave_barrier();
                 }
@@ -1866,7 +1887,7 @@ __device__ __forceinline__ void tile_mixed_occ2d(const PairArgs& a, int tile) {
         }
         pair_out(a, t, j, oj, act, (Fs + Fc) * 0.5, Ai, ls, lc);
     }
-    a.part[(int64_t)t.chunk * NS + t.i] = ls + lc;
+    a.part[(int64_t)t.tile * RVF_STRIP + t.lane] = ls + lc;
     if (a.counters) {
         unsigned long long c1 = nrays, c2 = naabb, c3 = ntri;
         for (int o = 32; o > 0; o >>= 1) {
@@ -1945,7 +1966,6 @@ __device__ __forceinline__ int capsule_entry(const double* ai, const double* aj,
 template <bool MIX, int S, bool SPLIT>
 __device__ __forceinline__ void tile_occ_queue(const PairArgs& a, int tile, double* lds) {
     const TileCtx t = tile_ctx(a, tile);
-    const Corners cwi = corners_of(a, t.oi, t.vi && a.duffy);
     const int64_t NS = a.NS;
     const int lane = t.lane;
     const int npi = t.vi ? a.elem_np[t.i] : 0;
@@ -1963,13 +1983,16 @@ __device__ __forceinline__ void tile_occ_queue(const PairArgs& a, int tile, doub
     int* lhint = lroot + 64;
     uint32_t* tq = (uint32_t*)(lhint + 64);
     float* stage = (float*)(tq + (SPLIT ? (MIX ? RVF_TQCAP_MIX + RVF_CQCAP : RVF_TQCAP) : 0));
-    int* lfr = (int*)(stage + 6 * 64);  // (mixed split) RVF_FRONTIER x 64 frontier subtrees
+    int* lfr = (int*)(stage + (MIX && SPLIT ? 6 * 64 : 0));  // (mixed split) RVF_FRONTIER x 64 frontier subtrees
+    // per-wave statistics (QS_*): rays cast, and of those the rays of pairs
+    // culled before any traversal, columns whose trace the wave skipped (all
+    // its pairs culled), the drains' counters
+    uint64_t* wst = (uint64_t*)(lfr + (MIX && SPLIT ? RVF_FRONTIER * 64 : 0));
+    if (lane < RVF_QSTATS) wst[lane] = 0;
+    __syncthreads();
     const double* __restrict__ P64 = a.P64;
     const float* __restrict__ P = a.P32;
     double ls = 0.0, lc = 0.0;
-    uint32_t nrays = 0, culled = 0;  // rays cast; of those, rays of pairs culled before any traversal
-    uint32_t skipped = 0;  // columns whose trace the wave skipped (all its pairs culled; lane 0)
-    QCount K;
     QCtx X;
     X.sbase = (int64_t)t.s * RVF_STRIP;
     X.lroot = lroot;
@@ -1983,7 +2006,9 @@ __device__ __forceinline__ void tile_occ_queue(const PairArgs& a, int tile, doub
         }
         bool act = t.vi && t.i < j + a.self_vf;
         const int64_t oj = a.slot_orig[j];
-        if (a.duffy && shares_corner(cwi, corners_of(a, oj, true))) act = false;
+        // (the row element's corners are read again here rather than held
+        // in registers across the column loop)
+        if (a.duffy && shares_corner(corners_of(a, t.oi, t.vi), corners_of(a, oj, true))) act = false;
         const bool rev = t.oi > oj;
         const int npj = a.elem_np[j];
         const int gj = a.elem_grp[j];
@@ -2038,14 +2063,16 @@ __device__ __forceinline__ void tile_occ_queue(const PairArgs& a, int tile, doub
         // masks and the trace are skipped; the sums below count the rays
         // instead, from the same kernel values
         const bool trace = __ballot(cap) != 0;
-        skipped += !trace && lane == 0 && __ballot(act && root >= 0) != 0;
+        const bool anyp = __ballot(act && root >= 0) != 0;
+        if (lane == 0 && !trace && anyp) wst[QS_SKIPPED]++;
         for (int p0 = 0; p0 < npmax; p0 += X.pch) {
             const int p1 = min(npmax, p0 + X.pch);
             const int nc = (p1 - p0) * npj;
             X.p0 = p0;
             if (!trace)
                 for (int w = lane; w < 2 * nc; w += 64) blk[w] = 0u;
-            // Phase A: need masks
+            // Phase A: need masks (ray counts as wave totals)
+            uint32_t wrays = 0, wculled = 0;
             for (int p = p0; p < p1 && trace; p++) {
                 const bool pv = act && p < npi && root >= 0;
                 if (MIX) {
@@ -2060,8 +2087,8 @@ __device__ __forceinline__ void tile_occ_queue(const PairArgs& a, int tile, doub
                                            qj32[q * 8 + 3], qj32[q * 8 + 4], qj32[q * 8 + 5], rx, ry, rz, r2);
                         const bool nd = pv && k != 0.0f;
                         const uint64_t m = __ballot(nd && cap);
-                        nrays += nd;
-                        culled += nd && !cap;
+                        wrays += __popcll(__ballot(nd));
+                        wculled += __popcll(__ballot(nd && !cap));
                         const int c = (p - p0) * npj + q;
                         if (lane == 0) { need[c] = m; blk[2 * c] = 0u; blk[2 * c + 1] = 0u; }
                     }
@@ -2077,21 +2104,25 @@ __device__ __forceinline__ void tile_occ_queue(const PairArgs& a, int tile, doub
                                                   qj64[q * 8 + 2], qj64[q * 8 + 3], qj64[q * 8 + 4], qj64[q * 8 + 5], Rr);
                         const bool nd = pv && K != 0.0;
                         const uint64_t m = __ballot(nd && cap);
-                        nrays += nd;
-                        culled += nd && !cap;
+                        wrays += __popcll(__ballot(nd));
+                        wculled += __popcll(__ballot(nd && !cap));
                         const int c = (p - p0) * npj + q;
                         if (lane == 0) { need[c] = m; blk[2 * c] = 0u; blk[2 * c + 1] = 0u; }
                     }
                 }
             }
+            if (lane == 0) { wst[QS_RAYS] += wrays; wst[QS_CULLED] += wculled; }
             __syncthreads();
             // Phase B: trace
             if (trace) {
+                QCount K;
                 if constexpr (SPLIT) split_drain<MIX, S>(a, X, nc, need, blk, lane, tq, K);
                 else queue_drain<MIX, S>(a, X, nc, need, blk, lane, K);
+                K.flush(wst, lane);
             }
             __syncthreads();
             // Phase C: sums in each lane's own order
+            uint32_t wcl = 0;  // rays of a skipped trace (all culled)
             for (int p = p0; p < p1; p++) {
                 if (MIX) {
                     const float dxi = P[(int64_t)(p * 8 + 0) * NS + t.i], dyi = P[(int64_t)(p * 8 + 1) * NS + t.i];
@@ -2107,8 +2138,7 @@ __device__ __forceinline__ void tile_occ_queue(const PairArgs& a, int tile, doub
                         const int c = (p - p0) * npj + q;
                         if (!trace) {  // as the need masks count them: every such ray is culled
                             const bool nd = act && p < npi && root >= 0 && k != 0.0f;
-                            nrays += nd;
-                            culled += nd;
+                            wcl += __popcll(__ballot(nd));
                         }
                         if ((blk[2 * c + (lane >> 5)] >> (lane & 31)) & 1u) {
                             k = 0.0f;
@@ -2136,8 +2166,7 @@ __device__ __forceinline__ void tile_occ_queue(const PairArgs& a, int tile, doub
                         const int c = (p - p0) * npj + q;
                         if (!trace) {  // as the need masks count them: every such ray is culled
                             const bool nd = act && p < npi && root >= 0 && K != 0.0;
-                            nrays += nd;
-                            culled += nd;
+                            wcl += __popcll(__ballot(nd));
                         }
                         if ((blk[2 * c + (lane >> 5)] >> (lane & 31)) & 1u) {
                             K = 0.0;
@@ -2150,6 +2179,7 @@ __device__ __forceinline__ void tile_occ_queue(const PairArgs& a, int tile, doub
                     if (p < npi) Fij += (wi * inner) * dAi;
                 }
             }
+            if (lane == 0 && !trace) { wst[QS_RAYS] += wcl; wst[QS_CULLED] += wcl; }
             __syncthreads();
         }
         double raw;
@@ -2168,22 +2198,18 @@ __device__ __forceinline__ void tile_occ_queue(const PairArgs& a, int tile, doub
         }
         pair_out(a, t, j, oj, act, raw, Ai, ls, lc);
     }
-    a.part[(int64_t)t.chunk * NS + t.i] = ls + lc;
-    if (a.counters) {
-        if (!MIX) K.c0 += culled;  // FP64 box tests: a culled ray's root test
-        unsigned long long v[9] = {nrays, K.c0, K.c1, K.iters, K.steps, K.tbatch, K.ebatch, culled, skipped};
+    a.part[(int64_t)t.tile * RVF_STRIP + t.lane] = ls + lc;
+    if (a.counters && lane == 0) {
+        unsigned long long v[9];
 #pragma unroll
-        for (int o = 32; o > 0; o >>= 1)
-#pragma unroll
-            for (int e = 0; e < 9; e++) v[e] += __shfl_xor(v[e], o, 64);
-        if (lane == 0) {
-            atomicAdd(&a.counters[1], v[0]);
-            if (MIX) { atomicAdd(&a.counters[0], v[1]); atomicAdd(&a.counters[8], v[2]); }
-            else { atomicAdd(&a.counters[2], v[1]); atomicAdd(&a.counters[3], v[2]); }
-            for (int e = 3; e < 7; e++) atomicAdd(&a.counters[e + 1], v[e]);
-            atomicAdd(&a.counters[9], v[7]);
-            atomicAdd(&a.counters[10], v[8]);
-        }
+        for (int e = 0; e < 9; e++) v[e] = wst[e];
+        if (!MIX) v[QS_C0] += v[QS_CULLED];  // FP64 box tests: a culled ray's root test
+        atomicAdd(&a.counters[1], v[QS_RAYS]);
+        if (MIX) { atomicAdd(&a.counters[0], v[QS_C0]); atomicAdd(&a.counters[8], v[QS_C1]); }
+        else { atomicAdd(&a.counters[2], v[QS_C0]); atomicAdd(&a.counters[3], v[QS_C1]); }
+        for (int e = QS_ITERS; e <= QS_EBATCH; e++) atomicAdd(&a.counters[e + 1], v[e]);
+        atomicAdd(&a.counters[9], v[QS_CULLED]);
+        atomicAdd(&a.counters[10], v[QS_SKIPPED]);
     }
 }
 
@@ -2321,7 +2347,7 @@ __device__ __forceinline__ void tile_mc(const PairArgs& a, int tile) {
         double raw = ((Ks * ri) * rj) / a.mc_nsamp;
         pair_out(a, t, j, oj, act, raw, Ai, ls, lc);
     }
-    a.part[(int64_t)t.chunk * NS + t.i] = ls + lc;
+    a.part[(int64_t)t.tile * RVF_STRIP + t.lane] = ls + lc;
     if (a.counters) {
         unsigned long long c1 = nrays;
         for (int o = 32; o > 0; o >>= 1) c1 += __shfl_xor(c1, o, 64);
@@ -2559,7 +2585,7 @@ __global__ __launch_bounds__(256) void k_reduce_seg(const double* __restrict__ p
             if (curg >= 0) Wr[(int64_t)curg * NS] += gs + gc;
             curg = g; gs = 0.0; gc = 0.0;
         }
-        double v = part[(int64_t)c * NS + r];
+        double v = part[((int64_t)chunks[c].first_tile + sr) * RVF_STRIP + (r - sr * RVF_STRIP)];
         neumaier(gs, gc, v);
         neumaier(ts, tc, v);
     }
@@ -2622,6 +2648,10 @@ __device__ __forceinline__ void run_tile(const PairArgs& a, int tile, double* ld
     else tile_mc(a, tile);
 }
 
+__global__ __launch_bounds__(64) void k_zero_ctr(int* __restrict__ c) {
+    if (threadIdx.x == 0) c[0] = 0;
+}
+
 // kernel tags: k_pair<tag> names the variant in profiles
 #define RVF_KIND_TAG(e, t) struct t { static constexpr int kind = RVF_K_##e; };
 RVF_PAIR_KINDS(RVF_KIND_TAG)
@@ -2633,10 +2663,31 @@ RVF_PAIR_KINDS(RVF_KIND_TAG)
 // are in the per-column setup, outside the traversal loop).
 constexpr int k_pair_waves(int kind) { return (kind == RVF_K_MIX_SPLIT3 || kind == RVF_K_MIX_SPLIT2) ? 4 : 1; }
 
+// Tile costs of the wave-queue kernels vary by orders of magnitude with where
+// the pairs sit relative to the occluders.  Their launches are persistent: as
+// many waves as fit the GPU claim tiles from a counter until none is left, so
+// no XCD idles while another still holds a run of expensive tiles (the
+// hardware hands a grid's workgroups to the XCDs in a fixed round robin).
+// Measured on the occluded 13,824-element cube: 265 -> 250 ms at 16 columns
+// per tile, 234 ms at 4 (FP64 730 -> 671 ms).
+constexpr bool k_pair_dynamic(int kind) { return kind >= RVF_K_FP64_QUEUE1 && kind <= RVF_K_MIX_SPLIT3; }
+
 template <class TAG>
 __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(k_pair_waves(TAG::kind)))) void k_pair(PairArgs a) {
     extern __shared__ double lds[];
-    run_tile<TAG::kind>(a, a.tile0 + (int)blockIdx.x * a.tile_step, lds);
+    if constexpr (!k_pair_dynamic(TAG::kind)) {
+        run_tile<TAG::kind>(a, a.tile0 + (int)blockIdx.x * a.tile_step, lds);
+    } else {
+        // every lane takes part in the claim (lane 0 adds 1, the others 0) and
+        // the index is read back with readfirstlane, so the loop and its exit
+        // are wave-uniform; each claim advances the counter, so every wave
+        // reaches w >= ntiles
+        int w = __builtin_amdgcn_readfirstlane(atomicAdd(a.tile_ctr, threadIdx.x == 0 ? 1 : 0));
+        while (w < a.ntiles) {
+            run_tile<TAG::kind>(a, a.tile0 + w * a.tile_step, lds);
+            w = __builtin_amdgcn_readfirstlane(atomicAdd(a.tile_ctr, threadIdx.x == 0 ? 1 : 0));
+        }
+    }
 }
 
 template <class TAG>
@@ -2682,7 +2733,20 @@ __global__ __launch_bounds__(256) void k_copy(const uint4* __restrict__ src, uin
 // ---------------------------------------------------------------------------
 template <class TAG>
 static void launch_direct(const PairArgs* a, size_t lds, hipStream_t st) {
-    hipLaunchKernelGGL(k_pair<TAG>, dim3((unsigned)a->ntiles), dim3(64), lds, st, *a);
+    if constexpr (k_pair_dynamic(TAG::kind)) {
+        // resident waves of the whole GPU (at least one), the claim counter
+        // zeroed by a kernel on the same stream (graph capture records no
+        // memset)
+        int dev = 0, ncu = 0, nb = 0;
+        (void)hipGetDevice(&dev);
+        (void)hipDeviceGetAttribute(&ncu, hipDeviceAttributeMultiprocessorCount, dev);
+        (void)hipOccupancyMaxActiveBlocksPerMultiprocessor(&nb, k_pair<TAG>, 64, lds);
+        const int grid = std::max(1, std::min(a->ntiles, std::max(nb, 1) * std::max(ncu, 1)));
+        hipLaunchKernelGGL(k_zero_ctr, dim3(1), dim3(64), 0, st, a->tile_ctr);
+        hipLaunchKernelGGL(k_pair<TAG>, dim3((unsigned)grid), dim3(64), lds, st, *a);
+    } else {
+        hipLaunchKernelGGL(k_pair<TAG>, dim3((unsigned)a->ntiles), dim3(64), lds, st, *a);
+    }
 }
 template <class TAG>
 static void launch_batched(const PairArgs* reqs, const int2* work, const int* ctl, int grid, size_t lds,

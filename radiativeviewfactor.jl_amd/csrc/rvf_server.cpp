@@ -34,6 +34,7 @@
 #include <deque>
 #include <map>
 #include <mutex>
+#include <set>
 #include <string>
 #include <thread>
 #include <vector>
@@ -46,6 +47,7 @@ int rvfi_plan_prologue(rvf_plan* p, hipStream_t st);
 int rvfi_plan_epilogue(rvf_plan* p, hipStream_t st);
 int rvfi_plan_collect(rvf_plan* p);
 int rvfi_plan_batch(const rvf_plan* p, PairArgs* a, int* kind, size_t* lds);
+int rvfi_plan_columns(const rvf_plan* p);
 size_t rvfi_plan_bytes(const rvf_plan* p);
 hipStream_t rvfi_ctx_stream(const rvf_ctx* c);
 int rvfi_ctx_device(const rvf_ctx* c);
@@ -83,6 +85,7 @@ struct Req {
     rvf_plan* plan = nullptr;
     PairArgs args{};
     int kind = 0, slot = -1, ntiles = 0, next_tile = 0;
+    int cols = RVF_CHUNK;              // columns per tile of the request's layout
     int stream = -1;                   // slot stream of a direct request
     bool epilogue = false;             // epilogue enqueued: retire once done_ev completes
     hipEvent_t ready_ev = nullptr;     // prologue done (context stream)
@@ -147,6 +150,35 @@ struct rvf_server {
 
 namespace {
 
+// Obstruction triangles the library will hold: one tree (in both traversal
+// forms) per distinct set of obstruction groups that a pair of element groups
+// sees, namely the obstruction groups other than the pair's own (the same
+// rule as rvf_obs_build), so a triangle is held once per set it belongs to.
+int64_t held_triangles(const Req& r) {
+    if (!r.has_obs || r.T == 0 || r.obs_groups.empty()) return 0;
+    std::vector<int32_t> tags(r.grp.begin(), r.grp.end());
+    std::sort(tags.begin(), tags.end());
+    tags.erase(std::unique(tags.begin(), tags.end()), tags.end());
+    std::map<int32_t, int64_t> per_group;
+    for (int32_t g : r.tri_group) per_group[g]++;
+    std::set<std::vector<int32_t>> seen;
+    std::vector<int32_t> act;
+    int64_t held = 0;
+    for (int32_t a : tags)
+        for (int32_t b : tags) {
+            act.clear();
+            for (int32_t g : r.obs_groups)
+                if (g != a && g != b) act.push_back(g);
+            std::sort(act.begin(), act.end());
+            if (act.empty() || !seen.insert(act).second) continue;
+            for (int32_t g : act) {
+                auto it = per_group.find(g);
+                if (it != per_group.end()) held += it->second;
+            }
+        }
+    return held;
+}
+
 // Device bytes a request will hold: mesh, obstruction trees, point tables,
 // row/column partials, F and audit arrays (upper estimate).
 size_t estimate_bytes(const Req& r) {
@@ -154,13 +186,19 @@ size_t estimate_bytes(const Req& r) {
     const int64_t tri_pts = nq <= 1 ? 1 : nq == 2 ? 3 : nq == 3 ? 7 : 13;
     const int64_t maxpts = std::max<int64_t>(nq * nq, tri_pts);
     const int64_t NS = cdiv(std::max<int64_t>(r.N, 1), RVF_STRIP) * RVF_STRIP;
-    const int64_t nchunks = cdiv(r.N, RVF_CHUNK) + 64;
+    // row partials: one 64-lane record per tile; tiles of chunks of the
+    // narrowest default width (w columns) number about N^2 / (128 w) plus one
+    // per chunk, and each of up to 64 extra chunks at group boundaries adds
+    // at most one tile per strip
+    const int64_t w = RVF_CHUNK_QUEUE;
+    const int64_t nchunks = cdiv(r.N, w) + 64;
+    const int64_t tiles = cdiv(r.N * r.N, 128 * w) + nchunks + 64 * (NS / RVF_STRIP);
     size_t b = 0;
     b += (size_t)r.Nn * 24 + (size_t)r.N * 64;                          // mesh
-    b += (size_t)r.T * 512;                                              // trees, triangles, items
+    b += (size_t)held_triangles(r) * 512;                                // trees, triangles, items
     b += (size_t)NS * maxpts * 8 * (8 + 8 + 4 + 4);                      // P64 Q64 P32 Q32
     b += (size_t)NS * 72;                                                // anchors, areas, sums, diagonal
-    b += (size_t)nchunks * NS * 8 + (size_t)(NS / RVF_STRIP) * NS * 8;   // part, partT
+    b += (size_t)tiles * RVF_STRIP * 8 + (size_t)(NS / RVF_STRIP) * NS * 8;  // part, partT
     if (r.opts.flags & RVF_WANT_F) b += (size_t)r.N * r.N * 8;
     if (r.opts.flags & RVF_AUDIT) b += (size_t)r.N * r.N * 8;
     return b + b / 8 + (1u << 20);
@@ -204,7 +242,10 @@ int pick_stream(const rvf_server* s) {
 
 // Large requests get direct launches of their tile ranges; smaller ones are
 // drained by the batched kernel.
-bool is_direct(const rvf_server* s, const Req* r) { return s->o.direct_tiles > 0 && r->ntiles >= s->o.direct_tiles; }
+// direct_tiles counts tiles of RVF_CHUNK columns: narrower tiles count pro rata
+bool is_direct(const rvf_server* s, const Req* r) {
+    return s->o.direct_tiles > 0 && (int64_t)r->ntiles * r->cols >= (int64_t)s->o.direct_tiles * RVF_CHUNK;
+}
 
 // Upload, build and enqueue the prologue of an admitted request.  The
 // prologue goes on the context stream, where plan creation queued its zero
@@ -229,6 +270,7 @@ int admit(rvf_server* s, Req* r) {
     }
     if (!rc) {
         r->ntiles = r->args.ntiles;
+        r->cols = rvfi_plan_columns(r->plan);
         s->h_args[r->slot] = r->args;  // pinned: the copy is asynchronous
         hipError_t e = hipMemcpyAsync(s->d_args + r->slot, s->h_args + r->slot, sizeof(PairArgs),
                                       hipMemcpyHostToDevice, st);

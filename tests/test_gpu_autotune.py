@@ -28,8 +28,14 @@ def _solve(ctx, dm, do, prec, tune=None, **kw):
     plan.run(); plan.sync()
     res = plan.fetch()
     name = plan.kernel_name()
+    cols = plan.columns
     plan.close()
-    return res, out, name
+    return res, out, (name, cols)
+
+
+def _default_columns(variant):
+    # wave-queue traversal variants run on tiles of 4 columns, the others 16
+    return 16 if variant == api.VARIANT_LANE else 4
 
 
 @pytest.mark.parametrize("prec", ["fp64", "mixed"])
@@ -38,19 +44,21 @@ def test_autotune_measures_all_variants_and_keeps_results(prec, tmp_path):
     m = _scene()
     dm = api.DeviceMesh(ctx, m)
     do = api.DeviceObstruction(ctx, dm, [7], "geometric")
-    ref, _, _ = _solve(ctx, dm, do, prec)
+    ref, _, (_, ref_cols) = _solve(ctx, dm, do, prec)
     table = api.TuneTable(str(tmp_path / "tune.json"))
-    got, (v, r, meas), name = _solve(ctx, dm, do, prec, tune=dict(sample_tiles=512, table=table))
+    got, (v, r, meas), (name, cols) = _solve(ctx, dm, do, prec, tune=dict(sample_tiles=512, table=table))
     variants = api.FP64_VARIANTS if prec == "fp64" else api.MIXED_VARIANTS
     measured = {e["variant"] for e in meas}
     assert set(variants) <= measured, (sorted(measured), variants)
     assert all(e["ms"] > 0 for e in meas)
-    # the first len(variants) entries rank the sample; then the library
-    # default (split 3 mixed, queue 1 FP64, refill 16) is timed twice on the
-    # larger set, then the three fastest others, then refill 8 and 32 of the
-    # choice.  Another configuration replaces the current choice only if it is
-    # faster than the best so far by more than twice the default's spread (at
-    # least 1 %).
+    # the first len(variants) entries rank the sample, each variant at its
+    # default tile width; then the library default (split 3 mixed, queue 1
+    # FP64, refill 16) is timed twice on the larger set, then the three
+    # fastest others, then refill 8 and 32 of the choice (wave-queue variants),
+    # then the choice's other tile widths of 4, 8 and 16 columns.  Another
+    # configuration replaces the current choice only if it is faster than the
+    # best so far by more than twice the default's spread (at least 1 %).
+    assert all(e["columns"] == _default_columns(e["variant"]) for e in meas[:len(variants)])
     vdef = api.VARIANT_SPLIT(3) if prec == "mixed" else api.VARIANT_QUEUE(1)
     ranked = [e["variant"] for e in sorted(meas[:len(variants)], key=lambda e: e["ms"]) if e["variant"] != vdef]
     again = meas[len(variants):]
@@ -59,26 +67,41 @@ def test_autotune_measures_all_variants_and_keeps_results(prec, tmp_path):
     assert [e["variant"] for e in again[2:5]] == ranked[:3]
     best_ms = min(d1["ms"], d2["ms"])
     margin = max(0.01, 2 * abs(d1["ms"] - d2["ms"]) / best_ms)
-    choice = (vdef, 16)
+    choice = (vdef, 16, 0)
     for e in again[2:5]:
         if e["ms"] < best_ms * (1 - margin):
-            best_ms, choice = e["ms"], (e["variant"], 16)
-    for e in again[5:]:
-        assert e["variant"] == choice[0] and e["refill_min"] in (8, 32)
+            best_ms, choice = e["ms"], (e["variant"], 16, 0)
+    k = 5
+    if choice[0] != api.VARIANT_LANE:
+        for e in again[k:k + 2]:
+            assert e["variant"] == choice[0] and e["refill_min"] in (8, 32)
+            assert e["columns"] == _default_columns(choice[0])
+            if e["ms"] < best_ms * (1 - margin):
+                best_ms, choice = e["ms"], (e["variant"], e["refill_min"], 0)
+        k += 2
+    widths = [e["columns"] for e in again[k:]]
+    assert sorted(widths) == sorted({4, 8, 16} - {_default_columns(choice[0])})
+    for e in again[k:]:
+        assert e["variant"] == choice[0] and e["refill_min"] == (0 if choice[0] == api.VARIANT_LANE else choice[1])
         if e["ms"] < best_ms * (1 - margin):
-            best_ms, choice = e["ms"], (e["variant"], e["refill_min"])
+            best_ms, choice = e["ms"], (choice[0], choice[1], e["columns"])
     if choice[0] == api.VARIANT_LANE:
-        choice = (choice[0], 0)
-    assert (v, r) == choice and v in variants
-    for a, b in zip(got, ref):
-        if isinstance(a, np.ndarray):
-            assert np.array_equal(a, b)
+        choice = (choice[0], 0, choice[2])
+    assert (v, r) == choice[:2] and v in variants
+    assert cols == (choice[2] or _default_columns(v))
+    names = ("F", "A", "rowsum", "F_group", "A_group", "tags", "audit_count", "audit_hash")
+    for nm, a, b in zip(names, got, ref):
+        if nm in ("rowsum", "F_group") and cols != ref_cols:
+            # sums over another tile width: the same terms added in another order
+            assert np.allclose(a, b, rtol=1e-13, atol=1e-15), nm
+        elif isinstance(a, np.ndarray):
+            assert np.array_equal(a, b), nm
         else:
-            assert a == b
+            assert a == b, nm
     # the stored choice is applied by a new plan without measuring
     table2 = api.TuneTable(str(tmp_path / "tune.json"))
     got2, (v2, r2, meas2), name2 = _solve(ctx, dm, do, prec, tune=dict(table=table2))
-    assert meas2 == [] and (v2, r2) == (v, r) and name2 == name
+    assert meas2 == [] and (v2, r2) == (v, r) and name2 == (name, cols)
     assert np.array_equal(got2[0], ref[0])
     do.close(); dm.close()
 
@@ -89,5 +112,5 @@ def test_autotune_unoccluded_plan_is_a_no_op():
     plan = api.Plan(ctx, dm, None, 3, "quad", "fp64")
     name = plan.kernel_name()
     v, r, meas = plan.autotune()
-    assert meas == [] and plan.kernel_name() == name
+    assert meas == [] and plan.kernel_name() == name and plan.columns == 16
     plan.close(); dm.close()
