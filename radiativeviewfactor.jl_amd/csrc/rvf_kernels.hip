@@ -1767,6 +1767,7 @@ __device__ __forceinline__ int capsule_frontier(const double* ai, const double* 
         return tmin <= tmax;
     };
 #endif
+#ifdef RVF_FRONTIER_RESTART
     int tests = 0;
     for (int lmax = 2;; lmax *= 4) {
         int k = 0;
@@ -1787,6 +1788,47 @@ __device__ __forceinline__ int capsule_frontier(const double* ai, const double* 
         }
         if (!over) return k;
     }
+#else
+    // Breadth first: each round replaces every open entry by the children
+    // its capsule meets, as long as the list keeps at most RVF_FRONTIER
+    // entries.  A round runs from the last entry to the first and writes the
+    // new list from the top of the column down, so an entry not yet read is
+    // never overwritten (entry u has the slots u .. w-1 to itself and the
+    // entries before it); the list stays in preorder.  Leaves, and entries
+    // that found no room (room only shrinks), are closed: stored as ~node.
+    int tests = 1;
+    if (!meets(nodes[root])) return 0;
+    out[0] = root;
+    int k = 1;
+    for (bool open = true; open;) {
+        open = false;
+        int w = RVF_FRONTIER;
+        for (int u = k - 1; u >= 0; u--) {
+            const int e = out[u * 64];
+            if (e >= 0) {
+                const int sc = nodes[e].start_count;
+                if ((sc & 15) == 0) {  // interior: children n + 1 and sc >> 4
+                    if (tests + 2 > limit) { out[0] = root; return 1; }
+                    tests += 2;
+                    const int l = e + 1, r = sc >> 4;
+                    const bool ml = meets(nodes[l]), mr = meets(nodes[r]);
+                    if ((int)ml + (int)mr <= w - u) {
+                        if (mr) out[--w * 64] = r;
+                        if (ml) out[--w * 64] = l;
+                        open = true;
+                        continue;
+                    }
+                }
+            }
+            out[--w * 64] = e >= 0 ? ~e : e;  // closed (a leaf, or no room)
+        }
+        k = RVF_FRONTIER - w;
+        for (int v = 0; v < k; v++) out[v * 64] = out[(w + v) * 64];
+    }
+    // the last round found every entry closed
+    for (int v = 0; v < k; v++) out[v * 64] = ~out[v * 64];
+    return k;
+#endif
 }
 
 // Can a ray between a point of element i and a point of element j reach the
