@@ -917,7 +917,7 @@ struct rvf_plan {
     hipEvent_t ev[8] = {};
     rvf_stats st{};
     bool ran = false;
-    int refill_min = 16;
+    int refill_min = 0;  // 0: the kernel's default (default_refill)
     int cw = 0;      // columns per tile of the current layout
     int cw_req = 0;  // columns asked for by rvf_plan_tune_tiles (0: the kernel's default)
     bool scratch = false;  // buffers are pieces of the static scratch array
@@ -935,6 +935,10 @@ static int pair_kind(const rvf_plan* p);
 static bool queue_kind(int kind) { return kind >= RVF_K_FP64_QUEUE1 && kind <= RVF_K_MIX_SPLIT3; }
 static int default_columns(int kind) { return queue_kind(kind) ? RVF_CHUNK_QUEUE : RVF_CHUNK; }
 static int plan_columns(const rvf_plan* p) { return p->cw_req ? p->cw_req : default_columns(pair_kind(p)); }
+// Lanes with an empty slot before a wave-queue refill.  Measured on the full
+// occluded cube / plates problems, mixed split 3: 16 -> 207.7 / 284.8 ms,
+// 32 -> 206.9 / 278.0 ms, 48 -> 213.5 / 284.1 ms.
+static int default_refill(int kind) { return kind >= RVF_K_MIX_SPLIT1 && kind <= RVF_K_MIX_SPLIT3 ? 32 : 16; }
 // spatially compact strips for occluded quadrature solves (coherent ray bundles)
 static bool plan_spatial(const rvf_plan* p) {
     return p->obs && !p->obs->trees.empty() && p->opts.method == RVF_METHOD_QUAD;
@@ -1177,7 +1181,7 @@ static PairArgs pair_args(const rvf_plan* p) {
         a.ttree_root = p->obs->d_troot.as<int32_t>(); a.tnodes32 = p->obs->d_tn32.as<BvhNode32>();
         a.ttris32 = p->obs->d_tt32.as<Tri32>(); a.tmap = p->obs->d_tmap.as<int32_t>();
         a.n_nodes = (int)p->obs->n_nodes;
-        a.refill_min = p->refill_min;
+        a.refill_min = p->refill_min ? p->refill_min : default_refill(pair_kind(p));
         a.verify = p->obs->soup_read == RVF_SOUP_REFERENCE ? 1 : 0;
         a.cull_limit = 256;  // box tests per pair; measured: 64 leaves 2 % of the node steps, more gains nothing
         a.hint = 1;  // (a second hint per pair saved 8 % of the node steps but cost more than it saved)
@@ -1418,7 +1422,7 @@ int rvf_plan_tune(rvf_plan* p, int32_t variant, int32_t refill_min) {
     p->opts.variant = variant;
     const int rc = plan_relayout(p);
     if (rc) { p->opts.variant = v0; return rc; }
-    p->refill_min = refill_min ? refill_min : 16;
+    p->refill_min = refill_min;
     if (p->gexec) { (void)hipGraphExecDestroy(p->gexec); p->gexec = nullptr; }  // re-captured on the next run
     return 0;
 }
@@ -1442,7 +1446,7 @@ int rvf_plan_autotune(rvf_plan* p, int64_t sample_tiles, rvf_tune_entry* ent, in
         return fail(RVF_E_ARG, "rvf_plan_autotune: bad arguments");
     *n = 0;
     if (variant) *variant = p->opts.variant;
-    if (refill_min) *refill_min = p->refill_min;
+    if (refill_min) *refill_min = p->refill_min ? p->refill_min : default_refill(pair_kind(p));
     if (!plan_occluded_quad(p) || p->mesh->dim2) return 0;  // curve meshes have one traversal form
     if (p->scratch) return fail(RVF_E_STATE, "rvf_plan_autotune: a plan in the static scratch array is not tuned");
     rvf_ctx* c = p->ctx;
@@ -1497,34 +1501,35 @@ int rvf_plan_autotune(rvf_plan* p, int64_t sample_tiles, rvf_tune_entry* ent, in
         if (occ_kind(RVF_VARIANT_QUEUE(S), mixed, ntri) >= 0) cands.push_back(RVF_VARIANT_QUEUE(S));
     for (int S = 1; S <= 3; S++)
         if (occ_kind(RVF_VARIANT_SPLIT(S), mixed, ntri) >= 0) cands.push_back(RVF_VARIANT_SPLIT(S));
+    // a measured configuration as it ran (refill 0 stands for the kernel's default)
     auto add = [&](int v, int refill, float ms) {
-        if (*n < cap)
-            ent[*n] = rvf_tune_entry{v, v == RVF_VARIANT_LANE ? 0 : refill, timed, (double)ms, p->cw, 0};
+        const int r = v == RVF_VARIANT_LANE ? 0 : refill ? refill : default_refill(pair_kind(p));
+        if (*n < cap) ent[*n] = rvf_tune_entry{v, r, timed, (double)ms, p->cw, 0};
         ++*n;
     };
     // 1. rank every variant (at its default tile width) on the sample
     std::vector<std::pair<float, int>> rank;
     for (int v : cands) {
         float ms = 0;
-        if ((rc = measure(v, 16, 0, frac, &ms))) break;
-        add(v, 16, ms);
+        if ((rc = measure(v, 0, 0, frac, &ms))) break;
+        add(v, 0, ms);
         rank.push_back({ms, v});
     }
     // 2. re-time the three fastest on the whole problem (or a four times
     //    larger sample), then the refill thresholds of the winner, then its
-    //    tile widths.  The library default (variant, refill 16, default
-    //    width) is timed twice there: another configuration replaces the
+    //    tile widths.  The library default (variant, refill and width at
+    //    their defaults) is timed twice there: another configuration replaces the
     //    current choice only if faster by more than twice their spread (and
     //    at least 1 %), so run-to-run noise does not swap the kernel that runs.
     const int vdef = default_variant(mixed, ntri);
-    int best_v = vdef, best_r = 16, best_c = 0;
+    int best_v = vdef, best_r = 0, best_c = 0;
     float best = 3.4e38f, margin = 0.01f;
     if (!rc) {
         float d1 = 0, d2 = 0;
-        if (!(rc = measure(vdef, 16, 0, big, &d1))) {
-            add(vdef, 16, d1);
-            if (!(rc = measure(vdef, 16, 0, big, &d2))) {
-                add(vdef, 16, d2);
+        if (!(rc = measure(vdef, 0, 0, big, &d1))) {
+            add(vdef, 0, d1);
+            if (!(rc = measure(vdef, 0, 0, big, &d2))) {
+                add(vdef, 0, d2);
                 best = std::min(d1, d2);
                 margin = std::max(0.01f, 2.0f * std::fabs(d1 - d2) / best);
             }
@@ -1536,15 +1541,18 @@ int rvf_plan_autotune(rvf_plan* p, int64_t sample_tiles, rvf_tune_entry* ent, in
         for (size_t k = 0, done = 0; k < rank.size() && done < 3; k++) {
             if (rank[k].second == vdef) continue;
             float ms = 0;
-            if ((rc = measure(rank[k].second, 16, 0, big, &ms))) break;
-            add(rank[k].second, 16, ms);
+            if ((rc = measure(rank[k].second, 0, 0, big, &ms))) break;
+            add(rank[k].second, 0, ms);
             done++;
-            if (better(ms)) { best = ms; best_v = rank[k].second; best_r = 16; }
+            if (better(ms)) { best = ms; best_v = rank[k].second; best_r = 0; }
         }
     }
     if (!rc && best_v != RVF_VARIANT_LANE) {
         const int bv = best_v;
-        for (int refill : {8, 32}) {
+        p->opts.variant = bv;
+        const int rdef = default_refill(pair_kind(p));
+        for (int refill : {8, 16, 32}) {
+            if (refill == rdef) continue;
             float ms = 0;
             if ((rc = measure(bv, refill, 0, big, &ms))) break;
             add(bv, refill, ms);
@@ -1573,7 +1581,7 @@ int rvf_plan_autotune(rvf_plan* p, int64_t sample_tiles, rvf_tune_entry* ent, in
     p->cw_req = best_c;
     if ((rc = plan_relayout(p))) return rc;
     if (variant) *variant = best_v;
-    if (refill_min) *refill_min = best_v == RVF_VARIANT_LANE ? 0 : best_r;
+    if (refill_min) *refill_min = best_v == RVF_VARIANT_LANE ? 0 : best_r ? best_r : default_refill(pair_kind(p));
     if (p->gexec) { (void)hipGraphExecDestroy(p->gexec); p->gexec = nullptr; }  // re-captured on the next run
     return 0;
 }

@@ -38,6 +38,15 @@ def _default_columns(variant):
     return 16 if variant == api.VARIANT_LANE else 4
 
 
+def _default_refill(variant, prec):
+    # lanes waiting before a refill: 32 for the mixed split variants, 16 for
+    # the other wave-queue variants (the lane variant has none)
+    if variant == api.VARIANT_LANE:
+        return 0
+    split = variant in (api.VARIANT_SPLIT(1), api.VARIANT_SPLIT(2), api.VARIANT_SPLIT(3))
+    return 32 if (prec == "mixed" and split) else 16
+
+
 @pytest.mark.parametrize("prec", ["fp64", "mixed"])
 def test_autotune_measures_all_variants_and_keeps_results(prec, tmp_path):
     ctx = api.context(0)
@@ -52,29 +61,34 @@ def test_autotune_measures_all_variants_and_keeps_results(prec, tmp_path):
     assert set(variants) <= measured, (sorted(measured), variants)
     assert all(e["ms"] > 0 for e in meas)
     # the first len(variants) entries rank the sample, each variant at its
-    # default tile width; then the library default (split 3 mixed, queue 1
-    # FP64, refill 16) is timed twice on the larger set, then the three
-    # fastest others, then refill 8 and 32 of the choice (wave-queue variants),
-    # then the choice's other tile widths of 4, 8 and 16 columns.  Another
+    # default tile width and refill threshold; then the library default
+    # (split 3 mixed, queue 1 FP64) is timed twice on the larger set, then the
+    # three fastest others, then the choice's other refill thresholds of 8, 16
+    # and 32 (wave-queue variants), then its other tile widths of 4, 8 and 16
+    # columns.  Another
     # configuration replaces the current choice only if it is faster than the
     # best so far by more than twice the default's spread (at least 1 %).
     assert all(e["columns"] == _default_columns(e["variant"]) for e in meas[:len(variants)])
+    assert all(e["refill_min"] == _default_refill(e["variant"], prec) for e in meas[:len(variants)])
+    rd = lambda v: _default_refill(v, prec)  # noqa: E731
     vdef = api.VARIANT_SPLIT(3) if prec == "mixed" else api.VARIANT_QUEUE(1)
     ranked = [e["variant"] for e in sorted(meas[:len(variants)], key=lambda e: e["ms"]) if e["variant"] != vdef]
     again = meas[len(variants):]
     d1, d2 = again[0], again[1]
-    assert (d1["variant"], d1["refill_min"]) == (vdef, 16) == (d2["variant"], d2["refill_min"])
+    assert (d1["variant"], d1["refill_min"]) == (vdef, rd(vdef)) == (d2["variant"], d2["refill_min"])
     assert [e["variant"] for e in again[2:5]] == ranked[:3]
     best_ms = min(d1["ms"], d2["ms"])
     margin = max(0.01, 2 * abs(d1["ms"] - d2["ms"]) / best_ms)
-    choice = (vdef, 16, 0)
+    choice = (vdef, rd(vdef), 0)
     for e in again[2:5]:
+        assert e["refill_min"] == rd(e["variant"])
         if e["ms"] < best_ms * (1 - margin):
-            best_ms, choice = e["ms"], (e["variant"], 16, 0)
+            best_ms, choice = e["ms"], (e["variant"], rd(e["variant"]), 0)
     k = 5
     if choice[0] != api.VARIANT_LANE:
+        assert sorted(e["refill_min"] for e in again[k:k + 2]) == sorted({8, 16, 32} - {rd(choice[0])})
         for e in again[k:k + 2]:
-            assert e["variant"] == choice[0] and e["refill_min"] in (8, 32)
+            assert e["variant"] == choice[0]
             assert e["columns"] == _default_columns(choice[0])
             if e["ms"] < best_ms * (1 - margin):
                 best_ms, choice = e["ms"], (e["variant"], e["refill_min"], 0)
@@ -82,11 +96,9 @@ def test_autotune_measures_all_variants_and_keeps_results(prec, tmp_path):
     widths = [e["columns"] for e in again[k:]]
     assert sorted(widths) == sorted({4, 8, 16} - {_default_columns(choice[0])})
     for e in again[k:]:
-        assert e["variant"] == choice[0] and e["refill_min"] == (0 if choice[0] == api.VARIANT_LANE else choice[1])
+        assert e["variant"] == choice[0] and e["refill_min"] == choice[1]
         if e["ms"] < best_ms * (1 - margin):
             best_ms, choice = e["ms"], (choice[0], choice[1], e["columns"])
-    if choice[0] == api.VARIANT_LANE:
-        choice = (choice[0], 0, choice[2])
     assert (v, r) == choice[:2] and v in variants
     assert cols == (choice[2] or _default_columns(v))
     names = ("F", "A", "rowsum", "F_group", "A_group", "tags", "audit_count", "audit_hash")
