@@ -182,7 +182,8 @@ def main():
                     help="one copy of known size, then one solve (for rocprofv3 --pmc runs)")
     ap.add_argument("--probe-precision", default="mixed", choices=["mixed", "fp64"])
     ap.add_argument("--section", default=None,
-                    choices=["occluded_plates", "curved_cylinder", "unoccluded_cube", "self_vf_sphere", "curves"],
+                    choices=["occluded_plates", "curved_cylinder", "unoccluded_cube", "self_vf_sphere", "curves",
+                             "serving"],
                     help="run only this secondary section and print its JSON (for profiler runs)")
     args = ap.parse_args()
     if args.gpus != 1:
@@ -199,7 +200,8 @@ def main():
               "curved_cylinder": lambda: secondary_cylinder(api, ctx),
               "unoccluded_cube": lambda: secondary_cube(api, ctx, args.steps),
               "self_vf_sphere": lambda: secondary_self_vf(api, ctx, args.steps),
-              "curves": lambda: secondary_curves(api, ctx, args.steps, 0.0)}[args.section]
+              "curves": lambda: secondary_curves(api, ctx, args.steps, 0.0),
+              "serving": lambda: serving_stream(api, ctx, args.serving_requests, args.serving_rate)}[args.section]
         print(json.dumps({args.section: fn()}))
         return
     mesh, obs, nquad, parity, desc = build_workload(args.workload)
@@ -345,7 +347,11 @@ def serving_stream(api, ctx, nreq, rate, seed=0x5EED0005):
     wmesh, wobs, wnq = M.random_box_enclosure(np.random.default_rng(seed + 1), 600, max_occluders=1,
                                               max_occluder_tris=500)
     outs = {}
-    for prec, direct in (("mixed", 0), ("fp64", 0), ("mixed_batched", -1)):
+    # the Poisson stream in each precision, the same requests with every tile
+    # batched, and the same requests submitted at once ("burst": the makespan
+    # is then the device's, not the arrivals')
+    for prec, direct, burst in (("mixed", 0, False), ("fp64", 0, False), ("mixed_batched", -1, False),
+                                ("mixed_burst", 0, True), ("fp64_burst", 0, True)):
         srv = api.Server(ctx, direct_tiles=direct)
         prec_name, prec = prec, prec.split("_")[0]
         # warm-up request (untimed): first use of the server's streams and of
@@ -360,7 +366,7 @@ def serving_stream(api, ctx, nreq, rate, seed=0x5EED0005):
 
         def submitter():
             for k, (mesh, obs, nq) in enumerate(reqs):
-                dt = arrivals[k] - (time.perf_counter() - t0)
+                dt = (0.0 if burst else arrivals[k]) - (time.perf_counter() - t0)
                 if dt > 0:
                     time.sleep(dt)
                 tickets[k] = srv.submit(mesh, nq, obs, "geometric", prec)
@@ -388,8 +394,10 @@ def serving_stream(api, ctx, nreq, rate, seed=0x5EED0005):
                           "rounds": c["rounds"] - c0["rounds"], "peak_device_bytes": c["peak_bytes"],
                           "tiles": tiles, "batched_tiles_share": (c["batched_tiles"] - c0["batched_tiles"]) / max(1, tiles),
                           "direct_tiles_option": "default (512)" if direct == 0 else "never (every tile batched)",
+                          "arrivals": "all at once" if burst else f"Poisson {rate}/s",
                           "worker_ms": {k: c[k] - c0[k] for k in ("launch_ms", "admit_ms", "wait_ms", "retire_ms")}}
     res["speedup_mixed_vs_fp64"] = res["fp64"]["makespan_s"] / res["mixed"]["makespan_s"]
+    res["speedup_burst_mixed_vs_fp64"] = res["fp64_burst"]["makespan_s"] / res["mixed_burst"]["makespan_s"]
     res["closure_err"] = float(max(np.abs(a[2] - b[2]).max() for a, b in zip(outs["mixed"], outs["fp64"])))
     res["max_abs_dF_group"] = float(max(np.abs(a[3] - b[3]).max() for a, b in zip(outs["mixed"], outs["fp64"])))
     # per request, untimed: standalone solves of both precisions with audits;
@@ -416,7 +424,7 @@ def serving_stream(api, ctx, nreq, rate, seed=0x5EED0005):
             do.close()
         dm.close()
         same = all(np.array_equal(outs[p][k][2], alone[p.split("_")[0]][2]) and
-                   np.array_equal(outs[p][k][3], alone[p.split("_")[0]][3]) for p in ("mixed", "fp64", "mixed_batched"))
+                   np.array_equal(outs[p][k][3], alone[p.split("_")[0]][3]) for p in outs)
         identical += same
         per.append({"elements": int(mesh.n_elems), "max_abs_dF": float(np.abs(Fm - Fd).max()),
                     "closure_err": float(np.abs(alone["mixed"][2] - alone["fp64"][2]).max()), "flips": flips})

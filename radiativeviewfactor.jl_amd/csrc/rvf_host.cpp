@@ -78,11 +78,21 @@ struct DBuf {
     template <class T> T* as() const { return (T*)p; }
 };
 
-static int upload(DBuf& b, const void* src, size_t bytes) {
+// Host <-> device copies ordered on one stream only.  A synchronous hipMemcpy
+// also waits for work on the device's other streams (measured in the request
+// server: up to 1.5 s behind another request's pair kernel), so every copy is
+// an asynchronous one on the context stream followed by a wait on that stream.
+static int copy_sync(void* dst, const void* src, size_t bytes, hipMemcpyKind kind, hipStream_t st) {
+    if (!bytes) return 0;
+    HIPCHK(hipMemcpyAsync(dst, src, bytes, kind, st));
+    HIPCHK(hipStreamSynchronize(st));
+    return 0;
+}
+
+static int upload(DBuf& b, const void* src, size_t bytes, hipStream_t st) {
     int rc = b.alloc(bytes);
     if (rc) return rc;
-    if (bytes && src) HIPCHK(hipMemcpy(b.p, src, bytes, hipMemcpyHostToDevice));
-    return 0;
+    return src ? copy_sync(b.p, src, bytes, hipMemcpyHostToDevice, st) : 0;
 }
 
 // ---------------------------------------------------------------------------
@@ -297,11 +307,12 @@ static int get_layout(const rvf_mesh* m, bool spatial, bool self, int cw, const 
         const int64_t NS = m->NS;
         layout_host(m, spatial, self, cw, L);
         int rc;
-        if ((rc = upload(L.d_slot_orig, L.slot_orig.data(), sizeof(int32_t) * NS)) ||
-            (rc = upload(L.d_slot_grp, L.slot_grp.data(), sizeof(int32_t) * NS)) ||
-            (rc = upload(L.d_chunks, L.chunks.data(), sizeof(Chunk) * L.chunks.size())) ||
-            (rc = upload(L.d_chunk_of, L.chunk_of.data(), sizeof(int32_t) * NS)) ||
-            (rc = upload(L.d_tiles, L.tiles.data(), sizeof(int2) * L.tiles.size())))
+        const hipStream_t st = m->ctx->stream;
+        if ((rc = upload(L.d_slot_orig, L.slot_orig.data(), sizeof(int32_t) * NS, st)) ||
+            (rc = upload(L.d_slot_grp, L.slot_grp.data(), sizeof(int32_t) * NS, st)) ||
+            (rc = upload(L.d_chunks, L.chunks.data(), sizeof(Chunk) * L.chunks.size(), st)) ||
+            (rc = upload(L.d_chunk_of, L.chunk_of.data(), sizeof(int32_t) * NS, st)) ||
+            (rc = upload(L.d_tiles, L.tiles.data(), sizeof(int2) * L.tiles.size(), st)))
             return rc;
         L.built = true;
     }
@@ -358,11 +369,12 @@ static int mesh_upload(rvf_ctx* ctx, const double* coords, int64_t n_nodes, int6
     for (int64_t bj = 0; bj < nb; bj++)
         for (int64_t bi = 0; bi <= bj; bi++) m->fill_tiles.push_back(make_int2((int)bi, (int)bj));
     int rc;
-    if ((rc = upload(m->coords, coords, sizeof(double) * 3 * n_nodes)) ||
-        (rc = upload(m->d_fam, m->fam.data(), n_elems)) ||
-        (rc = upload(m->d_nodes8, m->nodes8.data(), sizeof(int32_t) * 8 * n_elems)) ||
-        (rc = upload(m->d_grp, m->grp_rank.data(), sizeof(int32_t) * n_elems)) ||
-        (rc = upload(m->d_fill_tiles, m->fill_tiles.data(), sizeof(int2) * m->fill_tiles.size()))) {
+    const hipStream_t st = ctx->stream;
+    if ((rc = upload(m->coords, coords, sizeof(double) * 3 * n_nodes, st)) ||
+        (rc = upload(m->d_fam, m->fam.data(), n_elems, st)) ||
+        (rc = upload(m->d_nodes8, m->nodes8.data(), sizeof(int32_t) * 8 * n_elems, st)) ||
+        (rc = upload(m->d_grp, m->grp_rank.data(), sizeof(int32_t) * n_elems, st)) ||
+        (rc = upload(m->d_fill_tiles, m->fill_tiles.data(), sizeof(int2) * m->fill_tiles.size(), st))) {
         delete m;
         return rc;
     }
@@ -828,18 +840,19 @@ int rvf_obs_build(rvf_ctx* ctx, const rvf_mesh* mesh, const double* soup, const 
         }
     }
     int rc;
-    if ((rc = upload(o->d_items32, i32.data(), sizeof(Item32) * i32.size())) ||
-        (rc = upload(o->d_items64, i64.data(), sizeof(Item64) * i64.size())) ||
-        (rc = upload(o->d_root, o->root.data(), sizeof(int32_t) * std::max(1, G * G))) ||
-        (rc = upload(o->d_n64, n64.data(), sizeof(BvhNode64) * n64.size())) ||
-        (rc = upload(o->d_n32, n32.data(), sizeof(BvhNode32) * n32.size())) ||
-        (rc = upload(o->d_t64, t64.data(), sizeof(Tri64) * t64.size())) ||
-        (rc = upload(o->d_t32, t32.data(), sizeof(Tri32) * t32.size())) ||
-        (rc = upload(o->d_parent, parent.data(), sizeof(int32_t) * parent.size())) ||
-        (rc = upload(o->d_troot, o->troot.data(), sizeof(int32_t) * std::max(1, G * G))) ||
-        (rc = upload(o->d_tn32, tn32.data(), sizeof(BvhNode32) * tn32.size())) ||
-        (rc = upload(o->d_tt32, tt32.data(), sizeof(Tri32) * tt32.size())) ||
-        (rc = upload(o->d_tmap, tmap.data(), sizeof(int32_t) * tmap.size()))) {
+    const hipStream_t st = ctx->stream;
+    if ((rc = upload(o->d_items32, i32.data(), sizeof(Item32) * i32.size(), st)) ||
+        (rc = upload(o->d_items64, i64.data(), sizeof(Item64) * i64.size(), st)) ||
+        (rc = upload(o->d_root, o->root.data(), sizeof(int32_t) * std::max(1, G * G), st)) ||
+        (rc = upload(o->d_n64, n64.data(), sizeof(BvhNode64) * n64.size(), st)) ||
+        (rc = upload(o->d_n32, n32.data(), sizeof(BvhNode32) * n32.size(), st)) ||
+        (rc = upload(o->d_t64, t64.data(), sizeof(Tri64) * t64.size(), st)) ||
+        (rc = upload(o->d_t32, t32.data(), sizeof(Tri32) * t32.size(), st)) ||
+        (rc = upload(o->d_parent, parent.data(), sizeof(int32_t) * parent.size(), st)) ||
+        (rc = upload(o->d_troot, o->troot.data(), sizeof(int32_t) * std::max(1, G * G), st)) ||
+        (rc = upload(o->d_tn32, tn32.data(), sizeof(BvhNode32) * tn32.size(), st)) ||
+        (rc = upload(o->d_tt32, tt32.data(), sizeof(Tri32) * tt32.size(), st)) ||
+        (rc = upload(o->d_tmap, tmap.data(), sizeof(int32_t) * tmap.size(), st))) {
         delete o;
         return rc;
     }
@@ -1108,7 +1121,7 @@ static int plan_init(rvf_ctx* ctx, const rvf_mesh* mesh, const rvf_obs* obs, con
     auto h2d = [&](DBuf& b, const void* h, size_t bytes) -> int {
         if (!bytes) return 0;
         if (scratch) HIPCHK(hipMemcpyAsync(b.p, h, bytes, hipMemcpyHostToDevice, ctx->stream));
-        else HIPCHK(hipMemcpy(b.p, h, bytes, hipMemcpyHostToDevice));
+        else return copy_sync(b.p, h, bytes, hipMemcpyHostToDevice, ctx->stream);
         return 0;
     };
     if (scratch) {
@@ -1623,10 +1636,12 @@ int rvf_plan_run(rvf_plan* p) {
 // whose work has completed.
 static int plan_collect(rvf_plan* p) {
     int bad = 0;
-    HIPCHK(hipMemcpy(&bad, p->bad.p, sizeof(int), hipMemcpyDeviceToHost));
+    const hipStream_t st = p->ctx->stream;
+    int rc = copy_sync(&bad, p->bad.p, sizeof(int), hipMemcpyDeviceToHost, st);
+    if (rc) return rc;
     if (bad) return fail(RVF_E_ARG, "degenerate element: zero or non-finite area element at a quadrature point");
     unsigned long long cnt[16];
-    HIPCHK(hipMemcpy(cnt, p->counters.p, sizeof cnt, hipMemcpyDeviceToHost));
+    if ((rc = copy_sync(cnt, p->counters.p, sizeof cnt, hipMemcpyDeviceToHost, st))) return rc;
     p->st.retests = (int64_t)cnt[0];
     p->st.rays = (int64_t)cnt[1];
     p->st.aabb_tests = (int64_t)cnt[2];
@@ -1666,22 +1681,25 @@ int rvf_plan_fetch(rvf_plan* p, double* F, double* A, double* rowsum, double* F_
     int64_t N = m->N;
     int G = (int)m->tags.size();
     HIPCHK(hipSetDevice(p->ctx->device));
+    const hipStream_t st = p->ctx->stream;
+    auto d2h = [&](void* dst, const void* src, size_t bytes) { return copy_sync(dst, src, bytes, hipMemcpyDeviceToHost, st); };
+    int rc;
     if (F) {
         if (!p->F.p) return fail(RVF_E_STATE, "F was not requested (RVF_WANT_F)");
-        HIPCHK(hipMemcpy(F, p->F.p, sizeof(double) * N * N, hipMemcpyDeviceToHost));
+        if ((rc = d2h(F, p->F.p, sizeof(double) * N * N))) return rc;
     }
     const Layout* L = p->L;
     std::vector<double> hA(N), hAo(N), hW((size_t)G * m->NS);
-    HIPCHK(hipMemcpy(hA.data(), p->area.p, sizeof(double) * N, hipMemcpyDeviceToHost));      // slot order
-    HIPCHK(hipMemcpy(hAo.data(), p->area_orig.p, sizeof(double) * N, hipMemcpyDeviceToHost)); // element order
+    if ((rc = d2h(hA.data(), p->area.p, sizeof(double) * N))) return rc;      // slot order
+    if ((rc = d2h(hAo.data(), p->area_orig.p, sizeof(double) * N))) return rc; // element order
     if (A) std::memcpy(A, hAo.data(), sizeof(double) * N);
     if (rowsum) {
         std::vector<double> rs(N);
-        HIPCHK(hipMemcpy(rs.data(), p->rowsum.p, sizeof(double) * N, hipMemcpyDeviceToHost));
+        if ((rc = d2h(rs.data(), p->rowsum.p, sizeof(double) * N))) return rc;
         for (int64_t k = 0; k < N; k++) rowsum[L->slot_orig[k]] = rs[k];
     }
     if (F_group || A_group) {
-        HIPCHK(hipMemcpy(hW.data(), p->W.p, sizeof(double) * G * m->NS, hipMemcpyDeviceToHost));
+        if ((rc = d2h(hW.data(), p->W.p, sizeof(double) * G * m->NS))) return rc;
         // A_g F_gh = sum_{i in g} A_i sum_{j in h} F_ij (Results.jl:56-82); with W[h][k] =
         // sum over slots l > k in group h of F, the pairs below the slot diagonal
         // come from the symmetric raw integral: sum_{l in h} A_l W[g][l]
@@ -1697,7 +1715,7 @@ int rvf_plan_fetch(rvf_plan* p, double* F, double* A, double* rowsum, double* F_
         }
         if (p->diag.p) {  // self pairs: raw_ii = A_i F_ii, once
             std::vector<double> hd(N);
-            HIPCHK(hipMemcpy(hd.data(), p->diag.p, sizeof(double) * N, hipMemcpyDeviceToHost));
+            if ((rc = d2h(hd.data(), p->diag.p, sizeof(double) * N))) return rc;
             for (int64_t k = 0; k < N; k++) R[(size_t)L->slot_grp[k] * (G + 1)] += hd[k];
         }
         if (A_group) for (int g = 0; g < G; g++) A_group[g] = Ag[g];
@@ -1708,8 +1726,8 @@ int rvf_plan_fetch(rvf_plan* p, double* F, double* A, double* rowsum, double* F_
     if (group_tags) for (int g = 0; g < G; g++) group_tags[g] = m->tags[g];
     if (audit_count || audit_hash) {
         if (!p->audit_c.p) return fail(RVF_E_STATE, "audit was not requested (RVF_AUDIT with an obstruction set)");
-        if (audit_count) HIPCHK(hipMemcpy(audit_count, p->audit_c.p, sizeof(uint32_t) * N * N, hipMemcpyDeviceToHost));
-        if (audit_hash) HIPCHK(hipMemcpy(audit_hash, p->audit_h.p, sizeof(uint32_t) * N * N, hipMemcpyDeviceToHost));
+        if (audit_count) if ((rc = d2h(audit_count, p->audit_c.p, sizeof(uint32_t) * N * N))) return rc;
+        if (audit_hash) if ((rc = d2h(audit_hash, p->audit_h.p, sizeof(uint32_t) * N * N))) return rc;
     }
     return 0;
 }
