@@ -2271,7 +2271,16 @@ __device__ __forceinline__ double xorshift64(uint64_t& s) {
     uint64_t x = s;
     x ^= x << 13; x ^= x >> 7; x ^= x << 17;
     s = x;
-    return (double)(x >> 11) / (double)0x001FFFFFFFFFFFFFULL;
+    // (x >> 11) / (2^53 - 1), correctly rounded, without the division: for
+    // 1 <= m < 2^53, m / (2^53 - 1) = y (1 + 2^-53 / (1 - 2^-53)) with
+    // y = m 2^-53 exact, and the excess over y lies strictly between half an
+    // ulp of y and y's ulp times (1 + 2^-52), so the rounded quotient is the
+    // double after y (0 for m = 0).  Checked against the division for every m
+    // below 5,000 and above 2^53 - 5,000, at m = 2^e + {-2..2} and on 35 M
+    // random m; the Monte Carlo streams stay bit-identical to the oracle's.
+    const uint64_t m = x >> 11;
+    const double y = (double)m * 0x1p-53;
+    return m ? __longlong_as_double(__double_as_longlong(y) + 1) : 0.0;
 }
 // _sample_on_element (GPUMCKernels.jl:266-281)
 __device__ __forceinline__ void mc_sample(const PairArgs& a, int f, int64_t e, double u1, double u2, double* x,
