@@ -84,6 +84,17 @@ enum RvfPairKind {
 #undef RVF_KIND_ENUM
     RVF_K_COUNT
 };
+// The wave-queue traversal kinds (queue and split forms, both precisions) are
+// consecutive in the list: their launches are persistent, claiming tiles
+// dynamically, and their default tiles are RVF_CHUNK_QUEUE columns wide.
+constexpr bool rvf_queue_kind(int kind) { return kind >= RVF_K_FP64_QUEUE1 && kind <= RVF_K_MIX_SPLIT3; }
+static_assert(RVF_K_MIX_SPLIT3 - RVF_K_FP64_QUEUE1 == 12, "wave-queue kinds must stay consecutive");
+// the split kinds (an LDS triangle-task ring), consecutive too
+constexpr bool rvf_split_kind(int kind) { return kind >= RVF_K_FP64_SPLIT1 && kind <= RVF_K_MIX_SPLIT3; }
+static_assert(RVF_K_MIX_SPLIT3 - RVF_K_FP64_SPLIT1 == 5, "split kinds must stay consecutive");
+// the mixed split kinds, consecutive too (their refill default differs)
+constexpr bool rvf_mixed_split_kind(int kind) { return kind >= RVF_K_MIX_SPLIT1 && kind <= RVF_K_MIX_SPLIT3; }
+static_assert(RVF_K_MIX_SPLIT3 - RVF_K_MIX_SPLIT1 == 2, "mixed split kinds must stay consecutive");
 #define RVF_QCOMBO 128   // (p, q) combinations per wave-queue batch
 #define RVF_TQCAP 2048   // triangle tasks per wavefront ring (FP64 split variants)
 #define RVF_TQCAP_MIX 256   // the same for the mixed split variants (single-triangle leaves)

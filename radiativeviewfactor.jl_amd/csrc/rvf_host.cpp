@@ -945,13 +945,12 @@ static int pair_kind(const rvf_plan* p);
 // the occluded 13,824-element cube, mixed 250 / 240 / 234 ms at 16 / 8 / 4
 // columns); the other kernels amortise their per-tile set-up over
 // RVF_CHUNK columns.
-static bool queue_kind(int kind) { return kind >= RVF_K_FP64_QUEUE1 && kind <= RVF_K_MIX_SPLIT3; }
-static int default_columns(int kind) { return queue_kind(kind) ? RVF_CHUNK_QUEUE : RVF_CHUNK; }
+static int default_columns(int kind) { return rvf_queue_kind(kind) ? RVF_CHUNK_QUEUE : RVF_CHUNK; }
 static int plan_columns(const rvf_plan* p) { return p->cw_req ? p->cw_req : default_columns(pair_kind(p)); }
 // Lanes with an empty slot before a wave-queue refill.  Measured on the full
 // occluded cube / plates problems, mixed split 3: 16 -> 207.7 / 284.8 ms,
 // 32 -> 206.9 / 278.0 ms, 48 -> 213.5 / 284.1 ms.
-static int default_refill(int kind) { return kind >= RVF_K_MIX_SPLIT1 && kind <= RVF_K_MIX_SPLIT3 ? 32 : 16; }
+static int default_refill(int kind) { return rvf_mixed_split_kind(kind) ? 32 : 16; }
 // spatially compact strips for occluded quadrature solves (coherent ray bundles)
 static bool plan_spatial(const rvf_plan* p) {
     return p->obs && !p->obs->trees.empty() && p->opts.method == RVF_METHOD_QUAD;
@@ -1266,7 +1265,7 @@ static size_t pair_lds(const rvf_plan* p) {
     if (kind == RVF_K_MIX_OCC2D) return 0;
     if (kind == RVF_K_MIX_OCC_LANE) return 0;
     const size_t ring =
-        kind >= RVF_K_FP64_SPLIT1
+        rvf_split_kind(kind)
             ? (mixed ? (RVF_TQCAP_MIX + RVF_CQCAP) * sizeof(uint32_t) + 6 * 64 * sizeof(float) +
                            RVF_FRONTIER * 64 * sizeof(int32_t)
                      : RVF_TQCAP * sizeof(uint32_t))

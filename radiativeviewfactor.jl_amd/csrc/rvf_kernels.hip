@@ -2721,7 +2721,7 @@ constexpr int k_pair_waves(int kind) { return (kind == RVF_K_MIX_SPLIT3 || kind 
 // hardware hands a grid's workgroups to the XCDs in a fixed round robin).
 // Measured on the occluded 13,824-element cube: 265 -> 250 ms at 16 columns
 // per tile, 234 ms at 4 (FP64 730 -> 671 ms).
-constexpr bool k_pair_dynamic(int kind) { return kind >= RVF_K_FP64_QUEUE1 && kind <= RVF_K_MIX_SPLIT3; }
+constexpr bool k_pair_dynamic(int kind) { return rvf_queue_kind(kind); }
 
 template <class TAG>
 __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(k_pair_waves(TAG::kind)))) void k_pair(PairArgs a) {
