@@ -102,6 +102,7 @@ static_assert(RVF_K_MIX_SPLIT3 - RVF_K_MIX_SPLIT1 == 2, "mixed split kinds must 
 #define RVF_FRONTIER 16  // mixed split variants: subtrees a pair's rays start from (per lane, LDS)
 #define RVF_CQCAP 128    // mixed split variants: candidates for the exact FP64 test
 #define RVF_QSTATS 10    // per-wave statistics of the wave-queue kernels (LDS, u64)
+#define RVF_NCOUNTERS 80 // a plan's u64 counter slots: statistics, the tile claim counter (slot 15), 8 per-XCD claim counters
 // A split-variant triangle task packs (tag << RVF_TQ_TRI_BITS) | triangle into
 // 32 bits (tag = combination << 6 | lane < 2^13), so the split variants serve
 // obstruction sets of fewer than 2^19 triangles over all their trees.
@@ -128,6 +129,8 @@ struct PairArgs {
     int maxpts;               // per-element point stride in Q arrays
     int64_t N;                // elements
     const Chunk* chunks;
+    const int32_t* xorder;    // wave-queue kernels, whole launches: tile indices grouped by XCD, or null
+    int xseg[9];              // XCD x's tiles are xorder[xseg[x] .. xseg[x+1])
     const int32_t* slot_orig; // slot -> original element index (-1 = padding)
     const int32_t* elem_np;   // points per slot
     const int32_t* elem_grp;  // group rank per slot

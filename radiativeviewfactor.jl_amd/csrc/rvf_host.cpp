@@ -213,7 +213,9 @@ struct Layout {
     std::vector<int32_t> slot_orig, orig_slot, slot_grp, chunk_of;
     std::vector<Chunk> chunks;
     std::vector<int2> tiles;
-    DBuf d_slot_orig, d_slot_grp, d_chunks, d_chunk_of, d_tiles;
+    std::vector<int32_t> xorder;  // tile indices grouped by XCD (contiguous eighths of the strips)
+    int xseg[9] = {0};
+    DBuf d_slot_orig, d_slot_grp, d_chunks, d_chunk_of, d_tiles, d_xorder;
 };
 
 struct rvf_mesh {
@@ -297,6 +299,16 @@ static void layout_host(const rvf_mesh* m, bool spatial, bool self, int cw, Layo
         int64_t last = L.chunks[c].end - 1 + (self ? 1 : 0);
         for (int64_t s = 0; s * RVF_STRIP < last; s++) L.tiles.push_back(make_int2((int)s, (int)c));
     }
+    // the wave-queue kernels' claim order: XCD x takes the tiles of strips
+    // [x nstrips / 8, (x + 1) nstrips / 8), in tile-list order
+    const int64_t nst = std::max<int64_t>(1, NS / RVF_STRIP);
+    std::vector<int64_t> per(9, 0);
+    for (const int2& t : L.tiles) per[std::min<int64_t>(7, (int64_t)t.x * 8 / nst) + 1]++;
+    for (int x = 0; x < 8; x++) per[x + 1] += per[x];
+    for (int x = 0; x < 9; x++) L.xseg[x] = (int)per[x];
+    L.xorder.assign(L.tiles.size(), 0);
+    for (size_t t = 0; t < L.tiles.size(); t++)
+        L.xorder[per[std::min<int64_t>(7, (int64_t)L.tiles[t].x * 8 / nst)]++] = (int32_t)t;
 }
 
 // Layouts are cached per mesh and (spatial order, self-pair tiles, columns).
@@ -312,7 +324,8 @@ static int get_layout(const rvf_mesh* m, bool spatial, bool self, int cw, const 
             (rc = upload(L.d_slot_grp, L.slot_grp.data(), sizeof(int32_t) * NS, st)) ||
             (rc = upload(L.d_chunks, L.chunks.data(), sizeof(Chunk) * L.chunks.size(), st)) ||
             (rc = upload(L.d_chunk_of, L.chunk_of.data(), sizeof(int32_t) * NS, st)) ||
-            (rc = upload(L.d_tiles, L.tiles.data(), sizeof(int2) * L.tiles.size(), st)))
+            (rc = upload(L.d_tiles, L.tiles.data(), sizeof(int2) * L.tiles.size(), st)) ||
+            (rc = upload(L.d_xorder, L.xorder.data(), sizeof(int32_t) * L.xorder.size(), st)))
             return rc;
         L.built = true;
     }
@@ -1085,7 +1098,7 @@ static int plan_init(rvf_ctx* ctx, const rvf_mesh* mesh, const rvf_obs* obs, con
         (rc = src.get(p->partT, sizeof(double) * std::max<size_t>(nstrip, 1) * NS)) ||
         (rc = src.get(p->rowsum, sizeof(double) * NS)) || (rc = src.get(p->W, sizeof(double) * std::max(G, 1) * NS)) ||
         (self && (rc = src.get(p->diag, sizeof(double) * NS))) ||
-        (rc = src.get(p->counters, sizeof(unsigned long long) * 16)) ||
+        (rc = src.get(p->counters, sizeof(unsigned long long) * RVF_NCOUNTERS)) ||
         (rc = src.get(p->Tseg, sizeof(double) * RVF_RSEG * NS)) ||
         (rc = src.get(p->Wseg, sizeof(double) * RVF_RSEG * std::max(G, 1) * NS)))
         return rc;
@@ -1104,7 +1117,8 @@ static int plan_init(rvf_ctx* ctx, const rvf_mesh* mesh, const rvf_obs* obs, con
         ((rc = src.get(own->d_slot_orig, sizeof(int32_t) * NS)) || (rc = src.get(own->d_slot_grp, sizeof(int32_t) * NS)) ||
          (rc = src.get(own->d_chunks, sizeof(Chunk) * own->chunks.size())) ||
          (rc = src.get(own->d_chunk_of, sizeof(int32_t) * NS)) ||
-         (rc = src.get(own->d_tiles, sizeof(int2) * own->tiles.size()))))
+         (rc = src.get(own->d_tiles, sizeof(int2) * own->tiles.size())) ||
+         (rc = src.get(own->d_xorder, sizeof(int32_t) * own->xorder.size()))))
         return rc;
     if ((opts->flags & RVF_AUDIT) && obs) {
         if (scratch) return fail(RVF_E_ARG, "RVF_AUDIT needs device allocations");
@@ -1139,7 +1153,8 @@ static int plan_init(rvf_ctx* ctx, const rvf_mesh* mesh, const rvf_obs* obs, con
          (rc = h2d(own->d_slot_grp, own->slot_grp.data(), sizeof(int32_t) * NS)) ||
          (rc = h2d(own->d_chunks, own->chunks.data(), sizeof(Chunk) * own->chunks.size())) ||
          (rc = h2d(own->d_chunk_of, own->chunk_of.data(), sizeof(int32_t) * NS)) ||
-         (rc = h2d(own->d_tiles, own->tiles.data(), sizeof(int2) * own->tiles.size()))))
+         (rc = h2d(own->d_tiles, own->tiles.data(), sizeof(int2) * own->tiles.size())) ||
+         (rc = h2d(own->d_xorder, own->xorder.data(), sizeof(int32_t) * own->xorder.size()))))
         return rc;
     if (scratch) {
         // the host arrays above are released on return: finish the copies first
@@ -1177,6 +1192,10 @@ static PairArgs pair_args(const rvf_plan* p) {
     a.slot_orig = L->d_slot_orig.as<int32_t>();
     a.NS = m->NS; a.maxpts = p->maxpts; a.N = m->N;
     a.chunks = L->d_chunks.as<Chunk>();
+    // per-XCD claims for the mixed wave-queue kernels (the FP64 ones measured
+    // 1-2 % slower with them: 673 -> 681 ms on the occluded cube)
+    a.xorder = p->opts.precision == RVF_MIXED ? L->d_xorder.as<int32_t>() : nullptr;
+    for (int x = 0; x < 9; x++) a.xseg[x] = L->xseg[x];
     a.elem_np = p->elem_np.as<int32_t>(); a.elem_grp = L->d_slot_grp.as<int32_t>(); a.area = p->area.as<double>();
     a.P64 = p->P64.as<double>(); a.Q64 = p->Q64.as<double>(); a.P32 = p->P32.as<float>(); a.Q32 = p->Q32.as<float>();
     a.anchor = p->anchor.as<double>();
@@ -1203,7 +1222,7 @@ static PairArgs pair_args(const rvf_plan* p) {
     }
     a.audit_count = p->audit_c.as<uint32_t>(); a.audit_hash = p->audit_h.as<uint32_t>();
     a.counters = p->counters.as<unsigned long long>();
-    a.tile_ctr = (int*)(a.counters + 15);  // slots 0..10 hold the statistics
+    a.tile_ctr = (int*)(a.counters + 15);  // slots 0..10 hold the statistics; per-XCD counters follow
     a.coords = m->coords.as<double>(); a.fam = m->d_fam.as<int8_t>(); a.nodes8 = m->d_nodes8.as<int32_t>();
     a.duffy = p->opts.method == RVF_METHOD_DUFFY ? 1 : 0;
     a.mc_nsamp = p->opts.n_samples;
@@ -1495,6 +1514,7 @@ int rvf_plan_autotune(rvf_plan* p, int64_t sample_tiles, rvf_tune_entry* ent, in
         const int64_t cnt = want >= nt ? nt : want;  // (cnt - 1) * step < nt
         a.tile0 = 0;
         a.tile_step = (int)step;
+        a.xorder = nullptr;  // a strided sample claims from one counter
         a.ntiles = (int)std::min<int64_t>(cnt, 256);  // first launch of a kernel loads its code
         e = rvf_launch_pair(&a, kind, lds, st);
         HIPCHK(hipEventRecord(p->ev[6], st));

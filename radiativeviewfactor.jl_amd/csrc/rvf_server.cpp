@@ -198,7 +198,7 @@ size_t estimate_bytes(const Req& r) {
     b += (size_t)held_triangles(r) * 512;                                // trees, triangles, items
     b += (size_t)NS * maxpts * 8 * (8 + 8 + 4 + 4);                      // P64 Q64 P32 Q32
     b += (size_t)NS * 72;                                                // anchors, areas, sums, diagonal
-    b += (size_t)tiles * RVF_STRIP * 8 + (size_t)(NS / RVF_STRIP) * NS * 8;  // part, partT
+    b += (size_t)tiles * (RVF_STRIP * 8 + 12) + (size_t)(NS / RVF_STRIP) * NS * 8;  // part, tile lists, partT
     if (r.opts.flags & RVF_WANT_F) b += (size_t)r.N * r.N * 8;
     if (r.opts.flags & RVF_AUDIT) b += (size_t)r.N * r.N * 8;
     return b + b / 8 + (1u << 20);
