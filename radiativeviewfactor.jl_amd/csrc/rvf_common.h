@@ -29,6 +29,18 @@ struct __align__(16) BvhNode32 {
     int32_t start_count;  // leaves: (start << 4) | count (count <= 15); interior: right child << 4
 };
 
+// Node of the mixed traversal trees (32 B), padded FP32 bounds.  Leaves hold
+// one triangle: link = ~triangle (negative); interior: link = right child
+// (the left child is the next node).  exit = the preorder index after the
+// node's subtree, INT32_MAX on the tree's right spine, so a walk that
+// leaves a subtree always lands at or past that subtree's own exit.
+struct __align__(16) TNode32 {
+    float lo[3];
+    int32_t exit;
+    float hi[3];
+    int32_t link;
+};
+
 // Triangle for Moller-Trumbore, FP64: v0, e1 = v1 - v0, e2 = v2 - v0 as the
 // reference's reading of the soup yields them (72 B, padded to 80).
 struct __align__(16) Tri64 {
@@ -160,7 +172,7 @@ struct PairArgs {
     // SAH, small leaves), FP32 padded boxes; triangles in its leaf order and
     // their indices in the reference trees' order (exact test, leaf check)
     const int32_t* ttree_root; // [G*G] root node or -1
-    const BvhNode32* tnodes32;
+    const TNode32* tnodes32;
     const Tri32* ttris32;
     const int32_t* tmap;
     int n_nodes;              // M: items below M are nodes

@@ -776,7 +776,7 @@ int rvf_obs_build(rvf_ctx* ctx, const rvf_mesh* mesh, const double* soup, const 
         }
     }
     // traversal trees of the mixed split kernels, one per reference tree
-    std::vector<BvhNode32> tn32;
+    std::vector<TNode32> tn32;
     std::vector<Tri32> tt32(std::max<int64_t>(o->n_tris, 1));
     std::vector<int32_t> tmap(std::max<int64_t>(o->n_tris, 1), 0);
     std::map<int64_t, int32_t> troot_of;  // reference root -> traversal root
@@ -802,11 +802,11 @@ int rvf_obs_build(rvf_ctx* ctx, const rvf_mesh* mesh, const double* soup, const 
         if (base + (int64_t)sb.nodes.size() >= ((int64_t)1 << 27)) { delete o; return fail(RVF_E_ARG, "rvf_obs_build: too many nodes"); }
         for (size_t k = 0; k < sb.nodes.size(); k++) {
             const HostNode& h = sb.nodes[k];
-            BvhNode32 f;
+            TNode32 f;
             for (int c = 0; c < 3; c++) { f.lo[c] = f32_down(h.lo[c]) - pad; f.hi[c] = f32_up(h.hi[c]) + pad; }
-            f.miss = miss[k] ? (int32_t)(base + miss[k] - 1) : -1;
-            f.start_count = h.count ? (int32_t)(((tr.tri_base + h.start - 1) << 4) | h.count)
-                                    : (int32_t)((base + h.right - 1) << 4);
+            f.exit = miss[k] ? (int32_t)(base + miss[k] - 1) : INT32_MAX;
+            if (h.count > 1) { delete o; return fail(RVF_E_STATE, "bvh: traversal leaf with more than one triangle"); }
+            f.link = h.count ? ~(int32_t)(tr.tri_base + h.start - 1) : (int32_t)(base + h.right - 1);
             if (h.left && h.left != (int)k + 2) { delete o; return fail(RVF_E_STATE, "bvh: non-preorder layout"); }
             tn32.push_back(f);
         }
@@ -863,7 +863,7 @@ int rvf_obs_build(rvf_ctx* ctx, const rvf_mesh* mesh, const double* soup, const 
         (rc = upload(o->d_t32, t32.data(), sizeof(Tri32) * t32.size(), st)) ||
         (rc = upload(o->d_parent, parent.data(), sizeof(int32_t) * parent.size(), st)) ||
         (rc = upload(o->d_troot, o->troot.data(), sizeof(int32_t) * std::max(1, G * G), st)) ||
-        (rc = upload(o->d_tn32, tn32.data(), sizeof(BvhNode32) * tn32.size(), st)) ||
+        (rc = upload(o->d_tn32, tn32.data(), sizeof(TNode32) * tn32.size(), st)) ||
         (rc = upload(o->d_tt32, tt32.data(), sizeof(Tri32) * tt32.size(), st)) ||
         (rc = upload(o->d_tmap, tmap.data(), sizeof(int32_t) * tmap.size(), st))) {
         delete o;
@@ -1209,7 +1209,7 @@ static PairArgs pair_args(const rvf_plan* p) {
         a.nodes32 = p->obs->d_n32.as<BvhNode32>(); a.tris32 = p->obs->d_t32.as<Tri32>();
         a.parent = p->obs->d_parent.as<int32_t>();
         a.items32 = p->obs->d_items32.as<Item32>(); a.items64 = p->obs->d_items64.as<Item64>();
-        a.ttree_root = p->obs->d_troot.as<int32_t>(); a.tnodes32 = p->obs->d_tn32.as<BvhNode32>();
+        a.ttree_root = p->obs->d_troot.as<int32_t>(); a.tnodes32 = p->obs->d_tn32.as<TNode32>();
         a.ttris32 = p->obs->d_tt32.as<Tri32>(); a.tmap = p->obs->d_tmap.as<int32_t>();
         a.n_nodes = (int)p->obs->n_nodes;
         a.refill_min = p->refill_min ? p->refill_min : default_refill(pair_kind(p));

@@ -1364,23 +1364,25 @@ __device__ __forceinline__ int nstep(const BvhNode64& nd, int cur, const NRay<fa
     cnt = (pass && leaf) ? nd.count : 0;
     return (pass && !leaf) ? cur + 1 : nd.miss;
 }
-__device__ __forceinline__ int nstep(const BvhNode32& nd, int cur, const NRay<true>& N, int& first, int& cnt) {
+// Mixed traversal tree: single-triangle leaves, exits instead of miss links
+// (the next node is never negative; a walk past the tree lands at INT32_MAX).
+__device__ __forceinline__ int nstep(const TNode32& nd, int cur, const NRay<true>& N, int& first, int& cnt) {
     const float t1x = fmaf(nd.lo[0], N.ix, N.nx), t2x = fmaf(nd.hi[0], N.ix, N.nx);
     const float t1y = fmaf(nd.lo[1], N.iy, N.ny), t2y = fmaf(nd.hi[1], N.iy, N.ny);
     const float t1z = fmaf(nd.lo[2], N.iz, N.nz), t2z = fmaf(nd.hi[2], N.iz, N.nz);
     const float tmin = fmaxf(fmaxf(fmaxf(fminf(t1x, t2x), fminf(t1y, t2y)), fminf(t1z, t2z)), 0.0f);
     const float tmax = fminf(fminf(fminf(fmaxf(t1x, t2x), fmaxf(t1y, t2y)), fmaxf(t1z, t2z)), N.L);
     const bool pass = tmin <= tmax;
-    const int c = nd.start_count & 15;
-    first = nd.start_count >> 4;
-    cnt = (pass && c > 0) ? c : 0;
-    return (pass && c == 0) ? cur + 1 : nd.miss;
+    const bool leaf = nd.link < 0;
+    first = ~nd.link;
+    cnt = (pass && leaf) ? 1 : 0;
+    return (pass && !leaf) ? cur + 1 : nd.exit;
 }
 
 template <bool MIX>
-__device__ __forceinline__ const typename std::conditional<MIX, BvhNode32, BvhNode64>::type* nnodes(const PairArgs& a);
+__device__ __forceinline__ const typename std::conditional<MIX, TNode32, BvhNode64>::type* nnodes(const PairArgs& a);
 template <>
-__device__ __forceinline__ const BvhNode32* nnodes<true>(const PairArgs& a) { return a.tnodes32; }  // traversal tree
+__device__ __forceinline__ const TNode32* nnodes<true>(const PairArgs& a) { return a.tnodes32; }  // traversal tree
 template <>
 __device__ __forceinline__ const BvhNode64* nnodes<false>(const PairArgs& a) { return a.nodes64; }
 
@@ -1455,15 +1457,16 @@ __device__ __forceinline__ int tclass(const PairArgs& a, const SRay& S, int tri)
     return rvf_tri_classify(T.v0, T.e1, T.e2, T.E1, T.E2, R, 80.0f * uu * a.Sm);
 }
 
-// Mixed split traversal, after a node step of a slot: the first step in a
-// subtree reads its root, whose miss link ends the subtree's preorder range;
-// a walk that leaves the range (or the tree) moves on to the pair's next
-// frontier subtree, or ends after the last.  The slot's tag holds the
-// frontier index (bits 16-19) and count (bits 20-24).  Branch-free: the next
-// entry is read from LDS whether or not it is taken.
-__device__ __forceinline__ void frontier_next(const QCtx& X, int miss, int& cur, int& tag, int& end) {
-    end = end == -2 ? (miss >= 0 ? miss : 0x7fffffff) : end;
-    const bool leave = cur < 0 || cur >= end;
+// Mixed split traversal, after a node step of a slot (cur = the step's next
+// node, never negative): the first step in a subtree reads its root, whose
+// exit ends the subtree's preorder range; a walk that leaves the range (or
+// the tree, INT32_MAX) moves on to the pair's next frontier subtree, or ends
+// (cur = -1) after the last.  The slot's tag holds the frontier index (bits
+// 16-19) and count (bits 20-24).  Branch-free: the next entry is read from
+// LDS whether or not it is taken.
+__device__ __forceinline__ void frontier_next(const QCtx& X, int exit, int& cur, int& tag, int& end) {
+    end = end == -2 ? exit : end;
+    const bool leave = cur >= end;
     const int k = ((tag >> 16) & 15) + 1;
     const int nxt = X.lfr[min(k, RVF_FRONTIER - 1) * 64 + (tag & 63)];
     const bool more = leave && k < (tag >> 20);
@@ -1577,13 +1580,14 @@ __device__ __forceinline__ void split_drain(const PairArgs& a, const QCtx& X, in
                 if (!MIX) K.c0 += act;
                 K.steps += act;
                 cnt[s] = act ? cnt[s] : 0;
-                cur[s] = act ? nx : -1;
-                if (MIX) {
-                    int c2 = cur[s], t2 = tag[s], e2 = end[s];
-                    frontier_next(X, nd[s].miss, c2, t2, e2);
-                    cur[s] = act ? c2 : cur[s];
+                if constexpr (MIX) {
+                    int c2 = nx, t2 = tag[s], e2 = end[s];
+                    frontier_next(X, nd[s].exit, c2, t2, e2);
+                    cur[s] = act ? c2 : -1;
                     tag[s] = act ? t2 : tag[s];
                     end[s] = act ? e2 : end[s];
+                } else {
+                    cur[s] = act ? nx : -1;
                 }
             }
             // mixed: a slot whose step queued no leaf takes one more step, so
@@ -1591,7 +1595,7 @@ __device__ __forceinline__ void split_drain(const PairArgs& a, const QCtx& X, in
             // spread over more steps while each slot still queues at most
             // one leaf per iteration (the ring bound).  Measured: 2 steps
             // -5 % kernel time, 3 and 4 no better.
-            if (MIX) {
+            if constexpr (MIX) {
 #pragma unroll
                 for (int s = 0; s < S; s++)
                     nd[s] = *(decltype(&nodes[0]))((const char*)nodes + (uint32_t)max(cur[s], 0) * (uint32_t)sizeof(nd[0]));
@@ -1602,7 +1606,7 @@ __device__ __forceinline__ void split_drain(const PairArgs& a, const QCtx& X, in
                     const int nx = nstep(nd[s], max(cur[s], 0), R[s], f2, c2);
                     K.steps += act;
                     int cx = nx, tx = tag[s], ex = end[s];
-                    frontier_next(X, nd[s].miss, cx, tx, ex);
+                    frontier_next(X, nd[s].exit, cx, tx, ex);
                     first[s] = act ? f2 : first[s];
                     cnt[s] = act ? c2 : cnt[s];
                     cur[s] = act ? cx : cur[s];
@@ -1697,49 +1701,19 @@ __device__ __forceinline__ void split_drain(const PairArgs& a, const QCtx& X, in
     }
 }
 
-// The subtrees a pair's rays walk (mixed split traversal): the maximal nodes
-// of the traversal tree whose padded box the pair's capsule meets and whose
-// subtree holds at most `lmax` leaves, in preorder, written to out[k * 64]
-// (LDS, one column per lane); the finest such cut of at most RVF_FRONTIER
-// subtrees over lmax = 2, 8, 32, ... (RVF_FRONTIER <= 16: the count takes
-// 5 bits of a slot's tag)  Every triangle a ray of the pair can
-// reach lies in a leaf box the capsule meets, and every such leaf lies in one
-// of these subtrees, so walking each subtree from its root (stopping at its
-// root's miss link) visits every leaf the ray's walk from the root would pass.
-// Returns their number (0: the capsule meets no leaf box, nothing can block).
-// After `limit` box tests the walk stops and the root alone is returned.
-// Trees are preorder with miss links; an interior node's subtree spans the
-// nodes up to its miss link (2 lmax - 1 nodes hold lmax leaves), a node on the
-// tree's rightmost path (miss -1) is always descended.
+// The subtrees a pair's rays walk (mixed split traversal): a cut of the
+// traversal tree of at most RVF_FRONTIER nodes whose padded boxes the pair's
+// capsule meets, in preorder, written to out[k * 64] (LDS, one column per
+// lane; RVF_FRONTIER <= 16: the count takes 5 bits of a slot's tag).  Every
+// triangle a ray of the pair can reach lies in a leaf box the capsule meets,
+// and every such leaf lies in one of these subtrees, so walking each subtree
+// from its root to its root's exit visits every leaf the ray's walk from the
+// root would pass.  Returns their number (0: the capsule meets no leaf box,
+// nothing can block).  After `limit` box tests the walk stops and the root
+// alone is returned.
 __device__ __forceinline__ int capsule_frontier(const double* ai, const double* aj, double R,
-                                                const BvhNode32* __restrict__ nodes, int root, int limit, int* out,
+                                                const TNode32* __restrict__ nodes, int root, int limit, int* out,
                                                 float Sm) {
-#ifdef RVF_FRONTIER_FP64
-    double inv[3];
-    bool flat[3];
-#pragma unroll
-    for (int c = 0; c < 3; c++) {
-        const double d = aj[c] - ai[c];
-        flat[c] = d == 0.0;
-        inv[c] = flat[c] ? 0.0 : 1.0 / d;
-    }
-    auto meets = [&](const BvhNode32& nd) {
-        double tmin = 0.0, tmax = 1.0;
-#pragma unroll
-        for (int c = 0; c < 3; c++) {
-            const double lo = (double)nd.lo[c] - R, hi = (double)nd.hi[c] + R;
-            if (flat[c]) {
-                if (ai[c] < lo || ai[c] > hi) tmax = -1.0;
-            } else {
-                double t1 = (lo - ai[c]) * inv[c], t2 = (hi - ai[c]) * inv[c];
-                if (t1 > t2) { const double t = t1; t1 = t2; t2 = t; }
-                tmin = fmax(tmin, t1);
-                tmax = fmin(tmax, t2);
-            }
-        }
-        return tmin <= tmax;
-    };
-#else
     // FP32 slab test of the segment a_i -> a_j (t in [0, 1]) against the box
     // grown by R, in the traversal's FMA form.  The boxes are padded outward
     // by 128 u Sm; the anchors' rounding to FP32 (u Sm), the FMA form's
@@ -1756,7 +1730,7 @@ __device__ __forceinline__ int capsule_frontier(const double* ai, const double* 
         iv[c] = fminf(fmaxf(__builtin_amdgcn_rcpf(d), -1e30f), 1e30f);
         nn[c] = -o[c] * iv[c];
     }
-    auto meets = [&](const BvhNode32& nd) {
+    auto meets = [&](const TNode32& nd) {
         float tmin = 0.0f, tmax = 1.0001f;
 #pragma unroll
         for (int c = 0; c < 3; c++) {
@@ -1766,29 +1740,6 @@ __device__ __forceinline__ int capsule_frontier(const double* ai, const double* 
         }
         return tmin <= tmax;
     };
-#endif
-#ifdef RVF_FRONTIER_RESTART
-    int tests = 0;
-    for (int lmax = 2;; lmax *= 4) {
-        int k = 0;
-        bool over = false;
-        for (int n = root; n >= 0;) {
-            if (tests++ >= limit) { out[0] = root; return 1; }
-            const BvhNode32 nd = nodes[n];
-            if (!meets(nd)) { n = nd.miss; continue; }
-            const bool leaf = (nd.start_count & 15) != 0;
-            if (leaf || (nd.miss >= 0 && nd.miss - n <= 2 * lmax - 1)) {
-                if (k == RVF_FRONTIER) { over = true; break; }
-                out[k * 64] = n;
-                k++;
-                n = nd.miss;
-            } else {
-                n = n + 1;
-            }
-        }
-        if (!over) return k;
-    }
-#else
     // Breadth first: each round replaces every open entry by the children
     // its capsule meets, as long as the list keeps at most RVF_FRONTIER
     // entries.  A round runs from the last entry to the first and writes the
@@ -1806,11 +1757,11 @@ __device__ __forceinline__ int capsule_frontier(const double* ai, const double* 
         for (int u = k - 1; u >= 0; u--) {
             const int e = out[u * 64];
             if (e >= 0) {
-                const int sc = nodes[e].start_count;
-                if ((sc & 15) == 0) {  // interior: children n + 1 and sc >> 4
+                const int lk = nodes[e].link;
+                if (lk >= 0) {  // interior: children e + 1 and lk
                     if (tests + 2 > limit) { out[0] = root; return 1; }
                     tests += 2;
-                    const int l = e + 1, r = sc >> 4;
+                    const int l = e + 1, r = lk;
                     const bool ml = meets(nodes[l]), mr = meets(nodes[r]);
                     if ((int)ml + (int)mr <= w - u) {
                         if (mr) out[--w * 64] = r;
@@ -1828,7 +1779,6 @@ __device__ __forceinline__ int capsule_frontier(const double* ai, const double* 
     // the last round found every entry closed
     for (int v = 0; v < k; v++) out[v * 64] = ~out[v * 64];
     return k;
-#endif
 }
 
 // Can a ray between a point of element i and a point of element j reach the
