@@ -940,6 +940,14 @@ __device__ __forceinline__ void tile_mixed_occ(const PairArgs& a, int tile) {
 __device__ __forceinline__ int lane_rank(uint64_t m) {
     return (int)__builtin_amdgcn_mbcnt_hi((uint32_t)(m >> 32), __builtin_amdgcn_mbcnt_lo((uint32_t)m, 0u));
 }
+// Point data of a column element, which no pair kernel writes, read at
+// wave-uniform addresses: through a constant-address-space pointer the loads
+// go to the scalar cache and land in SGPRs.
+template <class T>
+using cptr = const __attribute__((address_space(4))) T*;
+template <class T>
+__device__ __forceinline__ cptr<T> as_const(const T* p) { return (cptr<T>)p; }
+
 __device__ __forceinline__ uint64_t uniform64(uint64_t x) {
     uint32_t lo = __builtin_amdgcn_readfirstlane((uint32_t)x);
     uint32_t hi = __builtin_amdgcn_readfirstlane((uint32_t)(x >> 32));
@@ -1049,6 +1057,7 @@ struct QCount {
     uint32_t c0 = 0, c1 = 0;     // FP64: box tests, triangle tests; mixed: exact re-tests, certain hits
     uint32_t iters = 0;          // traversal loop iterations (uniform)
     uint32_t steps = 0;          // slot steps that advanced a ray
+    uint32_t wsteps = 0;         // the same as a wave total (uniform; split kernels: a ballot's popcount, no VALU)
     uint32_t tbatch = 0;         // triangle-task batches (uniform)
     uint32_t ebatch = 0;         // exact-test batches (uniform)
     // adds the drain's wave totals to the per-wave block (every lane active)
@@ -1059,7 +1068,7 @@ struct QCount {
 #pragma unroll
             for (int e = 0; e < 3; e++) v[e] += __shfl_xor(v[e], o, 64);
         if (lane == 0) {
-            w[QS_C0] += v[0]; w[QS_C1] += v[1]; w[QS_STEPS] += v[2];
+            w[QS_C0] += v[0]; w[QS_C1] += v[1]; w[QS_STEPS] += v[2] + wsteps;
             w[QS_ITERS] += iters; w[QS_TBATCH] += tbatch; w[QS_EBATCH] += ebatch;
         }
     }
@@ -1578,7 +1587,7 @@ __device__ __forceinline__ void split_drain(const PairArgs& a, const QCtx& X, in
                 const bool act = cur[s] >= 0;
                 const int nx = nstep(nd[s], max(cur[s], 0), R[s], first[s], cnt[s]);
                 if (!MIX) K.c0 += act;
-                K.steps += act;
+                K.wsteps += (uint32_t)__popcll(__ballot(act));
                 cnt[s] = act ? cnt[s] : 0;
                 if constexpr (MIX) {
                     int c2 = nx, t2 = tag[s], e2 = end[s];
@@ -1604,7 +1613,7 @@ __device__ __forceinline__ void split_drain(const PairArgs& a, const QCtx& X, in
                     const bool act = cur[s] >= 0 && cnt[s] == 0;
                     int f2 = 0, c2 = 0;
                     const int nx = nstep(nd[s], max(cur[s], 0), R[s], f2, c2);
-                    K.steps += act;
+                    K.wsteps += (uint32_t)__popcll(__ballot(act));
                     int cx = nx, tx = tag[s], ex = end[s];
                     frontier_next(X, nd[s].exit, cx, tx, ex);
                     first[s] = act ? f2 : first[s];
@@ -2035,8 +2044,8 @@ __device__ __forceinline__ void tile_occ_queue(const PairArgs& a, int tile, doub
         lhint[lane] = -1;
         X.pch = max(1, RVF_QCOMBO / npj);
         X.magic = (65536u + (uint32_t)npj - 1u) / (uint32_t)npj;
-        const double* __restrict__ qj64 = a.Q64 + j * a.maxpts * 8;
-        const float* __restrict__ qj32 = a.Q32 + j * a.maxpts * 8;
+        const cptr<double> qj64 = as_const(a.Q64 + j * a.maxpts * 8);
+        const cptr<float> qj32 = as_const(a.Q32 + j * a.maxpts * 8);
         if (!MIX)
             for (int q = 0; q < npj; q++) accq[q * 64 + lane] = 0.0;
         // mixed mode: anchored offsets of the column element
@@ -2050,7 +2059,8 @@ __device__ __forceinline__ void tile_occ_queue(const PairArgs& a, int tile, doub
             X.fjx = (float)a.anchor[j * 4 + 0]; X.fjy = (float)a.anchor[j * 4 + 1]; X.fjz = (float)a.anchor[j * 4 + 2];
         }
         double Fij = 0.0, Fs = 0.0, Fc = 0.0;
-        uint32_t bc = 0, bh = 0;
+        uint32_t bc = 0, bh = 0;  // audit: blocked rays of the pair and their order-free hash
+        const bool aud = a.audit_count != nullptr;
         // when no pair of the wave can reach an occluder (all culled), the need
         // masks and the trace are skipped; the sums below count the rays
         // instead, from the same kernel values
@@ -2132,8 +2142,9 @@ __device__ __forceinline__ void tile_occ_queue(const PairArgs& a, int tile, doub
                             const bool nd = act && p < npi && root >= 0 && k != 0.0f;
                             wcl += __popcll(__ballot(nd));
                         }
-                        if ((blk[2 * c + (lane >> 5)] >> (lane & 31)) & 1u) {
-                            k = 0.0f;
+                        const bool blocked = (blk[2 * c + (lane >> 5)] >> (lane & 31)) & 1u;
+                        k = blocked ? 0.0f : k;
+                        if (aud && blocked) {
                             bc++;
                             bh += mix32((uint32_t)(rev ? q * npi + p : p * npj + q) + 0x9e3779b9U);
                         }
@@ -2160,8 +2171,9 @@ __device__ __forceinline__ void tile_occ_queue(const PairArgs& a, int tile, doub
                             const bool nd = act && p < npi && root >= 0 && K != 0.0;
                             wcl += __popcll(__ballot(nd));
                         }
-                        if ((blk[2 * c + (lane >> 5)] >> (lane & 31)) & 1u) {
-                            K = 0.0;
+                        const bool blocked = (blk[2 * c + (lane >> 5)] >> (lane & 31)) & 1u;
+                        K = blocked ? 0.0 : K;
+                        if (aud && blocked) {
                             bc++;
                             bh += mix32((uint32_t)(rev ? q * npi + p : p * npj + q) + 0x9e3779b9U);
                         }
