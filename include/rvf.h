@@ -168,7 +168,7 @@ int rvf_plan_device_ptr(const rvf_plan* plan, int which, void** ptr);
 /* Tuning knobs of an existing plan (no counterpart in the reference; used by
  * the autotuner): traversal variant (RVF_VARIANT_*, 0 = default) and the
  * wave-queue refill threshold (lanes with an empty slot before a refill,
- * 1..64; 0 = the variant's default: 32 for the mixed split variants, 16
+ * 1..64; 0 = the variant's default: 24 for the mixed split variants, 16
  * otherwise).  F, A and every visibility decision do not depend on either. */
 int rvf_plan_tune(rvf_plan* plan, int32_t variant, int32_t refill_min);
 /* Columns (column elements) per pair tile, 1..16; 0 = the pair kernel's
@@ -195,7 +195,7 @@ typedef struct {
  * of the plan's current layout, 0 = 8192; the same share of the pairs for
  * every width), times the library default twice and the three fastest others
  * once over the whole problem (or four times the share when the problem is
- * larger), tries the refill thresholds of 8, 16 and 32 other than the
+ * larger), tries the refill thresholds of 8, 16, 24 and 32 other than the
  * choice's default for it, then its tile widths of 4, 8 and 16 columns, and
  * keeps the default unless another configuration is faster by more than twice
  * the default's run-to-run spread (at least 1 %), applied as rvf_plan_tune

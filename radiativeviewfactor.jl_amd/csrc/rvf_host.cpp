@@ -963,7 +963,7 @@ static int plan_columns(const rvf_plan* p) { return p->cw_req ? p->cw_req : defa
 // Lanes with an empty slot before a wave-queue refill.  Measured on the full
 // occluded cube / plates problems, mixed split 3: 16 -> 207.7 / 284.8 ms,
 // 32 -> 206.9 / 278.0 ms, 48 -> 213.5 / 284.1 ms.
-static int default_refill(int kind) { return rvf_mixed_split_kind(kind) ? 32 : 16; }
+static int default_refill(int kind) { return rvf_mixed_split_kind(kind) ? 24 : 16; }
 // spatially compact strips for occluded quadrature solves (coherent ray bundles)
 static bool plan_spatial(const rvf_plan* p) {
     return p->obs && !p->obs->trees.empty() && p->opts.method == RVF_METHOD_QUAD;
@@ -1583,7 +1583,7 @@ int rvf_plan_autotune(rvf_plan* p, int64_t sample_tiles, rvf_tune_entry* ent, in
         const int bv = best_v;
         p->opts.variant = bv;
         const int rdef = default_refill(pair_kind(p));
-        for (int refill : {8, 16, 32}) {
+        for (int refill : {8, 16, 24, 32}) {
             if (refill == rdef) continue;
             float ms = 0;
             if ((rc = measure(bv, refill, 0, big, &ms))) break;

@@ -39,12 +39,12 @@ def _default_columns(variant):
 
 
 def _default_refill(variant, prec):
-    # lanes waiting before a refill: 32 for the mixed split variants, 16 for
+    # lanes waiting before a refill: 24 for the mixed split variants, 16 for
     # the other wave-queue variants (the lane variant has none)
     if variant == api.VARIANT_LANE:
         return 0
     split = variant in (api.VARIANT_SPLIT(1), api.VARIANT_SPLIT(2), api.VARIANT_SPLIT(3))
-    return 32 if (prec == "mixed" and split) else 16
+    return 24 if (prec == "mixed" and split) else 16
 
 
 @pytest.mark.parametrize("prec", ["fp64", "mixed"])
@@ -63,8 +63,8 @@ def test_autotune_measures_all_variants_and_keeps_results(prec, tmp_path):
     # the first len(variants) entries rank the sample, each variant at its
     # default tile width and refill threshold; then the library default
     # (split 3 mixed, queue 1 FP64) is timed twice on the larger set, then the
-    # three fastest others, then the choice's other refill thresholds of 8, 16
-    # and 32 (wave-queue variants), then its other tile widths of 4, 8 and 16
+    # three fastest others, then the choice's other refill thresholds of 8, 16,
+    # 24 and 32 (wave-queue variants), then its other tile widths of 4, 8 and 16
     # columns.  Another
     # configuration replaces the current choice only if it is faster than the
     # best so far by more than twice the default's spread (at least 1 %).
@@ -86,13 +86,14 @@ def test_autotune_measures_all_variants_and_keeps_results(prec, tmp_path):
             best_ms, choice = e["ms"], (e["variant"], rd(e["variant"]), 0)
     k = 5
     if choice[0] != api.VARIANT_LANE:
-        assert sorted(e["refill_min"] for e in again[k:k + 2]) == sorted({8, 16, 32} - {rd(choice[0])})
-        for e in again[k:k + 2]:
+        others = sorted({8, 16, 24, 32} - {rd(choice[0])})
+        assert sorted(e["refill_min"] for e in again[k:k + len(others)]) == others
+        for e in again[k:k + len(others)]:
             assert e["variant"] == choice[0]
             assert e["columns"] == _default_columns(choice[0])
             if e["ms"] < best_ms * (1 - margin):
                 best_ms, choice = e["ms"], (e["variant"], e["refill_min"], 0)
-        k += 2
+        k += len(others)
     widths = [e["columns"] for e in again[k:]]
     assert sorted(widths) == sorted({4, 8, 16} - {_default_columns(choice[0])})
     for e in again[k:]:
