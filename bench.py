@@ -72,7 +72,7 @@ def run_mode(api, ctx, dm, do, nquad, precision, steps, warmup, audit=False, tun
         # configuration over a tile sample and keep the fastest
         t0 = time.perf_counter()
         v, r, meas = plan.autotune(sample_tiles=tune["sample_tiles"])
-        tune.update(variant=v, refill_min=r, columns=plan.columns, kernel=plan.kernel_name(),
+        tune.update(variant=v, refill_min=r, columns=plan.columns, waves=plan.waves, kernel=plan.kernel_name(),
                     seconds=time.perf_counter() - t0, measurements=meas)
     for _ in range(warmup):
         plan.run(); plan.sync()
@@ -112,6 +112,7 @@ def graph_steady_state(api, ctx, dm, do, nquad, precision, tune, reps=10):
     if tune is not None and do is not None and "variant" in tune:
         plan.tune(tune["variant"], tune["refill_min"])
         plan.tune_tiles(tune["columns"])
+        plan.tune_waves(tune["waves"])
     plan.run(); plan.sync()  # capture
     ms = []
     for _ in range(reps):
@@ -246,6 +247,7 @@ def main():
             if tune_d and "variant" in tune_d:
                 pa.tune(tune_d["variant"], tune_d["refill_min"])
                 pa.tune_tiles(tune_d["columns"])
+                pa.tune_waves(tune_d["waves"])
             pa.run(); pa.sync()
             *_, ca, ha = pa.fetch(F=False)
             ver["audited_kernels"] = {"fp64": pa.kernel_name()}
@@ -254,6 +256,7 @@ def main():
             if tune_m and "variant" in tune_m:
                 pb.tune(tune_m["variant"], tune_m["refill_min"])
                 pb.tune_tiles(tune_m["columns"])
+                pb.tune_waves(tune_m["waves"])
             pb.run(); pb.sync()
             *_, cb, hb = pb.fetch(F=False)
             ver["audited_kernels"]["mixed"] = pb.kernel_name()

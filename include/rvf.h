@@ -98,6 +98,9 @@ typedef struct rvf_stats {
     int64_t culled_waves;
     /* columns (column elements) per pair tile of the plan's layout */
     int64_t tile_columns;
+    /* pair-kernel build the plan launches (rvf_plan_tune_waves): 3 = the mixed
+     * split kernels' three-waves-per-SIMD build, 0 = the kernel's default */
+    int64_t waves;
 } rvf_stats;
 
 /* --- lifecycle (no reference counterpart: the reference is one Julia process) */
@@ -177,6 +180,14 @@ int rvf_plan_tune(rvf_plan* plan, int32_t variant, int32_t refill_min);
  * every visibility decision do not depend on it; row sums and group sums are
  * added in another order (differences of a few ulps). */
 int rvf_plan_tune_tiles(rvf_plan* plan, int32_t columns);
+/* Occupancy of the pair kernel (no reference counterpart): resident waves per
+ * SIMD its build is compiled for.  0 = the kernel's default; the mixed split
+ * traversal kernels (default 4 waves at 128 VGPRs) also have a build for 3
+ * (every value in registers, no spills), chosen with 3 (4 = their default).
+ * Other values, or 3 / 4 with another pair kernel, return RVF_E_ARG.  The
+ * setting goes back to 0 when rvf_plan_tune moves to another kernel.  F, A and
+ * every visibility decision do not depend on it. */
+int rvf_plan_tune_waves(rvf_plan* plan, int32_t waves);
 /* Name of the pair kernel the plan launches (as profilers show it), e.g.
  * "k_pair<mixed_occ_split3>" (no reference counterpart; for measurement). */
 int rvf_plan_kernel(const rvf_plan* plan, char* name, int32_t cap);
@@ -187,7 +198,7 @@ typedef struct {
     int64_t tiles;       /* tiles timed */
     double ms;           /* pair-kernel time over those tiles */
     int32_t columns;     /* columns per tile of the layout timed */
-    int32_t reserved;
+    int32_t waves;       /* rvf_plan_tune_waves setting timed (0 = the kernel's default) */
 } rvf_tune_entry;
 /* Autotuner (no reference counterpart): runs the plan's prologue, then times
  * its pair kernel under every traversal variant valid for its precision (each
@@ -196,11 +207,12 @@ typedef struct {
  * every width), times the library default twice and the three fastest others
  * once over the whole problem (or four times the share when the problem is
  * larger), tries the refill thresholds of 8, 16, 24 and 32 other than the
- * choice's default for it, then its tile widths of 4, 8 and 16 columns, and
- * keeps the default unless another configuration is faster by more than twice
- * the default's run-to-run spread (at least 1 %), applied as rvf_plan_tune
- * and rvf_plan_tune_tiles would (*variant, *refill_min; the width in
- * rvf_stats.tile_columns).  Writes up to
+ * choice's default for it, then its tile widths of 4, 8 and 16 columns, then
+ * (mixed split kernels) their three-wave build, and keeps the default unless
+ * another configuration is faster by more than twice the default's
+ * run-to-run spread (at least 1 %), applied as rvf_plan_tune,
+ * rvf_plan_tune_tiles and rvf_plan_tune_waves would (*variant, *refill_min;
+ * the width and build in rvf_stats.tile_columns and .waves).  Writes up to
  * cap measurements to entries and their count to *n.  Plans without
  * obstruction trees have a single pair kernel: *n = 0 and nothing changes.
  * F, A and every visibility decision of later runs are the same whichever

@@ -54,7 +54,7 @@ class ServerCounters(C.Structure):
 
 class TuneEntry(C.Structure):
     _fields_ = [("variant", C.c_int32), ("refill_min", C.c_int32), ("tiles", C.c_int64), ("ms", C.c_double),
-                ("columns", C.c_int32), ("reserved", C.c_int32)]
+                ("columns", C.c_int32), ("waves", C.c_int32)]
 
 
 class Stats(C.Structure):
@@ -64,7 +64,8 @@ class Stats(C.Structure):
                 ("retests", C.c_int64), ("singular_pairs", C.c_int64), ("aabb_tests", C.c_int64),
                 ("tri_tests", C.c_int64), ("trav_iters", C.c_int64), ("slot_steps", C.c_int64),
                 ("task_batches", C.c_int64), ("exact_batches", C.c_int64), ("sure_hits", C.c_int64),
-                ("culled_rays", C.c_int64), ("culled_waves", C.c_int64), ("tile_columns", C.c_int64)]
+                ("culled_rays", C.c_int64), ("culled_waves", C.c_int64), ("tile_columns", C.c_int64),
+                ("waves", C.c_int64)]
 
     def as_dict(self):
         return {k: getattr(self, k) for k, _ in self._fields_}
@@ -104,6 +105,7 @@ def lib():
         "rvf_plan_device_ptr": ([P, C.c_int, PP], C.c_int),
         "rvf_plan_tune": ([P, C.c_int32, C.c_int32], C.c_int),
         "rvf_plan_tune_tiles": ([P, C.c_int32], C.c_int),
+        "rvf_plan_tune_waves": ([P, C.c_int32], C.c_int),
         "rvf_plan_kernel": ([P, C.c_char_p, C.c_int32], C.c_int),
         "rvf_plan_autotune": ([P, I64, P, I32, C.POINTER(C.c_int32), C.POINTER(C.c_int32), C.POINTER(C.c_int32)],
                               C.c_int),

@@ -222,15 +222,15 @@ class DeviceObstruction:
 class TuneTable:
     """Persisted autotuner choices: a JSON file mapping a problem class
     (precision, method, nquad, element count and occluder triangle count
-    rounded to powers of two) to the fastest (variant, refill_min) measured
-    for it on this machine."""
+    rounded to powers of two) to the fastest (variant, refill_min, columns,
+    waves) measured for it on this machine."""
 
     def __init__(self, path: str):
         self.path = path
         self.entries = {}
         if os.path.exists(path):
             with open(path) as f:
-                self.entries = {k: tuple(v) + (0,) * (3 - len(v)) for k, v in json.load(f).items()}
+                self.entries = {k: tuple(v) + (0,) * (4 - len(v)) for k, v in json.load(f).items()}
 
     @staticmethod
     def key(plan: "Plan") -> Optional[str]:
@@ -251,8 +251,9 @@ class TuneTable:
         return self.entries[key]
 
     def __setitem__(self, key, value):
-        # (variant, refill_min, columns per tile; 0 = the kernel's default)
-        self.entries[key] = (int(value[0]), int(value[1]), int(value[2]))
+        # (variant, refill_min, columns per tile, waves per SIMD; 0 = the kernel's default)
+        value = tuple(value) + (0,) * (4 - len(value))
+        self.entries[key] = (int(value[0]), int(value[1]), int(value[2]), int(value[3]))
 
     def save(self):
         d = os.path.dirname(os.path.abspath(self.path))
@@ -396,6 +397,19 @@ class Plan:
         check(_lib.lib().rvf_plan_stats(self.h, C.byref(st)))
         return int(st.tile_columns)
 
+    def tune_waves(self, waves: int = 0):
+        """Occupancy of the pair kernel: 0 = its default build; the mixed split
+        traversal kernels also have a build for 3 waves per SIMD (4 = their
+        default).  F, A and visibility decisions are unchanged."""
+        check(_lib.lib().rvf_plan_tune_waves(self.h, waves))
+
+    @property
+    def waves(self) -> int:
+        """The pair-kernel build in use (3: the three-wave build; 0: default)."""
+        st = _lib.Stats()
+        check(_lib.lib().rvf_plan_stats(self.h, C.byref(st)))
+        return int(st.waves)
+
     def autotune(self, sample_tiles: int = 0, table: Optional["TuneTable"] = None):
         """Pick the fastest traversal configuration for this plan.
 
@@ -404,24 +418,25 @@ class Plan:
         times every variant over a tile sample (rvf_plan_autotune) and the
         winner is stored in the table.  Returns (variant, refill_min,
         measurements), measurements being [] for a table hit; the tile width
-        chosen is `plan.columns`.  F, A and visibility decisions of later runs
+        and build chosen are `plan.columns` and `plan.waves`.  F, A and visibility decisions of later runs
         do not depend on the choice."""
         key = table.key(self) if table is not None else None
         if key is not None and key in table:
-            v, r, c = table[key]
+            v, r, c, w = table[key]
             self.tune(v, r)
             self.tune_tiles(c)
+            self.tune_waves(w)
             return v, r, []
         ents = (_lib.TuneEntry * 32)()
         n, v, r = C.c_int32(), C.c_int32(), C.c_int32()
         check(_lib.lib().rvf_plan_autotune(self.h, sample_tiles, ents, 32, C.byref(n), C.byref(v), C.byref(r)))
         meas = [{"variant": e.variant, "refill_min": e.refill_min, "tiles": e.tiles, "ms": e.ms,
-                 "columns": e.columns} for e in ents[:n.value]]
+                 "columns": e.columns, "waves": e.waves} for e in ents[:n.value]]
         if not meas:
             return v.value, r.value, []
         self.opts.variant = v.value
         if table is not None and key is not None:
-            table[key] = (v.value, r.value, self.columns)
+            table[key] = (v.value, r.value, self.columns, self.waves)
             table.save()
         return v.value, r.value, meas
 

@@ -180,6 +180,7 @@ struct PairArgs {
     int verify;               // mixed: confirm FP64 hits by the leaf's exact slab chain
     int cull_limit;           // mixed occluded: box tests a pair's capsule walk may spend (1: root box only)
     int hint;                 // mixed split traversal: try the pair's last certain blocker before walking
+    int waves;                // mixed split kinds, direct launches: 3 = the build for three waves per SIMD, else 0
     float Sm;                 // mixed: scene coordinate magnitude (FP32 error bounds)
     // audit (nullable): per pair blocked-ray count / order-free hash at [i + N*j]
     uint32_t* audit_count;

@@ -946,6 +946,7 @@ struct rvf_plan {
     int refill_min = 0;  // 0: the kernel's default (default_refill)
     int cw = 0;      // columns per tile of the current layout
     int cw_req = 0;  // columns asked for by rvf_plan_tune_tiles (0: the kernel's default)
+    int waves = 0;   // rvf_plan_tune_waves: 3 = the mixed split kernels' three-wave build, 0 = default
     bool scratch = false;  // buffers are pieces of the static scratch array
 };
 
@@ -1216,6 +1217,7 @@ static PairArgs pair_args(const rvf_plan* p) {
         a.verify = p->obs->soup_read == RVF_SOUP_REFERENCE ? 1 : 0;
         a.cull_limit = 256;  // box tests per pair; measured: 64 leaves 2 % of the node steps, more gains nothing
         a.hint = 1;  // (a second hint per pair saved 8 % of the node steps but cost more than it saved)
+        a.waves = rvf_mixed_split_kind(pair_kind(p)) ? p->waves : 0;
         a.Sm = p->obs->Sm;
     } else {
         a.Sm = 1.0f;
@@ -1419,7 +1421,9 @@ int rvf_plan_kernel(const rvf_plan* p, char* name, int32_t cap) {
     };
     const int k = pair_kind(p);
     if (k < 0 || k >= RVF_K_COUNT) return fail(RVF_E_STATE, "rvf_plan_kernel: no pair kernel");
-    std::snprintf(name, (size_t)cap, "%s", names[k]);
+    // the three-wave build of the mixed split kernels: k_pair_w3<tag>
+    if (rvf_mixed_split_kind(k) && p->waves == 3) std::snprintf(name, (size_t)cap, "k_pair_w3%s", names[k] + 6);
+    else std::snprintf(name, (size_t)cap, "%s", names[k]);
     return 0;
 }
 
@@ -1454,6 +1458,22 @@ int rvf_plan_tune(rvf_plan* p, int32_t variant, int32_t refill_min) {
     const int rc = plan_relayout(p);
     if (rc) { p->opts.variant = v0; return rc; }
     p->refill_min = refill_min;
+    if (!rvf_mixed_split_kind(pair_kind(p))) p->waves = 0;  // the three-wave build exists for these alone
+    if (p->gexec) { (void)hipGraphExecDestroy(p->gexec); p->gexec = nullptr; }  // re-captured on the next run
+    return 0;
+}
+
+int rvf_plan_tune_waves(rvf_plan* p, int32_t waves) {
+    if (!p) return fail(RVF_E_ARG, "rvf_plan_tune_waves: null plan");
+    const int k = pair_kind(p);
+    const bool split = rvf_mixed_split_kind(k);
+    if (waves == 4 && split) waves = 0;  // their default build
+    if (waves != 0 && !(waves == 3 && split))
+        return fail(RVF_E_ARG, "rvf_plan_tune_waves: waves must be 0 (the kernel's build), or 3 or 4 for the mixed "
+                               "split traversal kernels");
+    HIPCHK(hipSetDevice(p->ctx->device));
+    HIPCHK(hipStreamSynchronize(p->ctx->stream));
+    p->waves = waves;
     if (p->gexec) { (void)hipGraphExecDestroy(p->gexec); p->gexec = nullptr; }  // re-captured on the next run
     return 0;
 }
@@ -1493,16 +1513,17 @@ int rvf_plan_autotune(rvf_plan* p, int64_t sample_tiles, rvf_tune_entry* ent, in
     const double frac = std::min(1.0, (double)(sample_tiles ? sample_tiles : 8192) / (double)nt0);
     const double big = std::min(1.0, 4.0 * frac);
     const bool mixed = p->opts.precision == RVF_MIXED;
-    const int v0 = p->opts.variant, r0 = p->refill_min, c0 = p->cw_req;
+    const int v0 = p->opts.variant, r0 = p->refill_min, c0 = p->cw_req, w0 = p->waves;
     // Times the pair kernel over a share f of the tiles taken at a fixed
     // stride through the tile list (all tiles when f = 1).  Tile costs vary
     // with where the pair sits relative to the occluders, so a strided sample
     // follows the whole problem's mix where contiguous ranges do not.
     int64_t timed = 0;
-    auto measure = [&](int v, int refill, int cols, double f, float* ms) -> int {
+    auto measure = [&](int v, int refill, int cols, double f, float* ms, int waves = 0) -> int {
         p->opts.variant = v;
         p->refill_min = refill;
         p->cw_req = cols;
+        p->waves = waves;
         int e = plan_relayout(p);
         if (e) return e;
         const int64_t nt = (int64_t)p->L->tiles.size();
@@ -1534,9 +1555,9 @@ int rvf_plan_autotune(rvf_plan* p, int64_t sample_tiles, rvf_tune_entry* ent, in
     for (int S = 1; S <= 3; S++)
         if (occ_kind(RVF_VARIANT_SPLIT(S), mixed, ntri) >= 0) cands.push_back(RVF_VARIANT_SPLIT(S));
     // a measured configuration as it ran (refill 0 stands for the kernel's default)
-    auto add = [&](int v, int refill, float ms) {
+    auto add = [&](int v, int refill, float ms, int waves = 0) {
         const int r = v == RVF_VARIANT_LANE ? 0 : refill ? refill : default_refill(pair_kind(p));
-        if (*n < cap) ent[*n] = rvf_tune_entry{v, r, timed, (double)ms, p->cw, 0};
+        if (*n < cap) ent[*n] = rvf_tune_entry{v, r, timed, (double)ms, p->cw, waves};
         ++*n;
     };
     // 1. rank every variant (at its default tile width) on the sample
@@ -1602,15 +1623,28 @@ int rvf_plan_autotune(rvf_plan* p, int64_t sample_tiles, rvf_tune_entry* ent, in
             if (better(ms)) { best = ms; best_c = cols; }
         }
     }
+    // 3. occupancy: the mixed split kernels' three-wave build (no spills)
+    int best_w = 0;
+    if (!rc) {
+        p->opts.variant = best_v;
+        if (rvf_mixed_split_kind(pair_kind(p))) {
+            float ms = 0;
+            if (!(rc = measure(best_v, best_r, best_c, big, &ms, 3))) {
+                add(best_v, best_r, ms, 3);
+                if (better(ms)) { best = ms; best_w = 3; }
+            }
+        }
+    }
     if (*n > cap) *n = cap;
     if (rc) {
-        p->opts.variant = v0; p->refill_min = r0; p->cw_req = c0;
+        p->opts.variant = v0; p->refill_min = r0; p->cw_req = c0; p->waves = w0;
         (void)plan_relayout(p);
         return rc;
     }
     p->opts.variant = best_v;
     p->refill_min = best_r;
     p->cw_req = best_c;
+    p->waves = best_w;
     if ((rc = plan_relayout(p))) return rc;
     if (variant) *variant = best_v;
     if (refill_min) *refill_min = best_v == RVF_VARIANT_LANE ? 0 : best_r ? best_r : default_refill(pair_kind(p));
@@ -1755,6 +1789,7 @@ int rvf_plan_stats(const rvf_plan* p, rvf_stats* st) {
     if (!p || !st) return fail(RVF_E_ARG, "null plan");
     *st = p->st;
     st->tile_columns = p->cw;
+    st->waves = p->waves;
     return 0;
 }
 
@@ -1891,7 +1926,7 @@ void rvf_ctx_destroy(rvf_ctx* c) {
 }
 
 const char* rvf_last_error(void) { return g_err.c_str(); }
-int rvf_version(void) { return 2; }  // 2: rvf_stats.tile_columns, rvf_tune_entry.columns, rvf_plan_tune_tiles
+int rvf_version(void) { return 3; }  // 3: rvf_plan_tune_waves, rvf_stats.waves, rvf_tune_entry.waves
 
 int rvf_quad_rule(int32_t nquad, int32_t tri, double* pts, double* wts) {
     if (!pts || !wts) return fail(RVF_E_ARG, "null output");

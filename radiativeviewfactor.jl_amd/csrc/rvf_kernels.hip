@@ -2701,8 +2701,7 @@ constexpr int k_pair_waves(int kind) { return (kind == RVF_K_MIX_SPLIT3 || kind 
 constexpr bool k_pair_dynamic(int kind) { return rvf_queue_kind(kind); }
 
 template <class TAG>
-__global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(k_pair_waves(TAG::kind)))) void k_pair(PairArgs a) {
-    extern __shared__ double lds[];
+__device__ __forceinline__ void pair_body(const PairArgs& a, double* lds) {
     if constexpr (!k_pair_dynamic(TAG::kind)) {
         run_tile<TAG::kind>(a, a.tile0 + (int)blockIdx.x * a.tile_step, lds);
     } else {
@@ -2730,6 +2729,21 @@ __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(k_pair_waves
             }
         }
     }
+}
+
+template <class TAG>
+__global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(k_pair_waves(TAG::kind)))) void k_pair(PairArgs a) {
+    extern __shared__ double lds[];
+    pair_body<TAG>(a, lds);
+}
+
+// The mixed split kernels also built for three resident waves per SIMD (room
+// for every value in registers: no spills) instead of their default four:
+// an occupancy the autotuner can choose (rvf_plan_tune_waves).
+template <class TAG>
+__global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(3))) void k_pair_w3(PairArgs a) {
+    extern __shared__ double lds[];
+    pair_body<TAG>(a, lds);
 }
 
 template <class TAG>
@@ -2773,19 +2787,30 @@ __global__ __launch_bounds__(256) void k_copy(const uint4* __restrict__ src, uin
 // ---------------------------------------------------------------------------
 // Launch wrappers (host side of this translation unit)
 // ---------------------------------------------------------------------------
+// Persistent launch: as many workgroups as fit the GPU (at least one), the
+// claim counters zeroed by a kernel on the same stream (graph capture records
+// no memset).
+template <class KERN>
+static void launch_persistent(KERN kern, const PairArgs* a, size_t lds, hipStream_t st) {
+    int dev = 0, ncu = 0, nb = 0;
+    (void)hipGetDevice(&dev);
+    (void)hipDeviceGetAttribute(&ncu, hipDeviceAttributeMultiprocessorCount, dev);
+    (void)hipOccupancyMaxActiveBlocksPerMultiprocessor(&nb, kern, 64, lds);
+    const int grid = std::max(1, std::min(a->ntiles, std::max(nb, 1) * std::max(ncu, 1)));
+    hipLaunchKernelGGL(k_zero_ctr, dim3(1), dim3(64), 0, st, a->tile_ctr);
+    hipLaunchKernelGGL(kern, dim3((unsigned)grid), dim3(64), lds, st, *a);
+}
+
 template <class TAG>
 static void launch_direct(const PairArgs* a, size_t lds, hipStream_t st) {
     if constexpr (k_pair_dynamic(TAG::kind)) {
-        // resident waves of the whole GPU (at least one), the claim counter
-        // zeroed by a kernel on the same stream (graph capture records no
-        // memset)
-        int dev = 0, ncu = 0, nb = 0;
-        (void)hipGetDevice(&dev);
-        (void)hipDeviceGetAttribute(&ncu, hipDeviceAttributeMultiprocessorCount, dev);
-        (void)hipOccupancyMaxActiveBlocksPerMultiprocessor(&nb, k_pair<TAG>, 64, lds);
-        const int grid = std::max(1, std::min(a->ntiles, std::max(nb, 1) * std::max(ncu, 1)));
-        hipLaunchKernelGGL(k_zero_ctr, dim3(1), dim3(64), 0, st, a->tile_ctr);
-        hipLaunchKernelGGL(k_pair<TAG>, dim3((unsigned)grid), dim3(64), lds, st, *a);
+        if constexpr (rvf_mixed_split_kind(TAG::kind)) {
+            if (a->waves == 3) {
+                launch_persistent(k_pair_w3<TAG>, a, lds, st);
+                return;
+            }
+        }
+        launch_persistent(k_pair<TAG>, a, lds, st);
     } else {
         hipLaunchKernelGGL(k_pair<TAG>, dim3((unsigned)a->ntiles), dim3(64), lds, st, *a);
     }

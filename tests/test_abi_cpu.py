@@ -87,12 +87,15 @@ def test_tune_table_round_trip(tmp_path):
     assert api.TuneTable.key(NS(opts=o, dmesh=plan.dmesh, dobs=None)) is None
     t = api.TuneTable(str(tmp_path / "sub" / "t.json"))
     assert k not in t
-    t[k] = (api.VARIANT_SPLIT(3), 16, 8)
+    t[k] = (api.VARIANT_SPLIT(3), 16, 8, 3)
     t.save()
     t2 = api.TuneTable(str(tmp_path / "sub" / "t.json"))
-    assert k in t2 and t2[k] == (api.VARIANT_SPLIT(3), 16, 8)
-    # an entry written without a tile width reads as the kernel's default (0)
+    assert k in t2 and t2[k] == (api.VARIANT_SPLIT(3), 16, 8, 3)
+    # entries written without a tile width or a build read as the kernel's
+    # defaults (0)
     import json
     with open(tmp_path / "old.json", "w") as f:
         json.dump({k: [api.VARIANT_QUEUE(1), 16]}, f)
-    assert api.TuneTable(str(tmp_path / "old.json"))[k] == (api.VARIANT_QUEUE(1), 16, 0)
+    assert api.TuneTable(str(tmp_path / "old.json"))[k] == (api.VARIANT_QUEUE(1), 16, 0, 0)
+    t[k] = (api.VARIANT_QUEUE(2), 8, 4)
+    assert t[k] == (api.VARIANT_QUEUE(2), 8, 4, 0)

@@ -33,6 +33,10 @@ def _solve(ctx, dm, do, prec, tune=None, **kw):
     return res, out, (name, cols)
 
 
+def _mixed_split(variant, prec):
+    return prec == "mixed" and variant in (api.VARIANT_SPLIT(1), api.VARIANT_SPLIT(2), api.VARIANT_SPLIT(3))
+
+
 def _default_columns(variant):
     # wave-queue traversal variants run on tiles of 4 columns, the others 16
     return 16 if variant == api.VARIANT_LANE else 4
@@ -65,7 +69,7 @@ def test_autotune_measures_all_variants_and_keeps_results(prec, tmp_path):
     # (split 3 mixed, queue 1 FP64) is timed twice on the larger set, then the
     # three fastest others, then the choice's other refill thresholds of 8, 16,
     # 24 and 32 (wave-queue variants), then its other tile widths of 4, 8 and 16
-    # columns.  Another
+    # columns, then (mixed split kernels) their three-wave build.  Another
     # configuration replaces the current choice only if it is faster than the
     # best so far by more than twice the default's spread (at least 1 %).
     assert all(e["columns"] == _default_columns(e["variant"]) for e in meas[:len(variants)])
@@ -94,14 +98,27 @@ def test_autotune_measures_all_variants_and_keeps_results(prec, tmp_path):
             if e["ms"] < best_ms * (1 - margin):
                 best_ms, choice = e["ms"], (e["variant"], e["refill_min"], 0)
         k += len(others)
-    widths = [e["columns"] for e in again[k:]]
+    nw = len({4, 8, 16} - {_default_columns(choice[0])})
+    widths = [e["columns"] for e in again[k:k + nw]]
     assert sorted(widths) == sorted({4, 8, 16} - {_default_columns(choice[0])})
-    for e in again[k:]:
-        assert e["variant"] == choice[0] and e["refill_min"] == choice[1]
+    for e in again[k:k + nw]:
+        assert e["variant"] == choice[0] and e["refill_min"] == choice[1] and e["waves"] == 0
         if e["ms"] < best_ms * (1 - margin):
             best_ms, choice = e["ms"], (choice[0], choice[1], e["columns"])
+    k += nw
+    waves = 0
+    if _mixed_split(choice[0], prec):
+        assert len(again) == k + 1
+        e = again[k]
+        assert (e["variant"], e["refill_min"], e["waves"]) == (choice[0], choice[1], 3)
+        assert e["columns"] == (choice[2] or _default_columns(choice[0]))
+        if e["ms"] < best_ms * (1 - margin):
+            waves = 3
+    else:
+        assert len(again) == k
     assert (v, r) == choice[:2] and v in variants
     assert cols == (choice[2] or _default_columns(v))
+    assert name.startswith("k_pair_w3<" if waves == 3 else "k_pair<"), name
     names = ("F", "A", "rowsum", "F_group", "A_group", "tags", "audit_count", "audit_hash")
     for nm, a, b in zip(names, got, ref):
         if nm in ("rowsum", "F_group") and cols != ref_cols:
@@ -127,3 +144,35 @@ def test_autotune_unoccluded_plan_is_a_no_op():
     v, r, meas = plan.autotune()
     assert meas == [] and plan.kernel_name() == name and plan.columns == 16
     plan.close(); dm.close()
+
+
+def test_three_wave_build_gives_identical_results():
+    """rvf_plan_tune_waves: the mixed split kernels' three-wave build runs the
+    same tiles with the same arithmetic, so F, the sums and every decision are
+    bit-identical; other settings are refused."""
+    ctx = api.context(0)
+    m = _scene()
+    dm = api.DeviceMesh(ctx, m)
+    do = api.DeviceObstruction(ctx, dm, [7], "geometric")
+    out = {}
+    for w in (0, 3, 4):
+        plan = api.Plan(ctx, dm, do, 3, "quad", "mixed", want_F=True, audit=True)
+        plan.tune_waves(w)
+        assert plan.waves == (3 if w == 3 else 0)
+        plan.run(); plan.sync()
+        out[w] = (plan.fetch(), plan.kernel_name())
+        for bad in (1, 2, 5, -1):
+            with pytest.raises(RuntimeError):
+                plan.tune_waves(bad)
+        plan.tune(api.VARIANT_QUEUE(2))  # another kernel: back to its default build
+        assert plan.waves == 0
+        with pytest.raises(RuntimeError):
+            plan.tune_waves(3)
+        plan.close()
+    assert out[0][1] == out[4][1] == "k_pair<mixed_occ_split3>" and out[3][1] == "k_pair_w3<mixed_occ_split3>"
+    for a, b in zip(out[0][0], out[3][0]):
+        if isinstance(a, np.ndarray):
+            assert np.array_equal(a, b)
+        else:
+            assert a == b
+    do.close(); dm.close()
