@@ -2074,9 +2074,10 @@ __device__ __forceinline__ void tile_occ_queue(const PairArgs& a, int tile, doub
             const int p1 = min(npmax, p0 + X.pch);
             const int nc = (p1 - p0) * npj;
             X.p0 = p0;
-            if (!trace)
-                for (int w = lane; w < 2 * nc; w += 64) blk[w] = 0u;
-            // Phase A: need masks (ray counts as wave totals)
+            for (int w = lane; w < 2 * nc; w += 64) blk[w] = 0u;
+            // Phase A: need masks (ray counts as wave totals): one ballot per
+            // combination, the culled pairs' lanes masked out with capm
+            const uint64_t capm = __ballot(cap);
             uint32_t wrays = 0, wculled = 0;
             for (int p = p0; p < p1 && trace; p++) {
                 const bool pv = act && p < npi && root >= 0;
@@ -2090,12 +2091,11 @@ __device__ __forceinline__ void tile_occ_queue(const PairArgs& a, int tile, doub
                         float rx, ry, rz, r2;
                         float k = vf_mixed(bx, by, bz, nxi, nyi, nzi, qj32[q * 8 + 0], qj32[q * 8 + 1], qj32[q * 8 + 2],
                                            qj32[q * 8 + 3], qj32[q * 8 + 4], qj32[q * 8 + 5], rx, ry, rz, r2);
-                        const bool nd = pv && k != 0.0f;
-                        const uint64_t m = __ballot(nd && cap);
-                        wrays += __popcll(__ballot(nd));
-                        wculled += __popcll(__ballot(nd && !cap));
+                        const uint64_t ndm = __ballot(pv && k != 0.0f);
+                        wrays += __popcll(ndm);
+                        wculled += __popcll(ndm & ~capm);
                         const int c = (p - p0) * npj + q;
-                        if (lane == 0) { need[c] = m; blk[2 * c] = 0u; blk[2 * c + 1] = 0u; }
+                        if (lane == 0) need[c] = ndm & capm;
                     }
                 } else {
                     const double xi = P64[(int64_t)(p * 8 + 0) * NS + t.i], yi = P64[(int64_t)(p * 8 + 1) * NS + t.i];
@@ -2107,12 +2107,11 @@ __device__ __forceinline__ void tile_occ_queue(const PairArgs& a, int tile, doub
                         Ray64 Rr;
                         const double K = vf_exact(xi, yi, zi, nxi, nyi, nzi, qj64[q * 8 + 0], qj64[q * 8 + 1],
                                                   qj64[q * 8 + 2], qj64[q * 8 + 3], qj64[q * 8 + 4], qj64[q * 8 + 5], Rr);
-                        const bool nd = pv && K != 0.0;
-                        const uint64_t m = __ballot(nd && cap);
-                        wrays += __popcll(__ballot(nd));
-                        wculled += __popcll(__ballot(nd && !cap));
+                        const uint64_t ndm = __ballot(pv && K != 0.0);
+                        wrays += __popcll(ndm);
+                        wculled += __popcll(ndm & ~capm);
                         const int c = (p - p0) * npj + q;
-                        if (lane == 0) { need[c] = m; blk[2 * c] = 0u; blk[2 * c + 1] = 0u; }
+                        if (lane == 0) need[c] = ndm & capm;
                     }
                 }
             }
