@@ -146,6 +146,44 @@ __device__ __forceinline__ void eval_point(int fam, const double* __restrict__ c
     *dA = d;
 }
 
+// eval_point with the element's node coordinates already in registers
+// (cx/cy/cz[k] for node k < the family's node count): the same arithmetic in
+// the same order, so the same bits.
+__device__ __forceinline__ void eval_point_r(int fam, const double* cx, const double* cy, const double* cz, double xi,
+                                             double eta, double* x, double* n, double* dA) {
+    double N[8], dx[8], de[8];
+    int nn = shape_fn(fam, xi, eta, N, dx, de);
+    if (fam >= 4) {
+        double p0 = 0.0, p1 = 0.0, p2 = 0.0, a0 = 0.0, a1 = 0.0, a2 = 0.0;
+#pragma unroll
+        for (int k = 0; k < 3; k++) {
+            if (k < nn) {
+                p0 = p0 + N[k] * cx[k]; p1 = p1 + N[k] * cy[k]; p2 = p2 + N[k] * cz[k];
+                a0 = a0 + dx[k] * cx[k]; a1 = a1 + dx[k] * cy[k]; a2 = a2 + dx[k] * cz[k];
+            }
+        }
+        double d = sqrt((a0 * a0 + a1 * a1) + a2 * a2);
+        x[0] = p0; x[1] = p1; x[2] = p2;
+        n[0] = -(a1 / d); n[1] = a0 / d; n[2] = 0.0;
+        *dA = d;
+        return;
+    }
+    double p0 = 0.0, p1 = 0.0, p2 = 0.0, a0 = 0.0, a1 = 0.0, a2 = 0.0, b0 = 0.0, b1 = 0.0, b2 = 0.0;
+#pragma unroll
+    for (int k = 0; k < 8; k++) {
+        if (k < nn) {
+            p0 = p0 + N[k] * cx[k]; p1 = p1 + N[k] * cy[k]; p2 = p2 + N[k] * cz[k];
+            a0 = a0 + dx[k] * cx[k]; a1 = a1 + dx[k] * cy[k]; a2 = a2 + dx[k] * cz[k];
+            b0 = b0 + de[k] * cx[k]; b1 = b1 + de[k] * cy[k]; b2 = b2 + de[k] * cz[k];
+        }
+    }
+    double c0 = a1 * b2 - a2 * b1, c1 = a2 * b0 - a0 * b2, c2 = a0 * b1 - a1 * b0;
+    double d = sqrt((c0 * c0 + c1 * c1) + c2 * c2);
+    x[0] = p0; x[1] = p1; x[2] = p2;
+    n[0] = c0 / d; n[1] = c1 / d; n[2] = c2 / d;
+    *dA = d;
+}
+
 // ---------------------------------------------------------------------------
 // Precompute: one thread per element, points in rule order.  Writes FP64 SoA
 // and AoS, the area A_e = sum_p w_p dA_p in p order (Assembly.jl:186-190 via
@@ -2256,13 +2294,23 @@ __device__ __forceinline__ double xorshift64(uint64_t& s) {
     return m ? __longlong_as_double(__double_as_longlong(y) + 1) : 0.0;
 }
 // _sample_on_element (GPUMCKernels.jl:266-281)
-__device__ __forceinline__ void mc_sample(const PairArgs& a, int f, int64_t e, double u1, double u2, double* x,
-                                          double* n, double* dA) {
-    double xi, eta;
+__device__ __forceinline__ void mc_param(int f, double u1, double u2, double& xi, double& eta) {
     if (f >= 4) { xi = u1; eta = 0.0; }  // curve element: u1 is the point on [-1, 1]
     else if (f == 0 || f == 2) { xi = 2.0 * u1 - 1.0; eta = 2.0 * u2 - 1.0; }
     else { xi = u1; eta = u2; if (xi + eta > 1.0) { xi = 1.0 - xi; eta = 1.0 - eta; } }
+}
+__device__ __forceinline__ void mc_sample(const PairArgs& a, int f, int64_t e, double u1, double u2, double* x,
+                                          double* n, double* dA) {
+    double xi, eta;
+    mc_param(f, u1, u2, xi, eta);
     eval_point(f, a.coords, a.nodes8 + 8 * e, xi, eta, x, n, dA);
+}
+// the same with the element's node coordinates in registers
+__device__ __forceinline__ void mc_sample_r(int f, const double* cx, const double* cy, const double* cz, double u1,
+                                            double u2, double* x, double* n, double* dA) {
+    double xi, eta;
+    mc_param(f, u1, u2, xi, eta);
+    eval_point_r(f, cx, cy, cz, xi, eta, x, n, dA);
 }
 
 // Curve meshes (the reference's GPU Monte Carlo has no curve form): the same
@@ -2327,6 +2375,19 @@ __device__ __forceinline__ void tile_mc(const PairArgs& a, int tile) {
     const int gi = t.vi ? a.elem_grp[t.i] : 0;
     const int64_t ii = t.vi ? t.oi : 0;
     const int s = a.mc_s;
+    // the row element's node coordinates, the same for every column and
+    // sample: loaded once (gathered per sample, 64 elements' nodes per wave
+    // made the L1 the bound: 1,400 cache accesses per sample and wavefront)
+    double cx[8], cy[8], cz[8];
+    {
+        const int32_t* nd = a.nodes8 + 8 * ii;
+        const int nni = fi >= 4 ? (fi == 4 ? 3 : 2) : (fi == 0 ? 8 : fi == 1 ? 6 : fi == 2 ? 4 : 3);
+#pragma unroll
+        for (int k = 0; k < 8; k++) {
+            const double* xa = a.coords + 3 * (int64_t)(nd[k < nni ? k : 0] - 1);
+            cx[k] = xa[0]; cy[k] = xa[1]; cz[k] = xa[2];
+        }
+    }
     double ls = 0.0, lc = 0.0;
     unsigned long long nrays = 0, naabb = 0, ntri = 0;
     for (int64_t j = t.jb; j < t.je; j++) {
@@ -2346,8 +2407,8 @@ __device__ __forceinline__ void tile_mc(const PairArgs& a, int tile) {
             double u1 = xorshift64(st), u2 = xorshift64(st);
             double si = (double)(kk / s), sj = (double)(kk % s);
             bool strat = kk < s * s;
-            if (a.dim2) mc_sample(a, fi, ii, mc_line_point(kk, u1, a.mc_nsamp), 0.0, xi, ni, &ai);
-            else mc_sample(a, fi, ii, strat ? (si + u1) / s : u1, strat ? (sj + u2) / s : u2, xi, ni, &ai);
+            if (a.dim2) mc_sample_r(fi, cx, cy, cz, mc_line_point(kk, u1, a.mc_nsamp), 0.0, xi, ni, &ai);
+            else mc_sample_r(fi, cx, cy, cz, strat ? (si + u1) / s : u1, strat ? (sj + u2) / s : u2, xi, ni, &ai);
             double u3 = xorshift64(st), u4 = xorshift64(st);
             if (a.dim2) mc_sample(a, fj, oj, mc_line_point(kk, u3, a.mc_nsamp), 0.0, xj, nj, &aj);
             else mc_sample(a, fj, oj, strat ? (si + u3) / s : u3, strat ? (sj + u4) / s : u4, xj, nj, &aj);
