@@ -2182,22 +2182,34 @@ __device__ __forceinline__ void tile_occ_queue(const PairArgs& a, int tile, doub
                     const float nzi = P[(int64_t)(p * 8 + 5) * NS + t.i];
                     const float bx = dax - dxi, by = day - dyi, bz = daz - dzi;
                     float inner = 0.0f;
-                    for (int q = 0; q < npj; q++) {
-                        float rx, ry, rz, r2;
-                        float k = vf_mixed(bx, by, bz, nxi, nyi, nzi, qj32[q * 8 + 0], qj32[q * 8 + 1], qj32[q * 8 + 2],
-                                           qj32[q * 8 + 3], qj32[q * 8 + 4], qj32[q * 8 + 5], rx, ry, rz, r2);
-                        const int c = combo_index(p, q, p0, npj);
-                        if (!trace) {  // as the need masks count them: every such ray is culled
-                            const bool nd = act && p < npi && root >= 0 && k != 0.0f;
-                            wcl += __popcll(__ballot(nd));
+                    if (trace) {
+                        for (int q = 0; q < npj; q++) {
+                            float rx, ry, rz, r2;
+                            float k = vf_mixed(bx, by, bz, nxi, nyi, nzi, qj32[q * 8 + 0], qj32[q * 8 + 1],
+                                               qj32[q * 8 + 2], qj32[q * 8 + 3], qj32[q * 8 + 4], qj32[q * 8 + 5], rx, ry,
+                                               rz, r2);
+                            const int c = combo_index(p, q, p0, npj);
+                            const bool blocked = (blk[2 * c + (lane >> 5)] >> (lane & 31)) & 1u;
+                            k = blocked ? 0.0f : k;
+                            if (aud && blocked) {
+                                bc++;
+                                bh += mix32((uint32_t)(rev ? q * npi + p : p * npj + q) + 0x9e3779b9U);
+                            }
+                            inner = fmaf(qj32[q * 8 + 6], k, inner);
                         }
-                        const bool blocked = (blk[2 * c + (lane >> 5)] >> (lane & 31)) & 1u;
-                        k = blocked ? 0.0f : k;
-                        if (aud && blocked) {
-                            bc++;
-                            bh += mix32((uint32_t)(rev ? q * npi + p : p * npj + q) + 0x9e3779b9U);
+                    } else {
+                        // no ray of the wave was traced (every blocked bit is
+                        // 0); the rays are counted as the need masks would
+                        // count them, every one culled
+                        const bool pv = act && p < npi && root >= 0;
+                        for (int q = 0; q < npj; q++) {
+                            float rx, ry, rz, r2;
+                            const float k = vf_mixed(bx, by, bz, nxi, nyi, nzi, qj32[q * 8 + 0], qj32[q * 8 + 1],
+                                                     qj32[q * 8 + 2], qj32[q * 8 + 3], qj32[q * 8 + 4], qj32[q * 8 + 5],
+                                                     rx, ry, rz, r2);
+                            wcl += __popcll(__ballot(pv && k != 0.0f));
+                            inner = fmaf(qj32[q * 8 + 6], k, inner);
                         }
-                        inner = fmaf(qj32[q * 8 + 6], k, inner);
                     }
                     if (p < npi) {
                         double wd = P64[(int64_t)(p * 8 + 6) * NS + t.i] * P64[(int64_t)(p * 8 + 7) * NS + t.i];
