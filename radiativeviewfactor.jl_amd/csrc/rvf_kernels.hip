@@ -2014,7 +2014,37 @@ __device__ __forceinline__ int capsule_entry(const double* ai, const double* aj,
     return -1;
 }
 
-template <bool MIX, int S, bool SPLIT>
+// Per-wave statistics block of the wave-queue tiles in LDS (its place in the
+// tile's LDS layout below), and its flush into the plan's counters.
+template <bool MIX, bool SPLIT>
+__device__ __forceinline__ uint64_t* occ_wst(const PairArgs& a, double* lds) {
+    const char* p = (const char*)(lds + (MIX ? 0 : a.maxpts * 64));
+    p += RVF_QCOMBO * (sizeof(uint64_t) + 2 * sizeof(uint32_t)) + 2 * 64 * sizeof(int);
+    p += (SPLIT ? (MIX ? RVF_TQCAP_MIX + RVF_CQCAP : RVF_TQCAP) : 0) * sizeof(uint32_t);
+    p += (MIX && SPLIT ? 6 * 64 * sizeof(float) + RVF_FRONTIER * 64 * sizeof(int) : 0);
+    return (uint64_t*)p;
+}
+template <bool MIX>
+__device__ __forceinline__ void occ_flush(const PairArgs& a, const uint64_t* wst) {
+    if (a.counters && threadIdx.x == 0) {
+        unsigned long long v[9];
+#pragma unroll
+        for (int e = 0; e < 9; e++) v[e] = wst[e];
+        if (!MIX) v[QS_C0] += v[QS_CULLED];  // FP64 box tests: a culled ray's root test
+        atomicAdd(&a.counters[1], v[QS_RAYS]);
+        if (MIX) { atomicAdd(&a.counters[0], v[QS_C0]); atomicAdd(&a.counters[8], v[QS_C1]); }
+        else { atomicAdd(&a.counters[2], v[QS_C0]); atomicAdd(&a.counters[3], v[QS_C1]); }
+        for (int e = QS_ITERS; e <= QS_EBATCH; e++) atomicAdd(&a.counters[e + 1], v[e]);
+        atomicAdd(&a.counters[9], v[QS_CULLED]);
+        atomicAdd(&a.counters[10], v[QS_SKIPPED]);
+    }
+}
+
+// WS: the per-wave statistics block is zeroed and flushed by the caller once
+// per wave (persistent launches; per tile, every wave's eleven counter
+// atomics contended with every other's: 26 ms of a tile loop that did no
+// other work), not per tile.
+template <bool MIX, int S, bool SPLIT, bool WS = false>
 __device__ __forceinline__ void tile_occ_queue(const PairArgs& a, int tile, double* lds) {
     const TileCtx t = tile_ctx(a, tile);
     const int64_t NS = a.NS;
@@ -2039,8 +2069,10 @@ __device__ __forceinline__ void tile_occ_queue(const PairArgs& a, int tile, doub
     // culled before any traversal, columns whose trace the wave skipped (all
     // its pairs culled), the drains' counters
     uint64_t* wst = (uint64_t*)(lfr + (MIX && SPLIT ? RVF_FRONTIER * 64 : 0));
-    if (lane < RVF_QSTATS) wst[lane] = 0;
-    __syncthreads();
+    if (!WS) {
+        if (lane < RVF_QSTATS) wst[lane] = 0;
+        __syncthreads();
+    }
     const double* __restrict__ P64 = a.P64;
     const float* __restrict__ P = a.P32;
     double ls = 0.0, lc = 0.0;
@@ -2264,18 +2296,7 @@ __device__ __forceinline__ void tile_occ_queue(const PairArgs& a, int tile, doub
         pair_out(a, t, j, oj, act, raw, Ai, ls, lc);
     }
     a.part[(int64_t)t.tile * RVF_STRIP + t.lane] = ls + lc;
-    if (a.counters && lane == 0) {
-        unsigned long long v[9];
-#pragma unroll
-        for (int e = 0; e < 9; e++) v[e] = wst[e];
-        if (!MIX) v[QS_C0] += v[QS_CULLED];  // FP64 box tests: a culled ray's root test
-        atomicAdd(&a.counters[1], v[QS_RAYS]);
-        if (MIX) { atomicAdd(&a.counters[0], v[QS_C0]); atomicAdd(&a.counters[8], v[QS_C1]); }
-        else { atomicAdd(&a.counters[2], v[QS_C0]); atomicAdd(&a.counters[3], v[QS_C1]); }
-        for (int e = QS_ITERS; e <= QS_EBATCH; e++) atomicAdd(&a.counters[e + 1], v[e]);
-        atomicAdd(&a.counters[9], v[QS_CULLED]);
-        atomicAdd(&a.counters[10], v[QS_SKIPPED]);
-    }
+    if (!WS) occ_flush<MIX>(a, wst);
 }
 
 // ---------------------------------------------------------------------------
@@ -2714,7 +2735,7 @@ __global__ __launch_bounds__(256) void k_reduce_fin(const double* __restrict__ T
 // work items from many meshes in one launch: each wavefront takes the next
 // item with one atomic and exits when the queue is empty.
 // ---------------------------------------------------------------------------
-template <int KIND>
+template <int KIND, bool WS = false>
 __device__ __forceinline__ void run_tile(const PairArgs& a, int tile, double* lds) {
     if (KIND == RVF_K_FP64_PC4) tile_fp64<4>(a, tile);
     else if (KIND == RVF_K_FP64_PC9) tile_fp64<9>(a, tile);
@@ -2725,19 +2746,19 @@ __device__ __forceinline__ void run_tile(const PairArgs& a, int tile, double* ld
     else if (KIND == RVF_K_MIX_PC16) tile_mixed<16, true>(a, tile);
     else if (KIND == RVF_K_MIX_PC16C) tile_mixed<16, false>(a, tile);
     else if (KIND == RVF_K_MIX_OCC_LANE) tile_mixed_occ(a, tile);
-    else if (KIND == RVF_K_FP64_QUEUE1) tile_occ_queue<false, 1, false>(a, tile, lds);
-    else if (KIND == RVF_K_FP64_QUEUE2) tile_occ_queue<false, 2, false>(a, tile, lds);
-    else if (KIND == RVF_K_FP64_QUEUE3) tile_occ_queue<false, 3, false>(a, tile, lds);
-    else if (KIND == RVF_K_MIX_QUEUE2) tile_occ_queue<true, 2, false>(a, tile, lds);
-    else if (KIND == RVF_K_MIX_QUEUE3) tile_occ_queue<true, 3, false>(a, tile, lds);
-    else if (KIND == RVF_K_MIX_QUEUE4) tile_occ_queue<true, 4, false>(a, tile, lds);
-    else if (KIND == RVF_K_MIX_QUEUE6) tile_occ_queue<true, 6, false>(a, tile, lds);
-    else if (KIND == RVF_K_FP64_SPLIT1) tile_occ_queue<false, 1, true>(a, tile, lds);
-    else if (KIND == RVF_K_FP64_SPLIT2) tile_occ_queue<false, 2, true>(a, tile, lds);
-    else if (KIND == RVF_K_FP64_SPLIT3) tile_occ_queue<false, 3, true>(a, tile, lds);
-    else if (KIND == RVF_K_MIX_SPLIT1) tile_occ_queue<true, 1, true>(a, tile, lds);
-    else if (KIND == RVF_K_MIX_SPLIT2) tile_occ_queue<true, 2, true>(a, tile, lds);
-    else if (KIND == RVF_K_MIX_SPLIT3) tile_occ_queue<true, 3, true>(a, tile, lds);
+    else if (KIND == RVF_K_FP64_QUEUE1) tile_occ_queue<false, 1, false, WS>(a, tile, lds);
+    else if (KIND == RVF_K_FP64_QUEUE2) tile_occ_queue<false, 2, false, WS>(a, tile, lds);
+    else if (KIND == RVF_K_FP64_QUEUE3) tile_occ_queue<false, 3, false, WS>(a, tile, lds);
+    else if (KIND == RVF_K_MIX_QUEUE2) tile_occ_queue<true, 2, false, WS>(a, tile, lds);
+    else if (KIND == RVF_K_MIX_QUEUE3) tile_occ_queue<true, 3, false, WS>(a, tile, lds);
+    else if (KIND == RVF_K_MIX_QUEUE4) tile_occ_queue<true, 4, false, WS>(a, tile, lds);
+    else if (KIND == RVF_K_MIX_QUEUE6) tile_occ_queue<true, 6, false, WS>(a, tile, lds);
+    else if (KIND == RVF_K_FP64_SPLIT1) tile_occ_queue<false, 1, true, WS>(a, tile, lds);
+    else if (KIND == RVF_K_FP64_SPLIT2) tile_occ_queue<false, 2, true, WS>(a, tile, lds);
+    else if (KIND == RVF_K_FP64_SPLIT3) tile_occ_queue<false, 3, true, WS>(a, tile, lds);
+    else if (KIND == RVF_K_MIX_SPLIT1) tile_occ_queue<true, 1, true, WS>(a, tile, lds);
+    else if (KIND == RVF_K_MIX_SPLIT2) tile_occ_queue<true, 2, true, WS>(a, tile, lds);
+    else if (KIND == RVF_K_MIX_SPLIT3) tile_occ_queue<true, 3, true, WS>(a, tile, lds);
     else if (KIND == RVF_K_FP64_2D) tile_fp64<8, true>(a, tile);
     else if (KIND == RVF_K_MIX_2D) tile_mixed<8, true, true>(a, tile);
     else if (KIND == RVF_K_FP64_OCC2D) tile_fp64_occ<true>(a, tile, lds);
@@ -2786,6 +2807,13 @@ __device__ __forceinline__ void pair_body(const PairArgs& a, double* lds) {
     if constexpr (!k_pair_dynamic(TAG::kind)) {
         run_tile<TAG::kind>(a, a.tile0 + (int)blockIdx.x * a.tile_step, lds);
     } else {
+        // the wave's statistics: zeroed here, summed over all its tiles,
+        // flushed once below
+        constexpr int KD = TAG::kind;
+        constexpr bool MIXQ = (KD >= RVF_K_MIX_QUEUE2 && KD <= RVF_K_MIX_QUEUE6) || rvf_mixed_split_kind(KD);
+        uint64_t* wst = occ_wst<MIXQ, rvf_split_kind(KD)>(a, lds);
+        if (threadIdx.x < RVF_QSTATS) wst[threadIdx.x] = 0;
+        __syncthreads();
         // every lane takes part in the claim (lane 0 adds 1, the others 0) and
         // the index is read back with readfirstlane, so the loop and its exit
         // are wave-uniform; each claim advances the counter, so every wave
@@ -2799,16 +2827,18 @@ __device__ __forceinline__ void pair_body(const PairArgs& a, double* lds) {
                 for (;;) {
                     const int w = __builtin_amdgcn_readfirstlane(atomicAdd(ctr, threadIdx.x == 0 ? 1 : 0));
                     if (w >= n) break;
-                    run_tile<TAG::kind>(a, a.xorder[base + w], lds);
+                    run_tile<TAG::kind, true>(a, a.xorder[base + w], lds);
                 }
             }
         } else {
             int w = __builtin_amdgcn_readfirstlane(atomicAdd(a.tile_ctr, threadIdx.x == 0 ? 1 : 0));
             while (w < a.ntiles) {
-                run_tile<TAG::kind>(a, a.tile0 + w * a.tile_step, lds);
+                run_tile<TAG::kind, true>(a, a.tile0 + w * a.tile_step, lds);
                 w = __builtin_amdgcn_readfirstlane(atomicAdd(a.tile_ctr, threadIdx.x == 0 ? 1 : 0));
             }
         }
+        __syncthreads();
+        occ_flush<MIXQ>(a, wst);
     }
 }
 
