@@ -962,8 +962,8 @@ static int pair_kind(const rvf_plan* p);
 static int default_columns(int kind) { return rvf_queue_kind(kind) ? RVF_CHUNK_QUEUE : RVF_CHUNK; }
 static int plan_columns(const rvf_plan* p) { return p->cw_req ? p->cw_req : default_columns(pair_kind(p)); }
 // Lanes with an empty slot before a wave-queue refill.  Measured on the full
-// occluded cube / plates problems, mixed split 3: 16 -> 207.7 / 284.8 ms,
-// 32 -> 206.9 / 278.0 ms, 48 -> 213.5 / 284.1 ms.
+// occluded cube, mixed split 3: 12, 16, 20, 24, 28, 32, 40, 48 -> 182.3,
+// 180.2, 179.1, 178.4, 178.4, 180.9, 183.7, 188.1 ms.
 static int default_refill(int kind) { return rvf_mixed_split_kind(kind) ? 24 : 16; }
 // spatially compact strips for occluded quadrature solves (coherent ray bundles)
 static bool plan_spatial(const rvf_plan* p) {
@@ -1194,7 +1194,7 @@ static PairArgs pair_args(const rvf_plan* p) {
     a.NS = m->NS; a.maxpts = p->maxpts; a.N = m->N;
     a.chunks = L->d_chunks.as<Chunk>();
     // per-XCD claims for the mixed wave-queue kernels (the FP64 ones measured
-    // 1-2 % slower with them: 673 -> 681 ms on the occluded cube)
+    // slower with them: 666.1 -> 668.5 ms on the occluded cube, one run)
     a.xorder = p->opts.precision == RVF_MIXED ? L->d_xorder.as<int32_t>() : nullptr;
     for (int x = 0; x < 9; x++) a.xseg[x] = L->xseg[x];
     a.elem_np = p->elem_np.as<int32_t>(); a.elem_grp = L->d_slot_grp.as<int32_t>(); a.area = p->area.as<double>();

@@ -2797,9 +2797,10 @@ constexpr int k_pair_waves(int kind) { return (kind == RVF_K_MIX_SPLIT3 || kind 
 // xseg[x+1]), from its own counter), then helps the other XCDs in turn.  An
 // XCD's waves then share the point data of fewer strips and the tree nodes
 // their rays pass in their own L2 (one per XCD).  Measured on the occluded
-// cube: 207 -> 194 ms (mixed; the FP64 kernels keep one counter, which they
-// ran 1 % faster with); the same partition with strips dealt round robin,
-// 195 ms.  A tile's results do not depend on which wave runs it.
+// cube: 207 -> 194 ms (mixed; the FP64 kernels keep one counter, with which
+// they ran 666.1 ms against 668.5 in one run); the same partition with strips
+// dealt round robin, 195 ms.  A tile's results do not depend on which wave
+// runs it.
 constexpr bool k_pair_dynamic(int kind) { return rvf_queue_kind(kind); }
 
 template <class TAG>
