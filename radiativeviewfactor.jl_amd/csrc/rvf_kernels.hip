@@ -1762,7 +1762,7 @@ __device__ __forceinline__ void split_drain(const PairArgs& a, const QCtx& X, in
 
 // The subtrees a pair's rays walk (mixed split traversal): a cut of the
 // traversal tree of at most RVF_FRONTIER nodes whose padded boxes the pair's
-// capsule meets, in preorder, written to out[k * 64] (LDS, one column per
+// capsule meets, nearest the pair's axis first, written to out[k * 64] (LDS, one column per
 // lane; RVF_FRONTIER <= 16: the count takes 5 bits of a slot's tag).  Every
 // triangle a ray of the pair can reach lies in a leaf box the capsule meets,
 // and every such leaf lies in one of these subtrees, so walking each subtree
@@ -1837,6 +1837,40 @@ __device__ __forceinline__ int capsule_frontier(const double* ai, const double* 
     }
     // the last round found every entry closed
     for (int v = 0; v < k; v++) out[v * 64] = ~out[v * 64];
+    // Order: entries whose boxes lie nearest the pair's axis first (a stable
+    // sort, preorder among equal keys).  Each entry is walked whole whatever
+    // the order, so visibility is unchanged; a blocked ray stops at its first
+    // blocker, and those lie mostly where the axis crosses the occluder.
+    // key = distance from the box centre to the anchors' line in units of
+    // (R + box half-diagonal) / 8, capped at 31 (5 bits above the node's 27).
+    // Measured on the occluded cube: node steps -15.8 %, 173.7 -> 165.3 ms;
+    // finer keys (16 per unit) 173.8 ms, distance in units of R alone 190.7,
+    // 4 and 6 per unit 165.9 and 165.3.
+    {
+        const float ax = (float)ai[0], ay = (float)ai[1], az = (float)ai[2];
+        float ux = (float)aj[0] - ax, uy = (float)aj[1] - ay, uz = (float)aj[2] - az;
+        const float un = __builtin_amdgcn_rsqf(ux * ux + uy * uy + uz * uz);
+        ux *= un; uy *= un; uz *= un;
+        for (int v = 0; v < k; v++) {
+            const int e = out[v * 64];
+            const TNode32 nd = nodes[e];
+            const float cx = 0.5f * (nd.lo[0] + nd.hi[0]) - ax, cy = 0.5f * (nd.lo[1] + nd.hi[1]) - ay;
+            const float cz = 0.5f * (nd.lo[2] + nd.hi[2]) - az;
+            const float px = cy * uz - cz * uy, py = cz * ux - cx * uz, pz = cx * uy - cy * ux;
+            const float hx = nd.hi[0] - nd.lo[0], hy = nd.hi[1] - nd.lo[1], hz = nd.hi[2] - nd.lo[2];
+            const float dist = sqrtf(px * px + py * py + pz * pz);
+            const float scale = (float)R + 0.5f * sqrtf(hx * hx + hy * hy + hz * hz);
+            const int q = (int)fminf(31.0f, fmaxf(0.0f, 8.0f * dist / scale));
+            out[v * 64] = (q << 27) | e;  // e < 2^27 (tree size limit)
+        }
+        for (int v = 1; v < k; v++) {
+            const int x = out[v * 64];
+            int w = v - 1;
+            while (w >= 0 && out[w * 64] > x) { out[(w + 1) * 64] = out[w * 64]; w--; }
+            out[(w + 1) * 64] = x;
+        }
+        for (int v = 0; v < k; v++) out[v * 64] &= (1 << 27) - 1;
+    }
     return k;
 }
 
