@@ -248,7 +248,9 @@ static inline uint32_t spread10(uint32_t v) {
 
 // Host part of a layout: slot order, group-pure column chunks of at most `cw`
 // columns and the tile list (chunk-major: each chunk's strips consecutively).
-static void layout_host(const rvf_mesh* m, bool spatial, bool self, int cw, Layout& L) {
+// Fails, before the tile list is built, when it would hold 2^31 tiles or more
+// (tile indices are 32-bit; about N^2 / (128 cw) tiles).
+static int layout_host(const rvf_mesh* m, bool spatial, bool self, int cw, Layout& L) {
     int64_t N = m->N, NS = m->NS;
     std::vector<int32_t> order(N);
     for (int64_t e = 0; e < N; e++) order[e] = (int32_t)e;
@@ -293,7 +295,13 @@ static void layout_host(const rvf_mesh* m, bool spatial, bool self, int cw, Layo
     }
     // tiles (strip s, chunk c) exist iff 64 s < last(c); with self pairs
     // iff 64 s <= last(c)
+    int64_t ntiles = 0;
+    for (const Chunk& c : L.chunks) ntiles += (c.end - 1 + (self ? 1 : 0) + RVF_STRIP - 1) / RVF_STRIP;
+    if (ntiles >= ((int64_t)1 << 31))
+        return fail(RVF_E_ARG, "mesh too large for one plan: " + std::to_string(ntiles) +
+                                   " pair tiles (at most 2^31 - 1; about N^2 / (128 x columns per tile))");
     L.tiles.clear();
+    L.tiles.reserve((size_t)ntiles);
     for (size_t c = 0; c < L.chunks.size(); c++) {
         L.chunks[c].first_tile = (int32_t)L.tiles.size();
         int64_t last = L.chunks[c].end - 1 + (self ? 1 : 0);
@@ -309,6 +317,7 @@ static void layout_host(const rvf_mesh* m, bool spatial, bool self, int cw, Layo
     L.xorder.assign(L.tiles.size(), 0);
     for (size_t t = 0; t < L.tiles.size(); t++)
         L.xorder[per[std::min<int64_t>(7, (int64_t)L.tiles[t].x * 8 / nst)]++] = (int32_t)t;
+    return 0;
 }
 
 // Layouts are cached per mesh and (spatial order, self-pair tiles, columns).
@@ -317,8 +326,8 @@ static int get_layout(const rvf_mesh* m, bool spatial, bool self, int cw, const 
     Layout& L = m->lay[(spatial ? 1 : 0) | (self ? 2 : 0) | (cw << 2)];
     if (!L.built) {
         const int64_t NS = m->NS;
-        layout_host(m, spatial, self, cw, L);
-        int rc;
+        int rc = layout_host(m, spatial, self, cw, L);
+        if (rc) return rc;
         const hipStream_t st = m->ctx->stream;
         if ((rc = upload(L.d_slot_orig, L.slot_orig.data(), sizeof(int32_t) * NS, st)) ||
             (rc = upload(L.d_slot_grp, L.slot_grp.data(), sizeof(int32_t) * NS, st)) ||
@@ -950,8 +959,9 @@ struct rvf_plan {
     bool scratch = false;  // buffers are pieces of the static scratch array
 };
 
-static int occ_kind(int variant, bool mixed, int64_t n_tris);
-static const char* occ_kind_error(int variant, int64_t n_tris);
+static int occ_kind(int variant, bool mixed, int64_t n_tris, int64_t pts);
+static const char* occ_kind_error(int variant, int64_t n_tris, bool mixed, int64_t pts);
+static int64_t plan_pts(const rvf_plan* p);
 static int pair_kind(const rvf_plan* p);
 
 // Columns per tile.  The wave-queue kernels claim tiles dynamically and run
@@ -1003,8 +1013,9 @@ static int plan_init(rvf_ctx* ctx, const rvf_mesh* mesh, const rvf_obs* obs, con
     if (obs && (obs->G != (int)mesh->tags.size() || obs->seg != mesh->dim2))
         return fail(RVF_E_ARG, "obstruction set built for another mesh");
     if (obs && !obs->trees.empty() && opts->method != RVF_METHOD_MC &&
-        occ_kind(opts->variant, opts->precision == RVF_MIXED, obs->n_tris) < 0)
-        return fail(RVF_E_ARG, std::string("rvf_plan_create: ") + occ_kind_error(opts->variant, obs->n_tris));
+        occ_kind(opts->variant, opts->precision == RVF_MIXED, obs->n_tris, 0) < 0)
+        return fail(RVF_E_ARG, std::string("rvf_plan_create: ") +
+                                   occ_kind_error(opts->variant, obs->n_tris, opts->precision == RVF_MIXED, 0));
     const bool scratch = src.base != nullptr;
     HIPCHK(hipSetDevice(ctx->device));
     p->ctx = ctx; p->mesh = mesh; p->obs = obs; p->opts = *opts;
@@ -1023,6 +1034,10 @@ static int plan_init(rvf_ctx* ctx, const rvf_mesh* mesh, const rvf_obs* obs, con
     for (int8_t f : mesh->fam) { if (f == 0 || f == 2) anyq = true; else anyt = true; }
     p->maxpts = mesh->dim2 ? p->nq_line : std::max(anyq ? p->nq_quad : 0, anyt ? p->nq_tri : 0);
     if (p->maxpts == 0) p->maxpts = 1;
+    if (obs && !obs->trees.empty() && opts->method != RVF_METHOD_MC && !mesh->dim2 &&
+        occ_kind(opts->variant, opts->precision == RVF_MIXED, obs->n_tris, plan_pts(p)) < 0)
+        return fail(RVF_E_ARG, std::string("rvf_plan_create: ") +
+                                   occ_kind_error(opts->variant, obs->n_tris, opts->precision == RVF_MIXED, plan_pts(p)));
     int64_t N = mesh->N, NS = mesh->NS;
     int G = (int)mesh->tags.size();
     const bool spatial = plan_spatial(p);
@@ -1031,7 +1046,7 @@ static int plan_init(rvf_ctx* ctx, const rvf_mesh* mesh, const rvf_obs* obs, con
     p->cw = plan_columns(p);
     int rc;
     if (scratch) {
-        layout_host(mesh, spatial, self, p->cw, *own_layout);
+        if ((rc = layout_host(mesh, spatial, self, p->cw, *own_layout))) return rc;
         p->L = own_layout;
     } else if ((rc = get_layout(mesh, spatial, self, p->cw, &p->L))) {
         return rc;
@@ -1239,17 +1254,24 @@ static bool plan_occluded_quad(const rvf_plan* p) {
 
 // Kernel kind for an occluded quadrature plan's variant, -1 if invalid.  The
 // split variants address triangles with RVF_TQ_TRI_BITS bits of a task word:
-// with 2^19 or more triangles over all trees they are invalid, and the mixed
-// default falls back to the queue form.
-static int default_variant(bool mixed, int64_t n_tris) {
-    if (!mixed) return RVF_VARIANT_QUEUE(1);
-    return n_tris < ((int64_t)1 << RVF_TQ_TRI_BITS) ? RVF_VARIANT_SPLIT(3) : RVF_VARIANT_QUEUE(3);
+// with 2^19 or more triangles over all trees they are invalid.  The mixed
+// split variants also index the row points (pts = 8 maxpts NS FP32 values,
+// 0 while unknown) with 32-bit offsets.  Past either bound the mixed default
+// falls back to the queue form.
+static constexpr int64_t RVF_SPLIT_PTS_MAX = ((int64_t)1 << 32) - 128;
+static int64_t plan_pts(const rvf_plan* p) { return 8 * (int64_t)p->maxpts * p->mesh->NS; }
+static bool split_ok(bool mixed, int64_t n_tris, int64_t pts) {
+    return n_tris < ((int64_t)1 << RVF_TQ_TRI_BITS) && (!mixed || pts < RVF_SPLIT_PTS_MAX);
 }
-static int occ_kind(int variant, bool mixed, int64_t n_tris) {
-    if (variant == RVF_VARIANT_DEFAULT) variant = default_variant(mixed, n_tris);
+static int default_variant(bool mixed, int64_t n_tris, int64_t pts) {
+    if (!mixed) return RVF_VARIANT_QUEUE(1);
+    return split_ok(mixed, n_tris, pts) ? RVF_VARIANT_SPLIT(3) : RVF_VARIANT_QUEUE(3);
+}
+static int occ_kind(int variant, bool mixed, int64_t n_tris, int64_t pts) {
+    if (variant == RVF_VARIANT_DEFAULT) variant = default_variant(mixed, n_tris, pts);
     if (variant == RVF_VARIANT_LANE) return mixed ? RVF_K_MIX_OCC_LANE : RVF_K_FP64_OCC_LANE;
     if (variant > 20) {  // split node / triangle-task form
-        if (n_tris >= ((int64_t)1 << RVF_TQ_TRI_BITS)) return -1;
+        if (!split_ok(mixed, n_tris, pts)) return -1;
         const int S = variant - 20;
         if (!mixed) return S == 1 ? RVF_K_FP64_SPLIT1 : S == 2 ? RVF_K_FP64_SPLIT2 : S == 3 ? RVF_K_FP64_SPLIT3 : -1;
         return S == 1 ? RVF_K_MIX_SPLIT1 : S == 2 ? RVF_K_MIX_SPLIT2 : S == 3 ? RVF_K_MIX_SPLIT3 : -1;
@@ -1259,9 +1281,11 @@ static int occ_kind(int variant, bool mixed, int64_t n_tris) {
     return S == 2 ? RVF_K_MIX_QUEUE2 : S == 3 ? RVF_K_MIX_QUEUE3 : S == 4 ? RVF_K_MIX_QUEUE4
          : S == 6 ? RVF_K_MIX_QUEUE6 : -1;
 }
-static const char* occ_kind_error(int variant, int64_t n_tris) {
+static const char* occ_kind_error(int variant, int64_t n_tris, bool mixed, int64_t pts) {
     if (variant > 20 && variant <= 23 && n_tris >= ((int64_t)1 << RVF_TQ_TRI_BITS))
         return "split traversal variants need fewer than 2^19 obstruction triangles over all trees";
+    if (variant > 20 && variant <= 23 && mixed && pts >= RVF_SPLIT_PTS_MAX)
+        return "mixed split traversal variants need 8 x (points per element) x (elements) below 2^32";
     return "unknown traversal variant for this precision";
 }
 
@@ -1272,7 +1296,7 @@ static int pair_kind(const rvf_plan* p) {
         if (plan_occluded_quad(p)) return mixed ? RVF_K_MIX_OCC2D : RVF_K_FP64_OCC2D;
         return mixed ? RVF_K_MIX_2D : RVF_K_FP64_2D;
     }
-    if (plan_occluded_quad(p)) return occ_kind(p->opts.variant, mixed, p->obs->n_tris);
+    if (plan_occluded_quad(p)) return occ_kind(p->opts.variant, mixed, p->obs->n_tris, plan_pts(p));
     int pc = p->maxpts;
     if (!mixed && pc > 9) pc = 9;  // register chunks of 9 points
     return rvf_pair_kind(mixed, 0, pc, p->opts.method == RVF_METHOD_MC);
@@ -1449,8 +1473,9 @@ int rvf_plan_tune(rvf_plan* p, int32_t variant, int32_t refill_min) {
     if (!p) return fail(RVF_E_ARG, "rvf_plan_tune: null plan");
     if (refill_min < 0 || refill_min > 64) return fail(RVF_E_ARG, "rvf_plan_tune: refill_min must be in 0..64");
     if (p->obs && !p->obs->trees.empty() && p->opts.method != RVF_METHOD_MC &&
-        occ_kind(variant, p->opts.precision == RVF_MIXED, p->obs->n_tris) < 0)
-        return fail(RVF_E_ARG, std::string("rvf_plan_tune: ") + occ_kind_error(variant, p->obs->n_tris));
+        occ_kind(variant, p->opts.precision == RVF_MIXED, p->obs->n_tris, plan_pts(p)) < 0)
+        return fail(RVF_E_ARG, std::string("rvf_plan_tune: ") +
+                                   occ_kind_error(variant, p->obs->n_tris, p->opts.precision == RVF_MIXED, plan_pts(p)));
     HIPCHK(hipSetDevice(p->ctx->device));
     HIPCHK(hipStreamSynchronize(p->ctx->stream));
     const int v0 = p->opts.variant;
@@ -1551,9 +1576,9 @@ int rvf_plan_autotune(rvf_plan* p, int64_t sample_tiles, rvf_tune_entry* ent, in
     const int64_t ntri = p->obs->n_tris;
     std::vector<int> cands = {RVF_VARIANT_LANE};
     for (int S = 1; S <= 6; S++)
-        if (occ_kind(RVF_VARIANT_QUEUE(S), mixed, ntri) >= 0) cands.push_back(RVF_VARIANT_QUEUE(S));
+        if (occ_kind(RVF_VARIANT_QUEUE(S), mixed, ntri, plan_pts(p)) >= 0) cands.push_back(RVF_VARIANT_QUEUE(S));
     for (int S = 1; S <= 3; S++)
-        if (occ_kind(RVF_VARIANT_SPLIT(S), mixed, ntri) >= 0) cands.push_back(RVF_VARIANT_SPLIT(S));
+        if (occ_kind(RVF_VARIANT_SPLIT(S), mixed, ntri, plan_pts(p)) >= 0) cands.push_back(RVF_VARIANT_SPLIT(S));
     // a measured configuration as it ran (refill 0 stands for the kernel's default)
     auto add = [&](int v, int refill, float ms, int waves = 0) {
         const int r = v == RVF_VARIANT_LANE ? 0 : refill ? refill : default_refill(pair_kind(p));
@@ -1574,7 +1599,7 @@ int rvf_plan_autotune(rvf_plan* p, int64_t sample_tiles, rvf_tune_entry* ent, in
     //    their defaults) is timed twice there: another configuration replaces the
     //    current choice only if faster by more than twice their spread (and
     //    at least 1 %), so run-to-run noise does not swap the kernel that runs.
-    const int vdef = default_variant(mixed, ntri);
+    const int vdef = default_variant(mixed, ntri, plan_pts(p));
     int best_v = vdef, best_r = 0, best_c = 0;
     float best = 3.4e38f, margin = 0.01f;
     if (!rc) {

@@ -1080,6 +1080,13 @@ struct QCtx {
     const int* lfr;      // LDS (mixed, split form): per lane, the frontier subtrees its pair's rays walk, [k * 64 + lane]
     const float* stage;  // LDS (mixed, split form): per lane [dax day daz fax fay faz] x 64
     float fjx, fjy, fjz; // column element's anchor as FP32
+    // (mixed, split form) the strip's FP32 row points and the column
+    // element's FP32 points, indexed with 32-bit offsets (the host keeps
+    // 8 maxpts NS below 2^32 for these kernels): 64-bit index products are
+    // quarter-rate instructions on this part
+    const float* p32;    // a.P32 + sbase
+    const float* q32;    // a.Q32 + j maxpts 8
+    uint32_t ns32;       // a.NS
 };
 
 // Per-wave statistics of the wave-queue kernels, kept in LDS (u64 each) so
@@ -1466,15 +1473,15 @@ __device__ __forceinline__ bool ttest(const PairArgs& a, const QCtx& X, int c, i
 
 // Mixed split traversal: the FP32 ray of combination c and lane b from the
 // staged anchored offsets, as the kernel evaluation forms r.  dd bounds the
-// direction's error (the classifier's margins).
-__device__ __forceinline__ SRay sray(const PairArgs& a, const QCtx& X, int c, int b) {
+// direction's error (the classifier's margins).  32-bit offsets into the
+// strip's points: 164.7 -> 162.8 ms on the occluded cube against 64-bit ones.
+__device__ __forceinline__ SRay sray(const PairArgs&, const QCtx& X, int c, int b) {
     int p, q;
     combo_pq(X, c, p, q);
-    const int64_t ib = X.sbase + b;
-    const int64_t NS = a.NS;
-    const float dxi = a.P32[(int64_t)(p * 8 + 0) * NS + ib], dyi = a.P32[(int64_t)(p * 8 + 1) * NS + ib];
-    const float dzi = a.P32[(int64_t)(p * 8 + 2) * NS + ib];
-    const float* qj = a.Q32 + (X.j * a.maxpts + q) * 8;
+    const uint32_t o = (uint32_t)(p * 8) * X.ns32 + (uint32_t)b;
+    const float dxi = X.p32[o], dyi = X.p32[o + X.ns32];
+    const float dzi = X.p32[o + 2 * X.ns32];
+    const float* qj = X.q32 + q * 8;
     const float qx = qj[0], qy = qj[1], qz = qj[2];
     const float dax = X.stage[0 * 64 + b], day = X.stage[1 * 64 + b], daz = X.stage[2 * 64 + b];
     const float fax = X.stage[3 * 64 + b], fay = X.stage[4 * 64 + b], faz = X.stage[5 * 64 + b];
@@ -2122,6 +2129,8 @@ __device__ __forceinline__ void tile_occ_queue(const PairArgs& a, int tile, doub
     double ls = 0.0, lc = 0.0;
     QCtx X;
     X.sbase = (int64_t)t.s * RVF_STRIP;
+    X.p32 = a.P32 + X.sbase;
+    X.ns32 = (uint32_t)a.NS;
     X.lroot = lroot;
     X.lhint = lhint;
     X.stage = stage;
@@ -2165,6 +2174,7 @@ __device__ __forceinline__ void tile_occ_queue(const PairArgs& a, int tile, doub
             }
         }
         X.j = j; X.oj = oj; X.npj = npj; X.gj = gj;
+        X.q32 = a.Q32 + j * a.maxpts * 8;
         X.revmask = __ballot(rev);
         lroot[lane] = root;
         lhint[lane] = -1;

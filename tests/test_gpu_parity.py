@@ -349,3 +349,23 @@ def test_split_variants_refused_past_task_index_width(orc):
     mx = gpu(m, 2, [3], "mixed", "geometric", audit=True)
     assert np.array_equal(mx["ac"][iu], g["ac"][iu]) and np.array_equal(mx["ah"][iu], g["ah"][iu])
     assert float(np.abs(mx["F"] - g["F"]).max()) <= 1e-6
+
+
+def test_plan_size_guards_refuse_before_allocating():
+    # 2 x 2048^2 = 8,388,608 elements at nquad 8 (64 points per element): the
+    # mixed split kernels index the row points (8 x 64 x N FP32 values = 2^32)
+    # with 32-bit offsets, so they are refused; the default mixed plan (the
+    # queue form) then needs ~1.4e11 pair tiles, past the 2^31 tile indices,
+    # and is refused before its tile list is built
+    m = M.parallel_plates(2048)
+    assert m.n_elems == 1 << 23
+    m.group_tri_soup = {3: np.array([[[0.4, 0.4, 0.5], [0.6, 0.4, 0.5], [0.5, 0.6, 0.5]]])}
+    m.soup_layout = "xyz_vertex"
+    ctx = api.context(0)
+    dm = api.DeviceMesh(ctx, m)
+    do = api.DeviceObstruction(ctx, dm, [3], "geometric")
+    with pytest.raises(api.RVFError, match="2\\^32"):
+        api.Plan(ctx, dm, do, 8, "quad", "mixed", want_F=False, variant=api.VARIANT_SPLIT(3))
+    with pytest.raises(api.RVFError, match="mesh too large for one plan"):
+        api.Plan(ctx, dm, do, 8, "quad", "mixed", want_F=False)
+    do.close(); dm.close()
