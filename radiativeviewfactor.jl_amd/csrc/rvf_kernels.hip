@@ -1080,12 +1080,17 @@ struct QCtx {
     const int* lfr;      // LDS (mixed, split form): per lane, the frontier subtrees its pair's rays walk, [k * 64 + lane]
     const float* stage;  // LDS (mixed, split form): per lane [dax day daz fax fay faz] x 64
     float fjx, fjy, fjz; // column element's anchor as FP32
-    // (mixed, split form) the strip's FP32 row points and the column
-    // element's FP32 points, indexed with 32-bit offsets (the host keeps
-    // 8 maxpts NS below 2^32 for these kernels): 64-bit index products are
-    // quarter-rate instructions on this part
+    // the strip's row points and the column element's points (FP32 and
+    // FP64), indexed with 32-bit offsets: every plan has fewer than 2^31 pair
+    // tiles of at most 16 columns, so NS < 2.1M and 8 maxpts NS < 2^30
+    // (maxpts <= RVF_MAXPTS = 64); the host also refuses the mixed split
+    // variants past 2^32.  Against 64-bit index arithmetic: mixed 164.5 ->
+    // 162.7 ms (FP32 points), then 163.4 -> 162.6 ms and FP64 669.1 -> 665.8
+    // ms (FP64 points) on the occluded cube.
     const float* p32;    // a.P32 + sbase
     const float* q32;    // a.Q32 + j maxpts 8
+    const double* p64;   // a.P64 + sbase
+    const double* q64;   // a.Q64 + j maxpts 8
     uint32_t ns32;       // a.NS
 };
 
@@ -1152,11 +1157,10 @@ __device__ __forceinline__ int combo_index(int p, int q, int p0, int npj) {
 __device__ __forceinline__ int qray_setup(const PairArgs& a, const QCtx& X, int c, int b, QRay<false>& R) {
     int p, q;
     combo_pq(X, c, p, q);
-    const int64_t ib = X.sbase + b;
-    const int64_t NS = a.NS;
-    const double xi = a.P64[(int64_t)(p * 8 + 0) * NS + ib], yi = a.P64[(int64_t)(p * 8 + 1) * NS + ib];
-    const double zi = a.P64[(int64_t)(p * 8 + 2) * NS + ib];
-    const double* qj = a.Q64 + (X.j * a.maxpts + q) * 8;
+    const uint32_t o = (uint32_t)(p * 8) * X.ns32 + (uint32_t)b;
+    const double xi = X.p64[o], yi = X.p64[o + X.ns32];
+    const double zi = X.p64[o + 2 * X.ns32];
+    const double* qj = X.q64 + q * 8;
     const double xj = qj[0], yj = qj[1], zj = qj[2];
     const bool rev = (X.revmask >> b) & 1;
     const double r0 = xj - xi, r1 = yj - yi, r2 = zj - zi;
@@ -1211,11 +1215,10 @@ __device__ __forceinline__ int qray_setup(const PairArgs& a, const QCtx& X, int 
 __device__ __forceinline__ bool qexact_hit(const PairArgs& a, const QCtx& X, int c, int b, int tri, int leaf) {
     int p, q;
     combo_pq(X, c, p, q);
-    const int64_t ib = X.sbase + b;
-    const int64_t NS = a.NS;
-    const double xi = a.P64[(int64_t)(p * 8 + 0) * NS + ib], yi = a.P64[(int64_t)(p * 8 + 1) * NS + ib];
-    const double zi = a.P64[(int64_t)(p * 8 + 2) * NS + ib];
-    const double* qj = a.Q64 + (X.j * a.maxpts + q) * 8;
+    const uint32_t o = (uint32_t)(p * 8) * X.ns32 + (uint32_t)b;
+    const double xi = X.p64[o], yi = X.p64[o + X.ns32];
+    const double zi = X.p64[o + 2 * X.ns32];
+    const double* qj = X.q64 + q * 8;
     const double xj = qj[0], yj = qj[1], zj = qj[2];
     const bool rev = (X.revmask >> b) & 1;
     const double r0 = xj - xi, r1 = yj - yi, r2 = zj - zi;
@@ -1455,11 +1458,10 @@ __device__ __forceinline__ const BvhNode64* nnodes<false>(const PairArgs& a) { r
 __device__ __forceinline__ bool ttest(const PairArgs& a, const QCtx& X, int c, int b, int tri, uint32_t&) {
     int p, q;
     combo_pq(X, c, p, q);
-    const int64_t ib = X.sbase + b;
-    const int64_t NS = a.NS;
-    const double xi = a.P64[(int64_t)(p * 8 + 0) * NS + ib], yi = a.P64[(int64_t)(p * 8 + 1) * NS + ib];
-    const double zi = a.P64[(int64_t)(p * 8 + 2) * NS + ib];
-    const double* qj = a.Q64 + (X.j * a.maxpts + q) * 8;
+    const uint32_t o = (uint32_t)(p * 8) * X.ns32 + (uint32_t)b;
+    const double xi = X.p64[o], yi = X.p64[o + X.ns32];
+    const double zi = X.p64[o + 2 * X.ns32];
+    const double* qj = X.q64 + q * 8;
     const double xj = qj[0], yj = qj[1], zj = qj[2];
     const bool rev = (X.revmask >> b) & 1;
     const double r0 = xj - xi, r1 = yj - yi, r2 = zj - zi;
@@ -1473,8 +1475,7 @@ __device__ __forceinline__ bool ttest(const PairArgs& a, const QCtx& X, int c, i
 
 // Mixed split traversal: the FP32 ray of combination c and lane b from the
 // staged anchored offsets, as the kernel evaluation forms r.  dd bounds the
-// direction's error (the classifier's margins).  32-bit offsets into the
-// strip's points: 164.7 -> 162.8 ms on the occluded cube against 64-bit ones.
+// direction's error (the classifier's margins).
 __device__ __forceinline__ SRay sray(const PairArgs&, const QCtx& X, int c, int b) {
     int p, q;
     combo_pq(X, c, p, q);
@@ -2130,6 +2131,7 @@ __device__ __forceinline__ void tile_occ_queue(const PairArgs& a, int tile, doub
     QCtx X;
     X.sbase = (int64_t)t.s * RVF_STRIP;
     X.p32 = a.P32 + X.sbase;
+    X.p64 = a.P64 + X.sbase;
     X.ns32 = (uint32_t)a.NS;
     X.lroot = lroot;
     X.lhint = lhint;
@@ -2175,6 +2177,7 @@ __device__ __forceinline__ void tile_occ_queue(const PairArgs& a, int tile, doub
         }
         X.j = j; X.oj = oj; X.npj = npj; X.gj = gj;
         X.q32 = a.Q32 + j * a.maxpts * 8;
+        X.q64 = a.Q64 + j * a.maxpts * 8;
         X.revmask = __ballot(rev);
         lroot[lane] = root;
         lhint[lane] = -1;
