@@ -1880,9 +1880,11 @@ __device__ __forceinline__ int capsule_frontier(const double* ai, const double* 
             const float cz = 0.5f * (nd.lo[2] + nd.hi[2]) - az;
             const float px = cy * uz - cz * uy, py = cz * ux - cx * uz, pz = cx * uy - cy * ux;
             const float hx = nd.hi[0] - nd.lo[0], hy = nd.hi[1] - nd.lo[1], hz = nd.hi[2] - nd.lo[2];
-            const float dist = sqrtf(px * px + py * py + pz * pz);
-            const float scale = (float)R + 0.5f * sqrtf(hx * hx + hy * hy + hz * hz);
-            const int q = (int)fminf(31.0f, fmaxf(0.0f, 8.0f * dist / scale));
+            // hardware square root and reciprocal: the key orders the walk and
+            // nothing else (160.7 against 161.2 ms with IEEE sqrtf and division)
+            const float dist = __builtin_amdgcn_sqrtf(px * px + py * py + pz * pz);
+            const float scale = (float)R + 0.5f * __builtin_amdgcn_sqrtf(hx * hx + hy * hy + hz * hz);
+            const int q = (int)fminf(31.0f, fmaxf(0.0f, 8.0f * dist * __builtin_amdgcn_rcpf(scale)));
             out[v * 64] = (q << 27) | e;  // e < 2^27 (tree size limit)
         }
         for (int v = 1; v < k; v++) {
