@@ -2843,8 +2843,20 @@ RVF_PAIR_KINDS(RVF_KIND_TAG)
 // Resident waves per SIMD the compiler must allow (1 = no constraint).  The
 // mixed split traversal waits on node fetches more than on arithmetic; at
 // 128 VGPRs four of its waves share a SIMD instead of three (its few spills
-// are in the per-column setup, outside the traversal loop).
-constexpr int k_pair_waves(int kind) { return (kind == RVF_K_MIX_SPLIT3 || kind == RVF_K_MIX_SPLIT2) ? 4 : 1; }
+// are in the per-column setup, outside the traversal loop).  Others held
+// just past a threshold, measured on the bench workloads: the FP64 queue
+// kernel (135 VGPRs, three waves) at 128: 664.5 -> 657.0 ms on the occluded
+// cube; Monte Carlo (232, two waves) at 168: 701 -> 653 ms on the cylinder;
+// FP64 points-in-registers (214) at 168: 23.7 -> 23.5 ms on the unoccluded
+// cube, Duffy pairs 119.0 -> 117.4 ms on the cylinder; curves (FP64 200 at
+// 168, mixed 136 at 128): 2.42 -> 2.29 and 0.56 -> 0.52 ms on the circle.
+// The mixed points-in-registers kernel (148) at 128 was slower (4.12 -> 4.25).
+constexpr int k_pair_waves(int kind) {
+    return (kind == RVF_K_MIX_SPLIT3 || kind == RVF_K_MIX_SPLIT2 || kind == RVF_K_FP64_QUEUE1 || kind == RVF_K_MIX_2D)
+               ? 4
+           : (kind == RVF_K_MC || kind == RVF_K_FP64_PC9 || kind == RVF_K_FP64_2D) ? 3
+                                                                                    : 1;
+}
 
 // Tile costs of the wave-queue kernels vary by orders of magnitude with where
 // the pairs sit relative to the occluders.  Their launches are persistent: as
