@@ -2850,12 +2850,15 @@ RVF_PAIR_KINDS(RVF_KIND_TAG)
 // FP64 points-in-registers (214) at 168: 23.7 -> 23.5 ms on the unoccluded
 // cube, Duffy pairs 119.0 -> 117.4 ms on the cylinder; curves (FP64 200 at
 // 168, mixed 136 at 128): 2.42 -> 2.29 and 0.56 -> 0.52 ms on the circle.
-// The mixed points-in-registers kernel (148) at 128 was slower (4.12 -> 4.25).
+// The mixed chunked-points kernel (196) at 168: 37.1 -> 30.3 ms on the cube
+// at nquad 5.  Slower, so left alone: the mixed points-in-registers kernels
+// at one more wave (9 points, 148 VGPRs at 128: 4.12 -> 4.25 ms; 16 points,
+// 232 at 168: 13.6 -> 15.3 ms at nquad 4).
 constexpr int k_pair_waves(int kind) {
     return (kind == RVF_K_MIX_SPLIT3 || kind == RVF_K_MIX_SPLIT2 || kind == RVF_K_FP64_QUEUE1 || kind == RVF_K_MIX_2D)
                ? 4
-           : (kind == RVF_K_MC || kind == RVF_K_FP64_PC9 || kind == RVF_K_FP64_2D) ? 3
-                                                                                    : 1;
+           : (kind == RVF_K_MC || kind == RVF_K_FP64_PC9 || kind == RVF_K_FP64_2D || kind == RVF_K_MIX_PC16C) ? 3
+                                                                                                               : 1;
 }
 
 // Tile costs of the wave-queue kernels vary by orders of magnitude with where
