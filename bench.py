@@ -93,14 +93,15 @@ def run_mode(api, ctx, dm, do, nquad, precision, steps, warmup, audit=False, tun
     return plan, total_ms / steps, np.array(lat), np.array(pair_ms)
 
 
-def end_to_end(plan, reps=3):
+def end_to_end(plan, reps=3, F_out=None):
     """Solve plus the D2H copy of F (N x N doubles), areas, row sums and the
-    group sums, as the reference's hook returns them (GPUAssembly.jl:83-104)."""
+    group sums, as the reference's hook returns them (GPUAssembly.jl:83-104):
+    into new pageable arrays each time, or (F_out) into one reused buffer."""
     ms = []
     for _ in range(reps):
         t0 = time.perf_counter()
         plan.run(); plan.sync()
-        plan.fetch()
+        plan.fetch(F_out=F_out)
         ms.append((time.perf_counter() - t0) * 1e3)
     return float(np.median(ms))
 
@@ -229,6 +230,9 @@ def main():
                                            min(args.warmup, 1), tune=tune_d, lat_min=10)
     st_d = plan_d.stats()
     e2e_m = end_to_end(plan_m)
+    Fpin = api.pinned_empty((N, N))
+    e2e_pin = end_to_end(plan_m, F_out=Fpin)
+    del Fpin
     graph_m = graph_steady_state(api, ctx, dm, do, nquad, "mixed", tune_m)
 
     # verification of mixed against FP64 on the same inputs (outside the timed region)
@@ -294,7 +298,9 @@ def main():
                        "solves": int(len(lat_m)), "fp64_p50": float(np.percentile(lat_d, 50)),
                        "fp64_p99": float(np.percentile(lat_d, 99)), "fp64_solves": int(len(lat_d))},
         "end_to_end": {"ms": e2e_m, "pairs_per_s": ordered / (e2e_m * 1e-3),
-                       "includes": "solve + D2H of F (N^2 x 8 B) + areas, row sums, group sums (PCIe-inclusive)"},
+                       "includes": "solve + D2H of F (N^2 x 8 B) + areas, row sums, group sums (PCIe-inclusive)",
+                       "ms_pinned_F": e2e_pin,
+                       "pinned_F": "the same into one reused page-locked F buffer (api.pinned_empty)"},
         "graph_replay": graph_m,
         "kernel_ms": {"mixed": {k: st_m[k] for k in ("precompute_ms", "pair_ms", "fill_ms", "reduce_ms")},
                       "fp64": {k: st_d[k] for k in ("precompute_ms", "pair_ms", "fill_ms", "reduce_ms")}},

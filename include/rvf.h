@@ -166,6 +166,12 @@ int rvf_plan_sync(rvf_plan* plan);
 int rvf_plan_fetch(rvf_plan* plan, double* F, double* A, double* rowsum, double* F_group, double* A_group,
                    int32_t* group_tags, uint32_t* audit_count, uint32_t* audit_hash);
 int rvf_plan_stats(const rvf_plan* plan, rvf_stats* st);
+/* Page-locked host memory (no counterpart in the reference: CUDA.jl copies F
+ * into pageable memory, src/GPUAssembly.jl:83-84).  rvf_plan_fetch into such
+ * a buffer is one DMA transfer at PCIe speed instead of a staged copy; reuse
+ * it across solves.  rvf_host_free releases it (null is ignored). */
+int rvf_host_alloc(size_t bytes, void** out);
+int rvf_host_free(void* ptr);
 /* device pointer of an output (0 = F, 1 = A, 2 = rowsum) for device-side use */
 int rvf_plan_device_ptr(const rvf_plan* plan, int which, void** ptr);
 /* Tuning knobs of an existing plan (no counterpart in the reference; used by

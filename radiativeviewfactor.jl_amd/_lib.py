@@ -102,6 +102,8 @@ def lib():
         "rvf_plan_sync": ([P], C.c_int),
         "rvf_plan_fetch": ([P, P, P, P, P, P, P, P, P], C.c_int),
         "rvf_plan_stats": ([P, P], C.c_int),
+        "rvf_host_alloc": ([C.c_size_t, PP], C.c_int),
+        "rvf_host_free": ([P], C.c_int),
         "rvf_plan_device_ptr": ([P, C.c_int, PP], C.c_int),
         "rvf_plan_tune": ([P, C.c_int32, C.c_int32], C.c_int),
         "rvf_plan_tune_tiles": ([P, C.c_int32], C.c_int),

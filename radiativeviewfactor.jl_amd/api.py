@@ -457,11 +457,21 @@ class Plan:
         check(_lib.lib().rvf_plan_stats(self.h, C.byref(s)))
         return s.as_dict()
 
-    def fetch(self, F=True):
+    def fetch(self, F=True, F_out: Optional[np.ndarray] = None):
+        """Host copies of the results.  F_out: an N x N float64 Fortran-order
+        array to fill instead of a new one (e.g. from pinned_empty, which makes
+        the copy one DMA transfer)."""
         mesh = self.dmesh.mesh
         N = mesh.n_elems
         G = len(self.dmesh.tags)
-        Fm = np.empty((N, N), order="F") if (F and self.want_F) else None
+        Fm = None
+        if F and self.want_F:
+            if F_out is not None:
+                if F_out.shape != (N, N) or F_out.dtype != np.float64 or not F_out.flags.f_contiguous:
+                    raise ValueError("F_out must be an N x N float64 Fortran-order array")
+                Fm = F_out
+            else:
+                Fm = np.empty((N, N), order="F")
         A = np.empty(N); rs = np.empty(N)
         Fg = np.empty((G, G), order="F"); Ag = np.empty(G); tags = np.empty(G, np.int32)
         ac = np.empty((N, N), np.uint32, order="F") if self.audit else None
@@ -480,6 +490,32 @@ class Plan:
             self.close()
         except Exception:
             pass
+
+
+class _PinnedBlock:
+    """Owns page-locked host memory from rvf_host_alloc; freed with the
+    last array that views it."""
+
+    def __init__(self, nbytes: int):
+        p = C.c_void_p()
+        check(_lib.lib().rvf_host_alloc(nbytes, C.byref(p)))
+        self.ptr, self.nbytes = p.value, nbytes
+
+    def __del__(self):
+        if getattr(self, "ptr", None):
+            _lib.lib().rvf_host_free(C.c_void_p(self.ptr))
+            self.ptr = None
+
+
+def pinned_empty(shape, dtype=np.float64, order: str = "F") -> np.ndarray:
+    """An uninitialised array in page-locked host memory (no reference
+    counterpart): Plan.fetch(F_out=...) into it copies F by DMA."""
+    dt = np.dtype(dtype)
+    n = int(np.prod(shape)) * dt.itemsize
+    blk = _PinnedBlock(max(n, 1))
+    buf = (C.c_char * max(n, 1)).from_address(blk.ptr)
+    buf._rvf_owner = blk  # the ctypes buffer keeps the block alive; the array keeps the buffer
+    return np.ndarray(shape, dtype=dt, buffer=buf, order=order)
 
 
 _CTX: Dict[int, Context] = {}

@@ -1752,6 +1752,22 @@ int rvf_plan_sync(rvf_plan* p) {
     return 0;
 }
 
+int rvf_host_alloc(size_t bytes, void** out) {
+    if (!out || bytes == 0) return fail(RVF_E_ARG, "rvf_host_alloc: bad arguments");
+    *out = nullptr;
+    const hipError_t e = hipHostMalloc(out, bytes, hipHostMallocDefault);
+    if (e != hipSuccess) {
+        *out = nullptr;
+        return fail(RVF_E_NOMEM, std::string("rvf_host_alloc: ") + hipGetErrorString(e));
+    }
+    return 0;
+}
+
+int rvf_host_free(void* ptr) {
+    if (ptr) HIPCHK(hipHostFree(ptr));
+    return 0;
+}
+
 int rvf_plan_fetch(rvf_plan* p, double* F, double* A, double* rowsum, double* F_group, double* A_group,
                    int32_t* group_tags, uint32_t* audit_count, uint32_t* audit_hash) {
     if (!p || !p->ran) return fail(RVF_E_STATE, "rvf_plan_fetch: plan has not run");

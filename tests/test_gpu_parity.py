@@ -351,6 +351,31 @@ def test_split_variants_refused_past_task_index_width(orc):
     assert float(np.abs(mx["F"] - g["F"]).max()) <= 1e-6
 
 
+def test_fetch_into_pinned_buffer():
+    # F copied into a reused page-locked buffer equals the pageable copy, bit
+    # for bit, over two solves (the buffer keeps no stale values); wrong shapes
+    # are refused
+    m = M.parallel_plates(10)
+    ctx = api.context(0)
+    dm = api.DeviceMesh(ctx, m)
+    plan = api.Plan(ctx, dm, None, 2, "quad", "mixed", want_F=True)
+    N = m.n_elems
+    buf = api.pinned_empty((N, N))
+    buf[:] = np.nan
+    for _ in range(2):
+        plan.run(); plan.sync()
+        F, A, rs, *_ = plan.fetch()
+        Fp, Ap, rsp, *_ = plan.fetch(F_out=buf)
+        assert Fp is buf and np.array_equal(Fp, F) and np.array_equal(Ap, A) and np.array_equal(rsp, rs)
+        buf[:] = -1.0
+    with pytest.raises(ValueError):
+        plan.fetch(F_out=np.empty((N, N)))  # C order
+    with pytest.raises(ValueError):
+        plan.fetch(F_out=api.pinned_empty((N, N - 1)))
+    del buf
+    plan.close(); dm.close()
+
+
 def test_plan_size_guards_refuse_before_allocating():
     # 2 x 2048^2 = 8,388,608 elements at nquad 8 (64 points per element): the
     # mixed split kernels index the row points (8 x 64 x N FP32 values = 2^32)
