@@ -2227,10 +2227,10 @@ __device__ __forceinline__ void tile_occ_queue(const PairArgs& a, int tile, doub
             for (int p = p0; p < p1 && trace; p++) {
                 const bool pv = act && p < npi && root >= 0;
                 if (MIX) {
-                    const float dxi = P[(int64_t)(p * 8 + 0) * NS + t.i], dyi = P[(int64_t)(p * 8 + 1) * NS + t.i];
-                    const float dzi = P[(int64_t)(p * 8 + 2) * NS + t.i];
-                    const float nxi = P[(int64_t)(p * 8 + 3) * NS + t.i], nyi = P[(int64_t)(p * 8 + 4) * NS + t.i];
-                    const float nzi = P[(int64_t)(p * 8 + 5) * NS + t.i];
+                    const float dxi = X.p32[(uint32_t)(p * 8 + 0) * X.ns32 + (uint32_t)lane], dyi = X.p32[(uint32_t)(p * 8 + 1) * X.ns32 + (uint32_t)lane];
+                    const float dzi = X.p32[(uint32_t)(p * 8 + 2) * X.ns32 + (uint32_t)lane];
+                    const float nxi = X.p32[(uint32_t)(p * 8 + 3) * X.ns32 + (uint32_t)lane], nyi = X.p32[(uint32_t)(p * 8 + 4) * X.ns32 + (uint32_t)lane];
+                    const float nzi = X.p32[(uint32_t)(p * 8 + 5) * X.ns32 + (uint32_t)lane];
                     const float bx = dax - dxi, by = day - dyi, bz = daz - dzi;
                     for (int q = 0; q < npj; q++) {
                         float rx, ry, rz, r2;
@@ -2243,11 +2243,11 @@ __device__ __forceinline__ void tile_occ_queue(const PairArgs& a, int tile, doub
                         if (lane == 0) need[c] = ndm & capm;
                     }
                 } else {
-                    const double xi = P64[(int64_t)(p * 8 + 0) * NS + t.i], yi = P64[(int64_t)(p * 8 + 1) * NS + t.i];
-                    const double zi = P64[(int64_t)(p * 8 + 2) * NS + t.i];
-                    const double nxi = P64[(int64_t)(p * 8 + 3) * NS + t.i];
-                    const double nyi = P64[(int64_t)(p * 8 + 4) * NS + t.i];
-                    const double nzi = P64[(int64_t)(p * 8 + 5) * NS + t.i];
+                    const double xi = X.p64[(uint32_t)(p * 8 + 0) * X.ns32 + (uint32_t)lane], yi = X.p64[(uint32_t)(p * 8 + 1) * X.ns32 + (uint32_t)lane];
+                    const double zi = X.p64[(uint32_t)(p * 8 + 2) * X.ns32 + (uint32_t)lane];
+                    const double nxi = X.p64[(uint32_t)(p * 8 + 3) * X.ns32 + (uint32_t)lane];
+                    const double nyi = X.p64[(uint32_t)(p * 8 + 4) * X.ns32 + (uint32_t)lane];
+                    const double nzi = X.p64[(uint32_t)(p * 8 + 5) * X.ns32 + (uint32_t)lane];
                     for (int q = 0; q < npj; q++) {
                         Ray64 Rr;
                         const double K = vf_exact(xi, yi, zi, nxi, nyi, nzi, qj64[q * 8 + 0], qj64[q * 8 + 1],
@@ -2274,10 +2274,10 @@ __device__ __forceinline__ void tile_occ_queue(const PairArgs& a, int tile, doub
             uint32_t wcl = 0;  // rays of a skipped trace (all culled)
             for (int p = p0; p < p1; p++) {
                 if (MIX) {
-                    const float dxi = P[(int64_t)(p * 8 + 0) * NS + t.i], dyi = P[(int64_t)(p * 8 + 1) * NS + t.i];
-                    const float dzi = P[(int64_t)(p * 8 + 2) * NS + t.i];
-                    const float nxi = P[(int64_t)(p * 8 + 3) * NS + t.i], nyi = P[(int64_t)(p * 8 + 4) * NS + t.i];
-                    const float nzi = P[(int64_t)(p * 8 + 5) * NS + t.i];
+                    const float dxi = X.p32[(uint32_t)(p * 8 + 0) * X.ns32 + (uint32_t)lane], dyi = X.p32[(uint32_t)(p * 8 + 1) * X.ns32 + (uint32_t)lane];
+                    const float dzi = X.p32[(uint32_t)(p * 8 + 2) * X.ns32 + (uint32_t)lane];
+                    const float nxi = X.p32[(uint32_t)(p * 8 + 3) * X.ns32 + (uint32_t)lane], nyi = X.p32[(uint32_t)(p * 8 + 4) * X.ns32 + (uint32_t)lane];
+                    const float nzi = X.p32[(uint32_t)(p * 8 + 5) * X.ns32 + (uint32_t)lane];
                     const float bx = dax - dxi, by = day - dyi, bz = daz - dzi;
                     float inner = 0.0f;
                     if (trace) {
@@ -2310,16 +2310,16 @@ __device__ __forceinline__ void tile_occ_queue(const PairArgs& a, int tile, doub
                         }
                     }
                     if (p < npi) {
-                        double wd = P64[(int64_t)(p * 8 + 6) * NS + t.i] * P64[(int64_t)(p * 8 + 7) * NS + t.i];
+                        double wd = X.p64[(uint32_t)(p * 8 + 6) * X.ns32 + (uint32_t)lane] * X.p64[(uint32_t)(p * 8 + 7) * X.ns32 + (uint32_t)lane];
                         neumaier(Fs, Fc, wd * (double)inner);
                     }
                 } else {
-                    const double xi = P64[(int64_t)(p * 8 + 0) * NS + t.i], yi = P64[(int64_t)(p * 8 + 1) * NS + t.i];
-                    const double zi = P64[(int64_t)(p * 8 + 2) * NS + t.i];
-                    const double nxi = P64[(int64_t)(p * 8 + 3) * NS + t.i];
-                    const double nyi = P64[(int64_t)(p * 8 + 4) * NS + t.i];
-                    const double nzi = P64[(int64_t)(p * 8 + 5) * NS + t.i];
-                    const double wi = P64[(int64_t)(p * 8 + 6) * NS + t.i], dAi = P64[(int64_t)(p * 8 + 7) * NS + t.i];
+                    const double xi = X.p64[(uint32_t)(p * 8 + 0) * X.ns32 + (uint32_t)lane], yi = X.p64[(uint32_t)(p * 8 + 1) * X.ns32 + (uint32_t)lane];
+                    const double zi = X.p64[(uint32_t)(p * 8 + 2) * X.ns32 + (uint32_t)lane];
+                    const double nxi = X.p64[(uint32_t)(p * 8 + 3) * X.ns32 + (uint32_t)lane];
+                    const double nyi = X.p64[(uint32_t)(p * 8 + 4) * X.ns32 + (uint32_t)lane];
+                    const double nzi = X.p64[(uint32_t)(p * 8 + 5) * X.ns32 + (uint32_t)lane];
+                    const double wi = X.p64[(uint32_t)(p * 8 + 6) * X.ns32 + (uint32_t)lane], dAi = X.p64[(uint32_t)(p * 8 + 7) * X.ns32 + (uint32_t)lane];
                     double inner = 0.0;
                     for (int q = 0; q < npj; q++) {
                         Ray64 Rr;
