@@ -242,7 +242,8 @@ int rvf_compute_view_factors(rvf_ctx* ctx, const rvf_mesh* mesh, const rvf_obs* 
  * F_dev is non-null, is written to that caller-owned device buffer (N x N
  * doubles, column-major), the other results to host memory as
  * rvf_compute_view_factors writes them.  A solve whose intermediates exceed
- * the array (at nquad 3 about 6,700 elements with occluders, 10,700 without)
+ * the array (at nquad 3 about 5,000 elements with occluders in mixed precision,
+ * 3,700 in FP64, 10,700 without occluders)
  * returns RVF_E_NOMEM with the size it needs; RVF_AUDIT, RVF_GRAPH and
  * RVF_TIMING are refused (RVF_E_ARG).
  * Every call on a GPU shares that GPU's array: calls that overlap on one GPU

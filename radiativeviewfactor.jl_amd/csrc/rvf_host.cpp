@@ -967,10 +967,20 @@ static int pair_kind(const rvf_plan* p);
 // Columns per tile.  The wave-queue kernels claim tiles dynamically and run
 // best on narrow ones (the tail of a launch is one tile's time; measured on
 // the occluded 13,824-element cube, mixed 250 / 240 / 234 ms at 16 / 8 / 4
-// columns); the other kernels amortise their per-tile set-up over
-// RVF_CHUNK columns.
-static int default_columns(int kind) { return rvf_queue_kind(kind) ? RVF_CHUNK_QUEUE : RVF_CHUNK; }
-static int plan_columns(const rvf_plan* p) { return p->cw_req ? p->cw_req : default_columns(pair_kind(p)); }
+// columns, later 160.4 / 158.0 / 158.2 ms at 4 / 2 / 1 and FP64 656 / 651 /
+// 649 ms; on the occluded plates mixed 205.3 / 204.3 / 205.9 and FP64 1111 /
+// 1103 / 1100 ms): two for the mixed split kernels, one for the FP64 queue
+// kernels, four for the other wave-queue kernels; the other kernels amortise
+// their per-tile set-up over RVF_CHUNK columns.
+static int default_columns(int kind) {
+    if (!rvf_queue_kind(kind)) return RVF_CHUNK;
+    if (rvf_mixed_split_kind(kind)) return 2;
+    if (kind >= RVF_K_FP64_QUEUE1 && kind <= RVF_K_FP64_QUEUE3) return 1;
+    return RVF_CHUNK_QUEUE;
+}
+static int plan_columns(const rvf_plan* p) {
+    return p->cw_req ? p->cw_req : default_columns(pair_kind(p));
+}
 // Lanes with an empty slot before a wave-queue refill.  Measured on the full
 // occluded cube, mixed split 3: 12, 16, 20, 24, 28, 32, 40, 48 -> 182.3,
 // 180.2, 179.1, 178.4, 178.4, 180.9, 183.7, 188.1 ms.
@@ -1640,7 +1650,7 @@ int rvf_plan_autotune(rvf_plan* p, int64_t sample_tiles, rvf_tune_entry* ent, in
     if (!rc) {
         p->opts.variant = best_v;
         const int cdef = plan_columns(p);  // the choice's default width (cw_req is 0 here)
-        for (int cols : {RVF_CHUNK_QUEUE, 2 * RVF_CHUNK_QUEUE, RVF_CHUNK}) {
+        for (int cols : {1, 2, RVF_CHUNK_QUEUE, 2 * RVF_CHUNK_QUEUE, RVF_CHUNK}) {
             if (cols == cdef) continue;
             float ms = 0;
             if ((rc = measure(best_v, best_r, cols, big, &ms))) break;

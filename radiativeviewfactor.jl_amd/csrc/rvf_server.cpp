@@ -187,10 +187,11 @@ size_t estimate_bytes(const Req& r) {
     const int64_t maxpts = std::max<int64_t>(nq * nq, tri_pts);
     const int64_t NS = cdiv(std::max<int64_t>(r.N, 1), RVF_STRIP) * RVF_STRIP;
     // row partials: one 64-lane record per tile; tiles of chunks of the
-    // narrowest default width (w columns) number about N^2 / (128 w) plus one
-    // per chunk, and each of up to 64 extra chunks at group boundaries adds
-    // at most one tile per strip
-    const int64_t w = RVF_CHUNK_QUEUE;
+    // request's default width (w columns: occluded solves 2 mixed, 1 FP64;
+    // 4 bounds every other kind) number about N^2 / (128 w) plus one per
+    // chunk, and each of up to 64 extra chunks at group boundaries adds at
+    // most one tile per strip
+    const int64_t w = r.T > 0 ? (r.opts.precision == RVF_MIXED ? 2 : 1) : RVF_CHUNK_QUEUE;
     const int64_t nchunks = cdiv(r.N, w) + 64;
     const int64_t tiles = cdiv(r.N * r.N, 128 * w) + nchunks + 64 * (NS / RVF_STRIP);
     size_t b = 0;

@@ -37,9 +37,19 @@ def _mixed_split(variant, prec):
     return prec == "mixed" and variant in (api.VARIANT_SPLIT(1), api.VARIANT_SPLIT(2), api.VARIANT_SPLIT(3))
 
 
-def _default_columns(variant):
-    # wave-queue traversal variants run on tiles of 4 columns, the others 16
-    return 16 if variant == api.VARIANT_LANE else 4
+def _default_columns(variant, prec):
+    # the mixed split variants run on tiles of 2 columns, the FP64 queue
+    # variants 1, the other wave-queue variants 4, the lane variant 16
+    if variant == api.VARIANT_LANE:
+        return 16
+    if _mixed_split(variant, prec):
+        return 2
+    if prec == "fp64" and variant in (api.VARIANT_QUEUE(1), api.VARIANT_QUEUE(2), api.VARIANT_QUEUE(3)):
+        return 1
+    return 4
+
+
+WIDTHS = {1, 2, 4, 8, 16}
 
 
 def _default_refill(variant, prec):
@@ -72,7 +82,7 @@ def test_autotune_measures_all_variants_and_keeps_results(prec, tmp_path):
     # columns, then (mixed split kernels) their three-wave build.  Another
     # configuration replaces the current choice only if it is faster than the
     # best so far by more than twice the default's spread (at least 1 %).
-    assert all(e["columns"] == _default_columns(e["variant"]) for e in meas[:len(variants)])
+    assert all(e["columns"] == _default_columns(e["variant"], prec) for e in meas[:len(variants)])
     assert all(e["refill_min"] == _default_refill(e["variant"], prec) for e in meas[:len(variants)])
     rd = lambda v: _default_refill(v, prec)  # noqa: E731
     vdef = api.VARIANT_SPLIT(3) if prec == "mixed" else api.VARIANT_QUEUE(1)
@@ -94,13 +104,13 @@ def test_autotune_measures_all_variants_and_keeps_results(prec, tmp_path):
         assert sorted(e["refill_min"] for e in again[k:k + len(others)]) == others
         for e in again[k:k + len(others)]:
             assert e["variant"] == choice[0]
-            assert e["columns"] == _default_columns(choice[0])
+            assert e["columns"] == _default_columns(choice[0], prec)
             if e["ms"] < best_ms * (1 - margin):
                 best_ms, choice = e["ms"], (e["variant"], e["refill_min"], 0)
         k += len(others)
-    nw = len({4, 8, 16} - {_default_columns(choice[0])})
+    nw = len(WIDTHS - {_default_columns(choice[0], prec)})
     widths = [e["columns"] for e in again[k:k + nw]]
-    assert sorted(widths) == sorted({4, 8, 16} - {_default_columns(choice[0])})
+    assert sorted(widths) == sorted(WIDTHS - {_default_columns(choice[0], prec)})
     for e in again[k:k + nw]:
         assert e["variant"] == choice[0] and e["refill_min"] == choice[1] and e["waves"] == 0
         if e["ms"] < best_ms * (1 - margin):
@@ -111,13 +121,13 @@ def test_autotune_measures_all_variants_and_keeps_results(prec, tmp_path):
         assert len(again) == k + 1
         e = again[k]
         assert (e["variant"], e["refill_min"], e["waves"]) == (choice[0], choice[1], 3)
-        assert e["columns"] == (choice[2] or _default_columns(choice[0]))
+        assert e["columns"] == (choice[2] or _default_columns(choice[0], prec))
         if e["ms"] < best_ms * (1 - margin):
             waves = 3
     else:
         assert len(again) == k
     assert (v, r) == choice[:2] and v in variants
-    assert cols == (choice[2] or _default_columns(v))
+    assert cols == (choice[2] or _default_columns(v, prec))
     assert name.startswith("k_pair_w3<" if waves == 3 else "k_pair<"), name
     names = ("F", "A", "rowsum", "F_group", "A_group", "tags", "audit_count", "audit_hash")
     for nm, a, b in zip(names, got, ref):

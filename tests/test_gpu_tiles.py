@@ -61,7 +61,10 @@ def test_occluded_results_do_not_depend_on_tile_width(orc, precision, variant):
     m, obs = _scene()
     out = _solve_widths(m, obs, 3, precision, variant)
     lane = variant == api.VARIANT_LANE
-    assert out[0]["cols"] == (16 if lane else 4)  # the kernel's default width
+    # the kernel's default width: mixed split (the mixed default) 2, FP64
+    # queue (the FP64 default) 1, the other wave-queue variants 4, lane 16
+    default = 16 if lane else (2 if precision == "mixed" else 1) if variant == 0 else 4
+    assert out[0]["cols"] == default
     for w in WIDTHS:
         assert out[w]["cols"] == w
         _same(out[w], out[0], audited=True)
@@ -97,6 +100,6 @@ def test_tile_width_out_of_range_is_refused():
     for bad in (-1, 17, 1 << 20):
         with pytest.raises(_lib.RVFError, match="columns must be in 0..16"):
             plan.tune_tiles(bad)
-    assert plan.columns == 4
+    assert plan.columns == 2  # the mixed split default
     plan.run(); plan.sync()  # the plan is unchanged and still runs
     plan.close(); do.close(); dm.close()
